@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""HMC trajectories/s on the Nx=288, Nt=128 honeycomb torus: batched mixed precision
+(FP32 MFMA force, FP64 action) against the plain FP64 form, with the force error.
+
+Workload: 12x12-cell honeycomb torus (Nx=288), Nt=128, beta=8, U=3, mu=0, EXP hopping,
+particle/hole basis, sigmaKappa=-1; HMC trajectories of length 1 with 40 leapfrog steps
+(41 force evaluations and 1 action evaluation each); fields start from N(0, U*beta/Nt)
+with per-chain seeds.  A "step" is one trajectory of every chain of the batch.
+
+Prints one JSON line (see the fields below).  Only one GPU is used; --gpus > 1 exits.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "HMC traj/s at Nx=288 Nt=128: batched FP32-force vs plain FP64; force rel err"
+L, NT, BETA, U, NSTEP, LENGTH = 12, 128, 8.0, 3.0, 40, 1.0
+PEAK = {"f32": 157.3, "f64": 78.6}  # dense MFMA TFLOP/s, MI355X
+KIND_NAMES = ["gemm64_rc_pre", "gemm64_rc_row", "gemm64_cr_pre", "gemm64_cc", "gemm32_rc_row", "gemm32_cr_pre",
+              "scale64", "scale32", "lu", "lu_solve", "diag64", "diag32", "copy64", "cvt32to64", "cvt64to32",
+              "phase"]
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed trajectories (per chain) of the batched form")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--chains", type=int, default=8, help="independent chains advanced together")
+    ap.add_argument("--group", type=int, default=8, help="time slices per FP32 group")
+    ap.add_argument("--plain-steps", type=int, default=1, help="timed trajectories of the plain FP64 form")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(kappa_tilde, ut, nx):
+    """The oracle (numpy/OpenBLAS complex128 restatement of the reference algorithm) on
+    the host: 2 forces + 2 evals at the full size, scaled to 41 forces + 1 eval."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import hubbard_ref as ref
+    act = ref.SumAction(ref.HubbardGaugeAction(ut), ref.HubbardFermiAction(kappa_tilde, 0.0, -1, ref.EXP))
+    phi = np.random.RandomState(7).normal(0, np.sqrt(ut), nx * NT) + 0j
+    act.force(phi)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        act.force(phi)
+    tf = (time.perf_counter() - t0) / 2
+    t0 = time.perf_counter()
+    for _ in range(2):
+        act.eval(phi)
+    te = (time.perf_counter() - t0) / 2
+    traj = (NSTEP + 1) * tf + te
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count()
+    return {"value": 1.0 / traj, "unit": "traj/s", "cores": cores, "kind": "port",
+            "sample": f"oracle (numpy complex128, OpenBLAS) force {tf:.3f} s and eval {te:.3f} s at Nx=288 "
+                      f"Nt=128, each timed twice; trajectory = {NSTEP + 1} forces + 1 eval",
+            "force_s": tf, "eval_s": te}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on exactly one GPU (the chains of the batch share it); "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import isle_amd as isle
+    from isle_amd import _lib
+    from isle_amd import action as A
+    from isle_amd.evolver import BatchedConstStepLeapfrog, NumpyRNG
+
+    dev = _lib.device()
+    lat = isle.honeycomb(L, nt=NT)
+    nx = lat.nx()
+    kt = lat.hopping() * BETA / NT
+    ut = U * BETA / NT
+    B = args.chains
+
+    def make(prec):
+        ferm = A.HubbardFermiAction(kt, 0.0, -1, A.HFAHopping.EXP, precision=prec, group=args.group)
+        return A.SumAction(A.HubbardGaugeAction(ut), ferm), ferm
+
+    # ---------------- batched mixed precision
+    act_m, ferm_m = make("mixed")
+    rngs = [NumpyRNG(1000 + b) for b in range(B)]
+    phis = torch.as_tensor(np.stack([r.normal(0, np.sqrt(ut), nx * NT) + 0j for r in rngs]), device=dev)
+    acts = np.array([act_m.eval(phis[b]) for b in range(B)])
+    ev = BatchedConstStepLeapfrog(act_m, LENGTH, NSTEP, rngs)
+    for _ in range(args.warmup):
+        phis, acts, _ = ev.evolve(phis, acts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        phis, acts, _ = ev.evolve(phis, acts)
+    torch.cuda.synchronize()
+    t_mixed = time.perf_counter() - t0
+    traj_per_s = B * args.steps / t_mixed
+    dh = np.array(ev.deltaH[args.warmup:]).ravel()
+    acc = np.array(ev.trajPoints[args.warmup:]).ravel()
+
+    # ---------------- force error and kernel roofline (mixed force of one chain batch)
+    act_p, ferm_p = make("fp64")
+    phi0 = phis[0:1].contiguous()
+    f_m = ferm_m.force_batch(phi0)[0].cpu().numpy()
+    f_p = ferm_p.force_batch(phi0)[0].cpu().numpy()
+    force_err = float(np.linalg.norm(f_m - f_p) / np.linalg.norm(f_p))
+    force_err_max = float(np.max(np.abs(f_m - f_p)) / np.max(np.abs(f_p)))
+    nk = len(KIND_NAMES)
+    prof = np.zeros(4 * nk)
+    g = ferm_m._group(NT)
+    _lib.check(_lib.load().isle_profile_kernels(ferm_m.hfm.handle, _lib.ptr(phis), 1, NT, B, g, 2, 0, 3,
+                                                _lib.dptr(prof), nk))
+    prof = prof.reshape(nk, 4)
+    dom = int(np.argmax(prof[:, 1]))
+    launches, ms_op, flops_op = prof[dom, 0], prof[dom, 1], prof[dom, 2]
+    avg_ms = ms_op / launches
+    achieved = flops_op / launches / (avg_ms * 1e-3) / 1e12
+    prec = "f32" if "32" in KIND_NAMES[dom] else "f64"
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK[prec], "unit": "TFLOP/s",
+                "frac": achieved / PEAK[prec], "traffic": None, "kernel": KIND_NAMES[dom],
+                "avg_launch_ms": avg_ms, "flops_per_launch": flops_op / launches,
+                "force_ms": float(prof[:, 1].sum()),
+                "breakdown_ms": {KIND_NAMES[k]: round(float(prof[k, 1]), 4) for k in range(nk) if prof[k, 0] > 0}}
+
+    # ---------------- plain FP64, one chain, unbatched
+    plain = None
+    if not args.no_plain:
+        rng = NumpyRNG(1000)
+        phi1 = torch.as_tensor((rng.normal(0, np.sqrt(ut), nx * NT) + 0j).reshape(1, -1), device=dev)
+        a1 = np.array([act_p.eval(phi1[0])])
+        ev_p = BatchedConstStepLeapfrog(act_p, LENGTH, NSTEP, [rng])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            phi1, a1, _ = ev_p.evolve(phi1, a1)
+        torch.cuda.synchronize()
+        plain = args.plain_steps / (time.perf_counter() - t0)
+
+    cpu = None if args.no_cpu else cpu_baseline(kt, ut, nx)
+    out = {
+        "metric": METRIC, "value": traj_per_s, "unit": "traj/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * t_mixed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 force (MFMA) + f64 action", "data": "synthetic",
+        "config": {"workload": "honeycomb 12x12 torus Nx=288 Nt=128 beta=8 U=3 mu=0 EXP particle-hole "
+                               "sigmaKappa=-1, HMC length 1 with 40 leapfrog steps",
+                   "nx": nx, "nt": NT, "chains": B, "group": g, "parallelism": "independent chains, 1 GPU"},
+        "plain_fp64_traj_per_s": plain,
+        "speedup_vs_plain_fp64": (traj_per_s / plain) if plain else None,
+        "force_rel_err": force_err, "force_rel_err_max": force_err_max,
+        "force_evals_per_s": traj_per_s * (NSTEP + 1),
+        "acceptance": float(acc.mean()) if acc.size else None,
+        "mean_dH": float(dh.mean()) if dh.size else None,
+        "mean_exp_minus_dH": float(np.exp(-dh).mean()) if dh.size else None,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

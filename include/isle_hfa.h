@@ -1,0 +1,128 @@
+/*
+ * isle_hfa.h -- C ABI of the gfx950 Hubbard fermion-action library (libisle_hfa.so).
+ *
+ * Plain pointers and sizes only.  "device" pointers are HIP device allocations;
+ * complex numbers are complex128, interleaved (re, im).  Fields phi are time-major:
+ * element (t, x) at index t*nx + x.  Batched entry points take `batch` fields stored
+ * back to back (stride nx*nt complex).  Every function returns ISLE_OK (0) or an
+ * error code; isle_last_error() gives the message of the last failure on the thread.
+ * Work is ordered on the stream set with isle_hfm_set_stream (default: null stream).
+ *
+ * Each entry point replaces the reference interface cited beside it
+ * (paths relative to the evanberkowitz/isle source tree).
+ */
+#ifndef ISLE_HFA_H
+#define ISLE_HFA_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ISLE_OK 0
+#define ISLE_ERR_VALUE 1   /* maps to ValueError   (std::invalid_argument) */
+#define ISLE_ERR_INDEX 2   /* maps to IndexError   (std::out_of_range)     */
+#define ISLE_ERR_RUNTIME 3 /* maps to RuntimeError (std::runtime_error)    */
+#define ISLE_ERR_HIP 4     /* HIP runtime failure                          */
+
+#define ISLE_HOPPING_DIA 0 /* HFAHopping::DIA, src/isle/cpp/action/hubbardFermiAction.hpp:16 */
+#define ISLE_HOPPING_EXP 1 /* HFAHopping::EXP                                              */
+#define ISLE_PARTICLE 0    /* Species::PARTICLE, src/isle/cpp/species.hpp:10-13            */
+#define ISLE_HOLE 1        /* Species::HOLE                                                */
+
+typedef struct isle_hfm isle_hfm; /* opaque fermion-matrix plan */
+
+/* Message of the last error on this thread. */
+const char* isle_last_error(void);
+
+/* HubbardFermiMatrixExp/Dia(kappaTilde, muTilde, sigmaKappa)
+ *   src/isle/cpp/bind/bind_hubbardFermiMatrix.cpp:26, hubbardFermiMatrixExp.cpp:52-68,
+ *   hubbardFermiMatrixDia.cpp:49-62.
+ * kappa_tilde: host, nx*nx row-major, symmetric. */
+int isle_hfm_create(isle_hfm** out, int hopping, int nx, const double* kappa_tilde, double mu_tilde,
+                    int sigma_kappa);
+int isle_hfm_destroy(isle_hfm* h);
+/* stream: a hipStream_t (e.g. torch.cuda.current_stream().cuda_stream) */
+int isle_hfm_set_stream(isle_hfm* h, void* stream);
+int isle_hfm_sync(isle_hfm* h);
+
+/* HubbardFermiMatrixExp::expKappa(species, inv) / HubbardFermiMatrixDia::K(species)
+ *   hubbardFermiMatrixExp.cpp:76-93, hubbardFermiMatrixDia.cpp:96-103.
+ * out_host: nx*nx real row-major (for DIA, inv is ignored and K is returned). */
+int isle_hfm_slice_matrix(isle_hfm* h, int species, int inv, double* out_host);
+
+/* HubbardFermiMatrix{Exp,Dia}::F(tp, phi, species, inv)
+ *   bind_hubbardFermiMatrix.cpp:29-30, hubbardFermiMatrixExp.cpp:119-141, hubbardFermiMatrixDia.cpp:118-131.
+ * out: device, nx*nx complex128 row-major. */
+int isle_hfm_F(isle_hfm* h, int tp, const double* phi, int nt, int species, int inv, double* out);
+
+/* logdetM(hfm, phi, species) -> complex, Im in [-pi, pi]
+ *   bind_hubbardFermiMatrix.cpp:51-52, hubbardFermiMatrixExp.cpp:615-625, hubbardFermiMatrixDia.cpp:571-597.
+ * Plain FP64, the reference algorithm.  out_host: 2 doubles. */
+int isle_logdetM(isle_hfm* h, const double* phi, int nt, int species, double* out_host);
+
+/* Batched logdetM for the action's eval (hubbardFermiAction.cpp:230-272).
+ * mode 0: reference algorithm in FP64; mode 1: FP64 products of groups of `group` slices
+ * formed in parallel first.  nspecies 1 (particle) or 2 (particle, hole); spin != 0
+ * evaluates at aux = -i*phi (spin basis).  out_host: [batch][nspecies][2]. */
+int isle_logdetM_batch(isle_hfm* h, const double* phi, int nt, int batch, int mode, int group, int nspecies,
+                       int spin, double* out_host);
+
+/* forceDirectSinglePart(hfm, phi, K, species) for each species (without the -i factor)
+ *   src/isle/cpp/action/hubbardFermiAction.cpp:16-60.
+ * mode 0: plain FP64 reference algorithm; mode 1: FP32 chains over groups of `group`
+ * slices on MFMA with an FP64 combination of the group products.
+ * out_dev: device [batch][nspecies][nx*nt] complex128. */
+int isle_force_parts(isle_hfm* h, const double* phi, int nt, int batch, int mode, int group, int nspecies,
+                     int spin, double* out_dev);
+
+/* solveM(hfm, phi, species, rhss): solves M x = b for every row b of rhss
+ *   bind_hubbardFermiMatrix.cpp:53-55, hubbardFermiMatrixExp.cpp:645-694, hubbardFermiMatrixDia.cpp:617-673.
+ * rhss, out: device [nrhs][nx*nt] complex128. */
+int isle_solveM(isle_hfm* h, const double* phi, int nt, int species, const double* rhss, int nrhs, double* out);
+
+/* Combination of the species terms into the fermion force, n complex elements
+ *   (hubbardFermiAction.cpp:243-272): mode 0: -i(fp - fh); 1: -i(fp - conj fp); 2: fh - fp. */
+int isle_combine_force(const double* fp, const double* fh, double* out, long n, int mode, int accumulate,
+                       void* stream);
+
+/* HubbardGaugeAction::force = -phi/utilde (hubbardGaugeAction.cpp:11-13). */
+int isle_gauge_force(const double* phi, double* out, long n, double utilde, int accumulate, void* stream);
+
+/* sum(phi) and the bilinear (phi, phi) of `batch` vectors of n complex elements with
+ * the given stride (complex elements); out_host [batch][4] = (Re s1, Im s1, Re s2, Im s2).
+ *   hubbardGaugeAction.cpp:7-9 (eval), hubbardFermiMatrixExp.cpp:610 (sum of phi). */
+int isle_field_sums(const double* phi, long n, long stride, int batch, double* out_host, void* stream);
+
+/* Leapfrog kick  pi += eps*Re(force) [ - eps*Re(phi)/utilde if phi != NULL and utilde > 0 ]
+ * and drift phi += eps*pi on n complex elements (src/isle/cpp/integrator.cpp:20-32). */
+int isle_leapfrog_kick(double* pi, const double* force, const double* phi, long n, double eps, double utilde,
+                       void* stream);
+int isle_leapfrog_drift(double* phi, const double* pi, long n, double eps, void* stream);
+
+/* out = c * in for n complex elements (the spin basis' aux = -i*phi, hubbardFermiAction.cpp:261). */
+int isle_cscale(const double* in, double* out, long n, double cr, double ci, void* stream);
+
+/* Instrumentation (no counterpart in the reference, which only has timeit benchmarks,
+ * benchmarks/logdet.py:46,85): replays a cached operation `reps` times with a HIP event
+ * after every launch.  op: 0 plain force, 1 mixed force, 2 plain logdet, 3 grouped logdet.
+ * For launch kind k < nkinds: out_host[4k..4k+3] = (launches per operation, ms per
+ * operation, algorithmic real flops per operation, 0).  Kinds: 0 GEMM64 real x complex
+ * (pre-scaled), 1 GEMM64 real x complex (row-scaled), 2 GEMM64 complex x real, 3 GEMM64
+ * complex x complex, 4 GEMM32 real x complex, 5 GEMM32 complex x real, 6-7 chain starts,
+ * 8 LU, 9 LU solve, 10-11 diagonal extraction, 12-14 copies, 15 phases. */
+int isle_profile_kernels(isle_hfm* h, const double* phi, int op, int nt, int batch, int group, int nspecies,
+                         int spin, int reps, double* out_host, int nkinds);
+
+/* logdet(matrix): LU with partial pivoting, sum log U_ii + i*pi*(swap parity), Im in [-pi, pi]
+ *   src/isle/cpp/math.hpp:265-334 (bound as isle.logdet).  a: device n*n complex128. */
+int isle_logdet(const double* a, int n, double* out_host, void* stream);
+
+/* invert(matrix): getrf + getri equivalent (src/isle/cpp/math.hpp:222-226).
+ * a, out: device n*n complex128 (a is not modified). */
+int isle_invert(const double* a, int n, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISLE_HFA_H */

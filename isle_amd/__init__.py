@@ -1,0 +1,27 @@
+"""isle_amd: the Hubbard fermion-action hot path of isle on AMD MI355X (gfx950).
+
+Drop-in names for the reference's ``isle`` / ``isle.action`` surface on this path
+(src/isle/__init__.py, src/isle/cpp_wrappers.py, src/isle/cpp/bind/*):
+
+    Vector, CDVector, Species, Lattice, isBipartite, loadLattice,
+    HubbardFermiMatrixExp, HubbardFermiMatrixDia, logdetM, solveM, logdet, invert,
+    leapfrog, action.{Action, SumAction, HubbardGaugeAction, HubbardFermiAction,
+    makeHubbardFermiAction, HFAHopping, HFABasis, HFAAlgorithm},
+    evolver.{ConstStepLeapfrog, BinarySelector, EvolutionStage}, random.NumpyRNG.
+
+Every numerical routine runs in the hand-written HIP kernels of libisle_hfa.so; there
+is no CPU fallback.
+"""
+from . import action, evolver
+from .core import (CDVector, HubbardFermiMatrixDia, HubbardFermiMatrixExp, Species, Vector, as_field, invert,
+                   logdet, logdetM, solveM, to_first_log_branch)
+from .integrator import leapfrog, leapfrog_batch
+from .lattice import Lattice, honeycomb, isBipartite, loadLattice
+
+random = evolver  # isle.random.NumpyRNG lives in isle_amd.evolver
+
+__all__ = [
+    "CDVector", "HubbardFermiMatrixDia", "HubbardFermiMatrixExp", "Lattice", "Species", "Vector", "action",
+    "as_field", "evolver", "honeycomb", "invert", "isBipartite", "leapfrog", "leapfrog_batch", "loadLattice",
+    "logdet", "logdetM", "random", "solveM", "to_first_log_branch",
+]

@@ -1,0 +1,267 @@
+"""Actions: the Action protocol, SumAction, HubbardGaugeAction and HubbardFermiAction.
+
+Mirrors the reference's ``isle.action`` module (src/isle/cpp/bind/bind_action.cpp:100-324,
+src/isle/cpp/action/*.cpp).  Fields are complex128 tensors on the GPU; ``eval`` returns a
+Python complex and ``force`` a device tensor (the complex -dS/dphi).
+
+Besides the reference's single-field ``eval``/``force``, HubbardFermiAction offers
+``eval_batch``/``force_batch`` over a [batch, nx*nt] tensor of independent fields, and a
+``precision`` switch:
+
+* ``"fp64"``  - the reference DIRECT_SINGLE algorithm, every product in FP64;
+* ``"mixed"`` - force from FP32 chains over groups of ``group`` time slices on MFMA,
+  combined in FP64; eval (the Metropolis action) from FP64 group products.
+"""
+import enum
+
+import numpy as np
+import torch
+
+from . import _lib
+from .core import (HubbardFermiMatrixDia, HubbardFermiMatrixExp, Lattice, as_field, isBipartite,
+                   to_first_log_branch)
+
+
+class HFAHopping(enum.IntEnum):
+    """hubbardFermiAction.hpp:16."""
+
+    DIA = 0
+    EXP = 1
+
+
+class HFABasis(enum.IntEnum):
+    """hubbardFermiAction.hpp:19."""
+
+    PARTICLE_HOLE = 0
+    SPIN = 1
+
+
+class HFAAlgorithm(enum.IntEnum):
+    """hubbardFermiAction.hpp:25."""
+
+    DIRECT_SINGLE = 0
+    DIRECT_SQUARE = 1
+
+
+class Action:
+    """Abstract action (src/isle/cpp/action/action.hpp:14-22).  Subclass and implement
+    ``eval(phi) -> complex`` and ``force(phi) -> tensor``.  ``a + b`` builds a SumAction."""
+
+    def eval(self, phi):
+        raise NotImplementedError
+
+    def force(self, phi):
+        raise NotImplementedError
+
+    def __add__(self, other):
+        return SumAction(self, other)
+
+
+class SumAction(Action):
+    """Sum of actions (sumAction.hpp:21-59, sumAction.cpp:46-58).  Holds references to
+    its terms (the reference keeps non-owning pointers alive through keep_alive)."""
+
+    def __init__(self, *actions):
+        self._actions = list(actions)
+
+    def add(self, action):
+        self._actions.append(action)
+        return action
+
+    def __getitem__(self, idx):
+        if idx >= len(self._actions) or idx < -len(self._actions):
+            raise IndexError("Index of action is out of range")
+        return self._actions[idx]
+
+    def __len__(self):
+        return len(self._actions)
+
+    def size(self):
+        return len(self._actions)
+
+    def clear(self):
+        self._actions.clear()
+
+    def eval(self, phi):
+        res = 0j
+        for a in self._actions:
+            res += complex(a.eval(phi))
+        return res
+
+    def force(self, phi):
+        phi = as_field(phi)
+        res = torch.zeros_like(phi)
+        for a in self._actions:
+            res = res + as_field(a.force(phi))
+        return res
+
+
+class HubbardGaugeAction(Action):
+    """S = (phi, phi)/(2 utilde), bilinear; force -phi/utilde (hubbardGaugeAction.cpp:7-13)."""
+
+    def __init__(self, utilde):
+        self._utilde = float(utilde)
+
+    @property
+    def utilde(self):
+        return self._utilde
+
+    def eval(self, phi):
+        phi = as_field(phi)
+        out = np.empty(4)
+        n = phi.numel()
+        _lib.check(_lib.load().isle_field_sums(_lib.ptr(phi), n, n, 1, _lib.dptr(out), _lib.stream_ptr()))
+        return complex(out[2], out[3]) / 2.0 / self._utilde
+
+    def force(self, phi):
+        phi = as_field(phi)
+        out = torch.empty_like(phi)
+        _lib.check(_lib.load().isle_gauge_force(_lib.ptr(phi), _lib.ptr(out), phi.numel(), self._utilde, 0,
+                                                _lib.stream_ptr()))
+        return out
+
+
+def _hole_shortcut_possible(basis, kappa, mu_tilde, sigma_kappa):
+    """hubbardFermiAction.cpp:114-157."""
+    import logging
+    log = logging.getLogger("HubbardFermiAction")
+    if basis == HFABasis.SPIN:
+        log.info("Not using shortcut for hole determinant, spin basis is not supported.")
+        return False
+    if not isBipartite(kappa):
+        log.info("Not using shortcut for hole determinant, lattice is not bipartite.")
+        return False
+    if mu_tilde != 0.0:
+        log.info("Not using shortcut for hole determinant, chemical potential is not zero.")
+        return False
+    if sigma_kappa != +1:
+        log.info("Not using shortcut for hole determinant, sigmaKappa is not +1")
+        return False
+    log.info("Using shortcut to calculate hole determinant from particle determinant.")
+    return True
+
+
+class HubbardFermiAction(Action):
+    """Fermion action -log det M(phi) M(-phi) (hubbardFermiAction.hpp:96-142).
+
+    Only the DIRECT_SINGLE algorithm runs on the GPU; DIRECT_SQUARE (a dense
+    (NxNt)^2 inverse of Q) raises NotImplementedError.
+    """
+
+    def __init__(self, kappaTilde, muTilde, sigmaKappa, hopping=HFAHopping.DIA,
+                 basis=HFABasis.PARTICLE_HOLE, algorithm=HFAAlgorithm.DIRECT_SINGLE,
+                 allowShortcut=False, precision="fp64", group=8):
+        if HFAAlgorithm(algorithm) != HFAAlgorithm.DIRECT_SINGLE:
+            raise NotImplementedError("HFAAlgorithm.DIRECT_SQUARE is not implemented on the GPU")
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        kappa = kappaTilde.hopping() if isinstance(kappaTilde, Lattice) else np.asarray(kappaTilde, dtype=np.float64)
+        self.hopping = HFAHopping(hopping)
+        self.basis = HFABasis(basis)
+        self.algorithm = HFAAlgorithm(algorithm)
+        cls = HubbardFermiMatrixExp if self.hopping == HFAHopping.EXP else HubbardFermiMatrixDia
+        self.hfm = cls(kappa, muTilde, sigmaKappa)
+        self.precision = precision
+        self.group = int(group)
+        # log of the condition-number bound of one time slice: the FP32 chains of the
+        # mixed force span at most `group` slices and are kept within cond <= e^3
+        r = float(np.max(np.sum(np.abs(kappa), axis=1))) if kappa.size else 0.0
+        if self.hopping == HFAHopping.EXP:
+            self._slice_spread = 2.0 * r
+        else:
+            mu = abs(float(muTilde))
+            lo = 1.0 - mu - r
+            self._slice_spread = float(np.log((1.0 + mu + r) / lo)) if lo > 0 else np.inf
+        self.shortcut = bool(allowShortcut) and _hole_shortcut_possible(self.basis, kappa, float(muTilde),
+                                                                         int(sigmaKappa))
+
+    def _mode(self):
+        return 0 if self.precision == "fp64" else 1
+
+    def _group(self, nt):
+        """Slices per FP32 group: the largest divisor of nt not above ``group`` whose
+        chain keeps the slice-conditioning bound g * spread <= 3."""
+        g = min(self.group, nt)
+        if self._slice_spread > 0:
+            g = max(1, min(g, int(3.0 / self._slice_spread)))
+        while nt % g:
+            g -= 1
+        return g
+
+    def _shape(self, phi):
+        nx = self.hfm.nx()
+        phi = as_field(phi, nx)
+        if phi.dim() == 1:
+            phi = phi.reshape(1, -1)
+        return phi, phi.shape[0], phi.shape[1] // nx
+
+    def logdets(self, phis):
+        """[batch][nspecies] complex log det M of each field (aux = -i phi in the spin basis)."""
+        phis, batch, nt = self._shape(phis)
+        nsp = 1 if self.shortcut else 2
+        out = np.empty((batch, nsp, 2))
+        _lib.check(_lib.load().isle_logdetM_batch(
+            self.hfm.handle, _lib.ptr(phis), nt, batch, self._mode(), self._group(nt), nsp,
+            int(self.basis == HFABasis.SPIN), _lib.dptr(out)))
+        return out[..., 0] + 1j * out[..., 1]
+
+    def eval_batch(self, phis):
+        """Action of each field in a [batch, nx*nt] tensor (hubbardFermiAction.cpp:230-264)."""
+        ld = self.logdets(phis)
+        res = []
+        for row in ld:
+            if self.shortcut:
+                res.append(-to_first_log_branch(row[0] + np.conj(row[0])))
+            else:
+                res.append(-to_first_log_branch(row[0] + row[1]))
+        return np.array(res)
+
+    def eval(self, phi):
+        return complex(self.eval_batch(phi)[0])
+
+    def force_batch(self, phis, out=None, accumulate=False):
+        """Force -dS/dphi of each field (hubbardFermiAction.cpp:243-272), [batch, nx*nt]."""
+        phis, batch, nt = self._shape(phis)
+        nx = self.hfm.nx()
+        nsp = 1 if self.shortcut else 2
+        parts = torch.empty((batch, nsp, nx * nt), dtype=torch.complex128, device=phis.device)
+        lib = _lib.load()
+        _lib.check(lib.isle_force_parts(self.hfm.handle, _lib.ptr(phis), nt, batch, self._mode(), self._group(nt),
+                                        nsp, int(self.basis == HFABasis.SPIN), _lib.ptr(parts)))
+        if out is None:
+            out = torch.empty((batch, nx * nt), dtype=torch.complex128, device=phis.device)
+            accumulate = False
+        mode = 2 if self.basis == HFABasis.SPIN else (1 if self.shortcut else 0)
+        for b in range(batch):
+            fp = parts[b, 0]
+            fh = parts[b, 1] if nsp == 2 else parts[b, 0]
+            _lib.check(lib.isle_combine_force(_lib.ptr(fp), _lib.ptr(fh), _lib.ptr(out[b]), nx * nt, mode,
+                                              int(bool(accumulate)), _lib.stream_ptr()))
+        return out
+
+    def force(self, phi):
+        return self.force_batch(phi)[0]
+
+
+def makeHubbardFermiAction(lat_or_kappa, *args, **kwargs):
+    """Run-time dispatch of HubbardFermiAction (bind_action.cpp:114-222).
+
+    makeHubbardFermiAction(kappaTilde, muTilde, sigmaKappa, hopping=DIA, basis=PARTICLE_HOLE,
+                           algorithm=DIRECT_SINGLE, allowShortcut=False)
+    makeHubbardFermiAction(lat, beta, muTilde, sigmaKappa, hopping=DIA, ...)  with
+    kappaTilde = lat.hopping() * beta / lat.nt() (bind_action.cpp:206-221).
+    Extra keyword arguments: precision ('fp64' | 'mixed'), group (slices per FP32 group).
+    """
+    if isinstance(lat_or_kappa, Lattice):
+        beta, mu, sk, *rest = args
+        kappa = lat_or_kappa.hopping() * beta / lat_or_kappa.nt()
+    else:
+        mu, sk, *rest = args
+        kappa = np.asarray(lat_or_kappa, dtype=np.float64)
+    names = ("hopping", "basis", "algorithm", "allowShortcut")
+    defaults = {"hopping": HFAHopping.DIA, "basis": HFABasis.PARTICLE_HOLE,
+                "algorithm": HFAAlgorithm.DIRECT_SINGLE, "allowShortcut": False}
+    for name, val in zip(names, rest):
+        defaults[name] = val
+    defaults.update(kwargs)
+    return HubbardFermiAction(kappa, mu, sk, **defaults)

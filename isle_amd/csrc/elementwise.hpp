@@ -1,0 +1,355 @@
+// Streaming kernels: phase vectors, chain initialisation, diagonal extraction,
+// precision conversion, force combination, reductions and leapfrog updates.
+// All are HBM/L2-bound; they read and write each element once, coalesced.
+#pragma once
+
+#include "common.hpp"
+
+namespace isle {
+
+// ---------------------------------------------------------------------------
+// phase vectors d[t][x] = exp(sgn * i * phi[((t - 1) mod nt) * nx + x]),
+// phi interleaved complex128.  Written as split planes in FP64 and (optionally) FP32.
+struct PhaseItem {
+    int chain;     // index into the batch of fields (pointer = phi + chain * phi_stride)
+    int spin;      // 0: exp(sgn*i*phi); 1: exp(sgn*phi)  (the spin basis uses aux = -i*phi)
+    double sgn;
+    double* re64;
+    double* im64;
+    float* re32;
+    float* im32;
+    int nx, nt;
+};
+
+__global__ void phase_kernel(const PhaseItem* __restrict__ items, const double* __restrict__ phi,
+                             long phi_stride) {
+    const PhaseItem it = items[blockIdx.y];
+    const double* p = phi + it.chain * phi_stride;
+    const long total = (long)it.nx * it.nt;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const int t = (int)(e / it.nx), x = (int)(e % it.nx);
+        const int tm1 = t == 0 ? it.nt - 1 : t - 1;
+        const long src = 2 * ((long)tm1 * it.nx + x);
+        const double a = p[src], b = p[src + 1];
+        // z = sgn*i*(a+ib) = -sgn*b + i*sgn*a ;  spin: z = sgn*(a+ib)
+        const double zr = it.spin ? it.sgn * a : -it.sgn * b;
+        const double zi = it.spin ? it.sgn * b : it.sgn * a;
+        const double mag = exp(zr);
+        double s, c;
+        sincos(zi, &s, &c);
+        const double vr = mag * c, vi = mag * s;
+        if (it.re64) {
+            it.re64[e] = vr;
+            it.im64[e] = vi;
+        }
+        if (it.re32) {
+            it.re32[e] = (float)vr;
+            it.im32[e] = (float)vi;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Chain start: out = diag(d) * A (ROW) or A * diag(d) (COL) for real A (n x n).
+template <typename T>
+struct ScaleItem {
+    const T* a;
+    const T* d_re;
+    const T* d_im;
+    T* out_re;
+    T* out_im;
+    int n;
+    int mode;  // SCALE_ROW / SCALE_COL
+};
+
+template <typename T>
+__global__ void scale_real_kernel(const ScaleItem<T>* __restrict__ items) {
+    const ScaleItem<T> it = items[blockIdx.y];
+    const long total = (long)it.n * it.n;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(e / it.n), j = (int)(e % it.n);
+        const int di = it.mode == 1 ? i : j;
+        const T v = it.a[e];
+        it.out_re[e] = v * it.d_re[di];
+        it.out_im[e] = v * it.d_im[di];
+    }
+}
+
+// Complex matrix scaled by a diagonal: out = X diag(d) (COL) or diag(d) X (ROW).
+template <typename T>
+struct CScaleItem {
+    const T* x_re;
+    const T* x_im;
+    const T* d_re;
+    const T* d_im;
+    T* out_re;
+    T* out_im;
+    int n;
+    int mode;
+};
+
+template <typename T>
+__global__ void scale_complex_kernel(const CScaleItem<T>* __restrict__ items) {
+    const CScaleItem<T> it = items[blockIdx.y];
+    const long total = (long)it.n * it.n;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(e / it.n), j = (int)(e % it.n);
+        const int di = it.mode == 1 ? i : j;
+        const T xr = it.x_re[e], xi = it.x_im[e];
+        const T dr = it.d_re[di], dm = it.d_im[di];
+        it.out_re[e] = xr * dr - xi * dm;
+        it.out_im[e] = xr * dm + xi * dr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// out[i] = scale * sum_k L[i][k] R[k][i]  (L == null: identity), FP64 accumulation.
+// Result written interleaved complex128 at out[2*i], optional per-i divisor q.
+template <typename T>
+struct DiagItem {
+    const T* l_re;
+    const T* l_im;
+    const T* r_re;
+    const T* r_im;
+    double* out;
+    int n;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void diag_prod_kernel(const DiagItem<T>* __restrict__ items) {
+    const DiagItem<T> it = items[blockIdx.y];
+    const int n = it.n;
+    const int ii = threadIdx.x >> 4;   // 0..15
+    const int kk = threadIdx.x & 15;   // k lane
+    const int i = blockIdx.x * 16 + ii;
+    if (blockIdx.x * 16 >= n) return;
+    double sr = 0, si = 0;
+    if (i < n) {
+        if (it.l_re) {
+            for (int k = kk; k < n; k += 16) {
+                const double ar = it.l_re[(long)i * n + k], ai = it.l_im[(long)i * n + k];
+                const double br = it.r_re[(long)k * n + i], bi = it.r_im[(long)k * n + i];
+                sr += ar * br - ai * bi;
+                si += ar * bi + ai * br;
+            }
+        } else if (kk == 0) {
+            sr = it.r_re[(long)i * n + i];
+            si = it.r_im[(long)i * n + i];
+        }
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off, 16);
+        si += __shfl_xor(si, off, 16);
+    }
+    if (kk == 0 && i < n) {
+        it.out[2 * i] = sr;
+        it.out[2 * i + 1] = si;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// precision conversion between split planes (n elements)
+template <typename S, typename D>
+__global__ void convert_kernel(const S* __restrict__ src, D* __restrict__ dst, long n) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+        dst[e] = (D)src[e];
+}
+
+// split planes -> interleaved complex128
+__global__ void split_to_interleaved_kernel(const double* __restrict__ re, const double* __restrict__ im,
+                                            double* __restrict__ out, long n) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        out[2 * e] = re[e];
+        out[2 * e + 1] = im[e];
+    }
+}
+
+// interleaved complex128 -> split planes
+__global__ void interleaved_to_split_kernel(const double* __restrict__ in, double* __restrict__ re,
+                                            double* __restrict__ im, long n) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        re[e] = in[2 * e];
+        im[e] = in[2 * e + 1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Combine the two species' diagonal terms into the fermion force (interleaved).
+//  mode 0 (particle/hole):          out = -i (fp - fh)
+//  mode 1 (particle/hole shortcut): out = -i (fp - conj(fp))
+//  mode 2 (spin):                   out = fh - fp
+// ``accumulate`` adds to out instead of overwriting.
+__global__ void combine_force_kernel(const double* __restrict__ fp, const double* __restrict__ fh,
+                                     double* __restrict__ out, long n, int mode, int accumulate) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const double pr = fp[2 * e], pi = fp[2 * e + 1];
+        double hr, hi;
+        if (mode == 1) {
+            hr = pr;
+            hi = -pi;
+        } else {
+            hr = fh[2 * e];
+            hi = fh[2 * e + 1];
+        }
+        double vr, vi;
+        if (mode == 2) {
+            vr = hr - pr;
+            vi = hi - pi;
+        } else {  // -i (a + i b) = b - i a
+            const double ar = pr - hr, ai = pi - hi;
+            vr = ai;
+            vi = -ar;
+        }
+        if (accumulate) {
+            out[2 * e] += vr;
+            out[2 * e + 1] += vi;
+        } else {
+            out[2 * e] = vr;
+            out[2 * e + 1] = vi;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-vector sums of phi (complex) and of phi*phi (bilinear), one workgroup per vector.
+// out[4*b + 0..3] = (Re sum phi, Im sum phi, Re sum phi^2, Im sum phi^2)
+__global__ __launch_bounds__(256) void sums_kernel(const double* __restrict__ phi, long n, long stride,
+                                                   double* __restrict__ out) {
+    __shared__ double s[4][256];
+    const double* p = phi + blockIdx.x * stride;
+    double a = 0, b = 0, c = 0, d = 0;
+    for (long e = threadIdx.x; e < n; e += blockDim.x) {
+        const double r = p[2 * e], i = p[2 * e + 1];
+        a += r;
+        b += i;
+        c += r * r - i * i;
+        d += 2 * r * i;
+    }
+    s[0][threadIdx.x] = a;
+    s[1][threadIdx.x] = b;
+    s[2][threadIdx.x] = c;
+    s[3][threadIdx.x] = d;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int q = 0; q < 4; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[4 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------------------
+// Leapfrog pieces on interleaved complex128 vectors (n complex elements).
+//  kick:  pi += eps * Re(force) [ - eps * Re(phi) / utilde  when utilde > 0 ]
+//  drift: phi += eps * pi
+__global__ void kick_kernel(double* __restrict__ pi, const double* __restrict__ force,
+                            const double* __restrict__ phi, long n, double eps, double inv_utilde) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        double f = force ? force[2 * e] : 0.0;
+        if (phi) f -= phi[2 * e] * inv_utilde;
+        pi[2 * e] += eps * f;
+    }
+}
+
+__global__ void drift_kernel(double* __restrict__ phi, const double* __restrict__ pi, long n, double eps) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        phi[2 * e] += eps * pi[2 * e];
+        phi[2 * e + 1] += eps * pi[2 * e + 1];
+    }
+}
+
+// gauge force -phi/utilde, accumulated or written
+__global__ void gauge_force_kernel(const double* __restrict__ phi, double* __restrict__ out, long n,
+                                   double inv_utilde, int accumulate) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const double vr = -phi[2 * e] * inv_utilde, vi = -phi[2 * e + 1] * inv_utilde;
+        if (accumulate) {
+            out[2 * e] += vr;
+            out[2 * e + 1] += vi;
+        } else {
+            out[2 * e] = vr;
+            out[2 * e + 1] = vi;
+        }
+    }
+}
+
+// real symmetric helper for exp(X) by Taylor series: out = I + X/1!(I + X/2(I + ...)), Horner steps
+// done with GEMMs; this kernel forms out = I + c * Y (n x n).
+__global__ void identity_plus_kernel(const double* __restrict__ y, double c, double* __restrict__ out, int n) {
+    const long total = (long)n * n;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(e / n), j = (int)(e % n);
+        out[e] = c * y[e] + (i == j ? 1.0 : 0.0);
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// itemised copies / conversions (count elements each)
+template <typename S, typename D>
+struct CopyItem {
+    const S* src;
+    D* dst;
+    long count;
+};
+
+template <typename S, typename D>
+__global__ void copy_items_kernel(const CopyItem<S, D>* __restrict__ items) {
+    const CopyItem<S, D> it = items[blockIdx.y];
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < it.count; e += (long)gridDim.x * blockDim.x)
+        it.dst[e] = (D)it.src[e];
+}
+
+// solveM helpers: rhss [nrhs][nt*n] interleaved -> per-slice split planes [t][2][n][nrhs]
+__global__ void rhs_transpose_kernel(const double* __restrict__ rhss, double* __restrict__ out, int n, int nt,
+                                     int nrhs) {
+    const long total = (long)nt * n * nrhs;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(e % nrhs);
+        const long rest = e / nrhs;
+        const int k = (int)(rest % n);
+        const int t = (int)(rest / n);
+        const long src = 2 * ((long)c * nt * n + (long)t * n + k);
+        double* blk = out + (long)t * 2 * n * nrhs;
+        blk[(long)k * nrhs + c] = rhss[src];
+        blk[(long)n * nrhs + (long)k * nrhs + c] = rhss[src + 1];
+    }
+}
+
+// z_t += z_{t-1} along t (blocks of 2*n*nrhs doubles)
+__global__ void prefix_sum_t_kernel(double* __restrict__ z, long blk, int nt) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < blk; e += (long)gridDim.x * blockDim.x) {
+        double acc = z[e];
+        for (int t = 1; t < nt; ++t) {
+            acc += z[(long)t * blk + e];
+            z[(long)t * blk + e] = acc;
+        }
+    }
+}
+
+// z_t -= x for t < nt-1 (blocks of blk doubles)
+__global__ void sub_last_kernel(double* __restrict__ z, const double* __restrict__ x, long blk, int count) {
+    const long total = blk * count;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
+        z[e] -= x[e % blk];
+}
+
+// out = c * in (complex scalar c) on interleaved complex128 vectors
+__global__ void cscale_vec_kernel(const double* __restrict__ in, double* __restrict__ out, long n, double cr,
+                                  double ci) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const double a = in[2 * e], b = in[2 * e + 1];
+        out[2 * e] = a * cr - b * ci;
+        out[2 * e + 1] = a * ci + b * cr;
+    }
+}
+
+inline unsigned ew_blocks(long n) {
+    long b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+}  // namespace isle

@@ -1,0 +1,1223 @@
+// Implementation of the fermion-matrix plan (see plan.hpp) and the C ABI.
+#include "plan.hpp"
+
+#include <cmath>
+#include <cstring>
+
+#include "../../include/isle_hfa.h"
+
+namespace isle {
+
+namespace {
+
+constexpr int EW_THREADS = 256;
+
+// Host-side staging of one cached operation.
+struct Builder {
+    OpCache& op;
+    std::vector<char> blob;
+    struct Pending {
+        K kind;
+        size_t off;
+        int count, mm, mn, flag;
+        double flops;
+    };
+    std::vector<Pending> pend;
+
+    explicit Builder(OpCache& o) : op(o) {}
+
+    template <typename It>
+    void add(K kind, const std::vector<It>& items, int mm = 0, int mn = 0, int flag = 0) {
+        if (items.empty()) return;
+        size_t off = (blob.size() + 63) & ~size_t(63);
+        blob.resize(off + items.size() * sizeof(It));
+        std::memcpy(blob.data() + off, items.data(), items.size() * sizeof(It));
+        pend.push_back({kind, off, (int)items.size(), mm, mn, flag, flops_of(kind, items)});
+    }
+    template <typename It>
+    static double flops_of(K, const std::vector<It>&) { return 0.0; }
+    template <typename T>
+    static double flops_of(K kind, const std::vector<GemmItem<T>>& items) {
+        const double reals = (kind == K::GEMM64_CC) ? 4.0 : 2.0;  // real GEMMs per complex product
+        double f = 0;
+        for (const auto& g : items) f += reals * 2.0 * g.m * (double)g.n * g.k;
+        return f;
+    }
+    double* d64(size_t count) {
+        op.bufs.emplace_back(new DevBuf(count * sizeof(double)));
+        return op.bufs.back()->as<double>();
+    }
+    float* d32(size_t count) {
+        op.bufs.emplace_back(new DevBuf(count * sizeof(float)));
+        return op.bufs.back()->as<float>();
+    }
+    int* di(size_t count) {
+        op.bufs.emplace_back(new DevBuf(count * sizeof(int)));
+        return op.bufs.back()->as<int>();
+    }
+    void finish() {
+        op.bufs.emplace_back(new DevBuf(blob.size() ? blob.size() : 64));
+        char* dev = op.bufs.back()->as<char>();
+        if (!blob.empty()) ISLE_HIP(hipMemcpy(dev, blob.data(), blob.size(), hipMemcpyHostToDevice));
+        for (auto& p : pend) op.steps.push_back({p.kind, dev + p.off, p.count, p.mm, p.mn, p.flag, p.flops, 0});
+    }
+};
+
+// complex n x n matrix (split planes) inside a block array
+template <typename T>
+struct CMat {
+    T* re;
+    T* im;
+};
+template <typename T>
+inline CMat<T> cmat(T* base, long idx, long nn) {
+    T* r = base + idx * 2 * nn;
+    return {r, r + nn};
+}
+
+inline double sigma_of(int species) { return species == PARTICLE ? -1.0 : 1.0; }
+
+template <typename T>
+GemmItem<T> gitem(const T* a_re, const T* a_im, const T* b_re, const T* b_im, CMat<T> c, const T* pre_re,
+                  const T* pre_im, const T* post_re, const T* post_im, int n) {
+    GemmItem<T> g;
+    g.a_re = a_re;
+    g.a_im = a_im;
+    g.b_re = b_re;
+    g.b_im = b_im;
+    g.c_re = c.re;
+    g.c_im = c.im;
+    g.pre_re = pre_re;
+    g.pre_im = pre_im;
+    g.post_re = post_re;
+    g.post_im = post_im;
+    g.m = g.n = g.k = n;
+    g.lda = g.ldb = g.ldc = n;
+    return g;
+}
+
+inline std::complex<double> to_first_log_branch(std::complex<double> z) {
+    return {z.real(), std::remainder(z.imag(), 2.0 * M_PI)};
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+Plan::Plan(int hopping, int nx, const double* kappa, double mu, int sk)
+    : hop_(hopping), nx_(nx), mu_(mu), sk_(sk), kappa_(kappa, kappa + (size_t)nx * nx) {
+    if (nx <= 0) throw std::invalid_argument("nx must be positive");
+    if (hopping != DIA && hopping != EXP) throw std::invalid_argument("unknown hopping");
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < nx; ++j)
+            if (kappa_[(size_t)i * nx + j] != kappa_[(size_t)j * nx + i])
+                throw std::invalid_argument("Hopping matrix is not symmetric.");
+    const int n = nx;
+    const size_t nn = (size_t)n * n;
+    double trk = 0;
+    for (int i = 0; i < n; ++i) trk += kappa_[(size_t)i * n + i];
+    std::vector<double> id(nn, 0.0);
+    for (int i = 0; i < n; ++i) id[(size_t)i * n + i] = 1.0;
+    ident64_.reset(new DevBuf(nn * sizeof(double)));
+    ISLE_HIP(hipMemcpy(ident64_->p, id.data(), nn * sizeof(double), hipMemcpyHostToDevice));
+    for (int s = 0; s < 2; ++s) {
+        a64_[s].reset(new DevBuf(nn * sizeof(double)));
+        a32_[s].reset(new DevBuf(nn * sizeof(float)));
+        e64_[s].reset(new DevBuf(nn * sizeof(double)));
+    }
+    if (hop_ == EXP) {
+        // computeExponential (hubbardFermiMatrixExp.cpp:24-45)
+        std::vector<double> xp(nn), xh(nn), mxp(nn), mxh(nn);
+        for (size_t e = 0; e < nn; ++e) {
+            const double idv = id[e];
+            xp[e] = -kappa_[e] + mu * idv;        // particle, inv
+            xh[e] = -sk * kappa_[e] - mu * idv;   // hole, inv
+            mxp[e] = -xp[e];
+            mxh[e] = -xh[e];
+        }
+        build_exp(xp, *a64_[PARTICLE]);
+        build_exp(xh, *a64_[HOLE]);
+        build_exp(mxp, *e64_[PARTICLE]);
+        build_exp(mxh, *e64_[HOLE]);
+        logdet_a_[PARTICLE] = -trk + n * mu;     // log det exp(X) = tr X
+        logdet_a_[HOLE] = -sk * trk - n * mu;
+    } else {
+        // K (hubbardFermiMatrixDia.cpp:96-103)
+        std::vector<double> kp(nn), kh(nn);
+        for (size_t e = 0; e < nn; ++e) {
+            kp[e] = (1 + mu) * id[e] - kappa_[e];
+            kh[e] = (1 - mu) * id[e] - sk * kappa_[e];
+        }
+        ISLE_HIP(hipMemcpy(a64_[PARTICLE]->p, kp.data(), nn * 8, hipMemcpyHostToDevice));
+        ISLE_HIP(hipMemcpy(a64_[HOLE]->p, kh.data(), nn * 8, hipMemcpyHostToDevice));
+        ISLE_HIP(hipMemcpy(e64_[PARTICLE]->p, id.data(), nn * 8, hipMemcpyHostToDevice));
+        ISLE_HIP(hipMemcpy(e64_[HOLE]->p, id.data(), nn * 8, hipMemcpyHostToDevice));
+    }
+    for (int s = 0; s < 2; ++s) {
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(ew_blocks(nn)), dim3(EW_THREADS), 0, nullptr,
+                           a64_[s]->as<double>(), a32_[s]->as<float>(), (long)nn);
+        ISLE_HIP(hipGetLastError());
+    }
+    ISLE_HIP(hipDeviceSynchronize());
+}
+
+// exp of a real symmetric matrix: scaling and squaring with a degree-18 Taylor
+// polynomial evaluated by Horner's rule, all products on the device.
+void Plan::build_exp(const std::vector<double>& x, DevBuf& out) {
+    const int n = nx_;
+    const size_t nn = (size_t)n * n;
+    double norm1 = 0;
+    for (int j = 0; j < n; ++j) {
+        double s = 0;
+        for (int i = 0; i < n; ++i) s += std::fabs(x[(size_t)i * n + j]);
+        norm1 = std::max(norm1, s);
+    }
+    int sq = 0;
+    while (norm1 / std::ldexp(1.0, sq) > 0.125) ++sq;
+    std::vector<double> y(nn);
+    for (size_t e = 0; e < nn; ++e) y[e] = std::ldexp(x[e], -sq);
+    DevBuf dy(nn * 8), dt(nn * 8), dtmp(nn * 8), ditem(sizeof(GemmItem<double>));
+    ISLE_HIP(hipMemcpy(dy.p, y.data(), nn * 8, hipMemcpyHostToDevice));
+    const int deg = 18;
+    auto gemm_real = [&](const double* a, const double* b, double* c) {
+        GemmItem<double> g = gitem<double>(a, nullptr, b, nullptr, CMat<double>{c, nullptr}, nullptr, nullptr,
+                                           nullptr, nullptr, n);
+        ISLE_HIP(hipMemcpy(ditem.p, &g, sizeof(g), hipMemcpyHostToDevice));
+        launch_gemm<double, false, false, false, SCALE_NONE>(ditem.as<GemmItem<double>>(), 1, n, n, nullptr);
+        ISLE_HIP(hipDeviceSynchronize());
+    };
+    // T = I + Y/deg
+    hipLaunchKernelGGL(identity_plus_kernel, dim3(ew_blocks(nn)), dim3(EW_THREADS), 0, nullptr, dy.as<double>(),
+                       1.0 / deg, dt.as<double>(), n);
+    ISLE_HIP(hipGetLastError());
+    for (int k = deg - 1; k >= 1; --k) {
+        gemm_real(dy.as<double>(), dt.as<double>(), dtmp.as<double>());
+        hipLaunchKernelGGL(identity_plus_kernel, dim3(ew_blocks(nn)), dim3(EW_THREADS), 0, nullptr,
+                           dtmp.as<double>(), 1.0 / k, dt.as<double>(), n);
+        ISLE_HIP(hipGetLastError());
+    }
+    for (int s = 0; s < sq; ++s) {
+        gemm_real(dt.as<double>(), dt.as<double>(), dtmp.as<double>());
+        ISLE_HIP(hipMemcpy(dt.p, dtmp.p, nn * 8, hipMemcpyDeviceToDevice));
+    }
+    // symmetrise (exact symmetry of exp of a symmetric matrix)
+    std::vector<double> h(nn);
+    ISLE_HIP(hipMemcpy(h.data(), dt.p, nn * 8, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            const double v = 0.5 * (h[(size_t)i * n + j] + h[(size_t)j * n + i]);
+            h[(size_t)i * n + j] = h[(size_t)j * n + i] = v;
+        }
+    ISLE_HIP(hipMemcpy(out.p, h.data(), nn * 8, hipMemcpyHostToDevice));
+}
+
+void Plan::slice_matrix(int species, int inv, double* out_host) {
+    const size_t nn = (size_t)nx_ * nx_;
+    const DevBuf& src = (hop_ == EXP && !inv) ? *e64_[species] : *a64_[species];
+    ISLE_HIP(hipMemcpy(out_host, src.p, nn * 8, hipMemcpyDeviceToHost));
+}
+
+// ---------------------------------------------------------------------------
+void Plan::run(OpCache& op, const double* phi) {
+    for (const Step& st : op.steps) launch_step(op, st, phi);
+}
+
+static bool is_gemm(K k) {
+    return k == K::GEMM64_RC_PRE || k == K::GEMM64_RC_ROW || k == K::GEMM64_CR_PRE || k == K::GEMM64_CC ||
+           k == K::GEMM32_RC_ROW || k == K::GEMM32_CR_PRE;
+}
+
+void Plan::tune(OpCache& op) {
+    hipEvent_t e0, e1;
+    ISLE_HIP(hipEventCreate(&e0));
+    ISLE_HIP(hipEventCreate(&e1));
+    for (Step& st : op.steps) {
+        if (!is_gemm(st.kind)) continue;
+        auto key = std::make_tuple((int)st.kind, st.max_m, st.max_n, st.count);
+        auto f = tuned_.find(key);
+        if (f != tuned_.end()) {
+            st.tile = f->second;
+            continue;
+        }
+        float best = 1e30f;
+        int best_tile = TILE_64;
+        for (int tile = 0; tile < N_TILES; ++tile) {
+            st.tile = tile;
+            launch_step(op, st, nullptr);  // warm up
+            ISLE_HIP(hipEventRecord(e0, stream_));
+            for (int r = 0; r < 3; ++r) launch_step(op, st, nullptr);
+            ISLE_HIP(hipEventRecord(e1, stream_));
+            ISLE_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            ISLE_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                best_tile = tile;
+            }
+        }
+        st.tile = best_tile;
+        tuned_[key] = best_tile;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+void Plan::profile(const double* phi, int opk, int nt, int batch, int group, int nspecies, int spin, int reps,
+                   double* out, int nkinds) {
+    OpCache& op = opk == 0   ? get_force_plain(nt, batch, nspecies, spin)
+                  : opk == 1 ? get_force_mixed(nt, batch, group, nspecies, spin)
+                  : opk == 2 ? get_logdet_plain(nt, batch, nspecies, spin)
+                             : get_logdet_grouped(nt, batch, group, nspecies, spin);
+    run(op, phi);  // warm up
+    const size_t ns = op.steps.size();
+    std::vector<hipEvent_t> ev(ns + 1);
+    for (auto& e : ev) ISLE_HIP(hipEventCreate(&e));
+    std::vector<double> ms(ns, 0.0);
+    for (int r = 0; r < reps; ++r) {
+        ISLE_HIP(hipEventRecord(ev[0], stream_));
+        for (size_t i = 0; i < ns; ++i) {
+            launch_step(op, op.steps[i], phi);
+            ISLE_HIP(hipEventRecord(ev[i + 1], stream_));
+        }
+        ISLE_HIP(hipEventSynchronize(ev[ns]));
+        for (size_t i = 0; i < ns; ++i) {
+            float t = 0;
+            ISLE_HIP(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+            ms[i] += t;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < 4 * nkinds; ++k) out[k] = 0;
+    for (size_t i = 0; i < ns; ++i) {
+        const int k = (int)op.steps[i].kind;
+        if (k >= nkinds) continue;
+        out[4 * k] += 1;
+        out[4 * k + 1] += ms[i] / reps;
+        out[4 * k + 2] += op.steps[i].flops;
+    }
+}
+
+void Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
+    const int n = nx_;
+    const long nn = (long)n * n;
+    hipStream_t s = stream_;
+    {
+        switch (st.kind) {
+        case K::PHASE:
+            hipLaunchKernelGGL(phase_kernel, dim3(ew_blocks((long)n * op.nt), st.count), dim3(EW_THREADS), 0, s,
+                               (const PhaseItem*)st.items, phi, op.phi_stride);
+            break;
+        case K::GEMM64_RC_PRE:
+            launch_gemm<double, false, true, true, SCALE_NONE>((const GemmItem<double>*)st.items, st.count,
+                                                               st.max_m, st.max_n, s, st.tile);
+            break;
+        case K::GEMM64_RC_ROW:
+            launch_gemm<double, false, true, false, SCALE_ROW>((const GemmItem<double>*)st.items, st.count,
+                                                               st.max_m, st.max_n, s, st.tile);
+            break;
+        case K::GEMM64_CR_PRE:
+            launch_gemm<double, true, false, true, SCALE_NONE>((const GemmItem<double>*)st.items, st.count,
+                                                               st.max_m, st.max_n, s, st.tile);
+            break;
+        case K::GEMM64_CC:
+            launch_gemm<double, true, true, false, SCALE_NONE>((const GemmItem<double>*)st.items, st.count,
+                                                               st.max_m, st.max_n, s, st.tile);
+            break;
+        case K::GEMM32_RC_ROW:
+            launch_gemm<float, false, true, false, SCALE_ROW>((const GemmItem<float>*)st.items, st.count, st.max_m,
+                                                              st.max_n, s, st.tile);
+            break;
+        case K::GEMM32_CR_PRE:
+            launch_gemm<float, true, false, true, SCALE_NONE>((const GemmItem<float>*)st.items, st.count, st.max_m,
+                                                              st.max_n, s, st.tile);
+            break;
+        case K::SCALE64:
+            hipLaunchKernelGGL(scale_real_kernel<double>, dim3(ew_blocks(nn), st.count), dim3(EW_THREADS), 0, s,
+                               (const ScaleItem<double>*)st.items);
+            break;
+        case K::SCALE32:
+            hipLaunchKernelGGL(scale_real_kernel<float>, dim3(ew_blocks(nn), st.count), dim3(EW_THREADS), 0, s,
+                               (const ScaleItem<float>*)st.items);
+            break;
+        case K::LU:
+            launch_lu((const LuItem*)st.items, st.count, st.max_m, st.flag, s);
+            break;
+        case K::SOLVE:
+            launch_lu_solve((const SolveItem*)st.items, st.count, st.max_m, st.max_n, s);
+            break;
+        case K::DIAG64:
+            hipLaunchKernelGGL(diag_prod_kernel<double>, dim3(cdiv(n, 16), st.count), dim3(256), 0, s,
+                               (const DiagItem<double>*)st.items);
+            break;
+        case K::DIAG32:
+            hipLaunchKernelGGL(diag_prod_kernel<float>, dim3(cdiv(n, 16), st.count), dim3(256), 0, s,
+                               (const DiagItem<float>*)st.items);
+            break;
+        case K::COPY64:
+            hipLaunchKernelGGL((copy_items_kernel<double, double>), dim3(ew_blocks(st.max_m), st.count),
+                               dim3(EW_THREADS), 0, s, (const CopyItem<double, double>*)st.items);
+            break;
+        case K::CVT32TO64:
+            hipLaunchKernelGGL((copy_items_kernel<float, double>), dim3(ew_blocks(st.max_m), st.count),
+                               dim3(EW_THREADS), 0, s, (const CopyItem<float, double>*)st.items);
+            break;
+        case K::CVT64TO32:
+            hipLaunchKernelGGL((copy_items_kernel<double, float>), dim3(ew_blocks(st.max_m), st.count),
+                               dim3(EW_THREADS), 0, s, (const CopyItem<double, float>*)st.items);
+            break;
+        }
+        ISLE_HIP(hipGetLastError());
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain FP64 DIRECT_SINGLE force (hubbardFermiAction.cpp:16-60), all (field, species)
+// pairs advanced in lock step:
+//   lefts   Y_{nt-1} = X_{nt-1},  Y_t = X_t Y_{t+1}  with X_t = D_t A_s (D_t = exp(sigma i phi_{t-1}))
+//   Mid     = (1 + Y_0)^{-1}
+//   right   R_{-1} = Mid, R_tau = R_{tau-1} X_tau
+//   f[nt-1] = diag(Y_0 Mid),  f[tau] = diag(Y_{tau+1} R_tau)
+OpCache& Plan::get_force_plain(int nt, int batch, int nspecies, int spin) {
+    auto key = std::make_tuple(0, nt, batch, 0, nspecies, spin);
+    auto found = cache_.find(key);
+    if (found != cache_.end()) return *found->second;
+    auto op = std::make_unique<OpCache>();
+    op->nt = nt;
+    op->batch = batch;
+    op->nspecies = nspecies;
+    op->phi_stride = 2L * nx_ * nt;
+    Builder b(*op);
+    const int n = nx_;
+    const long nn = (long)n * n;
+    const int nb = batch * nspecies;
+    double* ph = b.d64((size_t)nb * 2 * nt * n);
+    double* Y = b.d64((size_t)nb * nt * 2 * nn);
+    double* R = b.d64((size_t)nb * nt * 2 * nn);
+    double* LUb = b.d64((size_t)nb * 2 * nn);
+    int* piv = b.di((size_t)nb * n);
+    op->parts = b.d64((size_t)nb * nt * n * 2);
+    auto D = [&](int item, int t) { return CMat<double>{ph + (long)item * 2 * nt * n + (long)t * n,
+                                                        ph + (long)item * 2 * nt * n + (long)nt * n + (long)t * n}; };
+    auto mY = [&](int item, int t) { return cmat(Y, (long)item * nt + t, nn); };
+    auto mR = [&](int item, int t) { return cmat(R, (long)item * nt + t, nn); };
+    auto spec = [&](int item) { return item % nspecies; };
+
+    std::vector<PhaseItem> phs;
+    for (int item = 0; item < nb; ++item) {
+        PhaseItem p{item / nspecies, spin, sigma_of(spec(item)), D(item, 0).re, D(item, 0).im, nullptr, nullptr, n, nt};
+        phs.push_back(p);
+    }
+    b.add(K::PHASE, phs);
+    std::vector<ScaleItem<double>> sc;
+    for (int item = 0; item < nb; ++item)
+        sc.push_back({a64_[spec(item)]->as<double>(), D(item, nt - 1).re, D(item, nt - 1).im, mY(item, nt - 1).re,
+                      mY(item, nt - 1).im, n, SCALE_ROW});
+    b.add(K::SCALE64, sc);
+    for (int t = nt - 2; t >= 0; --t) {
+        std::vector<GemmItem<double>> g;
+        for (int item = 0; item < nb; ++item) {
+            const CMat<double> src = mY(item, t + 1);
+            g.push_back(gitem<double>(a64_[spec(item)]->as<double>(), nullptr, src.re, src.im, mY(item, t), nullptr,
+                                      nullptr, D(item, t).re, D(item, t).im, n));
+        }
+        b.add(K::GEMM64_RC_ROW, g, n, n);
+    }
+    std::vector<CopyItem<double, double>> cp;
+    for (int item = 0; item < nb; ++item) cp.push_back({mY(item, 0).re, LUb + (long)item * 2 * nn, 2 * nn});
+    b.add(K::COPY64, cp, (int)(2 * nn));
+    std::vector<LuItem> lu;
+    for (int item = 0; item < nb; ++item)
+        lu.push_back({LUb + (long)item * 2 * nn, LUb + (long)item * 2 * nn + nn, piv + (long)item * n, nullptr, n, n});
+    b.add(K::LU, lu, n, 0, 1);
+    std::vector<SolveItem> sv;
+    for (int item = 0; item < nb; ++item) {
+        SolveItem si{};
+        si.lu_re = LUb + (long)item * 2 * nn;
+        si.lu_im = si.lu_re + nn;
+        si.piv = piv + (long)item * n;
+        si.b_re = si.b_im = nullptr;
+        si.x_re = mR(item, nt - 1).re;
+        si.x_im = mR(item, nt - 1).im;
+        si.xsr = n;
+        si.xsc = 1;
+        si.n = n;
+        si.ld = n;
+        si.nrhs = n;
+        sv.push_back(si);
+    }
+    b.add(K::SOLVE, sv, n, n);
+    for (int tau = 0; tau < nt - 1; ++tau) {
+        std::vector<GemmItem<double>> g;
+        for (int item = 0; item < nb; ++item) {
+            const CMat<double> prev = tau == 0 ? mR(item, nt - 1) : mR(item, tau - 1);
+            g.push_back(gitem<double>(prev.re, prev.im, a64_[spec(item)]->as<double>(), nullptr, mR(item, tau),
+                                      D(item, tau).re, D(item, tau).im, nullptr, nullptr, n));
+        }
+        b.add(K::GEMM64_CR_PRE, g, n, n);
+    }
+    std::vector<DiagItem<double>> dg;
+    for (int item = 0; item < nb; ++item)
+        for (int tau = 0; tau < nt; ++tau) {
+            const CMat<double> l = tau == nt - 1 ? mY(item, 0) : mY(item, tau + 1);
+            const CMat<double> r = tau == nt - 1 ? mR(item, nt - 1) : mR(item, tau);
+            dg.push_back({l.re, l.im, r.re, r.im, op->parts + ((long)item * nt + tau) * 2 * n, n});
+        }
+    b.add(K::DIAG64, dg);
+    b.finish();
+    tune(*op);
+    auto& ref = *op;
+    cache_[key] = std::move(op);
+    return ref;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision DIRECT_SINGLE force.  Slices are split into ng = nt/g groups
+// [s_j, e_j].  Per group (FP32, MFMA, all groups/species/fields in one launch):
+//   Suf_{e_j} = X_{e_j},  Suf_t = X_t Suf_{t+1}  (Suf_{s_j} = P_j, the group product)
+// Top level (FP64, group products promoted exactly):
+//   GL_{ng-1} = I, GL_j = P_{j+1} GL_{j+1};  Mid = (1 + P_0 GL_0)^{-1}
+//   RR_0 = Mid, RR_j = RR_{j-1} P_{j-1};     K_j = GL_j RR_j
+// Per group again (FP32):  Rt_{s_j} = K_j X_{s_j}, Rt_tau = Rt_{tau-1} X_tau,
+//   f[tau] = diag(Suf_{tau+1} Rt_tau)  (tau < e_j),  f[e_j] = diag(Rt_{e_j}).
+// This is the reference's lefts/right factorisation applied inside each group with the
+// global factors folded into K_j; FP32 chains never span more than g slices.
+OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin) {
+    if (g <= 0 || nt % g != 0) throw std::invalid_argument("group size must divide nt");
+    auto key = std::make_tuple(1, nt, batch, g, nspecies, spin);
+    auto found = cache_.find(key);
+    if (found != cache_.end()) return *found->second;
+    auto op = std::make_unique<OpCache>();
+    op->nt = nt;
+    op->batch = batch;
+    op->nspecies = nspecies;
+    op->phi_stride = 2L * nx_ * nt;
+    Builder b(*op);
+    const int n = nx_;
+    const long nn = (long)n * n;
+    const int nb = batch * nspecies;
+    const int ng = nt / g;
+    float* ph = b.d32((size_t)nb * 2 * nt * n);
+    float* Suf = b.d32((size_t)nb * nt * 2 * nn);
+    float* Rt = b.d32((size_t)nb * nt * 2 * nn);
+    double* P64 = b.d64((size_t)nb * ng * 2 * nn);
+    double* GL = b.d64((size_t)nb * ng * 2 * nn);
+    double* RR = b.d64((size_t)nb * ng * 2 * nn);
+    double* KK = b.d64((size_t)nb * ng * 2 * nn);
+    float* K32 = b.d32((size_t)nb * ng * 2 * nn);
+    double* LUb = b.d64((size_t)nb * 2 * nn);
+    int* piv = b.di((size_t)nb * n);
+    op->parts = b.d64((size_t)nb * nt * n * 2);
+    auto D = [&](int item, int t) { return CMat<float>{ph + (long)item * 2 * nt * n + (long)t * n,
+                                                       ph + (long)item * 2 * nt * n + (long)nt * n + (long)t * n}; };
+    auto mS = [&](int item, int t) { return cmat(Suf, (long)item * nt + t, nn); };
+    auto mRt = [&](int item, int t) { return cmat(Rt, (long)item * nt + t, nn); };
+    auto m64 = [&](double* base, int item, int j) { return cmat(base, (long)item * ng + j, nn); };
+    auto mK32 = [&](int item, int j) { return cmat(K32, (long)item * ng + j, nn); };
+    auto spec = [&](int item) { return item % nspecies; };
+    const float* A32[2] = {a32_[0]->as<float>(), a32_[1]->as<float>()};
+
+    std::vector<PhaseItem> phs;
+    for (int item = 0; item < nb; ++item)
+        phs.push_back({item / nspecies, spin, sigma_of(spec(item)), nullptr, nullptr, D(item, 0).re, D(item, 0).im, n, nt});
+    b.add(K::PHASE, phs);
+    {
+        std::vector<ScaleItem<float>> sc;
+        for (int item = 0; item < nb; ++item)
+            for (int j = 0; j < ng; ++j) {
+                const int e = j * g + g - 1;
+                sc.push_back({A32[spec(item)], D(item, e).re, D(item, e).im, mS(item, e).re, mS(item, e).im, n,
+                              SCALE_ROW});
+            }
+        b.add(K::SCALE32, sc);
+    }
+    for (int q = 1; q < g; ++q) {
+        std::vector<GemmItem<float>> gi;
+        for (int item = 0; item < nb; ++item)
+            for (int j = 0; j < ng; ++j) {
+                const int t = j * g + g - 1 - q;
+                const CMat<float> src = mS(item, t + 1);
+                gi.push_back(gitem<float>(A32[spec(item)], nullptr, src.re, src.im, mS(item, t), nullptr, nullptr,
+                                          D(item, t).re, D(item, t).im, n));
+            }
+        b.add(K::GEMM32_RC_ROW, gi, n, n);
+    }
+    {
+        std::vector<CopyItem<float, double>> cv;
+        for (int item = 0; item < nb; ++item)
+            for (int j = 0; j < ng; ++j) cv.push_back({mS(item, j * g).re, m64(P64, item, j).re, 2 * nn});
+        b.add(K::CVT32TO64, cv, (int)(2 * nn));
+    }
+    // ---- top level (FP64)
+    if (ng == 1) {
+        std::vector<CopyItem<double, double>> cp;
+        for (int item = 0; item < nb; ++item) cp.push_back({m64(P64, item, 0).re, LUb + (long)item * 2 * nn, 2 * nn});
+        b.add(K::COPY64, cp, (int)(2 * nn));
+    } else {
+        std::vector<CopyItem<double, double>> cp;
+        for (int item = 0; item < nb; ++item) cp.push_back({m64(P64, item, ng - 1).re, m64(GL, item, ng - 2).re, 2 * nn});
+        b.add(K::COPY64, cp, (int)(2 * nn));
+        for (int j = ng - 3; j >= 0; --j) {
+            std::vector<GemmItem<double>> gi;
+            for (int item = 0; item < nb; ++item) {
+                const CMat<double> a = m64(P64, item, j + 1), bb = m64(GL, item, j + 1);
+                gi.push_back(gitem<double>(a.re, a.im, bb.re, bb.im, m64(GL, item, j), nullptr, nullptr, nullptr,
+                                           nullptr, n));
+            }
+            b.add(K::GEMM64_CC, gi, n, n);
+        }
+        std::vector<GemmItem<double>> gi;
+        for (int item = 0; item < nb; ++item) {
+            const CMat<double> a = m64(P64, item, 0), bb = m64(GL, item, 0);
+            gi.push_back(gitem<double>(a.re, a.im, bb.re, bb.im, cmat(LUb, item, nn), nullptr, nullptr, nullptr,
+                                       nullptr, n));
+        }
+        b.add(K::GEMM64_CC, gi, n, n);
+    }
+    {
+        std::vector<LuItem> lu;
+        for (int item = 0; item < nb; ++item)
+            lu.push_back({LUb + (long)item * 2 * nn, LUb + (long)item * 2 * nn + nn, piv + (long)item * n, nullptr, n, n});
+        b.add(K::LU, lu, n, 0, 1);
+        std::vector<SolveItem> sv;
+        for (int item = 0; item < nb; ++item) {
+            SolveItem si{};
+            si.lu_re = LUb + (long)item * 2 * nn;
+            si.lu_im = si.lu_re + nn;
+            si.piv = piv + (long)item * n;
+            si.x_re = m64(RR, item, 0).re;
+            si.x_im = m64(RR, item, 0).im;
+            si.xsr = n;
+            si.xsc = 1;
+            si.n = n;
+            si.ld = n;
+            si.nrhs = n;
+            sv.push_back(si);
+        }
+        b.add(K::SOLVE, sv, n, n);
+    }
+    for (int j = 1; j < ng; ++j) {
+        std::vector<GemmItem<double>> gi;
+        for (int item = 0; item < nb; ++item) {
+            const CMat<double> a = m64(RR, item, j - 1), bb = m64(P64, item, j - 1);
+            gi.push_back(gitem<double>(a.re, a.im, bb.re, bb.im, m64(RR, item, j), nullptr, nullptr, nullptr, nullptr, n));
+        }
+        b.add(K::GEMM64_CC, gi, n, n);
+    }
+    if (ng > 1) {
+        std::vector<GemmItem<double>> gi;
+        for (int item = 0; item < nb; ++item)
+            for (int j = 0; j < ng - 1; ++j) {
+                const CMat<double> a = m64(GL, item, j), bb = m64(RR, item, j);
+                gi.push_back(gitem<double>(a.re, a.im, bb.re, bb.im, m64(KK, item, j), nullptr, nullptr, nullptr,
+                                           nullptr, n));
+            }
+        b.add(K::GEMM64_CC, gi, n, n);
+    }
+    {
+        std::vector<CopyItem<double, float>> cv;
+        for (int item = 0; item < nb; ++item)
+            for (int j = 0; j < ng; ++j) {
+                const double* src = (j < ng - 1) ? m64(KK, item, j).re : m64(RR, item, ng - 1).re;
+                cv.push_back({src, mK32(item, j).re, 2 * nn});
+            }
+        b.add(K::CVT64TO32, cv, (int)(2 * nn));
+    }
+    // ---- local right chains (FP32)
+    for (int q = 0; q < g; ++q) {
+        std::vector<GemmItem<float>> gi;
+        for (int item = 0; item < nb; ++item)
+            for (int j = 0; j < ng; ++j) {
+                const int tau = j * g + q;
+                const CMat<float> prev = q == 0 ? mK32(item, j) : mRt(item, tau - 1);
+                gi.push_back(gitem<float>(prev.re, prev.im, A32[spec(item)], nullptr, mRt(item, tau), D(item, tau).re,
+                                          D(item, tau).im, nullptr, nullptr, n));
+            }
+        b.add(K::GEMM32_CR_PRE, gi, n, n);
+    }
+    {
+        std::vector<DiagItem<float>> dg;
+        for (int item = 0; item < nb; ++item)
+            for (int tau = 0; tau < nt; ++tau) {
+                const bool last = (tau % g) == g - 1;
+                const CMat<float> l = last ? CMat<float>{nullptr, nullptr} : mS(item, tau + 1);
+                const CMat<float> r = mRt(item, tau);
+                dg.push_back({l.re, l.im, r.re, r.im, op->parts + ((long)item * nt + tau) * 2 * n, n});
+            }
+        b.add(K::DIAG32, dg);
+    }
+    b.finish();
+    tune(*op);
+    auto& ref = *op;
+    cache_[key] = std::move(op);
+    return ref;
+}
+
+// ---------------------------------------------------------------------------
+// Plain FP64 logdetM, the reference's chains (hubbardFermiMatrixExp.cpp:574-612,
+// hubbardFermiMatrixDia.cpp:571-597):
+//   EXP particle: B = F(nt-1)...F(0), F(t) = E_p diag(exp(i phi_{t-1}))  (B <- E_p (D'_t B))
+//   EXP hole / DIA: aux = X_0 X_1 ... X_{nt-1}                            (aux <- (aux D_t) A_s)
+//   then LU of (1 + .) with its log-determinant.
+OpCache& Plan::get_logdet_plain(int nt, int batch, int nspecies, int spin) {
+    auto key = std::make_tuple(2, nt, batch, 0, nspecies, spin);
+    auto found = cache_.find(key);
+    if (found != cache_.end()) return *found->second;
+    auto op = std::make_unique<OpCache>();
+    op->nt = nt;
+    op->batch = batch;
+    op->nspecies = nspecies;
+    op->phi_stride = 2L * nx_ * nt;
+    Builder b(*op);
+    const int n = nx_;
+    const long nn = (long)n * n;
+    const int nb = batch * nspecies;
+    double* ph = b.d64((size_t)nb * 2 * nt * n);
+    double* W = b.d64((size_t)nb * 2 * 2 * nn);
+    int* piv = b.di((size_t)nb * n);
+    op->logdet = b.d64((size_t)nb * 2);
+    auto D = [&](int item, int t) { return CMat<double>{ph + (long)item * 2 * nt * n + (long)t * n,
+                                                        ph + (long)item * 2 * nt * n + (long)nt * n + (long)t * n}; };
+    auto mW = [&](int item, int k) { return cmat(W, (long)item * 2 + k, nn); };
+    auto spec = [&](int item) { return item % nspecies; };
+    auto bchain = [&](int item) { return hop_ == EXP && spec(item) == PARTICLE; };
+
+    std::vector<PhaseItem> phs;
+    for (int item = 0; item < nb; ++item) {
+        const double sgn = bchain(item) ? 1.0 : sigma_of(spec(item));
+        phs.push_back({item / nspecies, spin, sgn, D(item, 0).re, D(item, 0).im, nullptr, nullptr, n, nt});
+    }
+    b.add(K::PHASE, phs);
+    std::vector<ScaleItem<double>> sc;
+    for (int item = 0; item < nb; ++item) {
+        if (bchain(item))
+            sc.push_back({e64_[PARTICLE]->as<double>(), D(item, 0).re, D(item, 0).im, mW(item, 0).re, mW(item, 0).im, n,
+                          SCALE_COL});
+        else
+            sc.push_back({a64_[spec(item)]->as<double>(), D(item, 0).re, D(item, 0).im, mW(item, 0).re,
+                          mW(item, 0).im, n, SCALE_ROW});
+    }
+    b.add(K::SCALE64, sc);
+    for (int t = 1; t < nt; ++t) {
+        std::vector<GemmItem<double>> gb, gr;
+        for (int item = 0; item < nb; ++item) {
+            const CMat<double> prev = mW(item, (t - 1) & 1), cur = mW(item, t & 1);
+            if (bchain(item))
+                gb.push_back(gitem<double>(e64_[PARTICLE]->as<double>(), nullptr, prev.re, prev.im, cur, D(item, t).re,
+                                           D(item, t).im, nullptr, nullptr, n));
+            else
+                gr.push_back(gitem<double>(prev.re, prev.im, a64_[spec(item)]->as<double>(), nullptr, cur,
+                                           D(item, t).re, D(item, t).im, nullptr, nullptr, n));
+        }
+        b.add(K::GEMM64_RC_PRE, gb, n, n);
+        b.add(K::GEMM64_CR_PRE, gr, n, n);
+    }
+    std::vector<LuItem> lu;
+    for (int item = 0; item < nb; ++item) {
+        const CMat<double> last = mW(item, (nt - 1) & 1);
+        lu.push_back({last.re, last.im, piv + (long)item * n, op->logdet + 2 * item, n, n});
+    }
+    b.add(K::LU, lu, n, 0, 1);
+    b.finish();
+    tune(*op);
+    auto& ref = *op;
+    cache_[key] = std::move(op);
+    return ref;
+}
+
+// Grouped FP64 logdetM (inverse form for both species):  P_j = X_{s_j} ... X_{e_j}
+// for all groups in parallel, then A^{-1} = P_0 ... P_{ng-1}, then LU of 1 + A^{-1}.
+OpCache& Plan::get_logdet_grouped(int nt, int batch, int g, int nspecies, int spin) {
+    if (g <= 0 || nt % g != 0) throw std::invalid_argument("group size must divide nt");
+    auto key = std::make_tuple(3, nt, batch, g, nspecies, spin);
+    auto found = cache_.find(key);
+    if (found != cache_.end()) return *found->second;
+    auto op = std::make_unique<OpCache>();
+    op->nt = nt;
+    op->batch = batch;
+    op->nspecies = nspecies;
+    op->phi_stride = 2L * nx_ * nt;
+    Builder b(*op);
+    const int n = nx_;
+    const long nn = (long)n * n;
+    const int nb = batch * nspecies;
+    const int ng = nt / g;
+    double* ph = b.d64((size_t)nb * 2 * nt * n);
+    double* Wg = b.d64((size_t)nb * ng * 2 * 2 * nn);
+    double* T = b.d64((size_t)nb * 2 * 2 * nn);
+    int* piv = b.di((size_t)nb * n);
+    op->logdet = b.d64((size_t)nb * 2);
+    auto D = [&](int item, int t) { return CMat<double>{ph + (long)item * 2 * nt * n + (long)t * n,
+                                                        ph + (long)item * 2 * nt * n + (long)nt * n + (long)t * n}; };
+    auto mW = [&](int item, int j, int k) { return cmat(Wg, ((long)item * ng + j) * 2 + k, nn); };
+    auto mT = [&](int item, int k) { return cmat(T, (long)item * 2 + k, nn); };
+    auto spec = [&](int item) { return item % nspecies; };
+
+    std::vector<PhaseItem> phs;
+    for (int item = 0; item < nb; ++item)
+        phs.push_back({item / nspecies, spin, sigma_of(spec(item)), D(item, 0).re, D(item, 0).im, nullptr, nullptr, n, nt});
+    b.add(K::PHASE, phs);
+    std::vector<ScaleItem<double>> sc;
+    for (int item = 0; item < nb; ++item)
+        for (int j = 0; j < ng; ++j) {
+            const int e = j * g + g - 1;
+            sc.push_back({a64_[spec(item)]->as<double>(), D(item, e).re, D(item, e).im, mW(item, j, 0).re,
+                          mW(item, j, 0).im, n, SCALE_ROW});
+        }
+    b.add(K::SCALE64, sc);
+    for (int q = 1; q < g; ++q) {
+        std::vector<GemmItem<double>> gi;
+        for (int item = 0; item < nb; ++item)
+            for (int j = 0; j < ng; ++j) {
+                const int t = j * g + g - 1 - q;
+                const CMat<double> src = mW(item, j, (q - 1) & 1);
+                gi.push_back(gitem<double>(a64_[spec(item)]->as<double>(), nullptr, src.re, src.im, mW(item, j, q & 1),
+                                           nullptr, nullptr, D(item, t).re, D(item, t).im, n));
+            }
+        b.add(K::GEMM64_RC_ROW, gi, n, n);
+    }
+    auto P = [&](int item, int j) { return mW(item, j, (g - 1) & 1); };
+    std::vector<LuItem> lu;
+    if (ng == 1) {
+        for (int item = 0; item < nb; ++item)
+            lu.push_back({P(item, 0).re, P(item, 0).im, piv + (long)item * n, op->logdet + 2 * item, n, n});
+    } else {
+        for (int j = 1; j < ng; ++j) {
+            std::vector<GemmItem<double>> gi;
+            for (int item = 0; item < nb; ++item) {
+                const CMat<double> a = j == 1 ? P(item, 0) : mT(item, (j - 2) & 1);
+                const CMat<double> bb = P(item, j);
+                gi.push_back(gitem<double>(a.re, a.im, bb.re, bb.im, mT(item, (j - 1) & 1), nullptr, nullptr, nullptr,
+                                           nullptr, n));
+            }
+            b.add(K::GEMM64_CC, gi, n, n);
+        }
+        for (int item = 0; item < nb; ++item) {
+            const CMat<double> last = mT(item, (ng - 2) & 1);
+            lu.push_back({last.re, last.im, piv + (long)item * n, op->logdet + 2 * item, n, n});
+        }
+    }
+    b.add(K::LU, lu, n, 0, 1);
+    b.finish();
+    tune(*op);
+    auto& ref = *op;
+    cache_[key] = std::move(op);
+    return ref;
+}
+
+// ---------------------------------------------------------------------------
+void Plan::logdetM(const double* phi, int nt, int batch, int mode, int group, int nspecies, int spin,
+                   double* out_host) {
+    if (nt < 1) throw std::invalid_argument("nt must be positive");
+    OpCache& op = mode == 0 ? get_logdet_plain(nt, batch, nspecies, spin)
+                            : get_logdet_grouped(nt, batch, group, nspecies, spin);
+    run(op, phi);
+    // sums of phi per field
+    DevBuf sums((size_t)batch * 4 * 8);
+    hipLaunchKernelGGL(sums_kernel, dim3(batch), dim3(256), 0, stream_, phi, (long)nx_ * nt, 2L * nx_ * nt,
+                       sums.as<double>());
+    ISLE_HIP(hipGetLastError());
+    std::vector<double> ld((size_t)batch * nspecies * 2), sm((size_t)batch * 4);
+    ISLE_HIP(hipMemcpyAsync(ld.data(), op.logdet, ld.size() * 8, hipMemcpyDeviceToHost, stream_));
+    ISLE_HIP(hipMemcpyAsync(sm.data(), sums.p, sm.size() * 8, hipMemcpyDeviceToHost, stream_));
+    ISLE_HIP(hipStreamSynchronize(stream_));
+    const std::complex<double> I(0, 1);
+    for (int bi = 0; bi < batch; ++bi) {
+        std::complex<double> sphi(sm[4 * bi], sm[4 * bi + 1]);
+        if (spin) sphi = -I * sphi;  // sum of aux = -i phi
+        for (int s = 0; s < nspecies; ++s) {
+            const int item = bi * nspecies + s;
+            const std::complex<double> l(ld[2 * item], ld[2 * item + 1]);
+            std::complex<double> v;
+            if (mode == 0 && hop_ == EXP && s == PARTICLE) {
+                v = l;
+            } else {
+                // log det M = log det(1 + A^{-1}) - sigma_s i sum(phi) - [EXP] nt log det A_s
+                v = l - sigma_of(s) * I * sphi;
+                if (hop_ == EXP) v -= (double)nt * logdet_a_[s];
+            }
+            v = to_first_log_branch(v);
+            out_host[2 * item] = v.real();
+            out_host[2 * item + 1] = v.imag();
+        }
+    }
+}
+
+void Plan::force_parts(const double* phi, int nt, int batch, int mode, int group, int nspecies, int spin,
+                       double* out_dev) {
+    if (nt < 2) throw std::invalid_argument("nt < 2 in HubbardFermiAction algorithm DIRECT_SINGLE not supported");
+    OpCache& op = mode == 0 ? get_force_plain(nt, batch, nspecies, spin)
+                            : get_force_mixed(nt, batch, group, nspecies, spin);
+    run(op, phi);
+    const size_t bytes = (size_t)batch * nspecies * nt * nx_ * 2 * 8;
+    ISLE_HIP(hipMemcpyAsync(out_dev, op.parts, bytes, hipMemcpyDeviceToDevice, stream_));
+}
+
+// solveM (hubbardFermiMatrixExp.cpp:645-694, hubbardFermiMatrixDia.cpp:617-673).
+void Plan::solveM(const double* phi, int nt, int species, const double* rhss, int nrhs, double* out) {
+    const int n = nx_;
+    const long nn = (long)n * n;
+    hipStream_t s = stream_;
+    OpCache op;
+    op.nt = nt;
+    op.phi_stride = 0;
+    Builder b(op);
+    double* ph = b.d64((size_t)2 * nt * n);
+    double* part = b.d64((size_t)nt * 2 * nn);
+    double* Mz = hop_ == DIA ? b.d64((size_t)nt * 2 * nn) : part;  // matrices multiplying the rhs
+    double* Bt = b.d64((size_t)nt * 2 * n * nrhs);
+    double* Z = b.d64((size_t)nt * 2 * n * nrhs);
+    double* Xl = b.d64((size_t)2 * n * nrhs);
+    int* piv = b.di((size_t)nt * n);
+    auto D = [&](int t) { return CMat<double>{ph + (long)t * n, ph + (long)nt * n + (long)t * n}; };
+    auto mP = [&](int t) { return cmat(part, t, nn); };
+    auto mM = [&](int t) { return cmat(Mz, t, nn); };
+    std::vector<PhaseItem> phs{{0, 0, sigma_of(species), D(0).re, D(0).im, nullptr, nullptr, n, nt}};
+    b.add(K::PHASE, phs);
+    const double* A = a64_[species]->as<double>();
+    std::vector<ScaleItem<double>> sc{{A, D(0).re, D(0).im, mP(0).re, mP(0).im, n, SCALE_ROW}};
+    if (hop_ == DIA) sc.push_back({ident64_->as<double>(), D(0).re, D(0).im, mM(0).re, mM(0).im, n, SCALE_ROW});
+    b.add(K::SCALE64, sc);
+    for (int t = 1; t < nt; ++t) {
+        std::vector<GemmItem<double>> gi{gitem<double>(mP(t - 1).re, mP(t - 1).im, A, nullptr, mP(t), D(t).re, D(t).im,
+                                                       nullptr, nullptr, n)};
+        b.add(K::GEMM64_CR_PRE, gi, n, n);
+    }
+    b.finish();
+    run(op, phi);
+    if (hop_ == DIA) {
+        // W_t = part_{t-1} D_t (the product without the final K)
+        std::vector<CScaleItem<double>> cs;
+        for (int t = 1; t < nt; ++t)
+            cs.push_back({mP(t - 1).re, mP(t - 1).im, D(t).re, D(t).im, mM(t).re, mM(t).im, n, SCALE_COL});
+        if (!cs.empty()) {
+            DevBuf dcs(cs.size() * sizeof(CScaleItem<double>));
+            ISLE_HIP(hipMemcpy(dcs.p, cs.data(), cs.size() * sizeof(cs[0]), hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(scale_complex_kernel<double>, dim3(ew_blocks(nn), cs.size()), dim3(EW_THREADS), 0, s,
+                               dcs.as<CScaleItem<double>>());
+            ISLE_HIP(hipGetLastError());
+            ISLE_HIP(hipStreamSynchronize(s));
+        }
+    }
+    const long blk = 2L * n * nrhs;
+    hipLaunchKernelGGL(rhs_transpose_kernel, dim3(ew_blocks((long)nt * n * nrhs)), dim3(EW_THREADS), 0, s, rhss, Bt,
+                       n, nt, nrhs);
+    ISLE_HIP(hipGetLastError());
+    {
+        std::vector<GemmItem<double>> gi;
+        for (int t = 0; t < nt; ++t) {
+            GemmItem<double> g{};
+            g.a_re = mM(t).re;
+            g.a_im = mM(t).im;
+            g.b_re = Bt + t * blk;
+            g.b_im = Bt + t * blk + (long)n * nrhs;
+            g.c_re = Z + t * blk;
+            g.c_im = Z + t * blk + (long)n * nrhs;
+            g.m = n;
+            g.n = nrhs;
+            g.k = n;
+            g.lda = n;
+            g.ldb = nrhs;
+            g.ldc = nrhs;
+            gi.push_back(g);
+        }
+        DevBuf dg(gi.size() * sizeof(gi[0]));
+        ISLE_HIP(hipMemcpy(dg.p, gi.data(), gi.size() * sizeof(gi[0]), hipMemcpyHostToDevice));
+        launch_gemm<double, true, true, false, SCALE_NONE>(dg.as<GemmItem<double>>(), nt, n, nrhs, s);
+        hipLaunchKernelGGL(prefix_sum_t_kernel, dim3(ew_blocks(blk)), dim3(EW_THREADS), 0, s, Z, blk, nt);
+        ISLE_HIP(hipGetLastError());
+        ISLE_HIP(hipStreamSynchronize(s));
+    }
+    // LU of every partial product (+1 on the last)
+    std::vector<LuItem> lu;
+    for (int t = 0; t < nt; ++t) lu.push_back({mP(t).re, mP(t).im, piv + (long)t * n, nullptr, n, n});
+    DevBuf dlu(lu.size() * sizeof(LuItem));
+    ISLE_HIP(hipMemcpy(dlu.p, lu.data(), lu.size() * sizeof(LuItem), hipMemcpyHostToDevice));
+    if (nt > 1) launch_lu(dlu.as<LuItem>(), nt - 1, n, 0, s);
+    launch_lu(dlu.as<LuItem>() + (nt - 1), 1, n, 1, s);
+    auto sitem = [&](int t, const double* bre, long bsr, long bsc, double* xre, double* xim, long xsr, long xsc) {
+        SolveItem si{};
+        si.lu_re = mP(t).re;
+        si.lu_im = mP(t).im;
+        si.piv = piv + (long)t * n;
+        si.b_re = bre;
+        si.b_im = bre + (long)n * nrhs;
+        si.bsr = bsr;
+        si.bsc = bsc;
+        si.x_re = xre;
+        si.x_im = xim;
+        si.xsr = xsr;
+        si.xsc = xsc;
+        si.n = n;
+        si.ld = n;
+        si.nrhs = nrhs;
+        return si;
+    };
+    std::vector<SolveItem> s1{sitem(nt - 1, Z + (nt - 1) * blk, nrhs, 1, Xl, Xl + (long)n * nrhs, nrhs, 1)};
+    DevBuf ds1(sizeof(SolveItem));
+    ISLE_HIP(hipMemcpy(ds1.p, s1.data(), sizeof(SolveItem), hipMemcpyHostToDevice));
+    launch_lu_solve(ds1.as<SolveItem>(), 1, n, nrhs, s);
+    if (nt > 1) {
+        hipLaunchKernelGGL(sub_last_kernel, dim3(ew_blocks(blk * (nt - 1))), dim3(EW_THREADS), 0, s, Z, Xl, blk, nt - 1);
+        ISLE_HIP(hipGetLastError());
+    }
+    std::vector<SolveItem> sv;
+    for (int t = 0; t < nt; ++t)
+        sv.push_back(sitem(t, Z + t * blk, nrhs, 1, out + 2L * t * n, out + 2L * t * n + 1, 2, 2L * nt * n));
+    // the last slice solves the original z_{nt-1} again (Z was not modified there)
+    DevBuf dsv(sv.size() * sizeof(SolveItem));
+    ISLE_HIP(hipMemcpy(dsv.p, sv.data(), sv.size() * sizeof(SolveItem), hipMemcpyHostToDevice));
+    launch_lu_solve(dsv.as<SolveItem>(), nt, n, nrhs, s);
+    ISLE_HIP(hipStreamSynchronize(s));
+}
+
+void Plan::F(int tp, const double* phi, int nt, int species, int inv, double* out) {
+    const int n = nx_;
+    const long nn = (long)n * n;
+    if (tp < 0 || tp >= nt) throw std::out_of_range("time slice out of range");
+    // sign of the phase (hubbardFermiMatrixExp.cpp:128-131, hubbardFermiMatrixDia.cpp:127-130)
+    const bool plus = (species == PARTICLE && !inv) || (species == HOLE && inv);
+    OpCache op;
+    op.nt = nt;
+    Builder b(op);
+    double* ph = b.d64((size_t)2 * nt * n);
+    double* m = b.d64((size_t)2 * nn);
+    const double* D_re = ph + (long)tp * n;
+    const double* D_im = ph + (long)nt * n + (long)tp * n;
+    std::vector<PhaseItem> phs{{0, 0, plus ? 1.0 : -1.0, ph, ph + (long)nt * n, nullptr, nullptr, n, nt}};
+    b.add(K::PHASE, phs);
+    std::vector<ScaleItem<double>> sc;
+    if (hop_ == EXP)
+        sc.push_back({inv ? a64_[species]->as<double>() : e64_[species]->as<double>(), D_re, D_im, m, m + nn, n,
+                      inv ? SCALE_ROW : SCALE_COL});
+    else
+        sc.push_back({ident64_->as<double>(), D_re, D_im, m, m + nn, n, SCALE_ROW});
+    b.add(K::SCALE64, sc);
+    b.finish();
+    run(op, phi);
+    hipLaunchKernelGGL(split_to_interleaved_kernel, dim3(ew_blocks(nn)), dim3(EW_THREADS), 0, stream_, m, m + nn, out,
+                       nn);
+    ISLE_HIP(hipGetLastError());
+    ISLE_HIP(hipStreamSynchronize(stream_));
+}
+
+}  // namespace isle
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return ISLE_OK;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return ISLE_ERR_VALUE;
+    } catch (const std::out_of_range& e) {
+        g_last_error = e.what();
+        return ISLE_ERR_INDEX;
+    } catch (const isle::HipError& e) {
+        g_last_error = e.what();
+        return ISLE_ERR_HIP;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return ISLE_ERR_RUNTIME;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return ISLE_ERR_RUNTIME;
+    }
+}
+
+isle::Plan* P(isle_hfm* h) {
+    if (!h) throw std::invalid_argument("null fermion-matrix handle");
+    return reinterpret_cast<isle::Plan*>(h);
+}
+
+void check_species(int s) {
+    if (s != 0 && s != 1) throw std::invalid_argument("Unknown species");
+}
+}  // namespace
+
+extern "C" {
+
+const char* isle_last_error(void) { return g_last_error.c_str(); }
+
+int isle_hfm_create(isle_hfm** out, int hopping, int nx, const double* kappa_tilde, double mu_tilde,
+                    int sigma_kappa) {
+    return guarded([&] {
+        if (!out || !kappa_tilde) throw std::invalid_argument("null argument");
+        *out = reinterpret_cast<isle_hfm*>(new isle::Plan(hopping, nx, kappa_tilde, mu_tilde, sigma_kappa));
+    });
+}
+
+int isle_hfm_destroy(isle_hfm* h) {
+    return guarded([&] { delete reinterpret_cast<isle::Plan*>(h); });
+}
+
+int isle_hfm_set_stream(isle_hfm* h, void* stream) {
+    return guarded([&] { P(h)->set_stream(reinterpret_cast<hipStream_t>(stream)); });
+}
+
+int isle_hfm_slice_matrix(isle_hfm* h, int species, int inv, double* out_host) {
+    return guarded([&] {
+        check_species(species);
+        P(h)->slice_matrix(species, inv, out_host);
+    });
+}
+
+int isle_hfm_F(isle_hfm* h, int tp, const double* phi, int nt, int species, int inv, double* out) {
+    return guarded([&] {
+        check_species(species);
+        P(h)->F(tp, phi, nt, species, inv, out);
+    });
+}
+
+int isle_logdetM(isle_hfm* h, const double* phi, int nt, int species, double* out_host) {
+    return guarded([&] {
+        check_species(species);
+        double tmp[4];
+        P(h)->logdetM(phi, nt, 1, 0, 0, 2, 0, tmp);
+        out_host[0] = tmp[2 * species];
+        out_host[1] = tmp[2 * species + 1];
+    });
+}
+
+int isle_logdetM_batch(isle_hfm* h, const double* phi, int nt, int batch, int mode, int group, int nspecies,
+                       int spin, double* out_host) {
+    return guarded([&] {
+        if (batch < 1 || (nspecies != 1 && nspecies != 2)) throw std::invalid_argument("bad batch/nspecies");
+        P(h)->logdetM(phi, nt, batch, mode, group, nspecies, spin, out_host);
+    });
+}
+
+int isle_force_parts(isle_hfm* h, const double* phi, int nt, int batch, int mode, int group, int nspecies, int spin,
+                     double* out_dev) {
+    return guarded([&] {
+        if (batch < 1 || (nspecies != 1 && nspecies != 2)) throw std::invalid_argument("bad batch/nspecies");
+        P(h)->force_parts(phi, nt, batch, mode, group, nspecies, spin, out_dev);
+    });
+}
+
+int isle_solveM(isle_hfm* h, const double* phi, int nt, int species, const double* rhss, int nrhs, double* out) {
+    return guarded([&] {
+        check_species(species);
+        if (nrhs < 1) throw std::invalid_argument("need at least one right hand side");
+        P(h)->solveM(phi, nt, species, rhss, nrhs, out);
+    });
+}
+
+int isle_profile_kernels(isle_hfm* h, const double* phi, int op, int nt, int batch, int group, int nspecies,
+                         int spin, int reps, double* out_host, int nkinds) {
+    return guarded([&] {
+        if (op < 0 || op > 3 || reps < 1) throw std::invalid_argument("bad profile request");
+        P(h)->profile(phi, op, nt, batch, group, nspecies, spin, reps, out_host, nkinds);
+    });
+}
+
+int isle_hfm_sync(isle_hfm* h) {
+    return guarded([&] { P(h)->sync(); });
+}
+
+int isle_combine_force(const double* fp, const double* fh, double* out, long n, int mode, int accumulate,
+                       void* stream) {
+    return guarded([&] {
+        hipLaunchKernelGGL(isle::combine_force_kernel, dim3(isle::ew_blocks(n)), dim3(256), 0, (hipStream_t)stream,
+                           fp, fh, out, n, mode, accumulate);
+        ISLE_HIP(hipGetLastError());
+    });
+}
+
+int isle_gauge_force(const double* phi, double* out, long n, double utilde, int accumulate, void* stream) {
+    return guarded([&] {
+        hipLaunchKernelGGL(isle::gauge_force_kernel, dim3(isle::ew_blocks(n)), dim3(256), 0, (hipStream_t)stream,
+                           phi, out, n, 1.0 / utilde, accumulate);
+        ISLE_HIP(hipGetLastError());
+    });
+}
+
+int isle_field_sums(const double* phi, long n, long stride, int batch, double* out_host, void* stream) {
+    return guarded([&] {
+        isle::DevBuf d((size_t)batch * 4 * 8);
+        hipLaunchKernelGGL(isle::sums_kernel, dim3(batch), dim3(256), 0, (hipStream_t)stream, phi, n, stride,
+                           d.as<double>());
+        ISLE_HIP(hipGetLastError());
+        ISLE_HIP(hipMemcpyAsync(out_host, d.p, (size_t)batch * 4 * 8, hipMemcpyDeviceToHost, (hipStream_t)stream));
+        ISLE_HIP(hipStreamSynchronize((hipStream_t)stream));
+    });
+}
+
+int isle_leapfrog_kick(double* pi, const double* force, const double* phi, long n, double eps, double utilde,
+                       void* stream) {
+    return guarded([&] {
+        const double iu = (phi && utilde > 0) ? 1.0 / utilde : 0.0;
+        hipLaunchKernelGGL(isle::kick_kernel, dim3(isle::ew_blocks(n)), dim3(256), 0, (hipStream_t)stream, pi, force,
+                           iu > 0 ? phi : nullptr, n, eps, iu);
+        ISLE_HIP(hipGetLastError());
+    });
+}
+
+int isle_leapfrog_drift(double* phi, const double* pi, long n, double eps, void* stream) {
+    return guarded([&] {
+        hipLaunchKernelGGL(isle::drift_kernel, dim3(isle::ew_blocks(n)), dim3(256), 0, (hipStream_t)stream, phi, pi, n,
+                           eps);
+        ISLE_HIP(hipGetLastError());
+    });
+}
+
+int isle_cscale(const double* in, double* out, long n, double cr, double ci, void* stream) {
+    return guarded([&] {
+        hipLaunchKernelGGL(isle::cscale_vec_kernel, dim3(isle::ew_blocks(n)), dim3(256), 0, (hipStream_t)stream, in,
+                           out, n, cr, ci);
+        ISLE_HIP(hipGetLastError());
+    });
+}
+
+int isle_logdet(const double* a, int n, double* out_host, void* stream) {
+    return guarded([&] {
+        if (n < 1) throw std::invalid_argument("empty matrix");
+        hipStream_t s = (hipStream_t)stream;
+        const long nn = (long)n * n;
+        isle::DevBuf m(2 * nn * 8), piv(n * sizeof(int)), ld(2 * 8), item(sizeof(isle::LuItem));
+        hipLaunchKernelGGL(isle::interleaved_to_split_kernel, dim3(isle::ew_blocks(nn)), dim3(256), 0, s, a,
+                           m.as<double>(), m.as<double>() + nn, nn);
+        ISLE_HIP(hipGetLastError());
+        isle::LuItem it{m.as<double>(), m.as<double>() + nn, piv.as<int>(), ld.as<double>(), n, n};
+        ISLE_HIP(hipMemcpyAsync(item.p, &it, sizeof(it), hipMemcpyHostToDevice, s));
+        isle::launch_lu(item.as<isle::LuItem>(), 1, n, 0, s);
+        ISLE_HIP(hipMemcpyAsync(out_host, ld.p, 16, hipMemcpyDeviceToHost, s));
+        ISLE_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int isle_invert(const double* a, int n, double* out, void* stream) {
+    return guarded([&] {
+        if (n < 1) throw std::invalid_argument("empty matrix");
+        hipStream_t s = (hipStream_t)stream;
+        const long nn = (long)n * n;
+        isle::DevBuf m(2 * nn * 8), piv(n * sizeof(int)), item(sizeof(isle::LuItem)), sitem(sizeof(isle::SolveItem));
+        hipLaunchKernelGGL(isle::interleaved_to_split_kernel, dim3(isle::ew_blocks(nn)), dim3(256), 0, s, a,
+                           m.as<double>(), m.as<double>() + nn, nn);
+        ISLE_HIP(hipGetLastError());
+        isle::LuItem it{m.as<double>(), m.as<double>() + nn, piv.as<int>(), nullptr, n, n};
+        ISLE_HIP(hipMemcpyAsync(item.p, &it, sizeof(it), hipMemcpyHostToDevice, s));
+        isle::launch_lu(item.as<isle::LuItem>(), 1, n, 0, s);
+        isle::SolveItem si{};
+        si.lu_re = m.as<double>();
+        si.lu_im = m.as<double>() + nn;
+        si.piv = piv.as<int>();
+        si.x_re = out;
+        si.x_im = out + 1;
+        si.xsr = 2L * n;
+        si.xsc = 2;
+        si.n = n;
+        si.ld = n;
+        si.nrhs = n;
+        ISLE_HIP(hipMemcpyAsync(sitem.p, &si, sizeof(si), hipMemcpyHostToDevice, s));
+        isle::launch_lu_solve(sitem.as<isle::SolveItem>(), 1, n, n, s);
+        ISLE_HIP(hipStreamSynchronize(s));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,138 @@
+// Fermion-matrix plan: device copies of the slice matrices, cached per-operation
+// descriptor sets and workspaces, and the step lists that implement logdetM,
+// solveM and the DIRECT_SINGLE force in plain FP64 and in mixed precision.
+#pragma once
+
+#include <complex>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "common.hpp"
+#include "elementwise.hpp"
+#include "gemm.hpp"
+#include "lu.hpp"
+
+namespace isle {
+
+enum Species : int { PARTICLE = 0, HOLE = 1 };
+enum Hopping : int { DIA = 0, EXP = 1 };
+
+// Owning device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t b) : bytes(b) {
+        if (b) ISLE_HIP(hipMalloc(&p, b));
+    }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Kinds of recorded launches.
+enum class K : int {
+    GEMM64_RC_PRE,      // real A x complex B, B pre-scaled along k        (f64)
+    GEMM64_RC_ROW,      // real A x complex B, rows post-scaled            (f64)
+    GEMM64_CR_PRE,      // complex A x real B, A pre-scaled along k        (f64)
+    GEMM64_CC,          // complex x complex                               (f64)
+    GEMM32_RC_ROW,      // (f32)
+    GEMM32_CR_PRE,      // (f32)
+    SCALE64, SCALE32,   // chain start diag(d) A or A diag(d)
+    LU,                 // factorise (flag: + I), logdet
+    SOLVE,              // X = A^{-1} B with stored LU
+    DIAG64, DIAG32,     // diag(L R)
+    COPY64, CVT32TO64, CVT64TO32,
+    PHASE,              // phase vectors from the runtime phi
+};
+
+struct Step {
+    K kind;
+    const void* items;   // device descriptor array
+    int count;           // number of items
+    int max_m, max_n;    // launch geometry (GEMM: max m, n; LU/SOLVE: max n, max rhs)
+    int flag;            // LU: add identity
+    double flops;        // algorithmic real flops of the launch (GEMMs: 2mnk per real product)
+    int tile = 0;        // GEMM tile shape chosen by the autotuner (TILE_64 / TILE_32)
+};
+
+// A cached operation: descriptor blob + workspaces + step list.
+struct OpCache {
+    std::vector<std::unique_ptr<DevBuf>> bufs;
+    std::vector<Step> steps;
+    long phi_stride = 0;       // doubles between consecutive fields of the batch
+    double* logdet = nullptr;  // device [batch][nspecies][2]
+    double* parts = nullptr;   // device [batch][nspecies][nx*nt] interleaved
+    int nspecies = 0;
+    int batch = 0;
+    int nt = 0;
+};
+
+class Plan {
+public:
+    Plan(int hopping, int nx, const double* kappa_tilde, double mu_tilde, int sigma_kappa);
+
+    int nx() const { return nx_; }
+    int hopping() const { return hop_; }
+    void set_stream(hipStream_t s) { stream_ = s; }
+    hipStream_t stream() const { return stream_; }
+
+    // log det M for a batch of fields (consecutive, nx*nt complex each) and nspecies
+    // species (1: particle only, 2: particle and hole).  mode 0 follows the reference
+    // algorithm step by step; mode 1 forms FP64 products of groups of `group` slices
+    // in parallel first.  spin != 0 evaluates at aux = -i*phi.  out_host: [batch][nspecies][2].
+    void logdetM(const double* phi, int nt, int batch, int mode, int group, int nspecies, int spin,
+                 double* out_host);
+    // DIRECT_SINGLE force terms (without the -i prefactor) for each species, written
+    // interleaved to out_dev [batch][nspecies][nx*nt].  mode 0: plain FP64 reference
+    // algorithm; mode 1: FP32 local chains over groups of `group` slices on MFMA plus
+    // an FP64 combination of the group products.
+    void force_parts(const double* phi, int nt, int batch, int mode, int group, int nspecies, int spin,
+                     double* out_dev);
+    void solveM(const double* phi, int nt, int species, const double* rhss, int nrhs, double* out);
+    void F(int tp, const double* phi, int nt, int species, int inv, double* out);
+    void slice_matrix(int species, int inv, double* out_host);
+    void sync() { ISLE_HIP(hipStreamSynchronize(stream_)); }
+    // Time every launch of a cached force (op 0/1) or logdet (op 2/3) operation with HIP
+    // events, `reps` times.  Per launch kind k (enum K): out[4k..4k+3] =
+    // (launches per operation, ms per operation, algorithmic flops per operation, 0).
+    void profile(const double* phi, int op, int nt, int batch, int group, int nspecies, int spin, int reps,
+                 double* out, int nkinds);
+
+    double logdet_A(int species) const { return logdet_a_[species]; }
+
+private:
+    int hop_, nx_;
+    double mu_;
+    int sk_;
+    std::vector<double> kappa_;
+    hipStream_t stream_ = nullptr;
+    // inverse-form slice matrices A_s (EXP: exp(-k+mu), exp(-s k-mu); DIA: K_p, K_h)
+    std::unique_ptr<DevBuf> a64_[2], a32_[2];
+    // EXP non-inverse matrices exp(k-mu), exp(s k+mu)
+    std::unique_ptr<DevBuf> e64_[2];
+    double logdet_a_[2] = {0, 0};
+    std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<OpCache>> cache_;
+
+    std::unique_ptr<DevBuf> ident64_;
+
+    void build_exp(const std::vector<double>& x, DevBuf& out);
+    OpCache& get_force_plain(int nt, int batch, int nspecies, int spin);
+    OpCache& get_force_mixed(int nt, int batch, int group, int nspecies, int spin);
+    OpCache& get_logdet_plain(int nt, int batch, int nspecies, int spin);
+    OpCache& get_logdet_grouped(int nt, int batch, int group, int nspecies, int spin);
+    void run(OpCache& op, const double* phi);
+    void launch_step(OpCache& op, const Step& st, const double* phi);
+    // Autotuner: time every tile shape of each GEMM step once per (kind, m, n, items) and
+    // keep the fastest for that step.
+    void tune(OpCache& op);
+    std::map<std::tuple<int, int, int, int>, int> tuned_;
+};
+
+}  // namespace isle

@@ -1,0 +1,190 @@
+"""HMC evolution: EvolutionStage, BinarySelector, ConstStepLeapfrog, NumpyRNG and the
+trajectory loop, with the reference's semantics and random-number consumption.
+
+* ``NumpyRNG`` wraps numpy's MT19937 RandomState (src/isle/random.py:80-124);
+* ``EvolutionStage`` (src/isle/evolver/stage.py:10-117, persistence omitted);
+* ``BinarySelector.selectTrajPoint``: accept iff dE < 0 or exp(-dE) > uniform, the
+  uniform drawn only when dE >= 0 (src/isle/evolver/selector.py:20-30);
+* ``ConstStepLeapfrog.evolve``: pi = rng.normal(0, 1, N) + 0j, leapfrog, Metropolis with
+  E = sum(logWeights) + |pi|^2/2 (src/isle/evolver/leapfrog.py:38-66);
+* ``hmc``: the HMC loop of src/isle/drivers/hmc.py:41-84 without HDF5 output; a start
+  configuration that is not a stage is evaluated first (hmc.py:64-66).
+
+``BatchedConstStepLeapfrog`` advances many independent chains together (one field tensor
+[batch, nx*nt]); each chain has its own NumpyRNG, consumed exactly as a lone chain would.
+"""
+import numpy as np
+import torch
+
+from .core import as_field
+from .integrator import leapfrog_batch
+
+
+class NumpyRNG:
+    """MT19937 via numpy.random.RandomState (src/isle/random.py:80-124)."""
+
+    NAME = "np.MT19937"
+
+    def __init__(self, seed):
+        self._state = np.random.RandomState(seed)
+
+    def seed(self, seed):
+        self._state.seed(seed)
+
+    def uniform(self, low=0.0, high=1.0, size=None, cmplx=False):
+        if cmplx:
+            return self._state.uniform(low, high, size) + 1j * self._state.uniform(low, high, size)
+        return self._state.uniform(low, high, size)
+
+    def normal(self, mean=0.0, std=1.0, size=None, cmplx=False):
+        if cmplx:
+            return self._state.normal(mean, std, size) + 1j * self._state.normal(mean, std, size)
+        return self._state.normal(mean, std, size)
+
+    def choice(self, a, size=None, replace=True, p=None):
+        return self._state.choice(a, size, replace, p)
+
+    def get_state(self):
+        return self._state.get_state()
+
+    def set_state(self, state):
+        self._state.set_state(state)
+
+
+class EvolutionStage:
+    """Configuration + log weights at one point of the Markov chain (stage.py:10-68)."""
+
+    __slots__ = "phi", "trajPoint", "logWeights", "extra", "phi_RTP"
+
+    def __init__(self, phi, actVal, trajPoint=1, logWeights=None, extra=None, phi_RTP=None):
+        self.phi = phi
+        self.phi_RTP = phi_RTP
+        self.trajPoint = trajPoint
+        self.logWeights = logWeights if logWeights is not None else dict()
+        self.logWeights["actVal"] = actVal
+        self.extra = extra if extra is not None else dict()
+
+    @property
+    def actVal(self):
+        return self.logWeights["actVal"]
+
+    @actVal.setter
+    def actVal(self, actVal):
+        self.logWeights["actVal"] = actVal
+
+    def accept(self, phi, actVal, logWeights=None, extra=None, phi_RTP=None):
+        return self.__class__(phi, actVal, 1, logWeights, extra, phi_RTP)
+
+    def reject(self, extra=None):
+        return self.__class__(self.phi, self.actVal, 0, self.logWeights, extra, self.phi_RTP)
+
+    def sumLogWeights(self):
+        return sum(self.logWeights.values())
+
+
+class BinarySelector:
+    """Metropolis accept/reject (selector.py:9-44)."""
+
+    def __init__(self, rng):
+        self.rng = rng
+
+    def selectTrajPoint(self, energy0, energy1):
+        deltaE = np.real(energy1 - energy0)
+        return 1 if deltaE < 0 or np.exp(-deltaE) > self.rng.uniform(0, 1) else 0
+
+    def selectTrajectory(self, energy0, data0, energy1, data1):
+        return (energy1, data1, 1) if self.selectTrajPoint(energy0, energy1) == 1 else (energy0, data0, 0)
+
+
+class ConstStepLeapfrog:
+    """Leapfrog evolver with constant length and step count (leapfrog.py:15-103)."""
+
+    def __init__(self, action, length, nstep, rng, transform=None):
+        if transform is not None:
+            raise NotImplementedError("transforms are not supported")
+        self.action = action
+        self.length = length
+        self.nstep = nstep
+        self.rng = rng
+        self.selector = BinarySelector(rng)
+        self.transform = None
+        self.trajPoints = []
+        self.last_energies = None
+
+    def evolve(self, stage):
+        phi = as_field(stage.phi)
+        n = phi.numel()
+        pi_host = self.rng.normal(0, 1, n) + 0j
+        pi = torch.as_tensor(pi_host, device=phi.device)
+        phi1, pi1, acts = leapfrog_batch(phi.reshape(1, -1), pi.reshape(1, -1), self.action, self.length,
+                                         self.nstep)
+        actVal1 = complex(acts[0])
+        energy0 = stage.sumLogWeights() + np.linalg.norm(pi_host) ** 2 / 2
+        energy1 = actVal1 + 0 + np.linalg.norm(pi1.cpu().numpy()) ** 2 / 2
+        self.last_energies = (energy0, energy1)
+        trajPoint = self.selector.selectTrajPoint(energy0, energy1)
+        self.trajPoints.append(trajPoint)
+        return stage.accept(phi1[0], actVal1) if trajPoint == 1 else stage.reject()
+
+    def report(self):
+        return f"""<ConstStepLeapfrog> (0x{id(self):x})
+  length = {self.length}, nstep = {self.nstep}
+  acceptance rate = {np.mean(self.trajPoints)}"""
+
+
+class BatchedConstStepLeapfrog:
+    """ConstStepLeapfrog over independent chains advanced together on the GPU.
+
+    ``rngs`` holds one NumpyRNG per chain; chain b draws its momenta and its Metropolis
+    uniform from rngs[b] exactly as ConstStepLeapfrog would, so each chain reproduces a
+    lone run with the same seed (up to the arithmetic of the batched kernels)."""
+
+    def __init__(self, action, length, nstep, rngs):
+        self.action = action
+        self.length = length
+        self.nstep = nstep
+        self.rngs = list(rngs)
+        self.selectors = [BinarySelector(r) for r in self.rngs]
+        self.trajPoints = []
+        self.deltaH = []
+
+    def evolve(self, phis, actVals):
+        """phis: [batch, nx*nt] device tensor, actVals: numpy complex [batch].
+        Returns (phis', actVals', trajPoints)."""
+        batch, n = phis.shape
+        pi_host = np.stack([r.normal(0, 1, n) + 0j for r in self.rngs])
+        pi = torch.as_tensor(pi_host, device=phis.device)
+        phi1, pi1, acts1 = leapfrog_batch(phis, pi, self.action, self.length, self.nstep)
+        pi1h = pi1.cpu().numpy()
+        new_phi = phis.clone()
+        new_act = np.array(actVals, dtype=complex)
+        points = []
+        dh = []
+        for b in range(batch):
+            e0 = actVals[b] + np.linalg.norm(pi_host[b]) ** 2 / 2
+            e1 = acts1[b] + np.linalg.norm(pi1h[b]) ** 2 / 2
+            dh.append(np.real(e1 - e0))
+            tp = self.selectors[b].selectTrajPoint(e0, e1)
+            points.append(tp)
+            if tp == 1:
+                new_phi[b] = phi1[b]
+                new_act[b] = acts1[b]
+        self.trajPoints.append(points)
+        self.deltaH.append(dh)
+        return new_phi, new_act, points
+
+
+def hmc(stage, evolver, ntr):
+    """HMC loop (src/isle/drivers/hmc.py:41-84) without file output.
+
+    Returns the list of stages after each trajectory."""
+    if not isinstance(stage, EvolutionStage):
+        stage = EvolutionStage(stage, evolver.action.eval(stage), 1)
+    out = []
+    for _ in range(ntr):
+        try:
+            stage = evolver.evolve(stage)
+        except StopIteration:
+            break
+        out.append(stage)
+    return out

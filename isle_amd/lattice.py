@@ -1,0 +1,141 @@
+"""Lattices: hopping matrices, the bipartiteness test and lattice generators.
+
+Mirrors src/isle/cpp/lattice.hpp:25-163 (Lattice), src/isle/cpp/lattice.cpp:16-84
+(isBipartite) and src/isle/yamlio.py:16-50 (the ``!lattice`` YAML format).
+"""
+import sys
+
+import numpy as np
+import yaml
+
+
+class Lattice:
+    """Spatial lattice with a symmetric hopping matrix and a number of time slices."""
+
+    def __init__(self, nt, nx, name="", comment=""):
+        self._nt = int(nt)
+        self._hopping = np.zeros((int(nx), int(nx)))
+        self.name = name
+        self.comment = comment
+
+    def nx(self):
+        return self._hopping.shape[0]
+
+    def nt(self, nt=None):
+        """Return nt, or set it when given (lattice.hpp: nt(std::size_t))."""
+        if nt is not None:
+            self._nt = int(nt)
+        return self._nt
+
+    def lattSize(self):
+        return self.nx() * self._nt
+
+    def setNeighbor(self, i, j, strength):
+        """Set a symmetric hopping; zero strength removes the bond (lattice.hpp:80-95)."""
+        self._hopping[i, j] = strength
+        self._hopping[j, i] = strength
+
+    def areNeighbors(self, i, j):
+        return self._hopping[i, j] != 0
+
+    def hopping(self):
+        """Dense copy of the hopping matrix (nx x nx, float64)."""
+        return self._hopping.copy()
+
+    def getNeighbors(self, i):
+        return [(int(j), float(self._hopping[i, j])) for j in np.nonzero(self._hopping[i])[0]]
+
+    @classmethod
+    def from_hopping(cls, hopping, nt=0, name="", comment=""):
+        h = np.asarray(hopping, dtype=np.float64)
+        if h.shape[0] != h.shape[1] or not np.array_equal(h, h.T):
+            raise ValueError("hopping matrix must be square and symmetric")
+        lat = cls(nt, h.shape[0], name, comment)
+        lat._hopping[...] = h
+        return lat
+
+
+def isBipartite(lat_or_hopping):
+    """True iff the lattice is bipartite AND even/odd site labels alternate.
+
+    Depth-first 2-colouring in increasing neighbour order that does not recurse from
+    the last row, followed by the alternating-label check (lattice.cpp:16-84).
+    """
+    h = lat_or_hopping.hopping() if isinstance(lat_or_hopping, Lattice) else np.asarray(lat_or_hopping)
+    n = h.shape[0]
+    if n == 0:
+        return False
+    none, col_a = 0, 1
+    col_b = ~col_a
+    colors = [none] * n
+    colors[0] = col_a
+    neigh = [np.nonzero(h[i])[0].tolist() for i in range(n)]
+    stack_limit = sys.getrecursionlimit()
+    sys.setrecursionlimit(max(stack_limit, 10 * n + 100))
+
+    def descend(cur):
+        c = colors[cur]
+        for nxt in neigh[cur]:
+            if colors[nxt] == none:
+                colors[nxt] = ~c
+                if cur < n - 1 and not descend(nxt):
+                    return False
+            elif colors[nxt] == c:
+                return False
+        return True
+
+    try:
+        bip = descend(0)
+    finally:
+        sys.setrecursionlimit(stack_limit)
+    alternating = all((c == col_a) if i % 2 == 0 else (c == col_b) for i, c in enumerate(colors))
+    return bip and alternating
+
+
+class _LatticeLoader(yaml.SafeLoader):
+    pass
+
+
+def _construct_lattice(loader, node):
+    d = loader.construct_mapping(node, deep=True)
+    adjacency = d["adjacency"]
+    hopping = d["hopping"]
+    if not isinstance(hopping, list):
+        hopping = [hopping] * len(adjacency)
+    elif len(hopping) != len(adjacency):
+        raise RuntimeError("Lengths of adjacency matrix and list of hopping strengths do not match")
+    lat = Lattice(d.get("nt", 0), len(d["positions"]), d.get("name", ""), d.get("comment", ""))
+    for (i, j), hop in zip(adjacency, hopping):
+        lat.setNeighbor(i, j, hop)
+    return lat
+
+
+_LatticeLoader.add_constructor("!lattice", _construct_lattice)
+
+
+def loadLattice(text_or_file):
+    """Parse a ``!lattice`` YAML document (string or open file) with a safe loader."""
+    text = text_or_file.read() if hasattr(text_or_file, "read") else text_or_file
+    return yaml.load(text, Loader=_LatticeLoader)
+
+
+def honeycomb(l_i, l_j=None, periodic_j=True, kappa=1.0, nt=0):
+    """Honeycomb lattice of l_i x l_j cells, sites A(i,j) = 2(i*l_j + j), B = A + 1.
+
+    Bonds A(i,j)-B(i,j), A(i,j)-B(i-1,j), A(i,j)-B(i,j-1); periodic in i, and in j
+    unless ``periodic_j`` is False (a cylinder).  Labels alternate A/B, so the
+    torus is bipartite in the sense of isBipartite.
+    """
+    l_j = l_i if l_j is None else l_j
+    nx = 2 * l_i * l_j
+    lat = Lattice(nt, nx, f"honeycomb {l_i}x{l_j}{'' if periodic_j else ' (open j)'}")
+    for i in range(l_i):
+        for j in range(l_j):
+            a = 2 * (i * l_j + j)
+            for ii, jj in ((i, j), ((i - 1) % l_i, j), (i, j - 1)):
+                if jj < 0:
+                    if not periodic_j:
+                        continue
+                    jj %= l_j
+                lat.setNeighbor(a, 2 * (ii * l_j + jj) + 1, kappa)
+    return lat

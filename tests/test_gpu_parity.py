@@ -1,0 +1,227 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on identical inputs.
+
+Tolerances (stated per test): logdet / eval / plain FP64 force 1e-10 relative
+(imaginary parts compared modulo 2*pi); mixed-precision force 1e-5 relative (L2).
+"""
+import itertools
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import hubbard_ref as ref
+from tests.conftest import load_lattice
+from tests.helpers import mod2pi_diff, random_phi, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+isle = pytest.importorskip("isle_amd")
+from isle_amd import action as act  # noqa: E402
+
+LATS = ("two_sites", "triangle", "tube_3-3_1", "one_site")
+
+
+def _dev(x):
+    return torch.as_tensor(x, device="cuda")
+
+
+@pytest.mark.parametrize("hop", [ref.EXP, ref.DIA])
+def test_logdetM_vs_oracle(hop):
+    rng = np.random.RandomState(8613)
+    cls_g = isle.HubbardFermiMatrixExp if hop == ref.EXP else isle.HubbardFermiMatrixDia
+    cls_r = ref.HFMExp if hop == ref.EXP else ref.HFMDia
+    for name in LATS:
+        kappa = load_lattice(name)
+        nx = kappa.shape[0]
+        for nt, beta, mu, sk in itertools.product((2, 4, 8), (3,), (0.0, 0.5), (-1, 1)):
+            kt, mt = kappa * beta / nt, mu * beta / nt
+            g, r = cls_g(kt, mt, sk), cls_r(kt, mt, sk)
+            for species, (re_, im_) in itertools.product((0, 1), ((True, False), (False, True), (True, True))):
+                phi = random_phi(rng, nx * nt, re_, im_)
+                a = isle.logdetM(g, phi, isle.Species(species))
+                b = ref.logdetM(r, phi, species)
+                assert mod2pi_diff(a, b) < 1e-10 * max(1, abs(b)), (name, nt, mu, sk, species, a, b)
+
+
+@pytest.mark.parametrize("hop,basis", list(itertools.product((0, 1), (0, 1))))
+def test_action_plain_vs_oracle(hop, basis):
+    rng = np.random.RandomState(8613)
+    for name in LATS:
+        kappa = load_lattice(name)
+        nx = kappa.shape[0]
+        for nt, beta, sk in itertools.product((8, 16), (1, 4), (-1, 1)):
+            kt = kappa * beta / nt
+            g = act.HubbardFermiAction(kt, 0.0, sk, hop, basis)
+            r = ref.HubbardFermiAction(kt, 0.0, sk, hop, basis)
+            phi = random_phi(rng, nx * nt)
+            assert mod2pi_diff(g.eval(phi), r.eval(phi)) < 1e-10 * max(1, abs(r.eval(phi))), (name, nt, beta, sk)
+            fg = g.force(phi).cpu().numpy()
+            fr = r.force(phi)
+            assert np.max(np.abs(fg - fr)) < 1e-10 * max(1, np.max(np.abs(fr))), (name, nt, beta, sk)
+
+
+@pytest.mark.parametrize("basis", [0, 1])
+def test_shortcut_and_mu(basis):
+    rng = np.random.RandomState(5)
+    kappa = ref.honeycomb(3)
+    nt, beta = 8, 3.0
+    for mu, short in ((0.0, True), (0.3, False)):
+        g = act.HubbardFermiAction(kappa * beta / nt, mu * beta / nt, 1, 1, basis, 0, short)
+        r = ref.HubbardFermiAction(kappa * beta / nt, mu * beta / nt, 1, 1, basis, 0, short)
+        phi = rng.normal(0, 0.4, 18 * nt) + 0j
+        assert mod2pi_diff(g.eval(phi), r.eval(phi)) < 1e-10 * max(1, abs(r.eval(phi)))
+        assert rel_l2(g.force(phi).cpu().numpy(), r.force(phi)) < 1e-10
+
+
+def test_F_and_slice_matrices():
+    kappa = ref.honeycomb(3) * 0.25
+    phi = np.random.RandomState(2).normal(0, 0.3, 18 * 4) + 1j * np.random.RandomState(3).normal(0, 0.1, 18 * 4)
+    for hop in (ref.EXP, ref.DIA):
+        g = (isle.HubbardFermiMatrixExp if hop == ref.EXP else isle.HubbardFermiMatrixDia)(kappa, 0.1, -1)
+        r = (ref.HFMExp if hop == ref.EXP else ref.HFMDia)(kappa, 0.1, -1)
+        for tp, s, inv in itertools.product((0, 2), (0, 1), (False, True)):
+            np.testing.assert_allclose(g.F(tp, phi, s, inv).cpu().numpy(), r.F(tp, phi, s, inv), rtol=1e-13, atol=1e-14)
+        for s in (0, 1):
+            np.testing.assert_allclose(g.K(s), r.K(s), rtol=1e-14, atol=1e-15)
+        if hop == ref.EXP:
+            for s, inv in itertools.product((0, 1), (False, True)):
+                np.testing.assert_allclose(g.expKappa(s, inv), r.exp_kappa[(s, inv)], rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(g.M(phi, 0).cpu().numpy(), r.M(phi, 0), rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("hop", [ref.EXP, ref.DIA])
+def test_solveM(hop):
+    rng = np.random.RandomState(8613)
+    kappa = load_lattice("tube_3-3_1")
+    nx = kappa.shape[0]
+    for nt, sk, species in itertools.product((4, 8), (-1, 1), (0, 1)):
+        g = (isle.HubbardFermiMatrixExp if hop == ref.EXP else isle.HubbardFermiMatrixDia)(kappa / nt, 0.0, sk)
+        r = (ref.HFMExp if hop == ref.EXP else ref.HFMDia)(kappa / nt, 0.0, sk)
+        phi = random_phi(rng, nx * nt)
+        rhss = np.array([random_phi(rng, nx * nt) for _ in range(10)])
+        xg = isle.solveM(g, phi, species, rhss).cpu().numpy()
+        np.testing.assert_allclose(xg, ref.solveM(r, phi, species, rhss), rtol=1e-10, atol=1e-12)
+        m = r.M(phi, species)
+        np.testing.assert_allclose(m @ xg.T, rhss.T, rtol=1e-10, atol=1e-12)
+
+
+def test_logdet_and_invert():
+    rng = np.random.RandomState(7)
+    for n in (1, 5, 33, 100):
+        a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        assert mod2pi_diff(isle.logdet(a), ref.ilogdet(a)) < 1e-11 * max(1, abs(ref.ilogdet(a)))
+        np.testing.assert_allclose(isle.invert(a).cpu().numpy(), np.linalg.inv(a), rtol=1e-9, atol=1e-11)
+
+
+@pytest.mark.parametrize("cfg", [(3, 16, 4.0), (6, 64, 6.0)])
+def test_mixed_force(cfg):
+    """FP32 local chains + FP64 top level within 1e-5 (L2 relative) of the FP64 oracle."""
+    l, nt, beta = cfg
+    kappa = ref.honeycomb(l)
+    nx = kappa.shape[0]
+    u = 3.0 if l == 6 else 2.0
+    rng = np.random.RandomState(42)
+    kt = kappa * beta / nt
+    r = ref.HubbardFermiAction(kt, 0.0, -1, ref.EXP)
+    m = act.HubbardFermiAction(kt, 0.0, -1, 1, precision="mixed", group=8)
+    for _ in range(2):
+        phi = rng.normal(0, np.sqrt(u * beta / nt), nx * nt) + 0j
+        err = rel_l2(m.force(phi).cpu().numpy(), r.force(phi))
+        assert err < 1e-5, err
+        assert mod2pi_diff(m.eval(phi), r.eval(phi)) < 1e-10 * abs(r.eval(phi))
+
+
+def test_batched_force_matches_single():
+    kappa = ref.honeycomb(3)
+    nt, beta = 16, 4.0
+    rng = np.random.RandomState(3)
+    phis = rng.normal(0, 0.7, (5, 18 * nt)) + 0j
+    for prec in ("fp64", "mixed"):
+        a = act.HubbardFermiAction(kappa * beta / nt, 0.0, -1, 1, precision=prec)
+        fb = a.force_batch(_dev(phis)).cpu().numpy()
+        eb = a.eval_batch(_dev(phis))
+        for b in range(5):
+            np.testing.assert_allclose(fb[b], a.force(phis[b]).cpu().numpy(), rtol=1e-12, atol=1e-13)
+            assert abs(eb[b] - a.eval(phis[b])) < 1e-11 * abs(eb[b])
+
+
+def test_leapfrog_vs_oracle():
+    kappa = ref.honeycomb(3)
+    nt, beta, u = 16, 4.0, 2.0
+    ut = u * beta / nt
+    rng = np.random.RandomState(1075)
+    phi = rng.normal(0, np.sqrt(ut), 18 * nt) + 0j
+    pi = rng.normal(0, 1, 18 * nt) + 0j
+    g = act.SumAction(act.HubbardGaugeAction(ut), act.HubbardFermiAction(kappa * beta / nt, 0.0, -1, 1))
+    r = ref.SumAction(ref.HubbardGaugeAction(ut), ref.HubbardFermiAction(kappa * beta / nt, 0.0, -1, ref.EXP))
+    p1, q1, s1 = isle.leapfrog(phi, pi, g, 1.0, 20)
+    rp, rq, rs = ref.leapfrog(phi, pi, r, 1.0, 20)
+    assert rel_l2(p1.cpu().numpy(), rp) < 1e-11
+    assert rel_l2(q1.cpu().numpy(), rq) < 1e-11
+    assert mod2pi_diff(s1, rs) < 1e-10 * abs(rs)
+    # generic (unfused) path gives the same
+    class Wrapped(act.Action):
+        def __init__(self, a):
+            self.a = a
+
+        def eval(self, phi):
+            return self.a.eval(phi)
+
+        def force(self, phi):
+            return self.a.force(phi)
+
+    p2, q2, s2 = isle.leapfrog(phi, pi, Wrapped(g), 1.0, 20)
+    assert rel_l2(p2.cpu().numpy(), rp) < 1e-11
+
+
+def test_hmc_fixed_seed_matches_oracle():
+    """Same seed, same momenta, same Metropolis draws: Delta H and decisions agree (config 1 shape)."""
+    from isle_amd.evolver import ConstStepLeapfrog, EvolutionStage, NumpyRNG
+    kappa = ref.honeycomb(3)
+    nt, beta, u = 16, 4.0, 2.0
+    ut = u * beta / nt
+    g = act.SumAction(act.HubbardGaugeAction(ut), act.HubbardFermiAction(kappa * beta / nt, 0.0, -1, 1))
+    r = ref.SumAction(ref.HubbardGaugeAction(ut), ref.HubbardFermiAction(kappa * beta / nt, 0.0, -1, ref.EXP))
+    seed = 1075
+    rng_g, rng_r = NumpyRNG(seed), np.random.RandomState(seed)
+    phi0 = rng_r.normal(0, np.sqrt(ut), 18 * nt) + 0j
+    rng_g.normal(0, np.sqrt(ut), 18 * nt)
+    ev = ConstStepLeapfrog(g, 1.0, 20, rng_g)
+    stage = EvolutionStage(phi0, g.eval(phi0))
+    phi_r, act_r = phi0, r.eval(phi0)
+    for _ in range(4):
+        stage = ev.evolve(stage)
+        phi_r, act_r, tp, e0, e1 = ref.const_step_leapfrog(rng_r, r, phi_r, act_r, 1.0, 20)
+        assert tp == stage.trajPoint
+        dh_g = np.real(ev.last_energies[1] - ev.last_energies[0])
+        assert abs(dh_g - np.real(e1 - e0)) < 1e-8
+    assert rel_l2(stage.phi.cpu().numpy(), phi_r) < 1e-10
+
+
+def test_config2_gradient_check():
+    """Nx=72, Nt=64, beta=6, U=3: force vs central FD (h=1e-6) on 64 entries, and vs the
+    oracle force on 16 seeded fields."""
+    kappa = ref.honeycomb(6)
+    nx, nt, beta, u = 72, 64, 6.0, 3.0
+    kt = kappa * beta / nt
+    ut = u * beta / nt
+    g = act.HubbardFermiAction(kt, 0.0, -1, 1)
+    r = ref.HubbardFermiAction(kt, 0.0, -1, ref.EXP)
+    rng = np.random.RandomState(2000)
+    phi = rng.normal(0, np.sqrt(ut), nx * nt) + 0j
+    f = g.force(phi).cpu().numpy()
+    s0 = abs(g.eval(phi))
+    idx = rng.choice(nx * nt, 64, replace=False)
+    # Central differences: truncation O(h^2) plus round-off |S| * eta / h with eta ~ 1e-13 the
+    # relative accuracy of the FP64 action.  h = 1e-6 (the prescribed step) is round-off
+    # limited (~|S| 1e-7); h = 1e-4 gives a tight check.
+    for h, tol in ((1e-6, 5 * s0 * 1e-13 / 1e-6), (1e-4, 2e-6)):
+        for i in idx:
+            e = np.zeros(nx * nt, dtype=complex)
+            e[i] = h
+            d = g.eval(phi + e) - g.eval(phi - e)
+            d = complex(d.real, np.remainder(d.imag + np.pi, 2 * np.pi) - np.pi)
+            assert abs(-d / (2 * h) - f[i]) < tol, (h, i, -d / (2 * h), f[i])
+    for s in range(2000, 2016):
+        phi = np.random.RandomState(s).normal(0, np.sqrt(ut), nx * nt) + 0j
+        assert rel_l2(g.force(phi).cpu().numpy(), r.force(phi)) < 1e-10
