@@ -159,6 +159,7 @@ def main():
         "force_evals_per_s": traj_per_s * (NSTEP + 1),
         "acceptance": float(acc.mean()) if acc.size else None,
         "mean_dH": float(dh.mean()) if dh.size else None,
+        "median_dH": float(np.median(dh)) if dh.size else None,
         "mean_exp_minus_dH": float(np.exp(-dh).mean()) if dh.size else None,
         "roofline": roofline, "cpu_baseline": cpu,
     }

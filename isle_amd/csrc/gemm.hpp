@@ -9,9 +9,10 @@
 // slice costs exactly two real GEMMs and no extra pass over HBM.
 //
 // Tiling: 256 threads (4 waves in 2x2), output tile BM x BN per plane, K staged in
-// slabs of 16 through double-buffered LDS; the next slab is prefetched into registers
-// while the current one feeds the MFMAs.  Each slab is summed in a fresh accumulator
-// before being added to the running sum (blocked summation).
+// slabs of BK through double-buffered LDS; the next slab is prefetched into registers
+// while the current one feeds the MFMAs, so a K-deep product costs K/BK global round
+// trips.  Each group of 16 k is summed in a fresh accumulator before being added to the
+// running sum (blocked summation).
 #pragma once
 
 #include "common.hpp"
@@ -36,16 +37,16 @@ struct GemmItem {
     int lda, ldb, ldc;
 };
 
-constexpr int GEMM_BK = 16;
 constexpr int GEMM_THREADS = 256;
 
 // grid: x = tile index over (ceil(m/BM) x ceil(n/BN)) of the largest item, y = item.
 // AC/BC: A/B complex.  PRE: scale the (single) complex operand along k.  POST: SCALE_*.
-template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN>
+template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int GEMM_BK>
 __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* __restrict__ items,
                                                             int tiles_n_max) {
     static_assert(!(PRE && AC && BC), "pre-scaling needs exactly one complex operand");
     static_assert(BM % 32 == 0 && BN % 32 == 0, "tile must be a multiple of 32");
+    static_assert(GEMM_BK % 16 == 0, "K slab must be a multiple of 16");
     constexpr int MI = BM / 32;  // 16x16 MFMA blocks per wave along m
     constexpr int NJ = BN / 32;  // along n
     constexpr int AE = BM * GEMM_BK / GEMM_THREADS;  // A elements per thread per slab
@@ -158,55 +159,58 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* _
         const bool more = k0 + GEMM_BK < it.k;
         if (more) load_slab(k0 + GEMM_BK);
 
-        V4 part_re[MI][NJ], part_im[MI][NJ];
 #pragma unroll
-        for (int i = 0; i < MI; ++i)
+        for (int kg = 0; kg < GEMM_BK; kg += 16) {
+            V4 part_re[MI][NJ], part_im[MI][NJ];
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) {
-                part_re[i][j] = V4{0, 0, 0, 0};
-                part_im[i][j] = V4{0, 0, 0, 0};
-            }
+            for (int i = 0; i < MI; ++i)
 #pragma unroll
-        for (int ks = 0; ks < GEMM_BK / 4; ++ks) {
-            const int kk = ks * 4 + lk;
-            T ar[MI], ai[MI], br[NJ], bi[NJ];
+                for (int j = 0; j < NJ; ++j) {
+                    part_re[i][j] = V4{0, 0, 0, 0};
+                    part_im[i][j] = V4{0, 0, 0, 0};
+                }
 #pragma unroll
-            for (int i = 0; i < MI; ++i) {
-                ar[i] = As_re[buf][kk][wm + i * 16 + lr];
-                ai[i] = AC ? As_im[buf][kk][wm + i * 16 + lr] : T(0);
-            }
+            for (int ks = 0; ks < 4; ++ks) {
+                const int kk = kg + ks * 4 + lk;
+                T ar[MI], ai[MI], br[NJ], bi[NJ];
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) {
-                br[j] = Bs_re[buf][kk][wn + j * 16 + lr];
-                bi[j] = BC ? Bs_im[buf][kk][wn + j * 16 + lr] : T(0);
+                for (int i = 0; i < MI; ++i) {
+                    ar[i] = As_re[buf][kk][wm + i * 16 + lr];
+                    ai[i] = AC ? As_im[buf][kk][wm + i * 16 + lr] : T(0);
+                }
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) {
+                    br[j] = Bs_re[buf][kk][wn + j * 16 + lr];
+                    bi[j] = BC ? Bs_im[buf][kk][wn + j * 16 + lr] : T(0);
+                }
+#pragma unroll
+                for (int i = 0; i < MI; ++i)
+#pragma unroll
+                    for (int j = 0; j < NJ; ++j) {
+                        if (AC && BC) {
+                            part_re[i][j] = mfma16(ar[i], br[j], part_re[i][j]);
+                            part_re[i][j] = mfma16(-ai[i], bi[j], part_re[i][j]);
+                            part_im[i][j] = mfma16(ar[i], bi[j], part_im[i][j]);
+                            part_im[i][j] = mfma16(ai[i], br[j], part_im[i][j]);
+                        } else if (AC) {
+                            part_re[i][j] = mfma16(ar[i], br[j], part_re[i][j]);
+                            part_im[i][j] = mfma16(ai[i], br[j], part_im[i][j]);
+                        } else if (BC) {
+                            part_re[i][j] = mfma16(ar[i], br[j], part_re[i][j]);
+                            part_im[i][j] = mfma16(ar[i], bi[j], part_im[i][j]);
+                        } else {
+                            part_re[i][j] = mfma16(ar[i], br[j], part_re[i][j]);
+                        }
+                    }
             }
 #pragma unroll
             for (int i = 0; i < MI; ++i)
 #pragma unroll
                 for (int j = 0; j < NJ; ++j) {
-                    if (AC && BC) {
-                        part_re[i][j] = mfma16(ar[i], br[j], part_re[i][j]);
-                        part_re[i][j] = mfma16(-ai[i], bi[j], part_re[i][j]);
-                        part_im[i][j] = mfma16(ar[i], bi[j], part_im[i][j]);
-                        part_im[i][j] = mfma16(ai[i], br[j], part_im[i][j]);
-                    } else if (AC) {
-                        part_re[i][j] = mfma16(ar[i], br[j], part_re[i][j]);
-                        part_im[i][j] = mfma16(ai[i], br[j], part_im[i][j]);
-                    } else if (BC) {
-                        part_re[i][j] = mfma16(ar[i], br[j], part_re[i][j]);
-                        part_im[i][j] = mfma16(ar[i], bi[j], part_im[i][j]);
-                    } else {
-                        part_re[i][j] = mfma16(ar[i], br[j], part_re[i][j]);
-                    }
+                    acc_re[i][j] += part_re[i][j];
+                    if (CPLX) acc_im[i][j] += part_im[i][j];
                 }
         }
-#pragma unroll
-        for (int i = 0; i < MI; ++i)
-#pragma unroll
-            for (int j = 0; j < NJ; ++j) {
-                acc_re[i][j] += part_re[i][j];
-                if (CPLX) acc_im[i][j] += part_im[i][j];
-            }
         if (more) store_slab(buf ^ 1);
         __syncthreads();
         buf ^= 1;
@@ -238,22 +242,39 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* _
             }
 }
 
-// Tile shapes the launcher can pick (the plan's autotuner times them per step).
-enum : int { TILE_64 = 0, TILE_32 = 1, N_TILES = 2 };
+// Tile shapes the launcher can pick (the plan's autotuner times them per step):
+//   0: 64x64, K slab 16   1: 32x32, K slab 32   2: 32x32, K slab 64   3: 64x64, K slab 32
+enum : int { N_TILES = 4 };
+
+template <typename T, bool AC, bool BC, int BM, int BN, int BK>
+constexpr size_t gemm_lds_bytes() {
+    return sizeof(T) * 2 * BK * (BM * (AC ? 2 : 1) + BN * (BC ? 2 : 1));
+}
+
+// Launches the GEMM with tile shape `tile`; returns false if that shape does not fit
+// this operand type (LDS budget), in which case nothing is launched.
+template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int BK>
+inline bool launch_gemm_tile(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s) {
+    if constexpr (gemm_lds_bytes<T, AC, BC, BM, BN, BK>() > 96 * 1024) {
+        return false;
+    } else {
+        const int tm = (max_m + BM - 1) / BM, tn = (max_n + BN - 1) / BN;
+        hipLaunchKernelGGL((gemm_kernel<T, AC, BC, PRE, POST, BM, BN, BK>), dim3(tm * tn, nitems),
+                           dim3(GEMM_THREADS), 0, s, d_items, tn);
+        ISLE_HIP(hipGetLastError());
+        return true;
+    }
+}
 
 template <typename T, bool AC, bool BC, bool PRE, int POST>
-inline void launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s,
-                        int tile = TILE_64) {
-    if (tile == TILE_32) {
-        const int tm = (max_m + 31) / 32, tn = (max_n + 31) / 32;
-        hipLaunchKernelGGL((gemm_kernel<T, AC, BC, PRE, POST, 32, 32>), dim3(tm * tn, nitems), dim3(GEMM_THREADS), 0,
-                           s, d_items, tn);
-    } else {
-        const int tm = (max_m + 63) / 64, tn = (max_n + 63) / 64;
-        hipLaunchKernelGGL((gemm_kernel<T, AC, BC, PRE, POST, 64, 64>), dim3(tm * tn, nitems), dim3(GEMM_THREADS), 0,
-                           s, d_items, tn);
+inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s,
+                        int tile = 0) {
+    switch (tile) {
+    case 1: return launch_gemm_tile<T, AC, BC, PRE, POST, 32, 32, 32>(d_items, nitems, max_m, max_n, s);
+    case 2: return launch_gemm_tile<T, AC, BC, PRE, POST, 32, 32, 64>(d_items, nitems, max_m, max_n, s);
+    case 3: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 32>(d_items, nitems, max_m, max_n, s);
+    default: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 16>(d_items, nitems, max_m, max_n, s);
     }
-    ISLE_HIP(hipGetLastError());
 }
 
 }  // namespace isle

@@ -239,10 +239,10 @@ void Plan::tune(OpCache& op) {
             continue;
         }
         float best = 1e30f;
-        int best_tile = TILE_64;
+        int best_tile = 0;
         for (int tile = 0; tile < N_TILES; ++tile) {
             st.tile = tile;
-            launch_step(op, st, nullptr);  // warm up
+            if (!launch_step(op, st, nullptr)) continue;  // shape does not fit; warm up otherwise
             ISLE_HIP(hipEventRecord(e0, stream_));
             for (int r = 0; r < 3; ++r) launch_step(op, st, nullptr);
             ISLE_HIP(hipEventRecord(e1, stream_));
@@ -296,7 +296,7 @@ void Plan::profile(const double* phi, int opk, int nt, int batch, int group, int
     }
 }
 
-void Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
+bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
     const int n = nx_;
     const long nn = (long)n * n;
     hipStream_t s = stream_;
@@ -307,28 +307,28 @@ void Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
                                (const PhaseItem*)st.items, phi, op.phi_stride);
             break;
         case K::GEMM64_RC_PRE:
-            launch_gemm<double, false, true, true, SCALE_NONE>((const GemmItem<double>*)st.items, st.count,
-                                                               st.max_m, st.max_n, s, st.tile);
+            if (!launch_gemm<double, false, true, true, SCALE_NONE>((const GemmItem<double>*)st.items, st.count,
+                                                               st.max_m, st.max_n, s, st.tile)) return false;
             break;
         case K::GEMM64_RC_ROW:
-            launch_gemm<double, false, true, false, SCALE_ROW>((const GemmItem<double>*)st.items, st.count,
-                                                               st.max_m, st.max_n, s, st.tile);
+            if (!launch_gemm<double, false, true, false, SCALE_ROW>((const GemmItem<double>*)st.items, st.count,
+                                                               st.max_m, st.max_n, s, st.tile)) return false;
             break;
         case K::GEMM64_CR_PRE:
-            launch_gemm<double, true, false, true, SCALE_NONE>((const GemmItem<double>*)st.items, st.count,
-                                                               st.max_m, st.max_n, s, st.tile);
+            if (!launch_gemm<double, true, false, true, SCALE_NONE>((const GemmItem<double>*)st.items, st.count,
+                                                               st.max_m, st.max_n, s, st.tile)) return false;
             break;
         case K::GEMM64_CC:
-            launch_gemm<double, true, true, false, SCALE_NONE>((const GemmItem<double>*)st.items, st.count,
-                                                               st.max_m, st.max_n, s, st.tile);
+            if (!launch_gemm<double, true, true, false, SCALE_NONE>((const GemmItem<double>*)st.items, st.count,
+                                                               st.max_m, st.max_n, s, st.tile)) return false;
             break;
         case K::GEMM32_RC_ROW:
-            launch_gemm<float, false, true, false, SCALE_ROW>((const GemmItem<float>*)st.items, st.count, st.max_m,
-                                                              st.max_n, s, st.tile);
+            if (!launch_gemm<float, false, true, false, SCALE_ROW>((const GemmItem<float>*)st.items, st.count, st.max_m,
+                                                              st.max_n, s, st.tile)) return false;
             break;
         case K::GEMM32_CR_PRE:
-            launch_gemm<float, true, false, true, SCALE_NONE>((const GemmItem<float>*)st.items, st.count, st.max_m,
-                                                              st.max_n, s, st.tile);
+            if (!launch_gemm<float, true, false, true, SCALE_NONE>((const GemmItem<float>*)st.items, st.count, st.max_m,
+                                                              st.max_n, s, st.tile)) return false;
             break;
         case K::SCALE64:
             hipLaunchKernelGGL(scale_real_kernel<double>, dim3(ew_blocks(nn), st.count), dim3(EW_THREADS), 0, s,
@@ -367,6 +367,7 @@ void Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
         }
         ISLE_HIP(hipGetLastError());
     }
+    return true;
 }
 
 // ---------------------------------------------------------------------------

@@ -128,7 +128,7 @@ private:
     OpCache& get_logdet_plain(int nt, int batch, int nspecies, int spin);
     OpCache& get_logdet_grouped(int nt, int batch, int group, int nspecies, int spin);
     void run(OpCache& op, const double* phi);
-    void launch_step(OpCache& op, const Step& st, const double* phi);
+    bool launch_step(OpCache& op, const Step& st, const double* phi);
     // Autotuner: time every tile shape of each GEMM step once per (kind, m, n, items) and
     // keep the fastest for that step.
     void tune(OpCache& op);
