@@ -20,9 +20,9 @@ import numpy as np
 METRIC = "HMC traj/s at Nx=288 Nt=128: batched FP32-force vs plain FP64; force rel err"
 L, NT, BETA, U, NSTEP, LENGTH = 12, 128, 8.0, 3.0, 40, 1.0
 PEAK = {"f32": 157.3, "f64": 78.6}  # dense MFMA TFLOP/s, MI355X
-KIND_NAMES = ["gemm64_rc_pre", "gemm64_rc_row", "gemm64_cr_pre", "gemm64_cc", "gemm32_rc_row", "gemm32_cr_pre",
+KIND_NAMES = ["gemm64_rc_pre", "gemm64_rc_row", "gemm64_cr_pre", "gemm64_cc", "gemm32_rc_row", "gemm32_rc_pre",
               "scale64", "scale32", "lu", "lu_solve", "diag64", "diag32", "copy64", "cvt32to64", "cvt64to32",
-              "phase"]
+              "phase", "tcvt64to32"]
 
 
 def parse():

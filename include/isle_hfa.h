@@ -108,8 +108,9 @@ int isle_cscale(const double* in, double* out, long n, double cr, double ci, voi
  * For launch kind k < nkinds: out_host[4k..4k+3] = (launches per operation, ms per
  * operation, algorithmic real flops per operation, 0).  Kinds: 0 GEMM64 real x complex
  * (pre-scaled), 1 GEMM64 real x complex (row-scaled), 2 GEMM64 complex x real, 3 GEMM64
- * complex x complex, 4 GEMM32 real x complex, 5 GEMM32 complex x real, 6-7 chain starts,
- * 8 LU, 9 LU solve, 10-11 diagonal extraction, 12-14 copies, 15 phases. */
+ * complex x complex, 4 GEMM32 real x complex (row-scaled), 5 GEMM32 real x complex
+ * (pre-scaled), 6-7 chain starts, 8 LU, 9 LU solve, 10-11 diagonal extraction, 12-14
+ * copies, 15 phases, 16 transposed copy. */
 int isle_profile_kernels(isle_hfm* h, const double* phi, int op, int nt, int batch, int group, int nspecies,
                          int spin, int reps, double* out_host, int nkinds);
 

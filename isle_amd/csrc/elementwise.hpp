@@ -109,10 +109,11 @@ template <typename T>
 struct DiagItem {
     const T* l_re;
     const T* l_im;
-    const T* r_re;
+    const T* r_re;   // R, or R^T when rowdot != 0
     const T* r_im;
     double* out;
     int n;
+    int rowdot;      // 1: out[i] = sum_k L[i][k] RT[i][k]  (R stored transposed, both rows contiguous)
 };
 
 template <typename T>
@@ -125,7 +126,14 @@ __global__ __launch_bounds__(256) void diag_prod_kernel(const DiagItem<T>* __res
     if (blockIdx.x * 16 >= n) return;
     double sr = 0, si = 0;
     if (i < n) {
-        if (it.l_re) {
+        if (it.l_re && it.rowdot) {
+            for (int k = kk; k < n; k += 16) {
+                const double ar = it.l_re[(long)i * n + k], ai = it.l_im[(long)i * n + k];
+                const double br = it.r_re[(long)i * n + k], bi = it.r_im[(long)i * n + k];
+                sr += ar * br - ai * bi;
+                si += ar * bi + ai * br;
+            }
+        } else if (it.l_re) {
             for (int k = kk; k < n; k += 16) {
                 const double ar = it.l_re[(long)i * n + k], ai = it.l_im[(long)i * n + k];
                 const double br = it.r_re[(long)k * n + i], bi = it.r_im[(long)k * n + i];
@@ -299,6 +307,38 @@ __global__ void copy_items_kernel(const CopyItem<S, D>* __restrict__ items) {
     const CopyItem<S, D> it = items[blockIdx.y];
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < it.count; e += (long)gridDim.x * blockDim.x)
         it.dst[e] = (D)it.src[e];
+}
+
+// n x n complex matrices (re plane then im plane) copied transposed, with conversion
+template <typename S, typename D>
+struct TCopyItem {
+    const S* src;
+    D* dst;
+    int n;
+};
+
+template <typename S, typename D>
+__global__ void transpose_items_kernel(const TCopyItem<S, D>* __restrict__ items) {
+    __shared__ S tile[32][33];
+    const TCopyItem<S, D> it = items[blockIdx.z];
+    const int n = it.n;
+    const int tiles = (n + 31) / 32;
+    const int plane = blockIdx.y;
+    const long nn = (long)n * n;
+    for (int tb = blockIdx.x; tb < tiles * tiles; tb += gridDim.x) {
+        const int bi = tb / tiles, bj = tb % tiles;
+        const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+        for (int r = ty; r < 32; r += 8) {
+            const int i = bi * 32 + r, j = bj * 32 + tx;
+            tile[r][tx] = (i < n && j < n) ? it.src[plane * nn + (long)i * n + j] : S(0);
+        }
+        __syncthreads();
+        for (int r = ty; r < 32; r += 8) {
+            const int i = bj * 32 + r, j = bi * 32 + tx;  // dst[i][j] = src[j][i]
+            if (i < n && j < n) it.dst[plane * nn + (long)i * n + j] = (D)tile[tx][r];
+        }
+        __syncthreads();
+    }
 }
 
 // solveM helpers: rhss [nrhs][nt*n] interleaved -> per-slice split planes [t][2][n][nrhs]

@@ -19,7 +19,7 @@
 
 namespace isle {
 
-enum : int { SCALE_NONE = 0, SCALE_ROW = 1, SCALE_COL = 2 };
+enum : int { SCALE_NONE = 0, SCALE_ROW = 1, SCALE_COL = 2, SUBTRACT = 3 };  // POST: C = C - A B
 
 template <typename T>
 struct GemmItem {
@@ -39,11 +39,11 @@ struct GemmItem {
 
 constexpr int GEMM_THREADS = 256;
 
-// grid: x = tile index over (ceil(m/BM) x ceil(n/BN)) of the largest item, y = item.
-// AC/BC: A/B complex.  PRE: scale the (single) complex operand along k.  POST: SCALE_*.
+// One BM x BN output tile of item `it` (tile index `tile` in a ceil(m/BM) x tiles_n_max
+// grid).  AC/BC: A/B complex.  PRE: scale the (single) complex operand along k.
+// POST: SCALE_* or SUBTRACT.  Must be called by all 256 threads of the block.
 template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int GEMM_BK>
-__global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* __restrict__ items,
-                                                            int tiles_n_max) {
+__device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int tiles_n_max) {
     static_assert(!(PRE && AC && BC), "pre-scaling needs exactly one complex operand");
     static_assert(BM % 32 == 0 && BN % 32 == 0, "tile must be a multiple of 32");
     static_assert(GEMM_BK % 16 == 0, "K slab must be a multiple of 16");
@@ -53,9 +53,8 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* _
     constexpr int BE = BN * GEMM_BK / GEMM_THREADS;  // B elements per thread per slab
     constexpr bool CPLX = AC || BC;
 
-    const GemmItem<T> it = items[blockIdx.y];
-    const int tile_m = blockIdx.x / tiles_n_max;
-    const int tile_n = blockIdx.x % tiles_n_max;
+    const int tile_m = tile / tiles_n_max;
+    const int tile_n = tile % tiles_n_max;
     const int m0 = tile_m * BM;
     const int n0 = tile_n * BN;
     if (m0 >= it.m || n0 >= it.n) return;
@@ -228,6 +227,12 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* _
                 if (row < it.m && col < it.n) {
                     T vr = acc_re[i][j][r];
                     T vi = CPLX ? acc_im[i][j][r] : T(0);
+                    const long off = (long)row * it.ldc + col;
+                    if (POST == SUBTRACT) {
+                        it.c_re[off] -= vr;
+                        if (CPLX) it.c_im[off] -= vi;
+                        continue;
+                    }
                     if (POST != SCALE_NONE) {
                         const int di = POST == SCALE_ROW ? row : col;
                         const T pr = it.post_re[di], pi = it.post_im[di];
@@ -235,11 +240,18 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* _
                         vi = vr * pi + vi * pr;
                         vr = t;
                     }
-                    const long off = (long)row * it.ldc + col;
                     it.c_re[off] = vr;
                     if (CPLX || POST != SCALE_NONE) it.c_im[off] = vi;
                 }
             }
+}
+
+// grid: x = tile index over (ceil(m/BM) x ceil(n/BN)) of the largest item, y = item.
+template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int GEMM_BK>
+__global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* __restrict__ items,
+                                                            int tiles_n_max) {
+    const GemmItem<T> it = items[blockIdx.y];
+    gemm_tile<T, AC, BC, PRE, POST, BM, BN, GEMM_BK>(it, blockIdx.x, tiles_n_max);
 }
 
 // Tile shapes the launcher can pick (the plan's autotuner times them per step):

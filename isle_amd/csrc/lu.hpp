@@ -91,6 +91,7 @@ __global__ __launch_bounds__(LU_THREADS) void lu_kernel(const LuItem* __restrict
     extern __shared__ double lds[];
     __shared__ double sval[LU_THREADS / 64];
     __shared__ int sidx[LU_THREADS / 64];
+    __shared__ int perm_pos[2 * NB], perm_src[2 * NB], perm_cnt;
     const LuItem it = items[blockIdx.x];
     const int n = it.n, ld = it.ld, tid = threadIdx.x, nth = blockDim.x;
     double* re = it.re;
@@ -167,18 +168,52 @@ __global__ __launch_bounds__(LU_THREADS) void lu_kernel(const LuItem* __restrict
             re[(long)(k0 + r) * ld + k0 + c] = p_re[r * NB + c];
             im[(long)(k0 + r) * ld + k0 + c] = p_im[r * NB + c];
         }
-        // ---- 2. row interchanges on the other columns (in order), one column per thread
-        for (int c = tid; c < n; c += nth) {
-            if (c >= k0 && c < k0 + kb) continue;
+        // ---- 2. row interchanges on the other columns: the panel's swaps are composed
+        //         into one permutation of at most 2*NB rows, applied as a gather/scatter
+        if (tid == 0) {
+            int cnt = 0;
             for (int kk = 0; kk < kb; ++kk) {
-                const int p = it.piv[k0 + kk];
-                if (p != k0 + kk) {
-                    const long a = (long)(k0 + kk) * ld + c, b = (long)p * ld + c;
-                    const double tr = re[a], ti = im[a];
-                    re[a] = re[b];
-                    im[a] = im[b];
-                    re[b] = tr;
-                    im[b] = ti;
+                const int a = k0 + kk, p = it.piv[k0 + kk];
+                int ia = -1, ip = -1;
+                for (int q = 0; q < cnt; ++q) {
+                    if (perm_pos[q] == a) ia = q;
+                    if (perm_pos[q] == p) ip = q;
+                }
+                if (ia < 0) {
+                    perm_pos[cnt] = a;
+                    perm_src[cnt] = a;
+                    ia = cnt++;
+                }
+                if (ip < 0) {
+                    perm_pos[cnt] = p;
+                    perm_src[cnt] = p;
+                    ip = cnt++;
+                }
+                const int t = perm_src[ia];
+                perm_src[ia] = perm_src[ip];
+                perm_src[ip] = t;
+            }
+            perm_cnt = cnt;
+        }
+        __syncthreads();
+        {
+            const int cnt = perm_cnt;
+            for (int c = tid; c < n; c += nth) {
+                if (c >= k0 && c < k0 + kb) continue;
+                double vr[2 * NB], vi[2 * NB];
+#pragma unroll
+                for (int q = 0; q < 2 * NB; ++q) {
+                    if (q < cnt) {
+                        vr[q] = re[(long)perm_src[q] * ld + c];
+                        vi[q] = im[(long)perm_src[q] * ld + c];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 2 * NB; ++q) {
+                    if (q < cnt) {
+                        re[(long)perm_pos[q] * ld + c] = vr[q];
+                        im[(long)perm_pos[q] * ld + c] = vi[q];
+                    }
                 }
             }
         }
@@ -285,16 +320,10 @@ __global__ __launch_bounds__(LU_THREADS) void lu_kernel(const LuItem* __restrict
 }
 
 inline void launch_lu(const LuItem* d_items, int nitems, int max_n, int add_identity, hipStream_t s) {
-    if (max_n <= 288) {
-        const size_t lds = sizeof(double) * 4 * (size_t)max_n * 16;
-        ISLE_HIP(hipFuncSetAttribute((const void*)lu_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(lu_kernel<16>, dim3(nitems), dim3(LU_THREADS), lds, s, d_items, add_identity);
-    } else {
-        const size_t lds = sizeof(double) * 4 * (size_t)max_n * 8;
-        if (lds > 150 * 1024) throw std::invalid_argument("matrix too large for the LU kernel");
-        ISLE_HIP(hipFuncSetAttribute((const void*)lu_kernel<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(lu_kernel<8>, dim3(nitems), dim3(LU_THREADS), lds, s, d_items, add_identity);
-    }
+    const size_t lds = sizeof(double) * 4 * (size_t)max_n * 8;
+    if (lds > 150 * 1024) throw std::invalid_argument("matrix too large for the LU kernel");
+    ISLE_HIP(hipFuncSetAttribute((const void*)lu_kernel<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(lu_kernel<8>, dim3(nitems), dim3(LU_THREADS), lds, s, d_items, add_identity);
     ISLE_HIP(hipGetLastError());
 }
 

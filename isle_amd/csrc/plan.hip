@@ -223,7 +223,7 @@ void Plan::run(OpCache& op, const double* phi) {
 
 static bool is_gemm(K k) {
     return k == K::GEMM64_RC_PRE || k == K::GEMM64_RC_ROW || k == K::GEMM64_CR_PRE || k == K::GEMM64_CC ||
-           k == K::GEMM32_RC_ROW || k == K::GEMM32_CR_PRE;
+           k == K::GEMM32_RC_ROW || k == K::GEMM32_RC_PRE;
 }
 
 void Plan::tune(OpCache& op) {
@@ -326,9 +326,13 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
             if (!launch_gemm<float, false, true, false, SCALE_ROW>((const GemmItem<float>*)st.items, st.count, st.max_m,
                                                               st.max_n, s, st.tile)) return false;
             break;
-        case K::GEMM32_CR_PRE:
-            if (!launch_gemm<float, true, false, true, SCALE_NONE>((const GemmItem<float>*)st.items, st.count, st.max_m,
+        case K::GEMM32_RC_PRE:
+            if (!launch_gemm<float, false, true, true, SCALE_NONE>((const GemmItem<float>*)st.items, st.count, st.max_m,
                                                               st.max_n, s, st.tile)) return false;
+            break;
+        case K::TCVT64TO32:
+            hipLaunchKernelGGL((transpose_items_kernel<double, float>), dim3(64, 2, st.count), dim3(256), 0, s,
+                               (const TCopyItem<double, float>*)st.items);
             break;
         case K::SCALE64:
             hipLaunchKernelGGL(scale_real_kernel<double>, dim3(ew_blocks(nn), st.count), dim3(EW_THREADS), 0, s,
@@ -375,8 +379,9 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
 // pairs advanced in lock step:
 //   lefts   Y_{nt-1} = X_{nt-1},  Y_t = X_t Y_{t+1}  with X_t = D_t A_s (D_t = exp(sigma i phi_{t-1}))
 //   Mid     = (1 + Y_0)^{-1}
-//   right   R_{-1} = Mid, R_tau = R_{tau-1} X_tau
-//   f[nt-1] = diag(Y_0 Mid),  f[tau] = diag(Y_{tau+1} R_tau)
+//   right   R_{-1} = Mid, R_tau = R_{tau-1} X_tau, carried transposed (A_s is symmetric):
+//           RT_tau = A_s (D_tau RT_{tau-1}), so every product is a real x complex GEMM
+//   f[nt-1] = diag(Y_0 Mid),  f[tau] = diag(Y_{tau+1} R_tau) = sum_k Y_{tau+1}[i][k] RT_tau[i][k]
 OpCache& Plan::get_force_plain(int nt, int batch, int nspecies, int spin) {
     auto key = std::make_tuple(0, nt, batch, 0, nspecies, spin);
     auto found = cache_.find(key);
@@ -436,10 +441,10 @@ OpCache& Plan::get_force_plain(int nt, int batch, int nspecies, int spin) {
         si.lu_im = si.lu_re + nn;
         si.piv = piv + (long)item * n;
         si.b_re = si.b_im = nullptr;
-        si.x_re = mR(item, nt - 1).re;
+        si.x_re = mR(item, nt - 1).re;  // Mid^T: element (r, c) stored at [c][r]
         si.x_im = mR(item, nt - 1).im;
-        si.xsr = n;
-        si.xsc = 1;
+        si.xsr = 1;
+        si.xsc = n;
         si.n = n;
         si.ld = n;
         si.nrhs = n;
@@ -450,17 +455,17 @@ OpCache& Plan::get_force_plain(int nt, int batch, int nspecies, int spin) {
         std::vector<GemmItem<double>> g;
         for (int item = 0; item < nb; ++item) {
             const CMat<double> prev = tau == 0 ? mR(item, nt - 1) : mR(item, tau - 1);
-            g.push_back(gitem<double>(prev.re, prev.im, a64_[spec(item)]->as<double>(), nullptr, mR(item, tau),
+            g.push_back(gitem<double>(a64_[spec(item)]->as<double>(), nullptr, prev.re, prev.im, mR(item, tau),
                                       D(item, tau).re, D(item, tau).im, nullptr, nullptr, n));
         }
-        b.add(K::GEMM64_CR_PRE, g, n, n);
+        b.add(K::GEMM64_RC_PRE, g, n, n);
     }
     std::vector<DiagItem<double>> dg;
     for (int item = 0; item < nb; ++item)
         for (int tau = 0; tau < nt; ++tau) {
             const CMat<double> l = tau == nt - 1 ? mY(item, 0) : mY(item, tau + 1);
             const CMat<double> r = tau == nt - 1 ? mR(item, nt - 1) : mR(item, tau);
-            dg.push_back({l.re, l.im, r.re, r.im, op->parts + ((long)item * nt + tau) * 2 * n, n});
+            dg.push_back({l.re, l.im, r.re, r.im, op->parts + ((long)item * nt + tau) * 2 * n, n, 1});
         }
     b.add(K::DIAG64, dg);
     b.finish();
@@ -477,8 +482,10 @@ OpCache& Plan::get_force_plain(int nt, int batch, int nspecies, int spin) {
 // Top level (FP64, group products promoted exactly):
 //   GL_{ng-1} = I, GL_j = P_{j+1} GL_{j+1};  Mid = (1 + P_0 GL_0)^{-1}
 //   RR_0 = Mid, RR_j = RR_{j-1} P_{j-1};     K_j = GL_j RR_j
-// Per group again (FP32):  Rt_{s_j} = K_j X_{s_j}, Rt_tau = Rt_{tau-1} X_tau,
-//   f[tau] = diag(Suf_{tau+1} Rt_tau)  (tau < e_j),  f[e_j] = diag(Rt_{e_j}).
+// Per group again (FP32):  Rt_{s_j} = K_j X_{s_j}, Rt_tau = Rt_{tau-1} X_tau, carried
+// transposed (A_s symmetric): RT_{s_j} = A_s (D_{s_j} K_j^T), RT_tau = A_s (D_tau RT_{tau-1});
+//   f[tau] = diag(Suf_{tau+1} Rt_tau) = sum_k Suf_{tau+1}[i][k] RT_tau[i][k]  (tau < e_j),
+//   f[e_j] = diag(RT_{e_j}).
 // This is the reference's lefts/right factorisation applied inside each group with the
 // global factors folded into K_j; FP32 chains never span more than g slices.
 OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin) {
@@ -614,25 +621,25 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
         b.add(K::GEMM64_CC, gi, n, n);
     }
     {
-        std::vector<CopyItem<double, float>> cv;
+        std::vector<TCopyItem<double, float>> cv;
         for (int item = 0; item < nb; ++item)
             for (int j = 0; j < ng; ++j) {
                 const double* src = (j < ng - 1) ? m64(KK, item, j).re : m64(RR, item, ng - 1).re;
-                cv.push_back({src, mK32(item, j).re, 2 * nn});
+                cv.push_back({src, mK32(item, j).re, n});
             }
-        b.add(K::CVT64TO32, cv, (int)(2 * nn));
+        b.add(K::TCVT64TO32, cv);
     }
-    // ---- local right chains (FP32)
+    // ---- local right chains (FP32), transposed
     for (int q = 0; q < g; ++q) {
         std::vector<GemmItem<float>> gi;
         for (int item = 0; item < nb; ++item)
             for (int j = 0; j < ng; ++j) {
                 const int tau = j * g + q;
                 const CMat<float> prev = q == 0 ? mK32(item, j) : mRt(item, tau - 1);
-                gi.push_back(gitem<float>(prev.re, prev.im, A32[spec(item)], nullptr, mRt(item, tau), D(item, tau).re,
+                gi.push_back(gitem<float>(A32[spec(item)], nullptr, prev.re, prev.im, mRt(item, tau), D(item, tau).re,
                                           D(item, tau).im, nullptr, nullptr, n));
             }
-        b.add(K::GEMM32_CR_PRE, gi, n, n);
+        b.add(K::GEMM32_RC_PRE, gi, n, n);
     }
     {
         std::vector<DiagItem<float>> dg;
@@ -641,7 +648,7 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
                 const bool last = (tau % g) == g - 1;
                 const CMat<float> l = last ? CMat<float>{nullptr, nullptr} : mS(item, tau + 1);
                 const CMat<float> r = mRt(item, tau);
-                dg.push_back({l.re, l.im, r.re, r.im, op->parts + ((long)item * nt + tau) * 2 * n, n});
+                dg.push_back({l.re, l.im, r.re, r.im, op->parts + ((long)item * nt + tau) * 2 * n, n, 1});
             }
         b.add(K::DIAG32, dg);
     }
@@ -656,7 +663,8 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
 // Plain FP64 logdetM, the reference's chains (hubbardFermiMatrixExp.cpp:574-612,
 // hubbardFermiMatrixDia.cpp:571-597):
 //   EXP particle: B = F(nt-1)...F(0), F(t) = E_p diag(exp(i phi_{t-1}))  (B <- E_p (D'_t B))
-//   EXP hole / DIA: aux = X_0 X_1 ... X_{nt-1}                            (aux <- (aux D_t) A_s)
+//   EXP hole / DIA: aux = X_0 X_1 ... X_{nt-1}  carried transposed (A_s symmetric):
+//                   auxT <- A_s (D_t auxT), auxT_0 = A_s D_0;  det(1 + aux) = det(1 + auxT)
 //   then LU of (1 + .) with its log-determinant.
 OpCache& Plan::get_logdet_plain(int nt, int batch, int nspecies, int spin) {
     auto key = std::make_tuple(2, nt, batch, 0, nspecies, spin);
@@ -694,22 +702,18 @@ OpCache& Plan::get_logdet_plain(int nt, int batch, int nspecies, int spin) {
                           SCALE_COL});
         else
             sc.push_back({a64_[spec(item)]->as<double>(), D(item, 0).re, D(item, 0).im, mW(item, 0).re,
-                          mW(item, 0).im, n, SCALE_ROW});
+                          mW(item, 0).im, n, SCALE_COL});
     }
     b.add(K::SCALE64, sc);
     for (int t = 1; t < nt; ++t) {
-        std::vector<GemmItem<double>> gb, gr;
+        std::vector<GemmItem<double>> gb;
         for (int item = 0; item < nb; ++item) {
             const CMat<double> prev = mW(item, (t - 1) & 1), cur = mW(item, t & 1);
-            if (bchain(item))
-                gb.push_back(gitem<double>(e64_[PARTICLE]->as<double>(), nullptr, prev.re, prev.im, cur, D(item, t).re,
-                                           D(item, t).im, nullptr, nullptr, n));
-            else
-                gr.push_back(gitem<double>(prev.re, prev.im, a64_[spec(item)]->as<double>(), nullptr, cur,
-                                           D(item, t).re, D(item, t).im, nullptr, nullptr, n));
+            const double* A = bchain(item) ? e64_[PARTICLE]->as<double>() : a64_[spec(item)]->as<double>();
+            gb.push_back(gitem<double>(A, nullptr, prev.re, prev.im, cur, D(item, t).re, D(item, t).im, nullptr,
+                                       nullptr, n));
         }
         b.add(K::GEMM64_RC_PRE, gb, n, n);
-        b.add(K::GEMM64_CR_PRE, gr, n, n);
     }
     std::vector<LuItem> lu;
     for (int item = 0; item < nb; ++item) {

@@ -43,13 +43,14 @@ enum class K : int {
     GEMM64_CR_PRE,      // complex A x real B, A pre-scaled along k        (f64)
     GEMM64_CC,          // complex x complex                               (f64)
     GEMM32_RC_ROW,      // (f32)
-    GEMM32_CR_PRE,      // (f32)
+    GEMM32_RC_PRE,      // (f32)
     SCALE64, SCALE32,   // chain start diag(d) A or A diag(d)
     LU,                 // factorise (flag: + I), logdet
     SOLVE,              // X = A^{-1} B with stored LU
     DIAG64, DIAG32,     // diag(L R)
     COPY64, CVT32TO64, CVT64TO32,
     PHASE,              // phase vectors from the runtime phi
+    TCVT64TO32,         // transposed copy with conversion
 };
 
 struct Step {
