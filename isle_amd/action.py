@@ -219,15 +219,26 @@ class HubbardFermiAction(Action):
     def eval(self, phi):
         return complex(self.eval_batch(phi)[0])
 
+    def force_parts(self, phis):
+        """forceDirectSinglePart (hubbardFermiAction.cpp:16-60) of each species for each
+        field: [batch, nspecies, nx*nt] (one species with the hole shortcut; evaluated at
+        aux = -i phi in the spin basis)."""
+        phis, batch, nt = self._shape(phis)
+        nx = self.hfm.nx()
+        nsp = 1 if self.shortcut else 2
+        parts = torch.empty((batch, nsp, nx * nt), dtype=torch.complex128, device=phis.device)
+        _lib.check(_lib.load().isle_force_parts(self.hfm.handle, _lib.ptr(phis), nt, batch, self._mode(),
+                                                self._group(nt), nsp, int(self.basis == HFABasis.SPIN),
+                                                _lib.ptr(parts)))
+        return parts
+
     def force_batch(self, phis, out=None, accumulate=False):
         """Force -dS/dphi of each field (hubbardFermiAction.cpp:243-272), [batch, nx*nt]."""
         phis, batch, nt = self._shape(phis)
         nx = self.hfm.nx()
         nsp = 1 if self.shortcut else 2
-        parts = torch.empty((batch, nsp, nx * nt), dtype=torch.complex128, device=phis.device)
+        parts = self.force_parts(phis)
         lib = _lib.load()
-        _lib.check(lib.isle_force_parts(self.hfm.handle, _lib.ptr(phis), nt, batch, self._mode(), self._group(nt),
-                                        nsp, int(self.basis == HFABasis.SPIN), _lib.ptr(parts)))
         if out is None:
             out = torch.empty((batch, nx * nt), dtype=torch.complex128, device=phis.device)
             accumulate = False

@@ -51,7 +51,10 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
     constexpr int NJ = BN / 32;  // along n
     constexpr int AE = BM * GEMM_BK / GEMM_THREADS;  // A elements per thread per slab
     constexpr int BE = BN * GEMM_BK / GEMM_THREADS;  // B elements per thread per slab
+    static_assert(AE * GEMM_THREADS == BM * GEMM_BK && BE * GEMM_THREADS == BN * GEMM_BK, "slab/threads");
     constexpr bool CPLX = AC || BC;
+    // FP32 sums each group of 16 k in a fresh accumulator (blocked summation); FP64 does not need it
+    constexpr bool BLOCKED = sizeof(T) == 4;
 
     const int tile_m = tile / tiles_n_max;
     const int tile_n = tile % tiles_n_max;
@@ -73,7 +76,8 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
     const int lk = lane >> 4;
 
     typedef typename Acc4<T>::type V4;
-    V4 acc_re[MI][NJ], acc_im[MI][NJ];
+    constexpr int NP2 = (AC && BC) ? MI * NJ : 1;  // third product of the complex x complex case
+    V4 acc_re[MI][NJ], acc_im[MI][NJ], acc_p2[NP2];
 #pragma unroll
     for (int i = 0; i < MI; ++i)
 #pragma unroll
@@ -81,23 +85,18 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
             acc_re[i][j] = V4{0, 0, 0, 0};
             acc_im[i][j] = V4{0, 0, 0, 0};
         }
+#pragma unroll
+    for (int q = 0; q < NP2; ++q) acc_p2[q] = V4{0, 0, 0, 0};
 
-    // A slab: BM rows x 16 k; thread covers row la_m, k = la_k .. la_k+AE-1
-    constexpr int A_TPR = GEMM_BK / AE;  // threads per row
-    const int la_m = tid / A_TPR;
-    const int la_k = (tid % A_TPR) * AE;
-    // B slab: 16 k x BN cols; thread covers k row lb_k, cols lb_n .. lb_n+BE-1
-    constexpr int B_TPR = BN / BE;
-    const int lb_k = tid / B_TPR;
-    const int lb_n = (tid % B_TPR) * BE;
-
+    // Slab staging: element e = tid + 256 q of the A slab is (row e / BK, k e % BK) and of
+    // the B slab (k e / BN, col e % BN), so consecutive threads read consecutive addresses.
     T ra_re[AE], ra_im[AC ? AE : 1], rb_re[BE], rb_im[BC ? BE : 1];
 
     auto load_slab = [&](int k0) {
-        const int gm = m0 + la_m;
 #pragma unroll
         for (int q = 0; q < AE; ++q) {
-            const int gk = k0 + la_k + q;
+            const int e = tid + q * GEMM_THREADS;
+            const int gm = m0 + e / GEMM_BK, gk = k0 + e % GEMM_BK;
             T vr = 0, vi = 0;
             if (gm < it.m && gk < it.k) {
                 const long off = (long)gm * it.lda + gk;
@@ -113,21 +112,17 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
             ra_re[q] = vr;
             if (AC) ra_im[q] = vi;
         }
-        const int gk = k0 + lb_k;
-        T pr = 1, pi = 0;
-        if (PRE && BC && gk < it.k) {
-            pr = it.pre_re[gk];
-            pi = it.pre_im[gk];
-        }
 #pragma unroll
         for (int q = 0; q < BE; ++q) {
-            const int gn = n0 + lb_n + q;
+            const int e = tid + q * GEMM_THREADS;
+            const int gk = k0 + e / BN, gn = n0 + e % BN;
             T vr = 0, vi = 0;
             if (gk < it.k && gn < it.n) {
                 const long off = (long)gk * it.ldb + gn;
                 vr = it.b_re[off];
                 if (BC) vi = it.b_im[off];
                 if (PRE && BC) {
+                    const T pr = it.pre_re[gk], pi = it.pre_im[gk];
                     const T r = vr * pr - vi * pi;
                     vi = vr * pi + vi * pr;
                     vr = r;
@@ -140,13 +135,15 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
     auto store_slab = [&](int b) {
 #pragma unroll
         for (int q = 0; q < AE; ++q) {
-            As_re[b][la_k + q][la_m] = ra_re[q];
-            if (AC) As_im[b][la_k + q][la_m] = ra_im[q];
+            const int e = tid + q * GEMM_THREADS;
+            As_re[b][e % GEMM_BK][e / GEMM_BK] = ra_re[q];
+            if (AC) As_im[b][e % GEMM_BK][e / GEMM_BK] = ra_im[q];
         }
 #pragma unroll
         for (int q = 0; q < BE; ++q) {
-            Bs_re[b][lb_k][lb_n + q] = rb_re[q];
-            if (BC) Bs_im[b][lb_k][lb_n + q] = rb_im[q];
+            const int e = tid + q * GEMM_THREADS;
+            Bs_re[b][e / BN][e % BN] = rb_re[q];
+            if (BC) Bs_im[b][e / BN][e % BN] = rb_im[q];
         }
     };
 
@@ -160,14 +157,21 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
 
 #pragma unroll
         for (int kg = 0; kg < GEMM_BK; kg += 16) {
-            V4 part_re[MI][NJ], part_im[MI][NJ];
+            // complex x complex uses three real products (Gauss):
+            //   P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 - P2
+            constexpr int PB = BLOCKED ? MI * NJ : 1;
+            V4 part_re[BLOCKED ? MI : 1][BLOCKED ? NJ : 1], part_im[BLOCKED ? MI : 1][BLOCKED ? NJ : 1];
+            V4 part_p2[(BLOCKED && AC && BC) ? PB : 1];
+            if constexpr (BLOCKED) {
 #pragma unroll
-            for (int i = 0; i < MI; ++i)
+                for (int i = 0; i < MI; ++i)
 #pragma unroll
-                for (int j = 0; j < NJ; ++j) {
-                    part_re[i][j] = V4{0, 0, 0, 0};
-                    part_im[i][j] = V4{0, 0, 0, 0};
-                }
+                    for (int j = 0; j < NJ; ++j) {
+                        part_re[i][j] = V4{0, 0, 0, 0};
+                        part_im[i][j] = V4{0, 0, 0, 0};
+                        if (AC && BC) part_p2[i * NJ + j] = V4{0, 0, 0, 0};
+                    }
+            }
 #pragma unroll
             for (int ks = 0; ks < 4; ++ks) {
                 const int kk = kg + ks * 4 + lk;
@@ -186,33 +190,51 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
                 for (int i = 0; i < MI; ++i)
 #pragma unroll
                     for (int j = 0; j < NJ; ++j) {
+                        V4& cr = BLOCKED ? part_re[BLOCKED ? i : 0][BLOCKED ? j : 0] : acc_re[i][j];
+                        V4& ci = BLOCKED ? part_im[BLOCKED ? i : 0][BLOCKED ? j : 0] : acc_im[i][j];
                         if (AC && BC) {
-                            part_re[i][j] = mfma16(ar[i], br[j], part_re[i][j]);
-                            part_re[i][j] = mfma16(-ai[i], bi[j], part_re[i][j]);
-                            part_im[i][j] = mfma16(ar[i], bi[j], part_im[i][j]);
-                            part_im[i][j] = mfma16(ai[i], br[j], part_im[i][j]);
+                            V4& c2 = BLOCKED ? part_p2[BLOCKED ? i * NJ + j : 0] : acc_p2[(AC && BC) ? i * NJ + j : 0];
+                            cr = mfma16(ar[i], br[j], cr);
+                            c2 = mfma16(ai[i], bi[j], c2);
+                            ci = mfma16(ar[i] + ai[i], br[j] + bi[j], ci);
                         } else if (AC) {
-                            part_re[i][j] = mfma16(ar[i], br[j], part_re[i][j]);
-                            part_im[i][j] = mfma16(ai[i], br[j], part_im[i][j]);
+                            cr = mfma16(ar[i], br[j], cr);
+                            ci = mfma16(ai[i], br[j], ci);
                         } else if (BC) {
-                            part_re[i][j] = mfma16(ar[i], br[j], part_re[i][j]);
-                            part_im[i][j] = mfma16(ar[i], bi[j], part_im[i][j]);
+                            cr = mfma16(ar[i], br[j], cr);
+                            ci = mfma16(ar[i], bi[j], ci);
                         } else {
-                            part_re[i][j] = mfma16(ar[i], br[j], part_re[i][j]);
+                            cr = mfma16(ar[i], br[j], cr);
                         }
                     }
             }
+            if constexpr (BLOCKED) {
 #pragma unroll
-            for (int i = 0; i < MI; ++i)
+                for (int i = 0; i < MI; ++i)
 #pragma unroll
-                for (int j = 0; j < NJ; ++j) {
-                    acc_re[i][j] += part_re[i][j];
-                    if (CPLX) acc_im[i][j] += part_im[i][j];
-                }
+                    for (int j = 0; j < NJ; ++j) {
+                        if (AC && BC) {
+                            acc_re[i][j] += part_re[i][j] - part_p2[i * NJ + j];
+                            acc_im[i][j] += part_im[i][j] - part_re[i][j] - part_p2[i * NJ + j];
+                        } else {
+                            acc_re[i][j] += part_re[i][j];
+                            if (CPLX) acc_im[i][j] += part_im[i][j];
+                        }
+                    }
+            }
         }
         if (more) store_slab(buf ^ 1);
         __syncthreads();
         buf ^= 1;
+    }
+    if constexpr (!BLOCKED && AC && BC) {  // Gauss recombination of the three products
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                acc_im[i][j] = acc_im[i][j] - acc_re[i][j] - acc_p2[i * NJ + j];
+                acc_re[i][j] = acc_re[i][j] - acc_p2[i * NJ + j];
+            }
     }
 
     // ---- epilogue
@@ -246,17 +268,41 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
             }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (linear id mod 8), each with its
+// own L2.  xcd_map() re-labels the linear id so that all tiles of one item run on one
+// XCD (items x, x+8, x+16, ... on XCD x): an item's operands then come from HBM into one
+// L2 instead of up to eight.  Items beyond the last multiple of 8 keep the plain order.
+// The map is a bijection of [0, tiles*nitems) onto (tile, item) pairs.
+constexpr int N_XCD = 8;
+
+__device__ __forceinline__ void xcd_map(int tiles, int nitems, int& tile, int& item) {
+    const int lin = blockIdx.x + blockIdx.y * gridDim.x;
+    const int nfull = (nitems / N_XCD) * N_XCD;
+    if (lin < nfull * tiles) {
+        const int x = lin % N_XCD, slot = lin / N_XCD;
+        item = (slot / tiles) * N_XCD + x;
+        tile = slot % tiles;
+    } else {
+        const int r = lin - nfull * tiles;
+        item = nfull + r / tiles;
+        tile = r % tiles;
+    }
+}
+
 // grid: x = tile index over (ceil(m/BM) x ceil(n/BN)) of the largest item, y = item.
 template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int GEMM_BK>
 __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* __restrict__ items,
                                                             int tiles_n_max) {
-    const GemmItem<T> it = items[blockIdx.y];
-    gemm_tile<T, AC, BC, PRE, POST, BM, BN, GEMM_BK>(it, blockIdx.x, tiles_n_max);
+    int tile, item;
+    xcd_map(gridDim.x, gridDim.y, tile, item);
+    const GemmItem<T> it = items[item];
+    gemm_tile<T, AC, BC, PRE, POST, BM, BN, GEMM_BK>(it, tile, tiles_n_max);
 }
 
 // Tile shapes the launcher can pick (the plan's autotuner times them per step):
 //   0: 64x64, K slab 16   1: 32x32, K slab 32   2: 32x32, K slab 64   3: 64x64, K slab 32
-enum : int { N_TILES = 4 };
+//   4: 96x96, K slab 32   5: 96x96, K slab 16   (96 = 288 / 3: no partial tiles at Nx = 288)
+enum : int { N_TILES = 6 };
 
 template <typename T, bool AC, bool BC, int BM, int BN, int BK>
 constexpr size_t gemm_lds_bytes() {
@@ -285,6 +331,8 @@ inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int m
     case 1: return launch_gemm_tile<T, AC, BC, PRE, POST, 32, 32, 32>(d_items, nitems, max_m, max_n, s);
     case 2: return launch_gemm_tile<T, AC, BC, PRE, POST, 32, 32, 64>(d_items, nitems, max_m, max_n, s);
     case 3: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 32>(d_items, nitems, max_m, max_n, s);
+    case 4: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 96, 32>(d_items, nitems, max_m, max_n, s);
+    case 5: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 96, 16>(d_items, nitems, max_m, max_n, s);
     default: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 16>(d_items, nitems, max_m, max_n, s);
     }
 }

@@ -1,68 +1,67 @@
-// Batched complex FP64 LU with partial pivoting (one workgroup per matrix), its
-// log-determinant epilogue, and a right-hand-side-parallel triangular solve.
+// Batched complex FP64 LU with partial pivoting, its log-determinant, and a
+// right-hand-side-parallel triangular solve.
 //
 // Pivoting follows LAPACK zgetrf: the pivot of column k is the first row of maximal
-// |re| + |im| (izamax), full rows are swapped, and piv[k] records the swapped row.
+// |re| + |im| (izamax), full rows are swapped, and piv[k] records the swapped row, so
+// the factors are in LAPACK's layout (L unit lower with later interchanges applied).
 // The log-determinant is sum_i log(U_ii) + i*pi*(number of k with piv[k] != k),
 // with the imaginary part reduced by the IEEE remainder to [-pi, pi]
 // (src/isle/cpp/math.hpp:265-293).
+//
+// The factorisation is right-looking in panels of LU_NB columns, two launches per panel:
+//   lu_panel_kernel  (one workgroup per matrix, one matrix row per thread, held in
+//                     registers): pivoted factorisation of the panel, L11^{-1} by
+//                     recursive doubling, the panel's interchanges on the columns right
+//                     of the panel (the new A12 rows are staged in the work buffer, the
+//                     displaced rows written in place), and on the item's last panel
+//                     the log-determinant;
+//   lu_update_kernel (32x32 tiles, every CU): U12 = L11^{-1} A12 and
+//                     A22 -= L21 U12 on FP64 MFMA, plus the panel's interchanges on the
+//                     columns left of it.
 #pragma once
 
 #include "common.hpp"
 
 namespace isle {
 
+constexpr int LU_NB = 32;          // panel width
+constexpr int LU_MAX_N = 512;      // one panel row per thread
+constexpr int LU_UPD_THREADS = 256;
+constexpr int LU_TILE = 32;        // trailing-update tile
+constexpr int LU_SWAP_COLS = 64;   // columns per left-interchange workgroup
+
 struct LuItem {
     double* re;      // n x n, row-major, row stride ld (factorised in place)
     double* im;
     int* piv;        // length n (0-based row index swapped with row k)
     double* logdet;  // 2 doubles (re, im) or null
+    double* work;    // lu_work_doubles(n) doubles of scratch
     int n;
     int ld;
 };
 
-constexpr int LU_THREADS = 512;
+// Work layout: for every panel p, L11^{-1} and U11^{-1} of that panel ([NB][NB] re and im
+// planes each; a partial last panel is padded with the identity), then the staged A12
+// rows [NB][n] re and im planes, then the panel's composed interchange as ints
+// {count, dst[2NB], src[2NB]}.  The diagonal-block inverses stay valid after the
+// factorisation and are used by the triangular solve.
+constexpr int LU_DINV = 4 * LU_NB * LU_NB;  // doubles per panel: L11^{-1}, U11^{-1} (re, im)
+__host__ __device__ inline int lu_panels(int n) { return (n + LU_NB - 1) / LU_NB; }
+__host__ __device__ inline size_t lu_work_doubles(int n) {
+    return (size_t)LU_DINV * lu_panels(n) + 2 * (size_t)LU_NB * n + 2 * LU_NB + 2;
+}
+__device__ inline double* lu_linv(const LuItem& it, int panel) { return it.work + (size_t)LU_DINV * panel; }
+__device__ inline double* lu_uinv(const LuItem& it, int panel) {
+    return it.work + (size_t)LU_DINV * panel + 2 * LU_NB * LU_NB;
+}
+__device__ inline double* lu_stage(const LuItem& it) { return it.work + (size_t)LU_DINV * lu_panels(it.n); }
+__device__ inline int* lu_perm(const LuItem& it) {
+    return reinterpret_cast<int*>(lu_stage(it) + 2 * (size_t)LU_NB * it.n);
+}
 
 __device__ __forceinline__ double cabs1(double r, double i) { return fabs(r) + fabs(i); }
 
-// Block-wide argmax (first index on ties).  scratch: 2 * (LU_THREADS/64) entries.
-__device__ inline int block_argmax(double v, int idx, double* sval, int* sidx) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(v, off);
-        const int oi = __shfl_xor(idx, off);
-        if (ov > v || (ov == v && oi < idx)) {
-            v = ov;
-            idx = oi;
-        }
-    }
-    if (lane == 0) {
-        sval[wave] = v;
-        sidx[wave] = idx;
-    }
-    __syncthreads();
-    if (wave == 0) {
-        const int nw = blockDim.x >> 6;
-        v = lane < nw ? sval[lane] : -1.0;
-        idx = lane < nw ? sidx[lane] : 0x7fffffff;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(v, off);
-            const int oi = __shfl_xor(idx, off);
-            if (ov > v || (ov == v && oi < idx)) {
-                v = ov;
-                idx = oi;
-            }
-        }
-        if (lane == 0) sidx[0] = idx;
-    }
-    __syncthreads();
-    const int r = sidx[0];
-    __syncthreads();
-    return r;
-}
-
+// Block-wide sum over all threads (any multiple of 64 up to 1024).  s: >= 16 doubles.
 __device__ inline double block_sum(double v, double* s) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -70,261 +69,428 @@ __device__ inline double block_sum(double v, double* s) {
     if (lane == 0) s[wave] = v;
     __syncthreads();
     double t = 0;
-    if (threadIdx.x == 0) {
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s[w];
-        s[0] = t;
-    }
-    __syncthreads();
-    t = s[0];
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s[w];
     __syncthreads();
     return t;
 }
 
-// Blocked right-looking LU, one workgroup per matrix.  Per panel of NB columns:
-//   1. the (n-k0) x NB panel is factorised with partial pivoting entirely in LDS;
-//   2. its row interchanges are applied to all other columns (one column per thread);
-//   3. U12 = L11^{-1} A12 by forward substitution (one column per thread), kept in LDS;
-//   4. A22 -= L21 U12 with 4x4 complex register tiles fed from LDS.
-// Dynamic LDS: panel [n][NB] + U12 [NB][n], complex (split planes).
+// Panel LDS used by the column steps.
 template <int NB>
-__global__ __launch_bounds__(LU_THREADS) void lu_kernel(const LuItem* __restrict__ items, int add_identity) {
-    extern __shared__ double lds[];
-    __shared__ double sval[LU_THREADS / 64];
-    __shared__ int sidx[LU_THREADS / 64];
-    __shared__ int perm_pos[2 * NB], perm_src[2 * NB], perm_cnt;
+struct PanelShared {
+    unsigned long long* pkey;  // [NB] pivot keys, one per column
+    double (*xrow)[2 * NB];
+    int* xorig;
+};
+
+// Pivot key: the bits of |re|+|im| (a non-negative double, so its bit pattern orders like
+// its value) with the low 9 mantissa bits replaced by 511 - row.  The maximal key is the
+// largest candidate, ties (and values equal to 2^-43 relative) going to the first row,
+// as izamax.  Rows must be < 512.
+__device__ __forceinline__ unsigned long long pivot_key(double v, int row) {
+    return (static_cast<unsigned long long>(__double_as_longlong(v)) & ~0x1FFull) |
+           static_cast<unsigned long long>(511 - row);
+}
+
+// Column KK of the panel factorisation (then KK+1, ...): pivot search, row exchange
+// through LDS, scaling and rank-1 update of this thread's row.  Templated on the
+// column so every register index is static.
+template <int NB, int KK>
+__device__ __forceinline__ void panel_column(const PanelShared<NB>& sh, double (&ar)[NB], double (&ai)[NB], int& orig,
+                                             int kb, bool own, int tid, int* piv, int k0) {
+    if (KK < kb) {
+        if (own && tid >= KK) atomicMax(&sh.pkey[KK], pivot_key(cabs1(ar[KK], ai[KK]), tid));
+        __syncthreads();
+        const int p = 511 - static_cast<int>(sh.pkey[KK] & 0x1FFull);
+        if (tid == p) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                sh.xrow[0][2 * c] = ar[c];
+                sh.xrow[0][2 * c + 1] = ai[c];
+            }
+            sh.xorig[0] = orig;
+        }
+        if (tid == KK && p != KK) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                sh.xrow[1][2 * c] = ar[c];
+                sh.xrow[1][2 * c + 1] = ai[c];
+            }
+            sh.xorig[1] = orig;
+        }
+        __syncthreads();
+        if (p != KK && (tid == p || tid == KK)) {
+            const int src = tid == p ? 1 : 0;
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                ar[c] = sh.xrow[src][2 * c];
+                ai[c] = sh.xrow[src][2 * c + 1];
+            }
+            orig = sh.xorig[src];
+        }
+        if (tid == 0) piv[KK] = k0 + p;
+        if (own && tid > KK) {
+            const double ur = sh.xrow[0][2 * KK], ui = sh.xrow[0][2 * KK + 1];
+            double lr = ar[KK], li = ai[KK];
+            if (ur != 0.0 || ui != 0.0) {
+                const double den = ur * ur + ui * ui;
+                const double ir = ur / den, ii = -ui / den;
+                const double t = lr * ir - li * ii;
+                li = lr * ii + li * ir;
+                lr = t;
+            }
+            ar[KK] = lr;
+            ai[KK] = li;
+#pragma unroll
+            for (int c = KK + 1; c < NB; ++c) {
+                const double br = sh.xrow[0][2 * c], bi = sh.xrow[0][2 * c + 1];
+                ar[c] -= lr * br - li * bi;
+                ai[c] -= lr * bi + li * br;
+            }
+        }
+    }
+    if constexpr (KK + 1 < NB) panel_column<NB, KK + 1>(sh, ar, ai, orig, kb, own, tid, piv, k0);
+}
+
+template <int NB>
+__global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __restrict__ items, int k0,
+                                                            int add_identity) {
+    __shared__ unsigned long long pkey[NB];
+    __shared__ double xrow[2][2 * NB];  // [0]: the pivot row, [1]: the displaced row kk (re, im interleaved)
+    __shared__ int xorig[2];
+    __shared__ double lt_re[NB][NB + 1], lt_im[NB][NB + 1];  // L11, inverted in place
+    __shared__ double ut_re[NB][NB + 1], ut_im[NB][NB + 1];  // U11, inverted in place
+    __shared__ double tl_re[NB][NB + 1], tl_im[NB][NB + 1];  // doubling temporaries
+    __shared__ double tu_re[NB][NB + 1], tu_im[NB][NB + 1];
+    __shared__ int orig_of[NB];
+    __shared__ int slot_cnt, slot_dst[2 * NB], slot_src[2 * NB];
+    __shared__ double rsum[LU_MAX_N / 64];
+
     const LuItem it = items[blockIdx.x];
-    const int n = it.n, ld = it.ld, tid = threadIdx.x, nth = blockDim.x;
+    const int n = it.n;
+    if (k0 >= n) return;
+    const int ld = it.ld, tid = threadIdx.x, nth = blockDim.x;
+    const int rows = n - k0, kb = min(NB, rows);
+    const int panel = k0 / NB;
     double* re = it.re;
     double* im = it.im;
-    double* p_re = lds;                     // panel [n][NB]
-    double* p_im = lds + (size_t)n * NB;
-    double* u_re = lds + 2 * (size_t)n * NB;  // U12 [NB][n]
-    double* u_im = lds + 3 * (size_t)n * NB;
-
+    const bool own = tid < rows;
+    if (tid == 0) slot_cnt = 0;
+    if (tid < NB) pkey[tid] = 0ull;
     if (add_identity) {
-        for (int i = tid; i < n; i += nth) re[(long)i * ld + i] += 1.0;
-        __syncthreads();
+        for (int i = NB + tid; i < n; i += nth) re[(long)i * ld + i] += 1.0;
     }
 
-    for (int k0 = 0; k0 < n; k0 += NB) {
-        const int kb = min(NB, n - k0);
-        const int pr_rows = n - k0;
-        // ---- 1. load and factorise the panel in LDS
-        for (int e = tid; e < pr_rows * NB; e += nth) {
-            const int r = e / NB, c = e % NB;
-            double vr = 0, vi = 0;
+    double ar[NB], ai[NB];
+    int orig = tid;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        double vr = 0, vi = 0;
+        if (own && c < kb) {
+            vr = re[(long)(k0 + tid) * ld + k0 + c];
+            vi = im[(long)(k0 + tid) * ld + k0 + c];
+            if (add_identity && c == tid) vr += 1.0;
+        }
+        ar[c] = vr;
+        ai[c] = vi;
+    }
+
+    __syncthreads();
+    PanelShared<NB> sh{pkey, xrow, xorig};
+    panel_column<NB, 0>(sh, ar, ai, orig, kb, own, tid, it.piv + k0, k0);
+    if (own) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
             if (c < kb) {
-                vr = re[(long)(k0 + r) * ld + k0 + c];
-                vi = im[(long)(k0 + r) * ld + k0 + c];
+                re[(long)(k0 + tid) * ld + k0 + c] = ar[c];
+                im[(long)(k0 + tid) * ld + k0 + c] = ai[c];
             }
-            p_re[r * NB + c] = vr;
-            p_im[r * NB + c] = vi;
+        }
+        if (orig != tid) {  // at most 2*NB positions change: the panel rows and their partners
+            const int sl = atomicAdd(&slot_cnt, 1);
+            if (sl < 2 * NB) {
+                slot_dst[sl] = tid;
+                slot_src[sl] = orig;
+            }
+        }
+    }
+    if (tid < NB) {
+        orig_of[tid] = orig;
+        const bool in = tid < kb;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            lt_re[tid][c] = (in && c < tid) ? ar[c] : (c == tid ? 1.0 : 0.0);
+            lt_im[tid][c] = (in && c < tid) ? ai[c] : 0.0;
+            const bool u = in && c >= tid && c < kb;
+            double vr = u ? ar[c] : (c == tid ? 1.0 : 0.0), vi = u ? ai[c] : 0.0;
+            if (c == tid) {  // the diagonal of U11^{-1} is 1/u_ii
+                const double den = vr * vr + vi * vi;
+                vr = vr / den;
+                vi = -vi / den;
+            }
+            ut_re[tid][c] = vr;
+            ut_im[tid][c] = vi;
+        }
+    }
+    __syncthreads();
+    const int cnt = min(slot_cnt, 2 * NB);
+    int* perm = lu_perm(it);
+    for (int e = tid; e < cnt; e += nth) {
+        perm[1 + e] = slot_dst[e];
+        perm[1 + 2 * NB + e] = slot_src[e];
+    }
+    if (tid == 0) perm[0] = cnt;
+
+    // L11^{-1} and U11^{-1} by recursive doubling:
+    //   [A 0; C D]^{-1} = [A^-1 0; -D^-1 C A^-1  D^-1],  [A B; 0 D]^{-1} = [A^-1 -A^-1 B D^-1; 0 D^-1]
+    for (int b = 1; b < NB; b *= 2) {
+        const int work = (NB / (2 * b)) * b * b;
+        for (int e = tid; e < work; e += nth) {
+            const int blk = e / (b * b), i = (e / b) % b, j = e % b, o = blk * 2 * b;
+            double tr = 0, ti = 0, sr = 0, si = 0;
+            for (int q = j; q < b; ++q) {  // C A^-1
+                const double cr = lt_re[o + b + i][o + q], ci = lt_im[o + b + i][o + q];
+                const double xr = lt_re[o + q][o + j], xi = lt_im[o + q][o + j];
+                tr += cr * xr - ci * xi;
+                ti += cr * xi + ci * xr;
+            }
+            for (int q = 0; q <= j; ++q) {  // B D^-1
+                const double br = ut_re[o + i][o + b + q], bi = ut_im[o + i][o + b + q];
+                const double xr = ut_re[o + b + q][o + b + j], xi = ut_im[o + b + q][o + b + j];
+                sr += br * xr - bi * xi;
+                si += br * xi + bi * xr;
+            }
+            tl_re[o + b + i][o + j] = tr;
+            tl_im[o + b + i][o + j] = ti;
+            tu_re[o + i][o + b + j] = sr;
+            tu_im[o + i][o + b + j] = si;
         }
         __syncthreads();
-        for (int kk = 0; kk < kb; ++kk) {
-            double best = -1.0;
-            int bi = 0x7fffffff;
-            for (int r = kk + tid; r < pr_rows; r += nth) {
-                const double v = cabs1(p_re[r * NB + kk], p_im[r * NB + kk]);
-                if (v > best) {
-                    best = v;
-                    bi = r;
-                }
+        for (int e = tid; e < work; e += nth) {
+            const int blk = e / (b * b), i = (e / b) % b, j = e % b, o = blk * 2 * b;
+            double xr_ = 0, xi_ = 0, yr = 0, yi = 0;
+            for (int q = 0; q <= i; ++q) {  // D^-1 (C A^-1)
+                const double dr = lt_re[o + b + i][o + b + q], di = lt_im[o + b + i][o + b + q];
+                const double tr = tl_re[o + b + q][o + j], ti = tl_im[o + b + q][o + j];
+                xr_ += dr * tr - di * ti;
+                xi_ += dr * ti + di * tr;
             }
-            const int p = block_argmax(best, bi, sval, sidx);
-            if (tid == 0) it.piv[k0 + kk] = k0 + p;
-            if (p != kk && tid < NB) {
-                const double tr = p_re[kk * NB + tid], ti = p_im[kk * NB + tid];
-                p_re[kk * NB + tid] = p_re[p * NB + tid];
-                p_im[kk * NB + tid] = p_im[p * NB + tid];
-                p_re[p * NB + tid] = tr;
-                p_im[p * NB + tid] = ti;
+            for (int q = i; q < b; ++q) {  // A^-1 (B D^-1)
+                const double ar_ = ut_re[o + i][o + q], ai_ = ut_im[o + i][o + q];
+                const double tr = tu_re[o + q][o + b + j], ti = tu_im[o + q][o + b + j];
+                yr += ar_ * tr - ai_ * ti;
+                yi += ar_ * ti + ai_ * tr;
             }
-            __syncthreads();
-            const double dr = p_re[kk * NB + kk], di = p_im[kk * NB + kk];
-            const double den = dr * dr + di * di;
-            const double ir = dr / den, ii = -di / den;
-            const int rows = pr_rows - (kk + 1);
-            const int cols = kb - (kk + 1);
-            for (int r = kk + 1 + tid; r < pr_rows; r += nth) {
-                const double xr = p_re[r * NB + kk], xi = p_im[r * NB + kk];
-                p_re[r * NB + kk] = xr * ir - xi * ii;
-                p_im[r * NB + kk] = xr * ii + xi * ir;
-            }
-            __syncthreads();
-            if (cols > 0) {
-                for (int e = tid; e < rows * cols; e += nth) {
-                    const int r = kk + 1 + e / cols, c = kk + 1 + e % cols;
-                    const double lr_ = p_re[r * NB + kk], li = p_im[r * NB + kk];
-                    const double ur = p_re[kk * NB + c], ui = p_im[kk * NB + c];
-                    p_re[r * NB + c] -= lr_ * ur - li * ui;
-                    p_im[r * NB + c] -= lr_ * ui + li * ur;
-                }
-            }
-            __syncthreads();
-        }
-        for (int e = tid; e < pr_rows * kb; e += nth) {
-            const int r = e / kb, c = e % kb;
-            re[(long)(k0 + r) * ld + k0 + c] = p_re[r * NB + c];
-            im[(long)(k0 + r) * ld + k0 + c] = p_im[r * NB + c];
-        }
-        // ---- 2. row interchanges on the other columns: the panel's swaps are composed
-        //         into one permutation of at most 2*NB rows, applied as a gather/scatter
-        if (tid == 0) {
-            int cnt = 0;
-            for (int kk = 0; kk < kb; ++kk) {
-                const int a = k0 + kk, p = it.piv[k0 + kk];
-                int ia = -1, ip = -1;
-                for (int q = 0; q < cnt; ++q) {
-                    if (perm_pos[q] == a) ia = q;
-                    if (perm_pos[q] == p) ip = q;
-                }
-                if (ia < 0) {
-                    perm_pos[cnt] = a;
-                    perm_src[cnt] = a;
-                    ia = cnt++;
-                }
-                if (ip < 0) {
-                    perm_pos[cnt] = p;
-                    perm_src[cnt] = p;
-                    ip = cnt++;
-                }
-                const int t = perm_src[ia];
-                perm_src[ia] = perm_src[ip];
-                perm_src[ip] = t;
-            }
-            perm_cnt = cnt;
-        }
-        __syncthreads();
-        {
-            const int cnt = perm_cnt;
-            for (int c = tid; c < n; c += nth) {
-                if (c >= k0 && c < k0 + kb) continue;
-                double vr[2 * NB], vi[2 * NB];
-#pragma unroll
-                for (int q = 0; q < 2 * NB; ++q) {
-                    if (q < cnt) {
-                        vr[q] = re[(long)perm_src[q] * ld + c];
-                        vi[q] = im[(long)perm_src[q] * ld + c];
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < 2 * NB; ++q) {
-                    if (q < cnt) {
-                        re[(long)perm_pos[q] * ld + c] = vr[q];
-                        im[(long)perm_pos[q] * ld + c] = vi[q];
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        const int c0 = k0 + kb;
-        const int nt = n - c0;
-        if (nt <= 0) break;
-        // ---- 3. U12 = L11^{-1} A12, one column per thread, into global and LDS
-        for (int j = tid; j < nt; j += nth) {
-            double xr[NB], xi[NB];
-#pragma unroll
-            for (int r = 0; r < NB; ++r) {
-                if (r < kb) {
-                    double sr = re[(long)(k0 + r) * ld + c0 + j], si = im[(long)(k0 + r) * ld + c0 + j];
-#pragma unroll
-                    for (int q = 0; q < NB; ++q) {
-                        if (q < r) {
-                            const double lr_ = p_re[r * NB + q], li = p_im[r * NB + q];
-                            sr -= lr_ * xr[q] - li * xi[q];
-                            si -= lr_ * xi[q] + li * xr[q];
-                        }
-                    }
-                    xr[r] = sr;
-                    xi[r] = si;
-                    re[(long)(k0 + r) * ld + c0 + j] = sr;
-                    im[(long)(k0 + r) * ld + c0 + j] = si;
-                    u_re[r * n + j] = sr;
-                    u_im[r * n + j] = si;
-                } else {
-                    xr[r] = xi[r] = 0;
-                    u_re[r * n + j] = 0;
-                    u_im[r * n + j] = 0;
-                }
-            }
-        }
-        __syncthreads();
-        // ---- 4. A22 -= L21 U12 with 4x4 complex register tiles
-        const int tiles_r = (nt + 3) / 4, tiles_c = (nt + 3) / 4;
-        for (int tix = tid; tix < tiles_r * tiles_c; tix += nth) {
-            const int tr_ = tix / tiles_c, tc = tix % tiles_c;
-            double ar[4][4], ai[4][4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) ar[a][b] = ai[a][b] = 0;
-            int rows[4], cols[4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                rows[a] = min(tr_ + a * tiles_r, nt - 1);  // strided rows for coalescing
-                cols[a] = min(tc * 4 + a, nt - 1);
-            }
-#pragma unroll 2
-            for (int q = 0; q < NB; ++q) {
-                double lr_[4], li[4], ur[4], ui[4];
-#pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    lr_[a] = p_re[(kb + rows[a]) * NB + q];
-                    li[a] = p_im[(kb + rows[a]) * NB + q];
-                    ur[a] = u_re[q * n + cols[a]];
-                    ui[a] = u_im[q * n + cols[a]];
-                }
-#pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) {
-                        ar[a][b] += lr_[a] * ur[b] - li[a] * ui[b];
-                        ai[a][b] += lr_[a] * ui[b] + li[a] * ur[b];
-                    }
-            }
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const int r = tr_ + a * tiles_r;
-                if (r >= nt) continue;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int c = tc * 4 + b;
-                    if (c >= nt) continue;
-                    const long o = (long)(c0 + r) * ld + c0 + c;
-                    re[o] -= ar[a][b];
-                    im[o] -= ai[a][b];
-                }
-            }
+            lt_re[o + b + i][o + j] = -xr_;
+            lt_im[o + b + i][o + j] = -xi_;
+            ut_re[o + i][o + b + j] = -yr;
+            ut_im[o + i][o + b + j] = -yi;
         }
         __syncthreads();
     }
+    {
+        double* li_re = lu_linv(it, panel);
+        double* li_im = li_re + NB * NB;
+        double* ui_re = lu_uinv(it, panel);
+        double* ui_im = ui_re + NB * NB;
+        for (int e = tid; e < NB * NB; e += nth) {
+            li_re[e] = lt_re[e / NB][e % NB];
+            li_im[e] = lt_im[e / NB][e % NB];
+            ui_re[e] = ut_re[e / NB][e % NB];
+            ui_im[e] = ut_im[e / NB][e % NB];
+        }
+    }
 
-    if (it.logdet) {
-        double lr_ = 0, la = 0, cnt = 0;
+    const int nrt = n - k0 - NB;  // columns right of the panel
+    if (nrt > 0) {
+        // staged A12 = the new rows k0..k0+NB-1 of the columns right of the panel
+        double* st_re = lu_stage(it);
+        double* st_im = st_re + (size_t)NB * n;
+        for (int e = tid; e < NB * nrt; e += nth) {
+            const int q = e / nrt, c = e - q * nrt;
+            const long src = (long)(k0 + orig_of[q]) * ld + k0 + NB + c;
+            st_re[(long)q * n + c] = re[src];
+            st_im[(long)q * n + c] = im[src];
+        }
+        __syncthreads();  // every staged read precedes the displaced-row writes below
+        // displaced rows below the panel take their new content (always an original panel
+        // row, which this kernel does not write right of the panel)
+        for (int e = tid; e < cnt * nrt; e += nth) {
+            const int q = e / nrt, c = e - q * nrt;
+            const int dst = slot_dst[q];
+            if (dst < NB) continue;
+            const long so = (long)(k0 + slot_src[q]) * ld + k0 + NB + c;
+            const long d = (long)(k0 + dst) * ld + k0 + NB + c;
+            re[d] = re[so];
+            im[d] = im[so];
+        }
+    } else if (it.logdet) {
+        // the item's last panel: every U_ii and pivot is final
+        double lr_ = 0, la = 0, pc = 0;
         for (int i = tid; i < n; i += nth) {
             const double dr = re[(long)i * ld + i], di = im[(long)i * ld + i];
             lr_ += log(hypot(dr, di));
             la += atan2(di, dr);
-            cnt += (it.piv[i] != i) ? 1.0 : 0.0;
+            pc += (it.piv[i] != i) ? 1.0 : 0.0;
         }
-        lr_ = block_sum(lr_, sval);
-        la = block_sum(la, sval);
-        cnt = block_sum(cnt, sval);
+        lr_ = block_sum(lr_, rsum);
+        la = block_sum(la, rsum);
+        pc = block_sum(pc, rsum);
         if (tid == 0) {
-            if (((long)cnt) & 1) la += M_PI;
+            if (((long)pc) & 1) la += M_PI;
             it.logdet[0] = lr_;
             it.logdet[1] = remainder(la, 2.0 * M_PI);
         }
     }
 }
 
+// grid.x: tm*tm trailing tiles (tm = tiles per dimension of the largest item's A22), then
+// swap_wgs workgroups of LU_SWAP_COLS left columns each; grid.y: item.
+__global__ __launch_bounds__(LU_UPD_THREADS) void lu_update_kernel(const LuItem* __restrict__ items, int k0,
+                                                                    int tm) {
+    constexpr int T = LU_TILE, NB = LU_NB;
+    static_assert(T == 32 && NB == 32, "tile code assumes 32");
+    const LuItem it = items[blockIdx.y];
+    const int n = it.n, ld = it.ld, tid = threadIdx.x;
+    if (k0 >= n) return;
+    double* re = it.re;
+    double* im = it.im;
+    if ((int)blockIdx.x >= tm * tm) {
+        // ---- interchanges of this panel on LU_SWAP_COLS columns left of it
+        const int c0 = ((int)blockIdx.x - tm * tm) * LU_SWAP_COLS;
+        if (c0 >= k0) return;
+        const int* perm = lu_perm(it);
+        const int cnt = perm[0];
+        constexpr int PER = 2 * NB * LU_SWAP_COLS / LU_UPD_THREADS;
+        double vr[PER], vi[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = tid + j * LU_UPD_THREADS, q = e / LU_SWAP_COLS, c = c0 + e % LU_SWAP_COLS;
+            if (q < cnt && c < k0) {
+                const long o = (long)(k0 + perm[1 + 2 * NB + q]) * ld + c;
+                vr[j] = re[o];
+                vi[j] = im[o];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = tid + j * LU_UPD_THREADS, q = e / LU_SWAP_COLS, c = c0 + e % LU_SWAP_COLS;
+            if (q < cnt && c < k0) {
+                const long o = (long)(k0 + perm[1 + q]) * ld + c;
+                re[o] = vr[j];
+                im[o] = vi[j];
+            }
+        }
+        return;
+    }
+    // ---- trailing tile (I, J) of A22 = rows/cols c0.. of the matrix
+    const int c0 = k0 + NB, nt = n - c0;
+    const int I = blockIdx.x / tm, J = blockIdx.x % tm;
+    if (I * T >= nt || J * T >= nt) return;
+    __shared__ double sa_re[T][T + 1], sa_im[T][T + 1];  // L11^{-1}
+    __shared__ double sl_re[T][T + 1], sl_im[T][T + 1];  // L21 tile
+    __shared__ double sb_re[T][T + 1], sb_im[T][T + 1];  // staged A12 tile, then U12 tile
+    const double* li_re = lu_linv(it, k0 / NB);
+    const double* li_im = li_re + NB * NB;
+    const double* st_re = lu_stage(it);
+    const double* st_im = st_re + (size_t)NB * n;
+    for (int e = tid; e < T * T; e += LU_UPD_THREADS) {
+        const int r = e / T, c = e % T;
+        sa_re[r][c] = li_re[e];
+        sa_im[r][c] = li_im[e];
+        const int col = J * T + c;
+        sb_re[r][c] = col < nt ? st_re[(long)r * n + col] : 0.0;
+        sb_im[r][c] = col < nt ? st_im[(long)r * n + col] : 0.0;
+        const int row = I * T + r;
+        sl_re[r][c] = row < nt ? re[(long)(c0 + row) * ld + k0 + c] : 0.0;
+        sl_im[r][c] = row < nt ? im[(long)(c0 + row) * ld + k0 + c] : 0.0;
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
+    const int li = lane & 15, lk = lane >> 4;
+    // prefetch this lane's C entries: rows wm + lk + 4r, column wn + li
+    double cr[4], ci[4];
+    const int ccol = J * T + wn + li;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int crow = I * T + wm + lk + 4 * r;
+        cr[r] = ci[r] = 0.0;
+        if (crow < nt && ccol < nt) {
+            const long o = (long)(c0 + crow) * ld + c0 + ccol;
+            cr[r] = re[o];
+            ci[r] = im[o];
+        }
+    }
+    __syncthreads();
+    // U12 tile = L11^{-1} A12 tile
+    dbl4 rr = {0, 0, 0, 0}, ii = rr, ri = rr, ir = rr;
+#pragma unroll
+    for (int ks = 0; ks < NB; ks += 4) {
+        const double a_r = sa_re[wm + li][ks + lk], a_i = sa_im[wm + li][ks + lk];
+        const double b_r = sb_re[ks + lk][wn + li], b_i = sb_im[ks + lk][wn + li];
+        rr = mfma16(a_r, b_r, rr);
+        ii = mfma16(a_i, b_i, ii);
+        ri = mfma16(a_r, b_i, ri);
+        ir = mfma16(a_i, b_r, ir);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        sb_re[wm + lk + 4 * r][wn + li] = rr[r] - ii[r];
+        sb_im[wm + lk + 4 * r][wn + li] = ri[r] + ir[r];
+    }
+    __syncthreads();
+    if (I == 0) {  // U12 into rows k0.. of the matrix (read by no other workgroup of this launch)
+        for (int e = tid; e < T * T; e += LU_UPD_THREADS) {
+            const int r = e / T, c = e % T, col = J * T + c;
+            if (col < nt) {
+                re[(long)(k0 + r) * ld + c0 + col] = sb_re[r][c];
+                im[(long)(k0 + r) * ld + c0 + col] = sb_im[r][c];
+            }
+        }
+    }
+    // A22 tile -= L21 tile * U12 tile
+    rr = dbl4{0, 0, 0, 0};
+    ii = rr;
+    ri = rr;
+    ir = rr;
+#pragma unroll
+    for (int ks = 0; ks < NB; ks += 4) {
+        const double a_r = sl_re[wm + li][ks + lk], a_i = sl_im[wm + li][ks + lk];
+        const double b_r = sb_re[ks + lk][wn + li], b_i = sb_im[ks + lk][wn + li];
+        rr = mfma16(a_r, b_r, rr);
+        ii = mfma16(a_i, b_i, ii);
+        ri = mfma16(a_r, b_i, ri);
+        ir = mfma16(a_i, b_r, ir);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int crow = I * T + wm + lk + 4 * r;
+        if (crow < nt && ccol < nt) {
+            const long o = (long)(c0 + crow) * ld + c0 + ccol;
+            re[o] = cr[r] - (rr[r] - ii[r]);
+            im[o] = ci[r] - (ri[r] + ir[r]);
+        }
+    }
+}
+
+// Factorise nitems matrices (n <= max_n <= LU_MAX_N each); add_identity: factorise 1 + A.
 inline void launch_lu(const LuItem* d_items, int nitems, int max_n, int add_identity, hipStream_t s) {
-    const size_t lds = sizeof(double) * 4 * (size_t)max_n * 8;
-    if (lds > 150 * 1024) throw std::invalid_argument("matrix too large for the LU kernel");
-    ISLE_HIP(hipFuncSetAttribute((const void*)lu_kernel<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(lu_kernel<8>, dim3(nitems), dim3(LU_THREADS), lds, s, d_items, add_identity);
-    ISLE_HIP(hipGetLastError());
+    if (max_n > LU_MAX_N) throw std::invalid_argument("matrix too large for the LU kernel");
+    for (int k0 = 0; k0 < max_n; k0 += LU_NB) {
+        const int threads = ((max_n - k0 + 63) / 64) * 64;
+        hipLaunchKernelGGL(lu_panel_kernel<LU_NB>, dim3(nitems), dim3(threads), 0, s, d_items, k0,
+                           k0 == 0 ? add_identity : 0);
+        ISLE_HIP(hipGetLastError());
+        const int ntr = max_n - k0 - LU_NB;
+        const int tm = ntr > 0 ? (ntr + LU_TILE - 1) / LU_TILE : 0;
+        const int swap_wgs = (k0 + LU_SWAP_COLS - 1) / LU_SWAP_COLS;
+        if (tm * tm + swap_wgs > 0) {
+            hipLaunchKernelGGL(lu_update_kernel, dim3(tm * tm + swap_wgs, nitems), dim3(LU_UPD_THREADS), 0, s,
+                               d_items, k0, tm);
+            ISLE_HIP(hipGetLastError());
+        }
+    }
 }
 
 // X = A^{-1} B for the LU factors of A.  Element (row r, rhs c) of B is read from
@@ -335,6 +501,7 @@ struct SolveItem {
     const double* lu_re;
     const double* lu_im;
     const int* piv;
+    const double* dinv;  // the LU's work buffer (its per-panel L11^{-1}, U11^{-1})
     const double* b_re;
     const double* b_im;
     long bsr, bsc;
@@ -347,133 +514,125 @@ struct SolveItem {
 constexpr int SOLVE_COLS = 16;
 constexpr int SOLVE_THREADS = 256;
 
+// Dynamic LDS of lu_solve_kernel for matrices of order n.
+inline size_t lu_solve_lds(int n) {
+    const size_t np = (size_t)((n + 15) / 16) * 16;
+    return 2 * np * SOLVE_COLS * sizeof(double) + 2 * (size_t)n * sizeof(int);
+}
+
+// Blocked forward/back substitution on 16 right-hand sides per workgroup (X in LDS).
+// Per block of 16 rows: the product with the already solved rows runs on FP64 MFMA (the
+// K range split over the 4 waves, partial sums reduced through LDS), then the block is
+// multiplied by the inverse of its 16x16 diagonal block of L (or U), which is a diagonal
+// sub-block of the panel inverses the LU stored.
 __global__ __launch_bounds__(SOLVE_THREADS) void lu_solve_kernel(const SolveItem* __restrict__ items) {
     extern __shared__ double lds[];
+    __shared__ double part_r[4][16][17], part_i[4][16][17];
+    __shared__ double dg_r[16][17], dg_i[16][17];  // inverse of the diagonal block
+    __shared__ double y_r[16][17], y_i[16][17];    // right-hand side of the diagonal block
     const SolveItem it = items[blockIdx.y];
     const int n = it.n, ld = it.ld;
     const int c0 = blockIdx.x * SOLVE_COLS;
     if (c0 >= it.nrhs) return;
-    double* xr = lds;                         // [n][16]
-    double* xi = lds + (size_t)n * SOLVE_COLS;
-    const int tid = threadIdx.x;
-    const int c = tid & 15;   // rhs within block
-    const int r = tid >> 4;   // row lane within a 16-row block
+    const int nb = (n + 15) / 16, np = nb * 16;
+    double* xr = lds;  // [np][16]
+    double* xi = lds + (size_t)np * SOLVE_COLS;
+    int* sigma = reinterpret_cast<int*>(lds + 2 * (size_t)np * SOLVE_COLS);  // [n]
+    int* pv = sigma + n;                                                      // [n]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const double* lre = it.lu_re;
+    const double* lim = it.lu_im;
 
-    for (int e = tid; e < n * SOLVE_COLS; e += SOLVE_THREADS) {
-        const int row = e / SOLVE_COLS, cc = e % SOLVE_COLS;
-        const int gc = c0 + cc;
-        double vr = 0, vi = 0;
-        if (gc < it.nrhs) {
-            if (it.b_re) {
-                vr = it.b_re[gc * it.bsc + row * it.bsr];
-                vi = it.b_im[gc * it.bsc + row * it.bsr];
-            } else {
-                vr = (row == gc) ? 1.0 : 0.0;
-            }
-        }
-        xr[row * SOLVE_COLS + cc] = vr;
-        xi[row * SOLVE_COLS + cc] = vi;
-    }
+    // row permutation: original row r ends at position pos(r) after the swaps piv[0..n-1];
+    // sigma[pos(r)] = r, so (P B)[i] = B[sigma[i]]
+    for (int i = tid; i < n; i += SOLVE_THREADS) pv[i] = it.piv[i];
     __syncthreads();
-    if (tid < SOLVE_COLS) {  // row interchanges, applied in order
+    for (int r = tid; r < n; r += SOLVE_THREADS) {
+        int pos = r;
         for (int k = 0; k < n; ++k) {
-            const int p = it.piv[k];
-            if (p != k) {
-                const double tr = xr[k * SOLVE_COLS + tid], ti = xi[k * SOLVE_COLS + tid];
-                xr[k * SOLVE_COLS + tid] = xr[p * SOLVE_COLS + tid];
-                xi[k * SOLVE_COLS + tid] = xi[p * SOLVE_COLS + tid];
-                xr[p * SOLVE_COLS + tid] = tr;
-                xi[p * SOLVE_COLS + tid] = ti;
-            }
+            const int p = pv[k];
+            pos = pos == k ? p : (pos == p ? k : pos);
         }
+        sigma[pos] = r;
     }
     __syncthreads();
-    __shared__ double dr_[16][17], di_[16][17];  // diagonal 16x16 block of L or U
-    __shared__ double br_[2][SOLVE_COLS], bi_[2][SOLVE_COLS];
-    const int nb = (n + 15) / 16;
-    // forward substitution with unit lower L, blocks of 16 rows
-    for (int b = 0; b < nb; ++b) {
-        const int i = b * 16 + r;
-        double xr_ = 0, xi_ = 0;
-        if (i < n) {
-            xr_ = xr[i * SOLVE_COLS + c];
-            xi_ = xi[i * SOLVE_COLS + c];
-            const double* lr = it.lu_re + (long)i * ld;
-            const double* li = it.lu_im + (long)i * ld;
-            for (int k = 0; k < b * 16; ++k) {
-                const double ar = lr[k], ai = li[k];
-                const double br = xr[k * SOLVE_COLS + c], bi = xi[k * SOLVE_COLS + c];
-                xr_ -= ar * br - ai * bi;
-                xi_ -= ar * bi + ai * br;
-            }
-            const int kk = b * 16 + c;
-            dr_[r][c] = kk < n ? lr[kk] : 0.0;
-            di_[r][c] = kk < n ? li[kk] : 0.0;
-        }
-        __syncthreads();
-        for (int rr = 0; rr < 16 && b * 16 + rr < n; ++rr) {
-            if (r == rr) {
-                br_[rr & 1][c] = xr_;
-                bi_[rr & 1][c] = xi_;
-            }
-            __syncthreads();
-            if (r > rr && i < n) {
-                const double ar = dr_[r][rr], ai = di_[r][rr];
-                const double br = br_[rr & 1][c], bi = bi_[rr & 1][c];
-                xr_ -= ar * br - ai * bi;
-                xi_ -= ar * bi + ai * br;
+    for (int e = tid; e < np * SOLVE_COLS; e += SOLVE_THREADS) {
+        const int row = e / SOLVE_COLS, gc = c0 + e % SOLVE_COLS;
+        double vr = 0, vi = 0;
+        if (row < n && gc < it.nrhs) {
+            const int src = sigma[row];
+            if (it.b_re) {
+                vr = it.b_re[gc * it.bsc + src * it.bsr];
+                vi = it.b_im[gc * it.bsc + src * it.bsr];
+            } else {
+                vr = (src == gc) ? 1.0 : 0.0;
             }
         }
-        if (i < n) {
-            xr[i * SOLVE_COLS + c] = xr_;
-            xi[i * SOLVE_COLS + c] = xi_;
-        }
-        __syncthreads();
+        xr[e] = vr;
+        xi[e] = vi;
     }
-    // back substitution with U, blocks of 16 rows from the bottom
-    for (int b = nb - 1; b >= 0; --b) {
-        const int i = b * 16 + r;
-        const int kend = min(n, b * 16 + 16);
-        double xr_ = 0, xi_ = 0;
-        if (i < n) {
-            xr_ = xr[i * SOLVE_COLS + c];
-            xi_ = xi[i * SOLVE_COLS + c];
-            const double* ur = it.lu_re + (long)i * ld;
-            const double* ui = it.lu_im + (long)i * ld;
-            for (int k = kend; k < n; ++k) {
-                const double ar = ur[k], ai = ui[k];
-                const double br = xr[k * SOLVE_COLS + c], bi = xi[k * SOLVE_COLS + c];
-                xr_ -= ar * br - ai * bi;
-                xi_ -= ar * bi + ai * br;
+    __syncthreads();
+
+    const int ei = tid >> 4, ec = tid & 15;  // this thread's element of a 16x16 block
+    for (int sweep = 0; sweep < 2; ++sweep) {  // 0: L (unit lower), 1: U (upper)
+        for (int bi = 0; bi < nb; ++bi) {
+            const int b = sweep == 0 ? bi : nb - 1 - bi;
+            const int r0 = b * 16;
+            const int arow = r0 + li;
+            // ---- partial = A[r0..r0+15, K] X[K], K = [0, r0) (L) or [r0+16, n) (U)
+            dbl4 rr = {0, 0, 0, 0}, ii = rr, ri = rr, ir = rr;
+            const int kbeg = sweep == 0 ? 0 : r0 + 16;
+            const int kend = sweep == 0 ? r0 : np;
+#pragma unroll 4
+            for (int ks = kbeg + wave * 4; ks < kend; ks += 16) {
+                const int k = ks + lk;
+                const bool ok = arow < n && k < n;
+                const double a_r = ok ? lre[(long)arow * ld + k] : 0.0;
+                const double a_i = ok ? lim[(long)arow * ld + k] : 0.0;
+                const double b_r = xr[k * SOLVE_COLS + li], b_i = xi[k * SOLVE_COLS + li];
+                rr = mfma16(a_r, b_r, rr);
+                ii = mfma16(a_i, b_i, ii);
+                ri = mfma16(a_r, b_i, ri);
+                ir = mfma16(a_i, b_r, ir);
             }
-            const int kk = b * 16 + c;
-            dr_[r][c] = kk < n ? ur[kk] : 0.0;
-            di_[r][c] = kk < n ? ui[kk] : 0.0;
-        }
-        __syncthreads();
-        for (int rr = kend - 1 - b * 16; rr >= 0; --rr) {
-            if (r == rr) {
-                const double dr = dr_[rr][rr], di = di_[rr][rr];
-                const double den = dr * dr + di * di;
-                const double t = (xr_ * dr + xi_ * di) / den;
-                xi_ = (xi_ * dr - xr_ * di) / den;
-                xr_ = t;
-                br_[rr & 1][c] = xr_;
-                bi_[rr & 1][c] = xi_;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                part_r[wave][lk + 4 * r][li] = rr[r] - ii[r];
+                part_i[wave][lk + 4 * r][li] = ri[r] + ir[r];
+            }
+            {  // inverse of the 16x16 diagonal block: a diagonal sub-block of the panel inverse
+                const int off = r0 % LU_NB;
+                const double* d = it.dinv + (size_t)LU_DINV * (r0 / LU_NB) + (sweep == 0 ? 0 : 2 * LU_NB * LU_NB);
+                dg_r[ei][ec] = d[(off + ei) * LU_NB + off + ec];
+                dg_i[ei][ec] = d[LU_NB * LU_NB + (off + ei) * LU_NB + off + ec];
             }
             __syncthreads();
-            if (r < rr) {
-                const double ar = dr_[r][rr], ai = di_[r][rr];
-                const double br = br_[rr & 1][c], bi = bi_[rr & 1][c];
-                xr_ -= ar * br - ai * bi;
-                xi_ -= ar * bi + ai * br;
+            y_r[ei][ec] = xr[(r0 + ei) * SOLVE_COLS + ec] -
+                          (part_r[0][ei][ec] + part_r[1][ei][ec] + part_r[2][ei][ec] + part_r[3][ei][ec]);
+            y_i[ei][ec] = xi[(r0 + ei) * SOLVE_COLS + ec] -
+                          (part_i[0][ei][ec] + part_i[1][ei][ec] + part_i[2][ei][ec] + part_i[3][ei][ec]);
+            __syncthreads();
+            if (wave == 0) {  // X block = Dinv * Y
+                dbl4 pr = {0, 0, 0, 0}, pi = pr, qr = pr, qi = pr;
+#pragma unroll
+                for (int ks = 0; ks < 16; ks += 4) {
+                    const double a_r = dg_r[li][ks + lk], a_i = dg_i[li][ks + lk];
+                    const double b_r = y_r[ks + lk][li], b_i = y_i[ks + lk][li];
+                    pr = mfma16(a_r, b_r, pr);
+                    pi = mfma16(a_i, b_i, pi);
+                    qr = mfma16(a_r, b_i, qr);
+                    qi = mfma16(a_i, b_r, qi);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = lk + 4 * r;
+                    xr[(r0 + row) * SOLVE_COLS + li] = pr[r] - pi[r];
+                    xi[(r0 + row) * SOLVE_COLS + li] = qr[r] + qi[r];
+                }
             }
+            __syncthreads();
         }
-        if (i < n) {
-            xr[i * SOLVE_COLS + c] = xr_;
-            xi[i * SOLVE_COLS + c] = xi_;
-        }
-        __syncthreads();
     }
     for (int e = tid; e < n * SOLVE_COLS; e += SOLVE_THREADS) {
         const int row = e / SOLVE_COLS, cc = e % SOLVE_COLS;
@@ -486,8 +645,8 @@ __global__ __launch_bounds__(SOLVE_THREADS) void lu_solve_kernel(const SolveItem
 }
 
 inline void launch_lu_solve(const SolveItem* d_items, int nitems, int max_n, int max_rhs, hipStream_t s) {
-    const size_t lds = sizeof(double) * 2 * (size_t)max_n * SOLVE_COLS;
-    if (lds > 160 * 1024) throw std::invalid_argument("matrix too large for the LU solve kernel");
+    const size_t lds = lu_solve_lds(max_n);
+    if (max_n > LU_MAX_N || lds > 134 * 1024) throw std::invalid_argument("matrix too large for the LU solve kernel");
     ISLE_HIP(hipFuncSetAttribute((const void*)lu_solve_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     dim3 grid((max_rhs + SOLVE_COLS - 1) / SOLVE_COLS, nitems);
     hipLaunchKernelGGL(lu_solve_kernel, grid, dim3(SOLVE_THREADS), lds, s, d_items);

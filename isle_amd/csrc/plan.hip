@@ -431,8 +431,10 @@ OpCache& Plan::get_force_plain(int nt, int batch, int nspecies, int spin) {
     for (int item = 0; item < nb; ++item) cp.push_back({mY(item, 0).re, LUb + (long)item * 2 * nn, 2 * nn});
     b.add(K::COPY64, cp, (int)(2 * nn));
     std::vector<LuItem> lu;
+    double* luw = b.d64(nb * lu_work_doubles(n));
     for (int item = 0; item < nb; ++item)
-        lu.push_back({LUb + (long)item * 2 * nn, LUb + (long)item * 2 * nn + nn, piv + (long)item * n, nullptr, n, n});
+        lu.push_back({LUb + (long)item * 2 * nn, LUb + (long)item * 2 * nn + nn, piv + (long)item * n, nullptr,
+                      luw + item * lu_work_doubles(n), n, n});
     b.add(K::LU, lu, n, 0, 1);
     std::vector<SolveItem> sv;
     for (int item = 0; item < nb; ++item) {
@@ -440,6 +442,7 @@ OpCache& Plan::get_force_plain(int nt, int batch, int nspecies, int spin) {
         si.lu_re = LUb + (long)item * 2 * nn;
         si.lu_im = si.lu_re + nn;
         si.piv = piv + (long)item * n;
+        si.dinv = luw + item * lu_work_doubles(n);
         si.b_re = si.b_im = nullptr;
         si.x_re = mR(item, nt - 1).re;  // Mid^T: element (r, c) stored at [c][r]
         si.x_im = mR(item, nt - 1).im;
@@ -582,8 +585,10 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
     }
     {
         std::vector<LuItem> lu;
+        double* luw = b.d64(nb * lu_work_doubles(n));
         for (int item = 0; item < nb; ++item)
-            lu.push_back({LUb + (long)item * 2 * nn, LUb + (long)item * 2 * nn + nn, piv + (long)item * n, nullptr, n, n});
+            lu.push_back({LUb + (long)item * 2 * nn, LUb + (long)item * 2 * nn + nn, piv + (long)item * n, nullptr,
+                          luw + item * lu_work_doubles(n), n, n});
         b.add(K::LU, lu, n, 0, 1);
         std::vector<SolveItem> sv;
         for (int item = 0; item < nb; ++item) {
@@ -591,6 +596,7 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
             si.lu_re = LUb + (long)item * 2 * nn;
             si.lu_im = si.lu_re + nn;
             si.piv = piv + (long)item * n;
+            si.dinv = luw + item * lu_work_doubles(n);
             si.x_re = m64(RR, item, 0).re;
             si.x_im = m64(RR, item, 0).im;
             si.xsr = n;
@@ -716,9 +722,11 @@ OpCache& Plan::get_logdet_plain(int nt, int batch, int nspecies, int spin) {
         b.add(K::GEMM64_RC_PRE, gb, n, n);
     }
     std::vector<LuItem> lu;
+    double* luw = b.d64(nb * lu_work_doubles(n));
     for (int item = 0; item < nb; ++item) {
         const CMat<double> last = mW(item, (nt - 1) & 1);
-        lu.push_back({last.re, last.im, piv + (long)item * n, op->logdet + 2 * item, n, n});
+        lu.push_back({last.re, last.im, piv + (long)item * n, op->logdet + 2 * item, luw + item * lu_work_doubles(n),
+                      n, n});
     }
     b.add(K::LU, lu, n, 0, 1);
     b.finish();
@@ -781,9 +789,11 @@ OpCache& Plan::get_logdet_grouped(int nt, int batch, int g, int nspecies, int sp
     }
     auto P = [&](int item, int j) { return mW(item, j, (g - 1) & 1); };
     std::vector<LuItem> lu;
+    double* luw = b.d64(nb * lu_work_doubles(n));
+    auto work = [&](int item) { return luw + item * lu_work_doubles(n); };
     if (ng == 1) {
         for (int item = 0; item < nb; ++item)
-            lu.push_back({P(item, 0).re, P(item, 0).im, piv + (long)item * n, op->logdet + 2 * item, n, n});
+            lu.push_back({P(item, 0).re, P(item, 0).im, piv + (long)item * n, op->logdet + 2 * item, work(item), n, n});
     } else {
         for (int j = 1; j < ng; ++j) {
             std::vector<GemmItem<double>> gi;
@@ -797,7 +807,7 @@ OpCache& Plan::get_logdet_grouped(int nt, int batch, int g, int nspecies, int sp
         }
         for (int item = 0; item < nb; ++item) {
             const CMat<double> last = mT(item, (ng - 2) & 1);
-            lu.push_back({last.re, last.im, piv + (long)item * n, op->logdet + 2 * item, n, n});
+            lu.push_back({last.re, last.im, piv + (long)item * n, op->logdet + 2 * item, work(item), n, n});
         }
     }
     b.add(K::LU, lu, n, 0, 1);
@@ -933,7 +943,9 @@ void Plan::solveM(const double* phi, int nt, int species, const double* rhss, in
     }
     // LU of every partial product (+1 on the last)
     std::vector<LuItem> lu;
-    for (int t = 0; t < nt; ++t) lu.push_back({mP(t).re, mP(t).im, piv + (long)t * n, nullptr, n, n});
+    DevBuf luw(nt * lu_work_doubles(n) * sizeof(double));
+    for (int t = 0; t < nt; ++t)
+        lu.push_back({mP(t).re, mP(t).im, piv + (long)t * n, nullptr, luw.as<double>() + t * lu_work_doubles(n), n, n});
     DevBuf dlu(lu.size() * sizeof(LuItem));
     ISLE_HIP(hipMemcpy(dlu.p, lu.data(), lu.size() * sizeof(LuItem), hipMemcpyHostToDevice));
     if (nt > 1) launch_lu(dlu.as<LuItem>(), nt - 1, n, 0, s);
@@ -943,6 +955,7 @@ void Plan::solveM(const double* phi, int nt, int species, const double* rhss, in
         si.lu_re = mP(t).re;
         si.lu_im = mP(t).im;
         si.piv = piv + (long)t * n;
+        si.dinv = luw.as<double>() + t * lu_work_doubles(n);
         si.b_re = bre;
         si.b_im = bre + (long)n * nrhs;
         si.bsr = bsr;
@@ -1184,11 +1197,12 @@ int isle_logdet(const double* a, int n, double* out_host, void* stream) {
         if (n < 1) throw std::invalid_argument("empty matrix");
         hipStream_t s = (hipStream_t)stream;
         const long nn = (long)n * n;
-        isle::DevBuf m(2 * nn * 8), piv(n * sizeof(int)), ld(2 * 8), item(sizeof(isle::LuItem));
+        isle::DevBuf m(2 * nn * 8), piv(n * sizeof(int)), ld(2 * 8), item(sizeof(isle::LuItem)),
+            w(isle::lu_work_doubles(n) * sizeof(double));
         hipLaunchKernelGGL(isle::interleaved_to_split_kernel, dim3(isle::ew_blocks(nn)), dim3(256), 0, s, a,
                            m.as<double>(), m.as<double>() + nn, nn);
         ISLE_HIP(hipGetLastError());
-        isle::LuItem it{m.as<double>(), m.as<double>() + nn, piv.as<int>(), ld.as<double>(), n, n};
+        isle::LuItem it{m.as<double>(), m.as<double>() + nn, piv.as<int>(), ld.as<double>(), w.as<double>(), n, n};
         ISLE_HIP(hipMemcpyAsync(item.p, &it, sizeof(it), hipMemcpyHostToDevice, s));
         isle::launch_lu(item.as<isle::LuItem>(), 1, n, 0, s);
         ISLE_HIP(hipMemcpyAsync(out_host, ld.p, 16, hipMemcpyDeviceToHost, s));
@@ -1201,17 +1215,19 @@ int isle_invert(const double* a, int n, double* out, void* stream) {
         if (n < 1) throw std::invalid_argument("empty matrix");
         hipStream_t s = (hipStream_t)stream;
         const long nn = (long)n * n;
-        isle::DevBuf m(2 * nn * 8), piv(n * sizeof(int)), item(sizeof(isle::LuItem)), sitem(sizeof(isle::SolveItem));
+        isle::DevBuf m(2 * nn * 8), piv(n * sizeof(int)), item(sizeof(isle::LuItem)), sitem(sizeof(isle::SolveItem)),
+            w(isle::lu_work_doubles(n) * sizeof(double));
         hipLaunchKernelGGL(isle::interleaved_to_split_kernel, dim3(isle::ew_blocks(nn)), dim3(256), 0, s, a,
                            m.as<double>(), m.as<double>() + nn, nn);
         ISLE_HIP(hipGetLastError());
-        isle::LuItem it{m.as<double>(), m.as<double>() + nn, piv.as<int>(), nullptr, n, n};
+        isle::LuItem it{m.as<double>(), m.as<double>() + nn, piv.as<int>(), nullptr, w.as<double>(), n, n};
         ISLE_HIP(hipMemcpyAsync(item.p, &it, sizeof(it), hipMemcpyHostToDevice, s));
         isle::launch_lu(item.as<isle::LuItem>(), 1, n, 0, s);
         isle::SolveItem si{};
         si.lu_re = m.as<double>();
         si.lu_im = m.as<double>() + nn;
         si.piv = piv.as<int>();
+        si.dinv = w.as<double>();
         si.x_re = out;
         si.x_im = out + 1;
         si.xsr = 2L * n;

@@ -105,12 +105,35 @@ def test_solveM(hop):
         np.testing.assert_allclose(m @ xg.T, rhss.T, rtol=1e-10, atol=1e-12)
 
 
-def test_logdet_and_invert():
-    rng = np.random.RandomState(7)
-    for n in (1, 5, 33, 100):
-        a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
-        assert mod2pi_diff(isle.logdet(a), ref.ilogdet(a)) < 1e-11 * max(1, abs(ref.ilogdet(a)))
-        np.testing.assert_allclose(isle.invert(a).cpu().numpy(), np.linalg.inv(a), rtol=1e-9, atol=1e-11)
+def _pivot_heavy(rng, n):
+    """P L U with well-conditioned triangular factors (|re| + |im| of the entries of the unit
+    lower L below 1) and a random row permutation P: partial pivoting then swaps almost
+    every row, many of them with rows several panels further down."""
+    s = 0.5 / np.sqrt(n)
+    lo = np.tril(rng.uniform(-s, s, (n, n)) + 1j * rng.uniform(-s, s, (n, n)), -1) + np.eye(n)
+    up = np.triu(rng.uniform(-s, s, (n, n)) + 1j * rng.uniform(-s, s, (n, n)), 1) + np.diag(
+        rng.uniform(1, 2, n) * np.exp(1j * rng.uniform(-np.pi, np.pi, n)))
+    return (lo @ up)[rng.permutation(n)]
+
+
+@pytest.mark.parametrize("n", [1, 5, 16, 31, 32, 33, 63, 64, 65, 100, 288, 300, 384, 512])
+def test_logdet_and_invert(n):
+    """logdet (1e-11 relative, phase mod 2pi) and inverse (1e-10 relative Frobenius) against
+    LAPACK through numpy/scipy, on Gaussian and pivot-heavy matrices; the sizes cover one
+    partial panel, exact panel multiples, one row past a panel, and the largest order."""
+    rng = np.random.RandomState(7 + n)
+    mats = [rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)), _pivot_heavy(rng, n)]
+    if n > 1:
+        z = mats[0].copy()
+        z[0, 0] = 0.0  # zero leading pivot
+        mats.append(z)
+    for a in mats:
+        want = ref.ilogdet(a)
+        assert mod2pi_diff(isle.logdet(a), want) < 1e-11 * max(1, abs(want))
+        inv = np.linalg.inv(a)
+        got = isle.invert(a).cpu().numpy()
+        assert np.linalg.norm(got - inv) / np.linalg.norm(inv) < 1e-10
+        assert np.linalg.norm(a @ got - np.eye(n)) / np.sqrt(n) < 1e-10
 
 
 @pytest.mark.parametrize("cfg", [(3, 16, 4.0), (6, 64, 6.0)])
@@ -199,8 +222,9 @@ def test_hmc_fixed_seed_matches_oracle():
 
 
 def test_config2_gradient_check():
-    """Nx=72, Nt=64, beta=6, U=3: force vs central FD (h=1e-6) on 64 entries, and vs the
-    oracle force on 16 seeded fields."""
+    """Nx=72, Nt=64, beta=6, U=3: the gradient of the action through torch.autograd (its
+    backward is the force sweep) vs central FD (h=1e-6) on 64 entries, and the force vs
+    the oracle force on 16 seeded fields."""
     kappa = ref.honeycomb(6)
     nx, nt, beta, u = 72, 64, 6.0, 3.0
     kt = kappa * beta / nt
@@ -209,7 +233,12 @@ def test_config2_gradient_check():
     r = ref.HubbardFermiAction(kt, 0.0, -1, ref.EXP)
     rng = np.random.RandomState(2000)
     phi = rng.normal(0, np.sqrt(ut), nx * nt) + 0j
-    f = g.force(phi).cpu().numpy()
+    # dS/dphi from autograd on a complex leaf: grad = conj(dS/dphi) for L = Re S
+    leaf = torch.as_tensor(phi, device="cuda").requires_grad_(True)
+    s_val = isle.autograd.action_value(g, leaf)
+    s_val.real.backward()
+    f = -leaf.grad.conj().cpu().numpy()  # force = -dS/dphi
+    assert rel_l2(f, g.force(phi).cpu().numpy()) < 1e-14
     s0 = abs(g.eval(phi))
     idx = rng.choice(nx * nt, 64, replace=False)
     # Central differences: truncation O(h^2) plus round-off |S| * eta / h with eta ~ 1e-13 the
@@ -225,3 +254,42 @@ def test_config2_gradient_check():
     for s in range(2000, 2016):
         phi = np.random.RandomState(s).normal(0, np.sqrt(ut), nx * nt) + 0j
         assert rel_l2(g.force(phi).cpu().numpy(), r.force(phi)) < 1e-10
+
+
+@pytest.mark.parametrize("hop,basis,mu", [(ref.EXP, ref.PARTICLE_HOLE, 0.0), (ref.EXP, ref.SPIN, 0.3),
+                                          (ref.DIA, ref.PARTICLE_HOLE, 0.2), (ref.DIA, ref.SPIN, 0.5)])
+def test_autograd_logdetM_and_action(hop, basis, mu):
+    """torch.autograd through logdetM (per species) and the action, real and complex
+    fields: gradients against the oracle's derivative rule and central differences of the
+    GPU logdetM; 1e-9 relative."""
+    kappa = ref.honeycomb(3)
+    nx, nt, beta = 18, 16, 4.0
+    kt, mt = kappa * beta / nt, mu * beta / nt
+    g = act.HubbardFermiAction(kt, mt, -1, hop, basis)
+    r = ref.HubbardFermiAction(kt, mt, -1, hop, basis)
+    rng = np.random.RandomState(31)
+    phi = rng.normal(0, 0.4, nx * nt) + 1j * rng.normal(0, 0.1, nx * nt)
+    spin = basis == ref.SPIN
+    arg = (lambda p: -1j * p) if spin else (lambda p: p)
+    for species, k in ((0, r.kp), (1, r.kh)):
+        leaf = torch.as_tensor(phi, device="cuda").requires_grad_(True)
+        ld = isle.autograd.logdetM_value(g, leaf, species)
+        assert mod2pi_diff(complex(ld.item()), ref.logdetM(r.hfm, arg(phi), species)) < 1e-10 * max(1, abs(ld.item()))
+        ld.real.backward()
+        d = leaf.grad.conj().cpu().numpy()  # d logdetM / dphi
+        coef = (-1.0 if species == 0 else 1.0) if spin else (-1j if species == 0 else 1j)
+        want = coef * (ref.force_direct_single_part(r.hfm, arg(phi), k, species) - 1.0)
+        assert rel_l2(d, want) < 1e-9
+        h = 1e-5
+        for i in rng.choice(nx * nt, 4, replace=False):
+            e = np.zeros(nx * nt, dtype=complex)
+            e[i] = h
+            fd = (isle.autograd.logdetM_value(g, _dev(phi + e), species).item()
+                  - isle.autograd.logdetM_value(g, _dev(phi - e), species).item())
+            fd = complex(fd.real, np.remainder(fd.imag + np.pi, 2 * np.pi) - np.pi) / (2 * h)
+            assert abs(fd - d[i]) < 1e-7 * max(1, abs(d[i]))
+    # action on a real field (the HMC case): grad = -Re(force)
+    x = torch.as_tensor(phi.real.copy(), device="cuda").requires_grad_(True)
+    isle.autograd.action_value(g, x).real.backward()
+    want = -np.real(r.force(phi.real + 0j))
+    assert rel_l2(x.grad.cpu().numpy(), want) < 1e-9

@@ -76,6 +76,32 @@ def test_force_is_minus_gradient(hop, basis, mu):
         assert abs(fd - f[idx]) < 1e-7 * max(1, abs(f[idx])), (idx, fd, f[idx])
 
 
+@pytest.mark.parametrize("hop,basis,mu", [(ref.EXP, ref.PARTICLE_HOLE, 0.0), (ref.EXP, ref.SPIN, 0.3),
+                                          (ref.DIA, ref.PARTICLE_HOLE, 0.2), (ref.DIA, ref.SPIN, 0.5)])
+def test_species_logdet_gradient(hop, basis, mu):
+    """d logdetM_s/dphi = c_s (f_s - 1) with c = (-i, +i) in the particle/hole basis and
+    (-1, +1) in the spin basis (f_s = forceDirectSinglePart at phi, resp. aux = -i phi; the
+    -1 is the derivative of the explicit +-i sum(phi) term): the rule the autograd
+    wrapper's backward uses, checked against central differences of logdetM."""
+    kappa = ref.honeycomb(3)
+    nx, nt, beta = 18, 8, 3.0
+    act = _act(kappa, nt, beta, mu, -1, hop, basis, ref.DIRECT_SINGLE)
+    rng = np.random.RandomState(5)
+    phi = rng.normal(0, 0.4, nx * nt) + 1j * rng.normal(0, 0.1, nx * nt)
+    spin = basis == ref.SPIN
+    h = 1e-5
+    for species, k in ((ref.PARTICLE, act.kp), (ref.HOLE, act.kh)):
+        arg = (lambda p: -1j * p) if spin else (lambda p: p)
+        coef = (-1.0 if species == ref.PARTICLE else 1.0) if spin else (-1j if species == ref.PARTICLE else 1j)
+        grad = coef * (ref.force_direct_single_part(act.hfm, arg(phi), k, species) - 1.0)
+        for idx in rng.choice(nx * nt, 6, replace=False):
+            e = np.zeros(nx * nt, dtype=complex)
+            e[idx] = h
+            d = ref.logdetM(act.hfm, arg(phi + e), species) - ref.logdetM(act.hfm, arg(phi - e), species)
+            d = complex(d.real, np.remainder(d.imag + np.pi, 2 * np.pi) - np.pi)
+            assert abs(d / (2 * h) - grad[idx]) < 1e-7 * max(1, abs(grad[idx])), (species, idx)
+
+
 def test_sum_action():
     rng = np.random.RandomState(8613)
 
