@@ -35,7 +35,42 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=1, help="timed trajectories of the plain FP64 form")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch the integration eagerly, not from a HIP graph")
+    ap.add_argument("--force-only", action="store_true",
+                    help="profiling aid: run --steps batched mixed forces and one 1 GiB device copy "
+                         "(the calibration kernel for HBM byte counters), print timings, no HMC")
     return ap.parse_args()
+
+
+def force_only(args):
+    """Short workload for hardware-counter runs: the batched mixed force at Nx=288, Nt=128
+    for all chains, then a device-to-device copy of 1 GiB (known HBM traffic: 1 GiB read
+    and 1 GiB written) to scale the byte counters."""
+    import torch
+    import isle_amd as isle
+    from isle_amd import _lib
+    from isle_amd import action as A
+    dev = _lib.device()
+    lat = isle.honeycomb(L, nt=NT)
+    nx = lat.nx()
+    kt = lat.hopping() * BETA / NT
+    ut = U * BETA / NT
+    ferm = A.HubbardFermiAction(kt, 0.0, -1, A.HFAHopping.EXP, precision="mixed", group=args.group)
+    phis = torch.as_tensor(np.random.RandomState(5).normal(0, np.sqrt(ut), (args.chains, nx * NT)) + 0j, device=dev)
+    for _ in range(args.warmup):
+        ferm.force_batch(phis)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ferm.force_batch(phis)
+    torch.cuda.synchronize()
+    t_force = (time.perf_counter() - t0) / args.steps
+    src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)  # 1 GiB
+    torch.cuda.synchronize()
+    dst = src.clone()
+    torch.cuda.synchronize()
+    print(json.dumps({"force_only": True, "chains": args.chains, "force_ms": 1e3 * t_force,
+                      "copy_bytes": int(dst.numel() * 8)}))
 
 
 def cpu_baseline(kappa_tilde, ut, nx):
@@ -71,8 +106,11 @@ def main():
         print("bench.py runs on exactly one GPU (the chains of the batch share it); "
               "multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
-    import torch
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    if args.force_only:
+        force_only(args)
+        return
+    import torch
     import isle_amd as isle
     from isle_amd import _lib
     from isle_amd import action as A
@@ -102,7 +140,7 @@ def main():
     errs = np.linalg.norm(f_m - f_p, axis=1) / np.linalg.norm(f_p, axis=1)
     force_err = float(errs.max())
     force_err_max = float((np.abs(f_m - f_p).max(axis=1) / np.abs(f_p).max(axis=1)).max())
-    ev = BatchedConstStepLeapfrog(act_m, LENGTH, NSTEP, rngs)
+    ev = BatchedConstStepLeapfrog(act_m, LENGTH, NSTEP, rngs, graph=not args.no_graph)
     for _ in range(args.warmup):
         phis, acts, _ = ev.evolve(phis, acts)
     torch.cuda.synchronize()
@@ -145,7 +183,8 @@ def main():
         rng = NumpyRNG(1000)
         phi1 = torch.as_tensor((rng.normal(0, np.sqrt(ut), nx * NT) + 0j).reshape(1, -1), device=dev)
         a1 = np.array([act_p.eval(phi1[0])])
-        ev_p = BatchedConstStepLeapfrog(act_p, LENGTH, NSTEP, [rng])
+        ev_p = BatchedConstStepLeapfrog(act_p, LENGTH, NSTEP, [rng], graph=not args.no_graph)
+        phi1, a1, _ = ev_p.evolve(phi1, a1)  # warm-up (plan build, tuning, graph capture)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.plain_steps):

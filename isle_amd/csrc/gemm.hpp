@@ -15,6 +15,8 @@
 // running sum (blocked summation).
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace isle {
@@ -299,10 +301,195 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* _
     gemm_tile<T, AC, BC, PRE, POST, BM, BN, GEMM_BK>(it, tile, tiles_n_max);
 }
 
+// ---------------------------------------------------------------------------------------
+// FP32 chain product with direct global->LDS loads (gfx950 global_load_lds_dwordx4):
+//     C = POST(A * PRE(B)),  A real and SYMMETRIC (shared by all items), B complex planes.
+// Because A = A^T, the k-rows of A are its columns, so both operand slabs are plain copies
+// of row segments: slab rows k0..k0+31 of A (columns m0..) and of B (columns n0..).  The
+// copies go straight into LDS (no staging registers), three slabs in flight.  In LDS a row
+// of 4-float chunks is XOR-swizzled (chunk ^ 4 on odd rows) so the four k-rows read by one
+// MFMA step fall into different banks; the swizzle is applied by choosing each lane's
+// source chunk, since a wave's direct load writes 64 consecutive 16-byte chunks.
+// Requirements (checked by the launcher): k % 32 == 0, m, n, lda, ldb, ldc % 4 == 0.
+constexpr int RCD_BK = 32;
+
+constexpr int RCD_STAGES = 3;
+
+template <int BM, int BN>
+struct RcdLayout {
+    static constexpr int A = RCD_BK * BM;        // floats
+    static constexpr int B = RCD_BK * BN;        // per plane
+    static constexpr int PRE = 4 * 2 * RCD_BK;   // one copy of pre (re[32], im[32]) per wave
+    static constexpr int STAGE = A + 2 * B + PRE;
+    static constexpr size_t BYTES = sizeof(float) * STAGE * RCD_STAGES;
+};
+
+__device__ __forceinline__ int rcd_swz(int k, int m) { return ((((m >> 2) ^ ((k & 1) << 2))) << 2) | (m & 3); }
+
+template <int POST, bool PRE, int BM, int BN>
+__global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<float>* __restrict__ items,
+                                                                int tiles_n_max) {
+    typedef RcdLayout<BM, BN> LY;
+    constexpr int MI = BM / 32, NJ = BN / 32;
+    constexpr int ACH = RCD_BK * BM / 4, BCH = RCD_BK * BN / 4;  // 16-byte chunks per slab
+    static_assert(ACH % 256 == 0 && BCH % 256 == 0, "slab chunks must cover whole waves");
+    constexpr int AQ = ACH / 256, BQ = BCH / 256;                 // direct loads per wave per plane
+    extern __shared__ float rcd_lds[];
+
+    int tile, item;
+    xcd_map(gridDim.x, gridDim.y, tile, item);
+    const GemmItem<float> it = items[item];
+    const int m0 = (tile / tiles_n_max) * BM, n0 = (tile % tiles_n_max) * BN;
+    if (m0 >= it.m || n0 >= it.n) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
+    const int lr = lane & 15, lk = lane >> 4;
+    const int nslab = it.k / RCD_BK;
+
+    // (offsets as literal constants: the slab lambda must not name the layout's static members)
+    constexpr int OFF_BR = RCD_BK * BM, OFF_BI = RCD_BK * (BM + BN), OFF_PRE = RCD_BK * (BM + 2 * BN);
+    constexpr int STAGE_F = OFF_PRE + 4 * 2 * RCD_BK;
+    static_assert(STAGE_F == LY::STAGE, "layout");
+    auto issue = [&](int slab) {
+        float* st = rcd_lds + (slab % RCD_STAGES) * STAGE_F;
+        const int k0 = slab * RCD_BK;
+#pragma unroll
+        for (int q = 0; q < AQ; ++q) {
+            const int c = (wave * AQ + q) * 64 + lane;           // chunk slot of this lane
+            const int k = c / (BM / 4), slot = c % (BM / 4);
+            int col = m0 + 4 * (slot ^ ((k & 1) << 2));
+            if (col >= it.m) col = 0;                              // only feeds rows >= m of C
+            __builtin_amdgcn_global_load_lds(it.a_re + (long)(k0 + k) * it.lda + col,
+                                             st + (wave * AQ + q) * 256, 16, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < BQ; ++q) {
+            const int c = (wave * BQ + q) * 64 + lane;
+            const int k = c / (BN / 4), slot = c % (BN / 4);
+            int col = n0 + 4 * (slot ^ ((k & 1) << 2));
+            if (col >= it.n) col = 0;                              // only feeds columns >= n of C
+            const long off = (long)(k0 + k) * it.ldb + col;
+            __builtin_amdgcn_global_load_lds(it.b_re + off, st + OFF_BR + (wave * BQ + q) * 256, 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(it.b_im + off, st + OFF_BI + (wave * BQ + q) * 256, 16, 0, 0);
+        }
+        if (PRE && lane < 16) {  // every wave copies pre[k0..k0+31] (re then im) into its own 64 floats
+            const float* src = (lane < 8 ? it.pre_re : it.pre_im) + k0 + 4 * (lane & 7);
+            __builtin_amdgcn_global_load_lds(src, st + OFF_PRE + wave * 64, 16, 0, 0);
+        }
+    };
+    // direct loads issued per wave and slab (the pre copy is one instruction of 16 lanes)
+    constexpr int PER_SLAB = AQ + 2 * BQ + (PRE ? 1 : 0);
+
+    flt4 acc_re[MI][NJ], acc_im[MI][NJ];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) acc_re[i][j] = acc_im[i][j] = flt4{0, 0, 0, 0};
+
+    issue(0);
+    if (nslab > 1) issue(1);
+    for (int s = 0; s < nslab; ++s) {
+        // wait for this wave's copies of slab s (slab s+1 may stay in flight), then for all
+        // waves.  A plain s_barrier: __syncthreads() would also drain the copies in flight.
+        if (s + 1 < nslab)
+            asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(PER_SLAB) : "memory");
+        else
+            asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
+        if (s + 2 < nslab) issue(s + 2);  // into the buffer every wave finished reading
+        const float* st = rcd_lds + (s % RCD_STAGES) * LY::STAGE;
+        const float* As = st;
+        const float* Br = st + LY::A;
+        const float* Bi = st + LY::A + LY::B;
+        const float* pw = st + LY::A + 2 * LY::B + wave * 64;
+#pragma unroll
+        for (int kg = 0; kg < RCD_BK; kg += 16) {
+            flt4 pr_[MI][NJ], pi_[MI][NJ];  // blocked summation: each 16 k in a fresh accumulator
+#pragma unroll
+            for (int i = 0; i < MI; ++i)
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) pr_[i][j] = pi_[i][j] = flt4{0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                const int kk = kg + ks * 4 + lk;
+                float a[MI], br[NJ], bi[NJ];
+#pragma unroll
+                for (int i = 0; i < MI; ++i) a[i] = As[kk * BM + rcd_swz(kk, wm + i * 16 + lr)];
+                float cr = 1.0f, ci = 0.0f;
+                if (PRE) {
+                    cr = pw[kk];
+                    ci = pw[32 + kk];
+                }
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) {
+                    const int o = kk * BN + rcd_swz(kk, wn + j * 16 + lr);
+                    float vr = Br[o], vi = Bi[o];
+                    if (PRE) {
+                        const float t = vr * cr - vi * ci;
+                        vi = vr * ci + vi * cr;
+                        vr = t;
+                    }
+                    br[j] = vr;
+                    bi[j] = vi;
+                }
+#pragma unroll
+                for (int i = 0; i < MI; ++i)
+#pragma unroll
+                    for (int j = 0; j < NJ; ++j) {
+                        pr_[i][j] = mfma16(a[i], br[j], pr_[i][j]);
+                        pi_[i][j] = mfma16(a[i], bi[j], pi_[i][j]);
+                    }
+            }
+#pragma unroll
+            for (int i = 0; i < MI; ++i)
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) {
+                    acc_re[i][j] += pr_[i][j];
+                    acc_im[i][j] += pi_[i][j];
+                }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm + i * 16 + acc_row<float>(lk, r);
+                const int col = n0 + wn + j * 16 + lr;
+                if (row < it.m && col < it.n) {
+                    float vr = acc_re[i][j][r], vi = acc_im[i][j][r];
+                    if (POST != SCALE_NONE) {
+                        const int di = POST == SCALE_ROW ? row : col;
+                        const float pr = it.post_re[di], pi = it.post_im[di];
+                        const float t = vr * pr - vi * pi;
+                        vi = vr * pi + vi * pr;
+                        vr = t;
+                    }
+                    const long off = (long)row * it.ldc + col;
+                    it.c_re[off] = vr;
+                    it.c_im[off] = vi;
+                }
+            }
+}
+
 // Tile shapes the launcher can pick (the plan's autotuner times them per step):
 //   0: 64x64, K slab 16   1: 32x32, K slab 32   2: 32x32, K slab 64   3: 64x64, K slab 32
 //   4: 96x96, K slab 32   5: 96x96, K slab 16   (96 = 288 / 3: no partial tiles at Nx = 288)
-enum : int { N_TILES = 6 };
+//   6: 64x64 and 7: 96x96 direct-to-LDS (gemm_rcd_kernel): FP32, real symmetric A times a
+//      complex B only; other shapes/types report "does not fit"
+enum : int { N_TILES = 8 };
+
+template <int POST, bool PRE, int BM, int BN>
+inline bool launch_gemm_rcd(const GemmItem<float>* d_items, int nitems, int max_m, int max_n, hipStream_t s) {
+    const size_t lds = RcdLayout<BM, BN>::BYTES;
+    auto kern = gemm_rcd_kernel<POST, PRE, BM, BN>;
+    ISLE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds));
+    const int tm = (max_m + BM - 1) / BM, tn = (max_n + BN - 1) / BN;
+    hipLaunchKernelGGL(kern, dim3(tm * tn, nitems), dim3(GEMM_THREADS), lds, s, d_items, tn);
+    ISLE_HIP(hipGetLastError());
+    return true;
+}
 
 template <typename T, bool AC, bool BC, int BM, int BN, int BK>
 constexpr size_t gemm_lds_bytes() {
@@ -324,9 +511,21 @@ inline bool launch_gemm_tile(const GemmItem<T>* d_items, int nitems, int max_m, 
     }
 }
 
+// direct tiles need: FP32, A real, square and symmetric, B complex, no A pre-scaling,
+// k % 32 == 0, every dimension and leading dimension a multiple of 4 and 16-byte aligned
+// operand / phase pointers (the caller vouches for all of it with `direct_ok`)
 template <typename T, bool AC, bool BC, bool PRE, int POST>
 inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s,
-                        int tile = 0) {
+                        int tile = 0, bool direct_ok = false) {
+    if (tile >= 6) {
+        if constexpr (std::is_same<T, float>::value && !AC && BC && POST != SUBTRACT) {
+            if (!direct_ok) return false;
+            return tile == 6 ? launch_gemm_rcd<POST, PRE, 64, 64>(d_items, nitems, max_m, max_n, s)
+                             : launch_gemm_rcd<POST, PRE, 96, 96>(d_items, nitems, max_m, max_n, s);
+        } else {
+            return false;
+        }
+    }
     switch (tile) {
     case 1: return launch_gemm_tile<T, AC, BC, PRE, POST, 32, 32, 32>(d_items, nitems, max_m, max_n, s);
     case 2: return launch_gemm_tile<T, AC, BC, PRE, POST, 32, 32, 64>(d_items, nitems, max_m, max_n, s);

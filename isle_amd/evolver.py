@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from .core import as_field
-from .integrator import leapfrog_batch
+from .integrator import LeapfrogGraph, eval_batch as _eval_batch, leapfrog_batch
 
 
 class NumpyRNG:
@@ -139,7 +139,7 @@ class BatchedConstStepLeapfrog:
     uniform from rngs[b] exactly as ConstStepLeapfrog would, so each chain reproduces a
     lone run with the same seed (up to the arithmetic of the batched kernels)."""
 
-    def __init__(self, action, length, nstep, rngs):
+    def __init__(self, action, length, nstep, rngs, graph=False):
         self.action = action
         self.length = length
         self.nstep = nstep
@@ -147,6 +147,8 @@ class BatchedConstStepLeapfrog:
         self.selectors = [BinarySelector(r) for r in self.rngs]
         self.trajPoints = []
         self.deltaH = []
+        self.graph = bool(graph)  # replay the integration from a captured HIP graph
+        self._lf = None
 
     def evolve(self, phis, actVals):
         """phis: [batch, nx*nt] device tensor, actVals: numpy complex [batch].
@@ -154,7 +156,13 @@ class BatchedConstStepLeapfrog:
         batch, n = phis.shape
         pi_host = np.stack([r.normal(0, 1, n) + 0j for r in self.rngs])
         pi = torch.as_tensor(pi_host, device=phis.device)
-        phi1, pi1, acts1 = leapfrog_batch(phis, pi, self.action, self.length, self.nstep)
+        if self.graph:
+            if self._lf is None:
+                self._lf = LeapfrogGraph(self.action, batch, n, self.length, self.nstep)
+            phi1, pi1 = self._lf(phis, pi)
+            acts1 = _eval_batch(self.action, phi1)
+        else:
+            phi1, pi1, acts1 = leapfrog_batch(phis, pi, self.action, self.length, self.nstep)
         pi1h = pi1.cpu().numpy()
         new_phi = phis.clone()
         new_act = np.array(actVals, dtype=complex)

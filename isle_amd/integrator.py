@@ -68,22 +68,26 @@ def leapfrog_batch(phis, pis, action, length, nsteps, direction=1.0, evaluate=Tr
         kick(1.0)
         _drift(phi, pi, eps)
     kick(0.5)
-    acts = None
-    if evaluate:
-        if fused is not None:
-            ferm, utilde = fused
-            acts = ferm.eval_batch(phi)
-            if utilde > 0:
-                import numpy as np
-                sums = np.empty((phi.shape[0], 4))
-                n = phi.shape[1]
-                _lib.check(_lib.load().isle_field_sums(_lib.ptr(phi), n, n, phi.shape[0],
-                                                       _lib.dptr(sums), _lib.stream_ptr()))
-                acts = acts + (sums[:, 2] + 1j * sums[:, 3]) / 2.0 / utilde
-        else:
-            import numpy as np
-            acts = np.array([complex(action.eval(phi[b])) for b in range(phi.shape[0])])
+    acts = eval_batch(action, phi) if evaluate else None
     return phi, pi, acts
+
+
+def eval_batch(action, phi):
+    """Action of each field of a [batch, nx*nt] stack as a numpy complex array; gauge +
+    fermion sums are evaluated with the batched kernels."""
+    import numpy as np
+    fused = _split_fused(action)
+    if fused is None:
+        return np.array([complex(action.eval(phi[b])) for b in range(phi.shape[0])])
+    ferm, utilde = fused
+    acts = ferm.eval_batch(phi)
+    if utilde > 0:
+        sums = np.empty((phi.shape[0], 4))
+        n = phi.shape[1]
+        _lib.check(_lib.load().isle_field_sums(_lib.ptr(phi), n, n, phi.shape[0], _lib.dptr(sums),
+                                               _lib.stream_ptr()))
+        acts = acts + (sums[:, 2] + 1j * sums[:, 3]) / 2.0 / utilde
+    return acts
 
 
 def leapfrog(phi, pi, action, length, nsteps, direction=1):
@@ -92,3 +96,34 @@ def leapfrog(phi, pi, action, length, nsteps, direction=1):
     pi = as_field(pi)
     p1, q1, acts = leapfrog_batch(phi.reshape(1, -1), pi.reshape(1, -1), action, length, nsteps, direction)
     return p1[0], q1[0], complex(acts[0])
+
+
+class LeapfrogGraph:
+    """A leapfrog trajectory of a fixed [batch, nx*nt] stack of chains captured once in a
+    HIP graph (torch.cuda.CUDAGraph) and replayed: the 2*nsteps kicks/drifts and the
+    nsteps+1 batched force evaluations become one graph launch.  The action's plans are
+    built and tuned by an eager warm-up before the capture; the force path then makes no
+    host synchronisation, allocation or memset.  The action value is not part of the
+    graph (its log-determinants are read back to the host)."""
+
+    def __init__(self, action, batch, n, length, nsteps, direction=1.0):
+        dev = _lib.device()
+        self.phi = torch.zeros((batch, n), dtype=torch.complex128, device=dev)
+        self.pi = torch.zeros_like(self.phi)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            leapfrog_batch(self.phi, self.pi, action, length, nsteps, direction, evaluate=False)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.out_phi, self.out_pi, _ = leapfrog_batch(self.phi, self.pi, action, length, nsteps, direction,
+                                                          evaluate=False)
+
+    def __call__(self, phis, pis):
+        """Returns (phis', pis') as new tensors."""
+        self.phi.copy_(as_field(phis).reshape(self.phi.shape))
+        self.pi.copy_(as_field(pis).reshape(self.pi.shape))
+        self.graph.replay()
+        return self.out_phi.clone(), self.out_pi.clone()
