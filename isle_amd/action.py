@@ -150,7 +150,7 @@ class HubbardFermiAction(Action):
 
     def __init__(self, kappaTilde, muTilde, sigmaKappa, hopping=HFAHopping.DIA,
                  basis=HFABasis.PARTICLE_HOLE, algorithm=HFAAlgorithm.DIRECT_SINGLE,
-                 allowShortcut=False, precision="fp64", group=8):
+                 allowShortcut=False, precision="fp64", group=32):
         if HFAAlgorithm(algorithm) != HFAAlgorithm.DIRECT_SINGLE:
             raise NotImplementedError("HFAAlgorithm.DIRECT_SQUARE is not implemented on the GPU")
         if precision not in ("fp64", "mixed"):
@@ -163,8 +163,8 @@ class HubbardFermiAction(Action):
         self.hfm = cls(kappa, muTilde, sigmaKappa)
         self.precision = precision
         self.group = int(group)
-        # log of the condition-number bound of one time slice: the FP32 chains of the
-        # mixed force span at most `group` slices and are kept within cond <= e^3
+        # log of the condition-number bound of one time slice: the FP32 right chains of
+        # the mixed force span at most `group` slices and are kept within cond <= e^12
         r = float(np.max(np.sum(np.abs(kappa), axis=1))) if kappa.size else 0.0
         if self.hopping == HFAHopping.EXP:
             self._slice_spread = 2.0 * r
@@ -179,11 +179,12 @@ class HubbardFermiAction(Action):
         return 0 if self.precision == "fp64" else 1
 
     def _group(self, nt):
-        """Slices per FP32 group: the largest divisor of nt not above ``group`` whose
-        chain keeps the slice-conditioning bound g * spread <= 3."""
+        """Slices per group of the mixed force: the largest divisor of nt not above
+        ``group`` whose FP32 right chain keeps the slice-conditioning bound
+        g * spread <= 12 (cond <= e^12 over one group)."""
         g = min(self.group, nt)
         if self._slice_spread > 0:
-            g = max(1, min(g, int(3.0 / self._slice_spread)))
+            g = max(1, min(g, int(12.0 / self._slice_spread)))
         while nt % g:
             g -= 1
         return g

@@ -105,20 +105,20 @@ __global__ void scale_complex_kernel(const CScaleItem<T>* __restrict__ items) {
 // ---------------------------------------------------------------------------
 // out[i] = scale * sum_k L[i][k] R[k][i]  (L == null: identity), FP64 accumulation.
 // Result written interleaved complex128 at out[2*i], optional per-i divisor q.
-template <typename T>
+template <typename TL, typename TR = TL>
 struct DiagItem {
-    const T* l_re;
-    const T* l_im;
-    const T* r_re;   // R, or R^T when rowdot != 0
-    const T* r_im;
+    const TL* l_re;
+    const TL* l_im;
+    const TR* r_re;  // R, or R^T when rowdot != 0
+    const TR* r_im;
     double* out;
     int n;
     int rowdot;      // 1: out[i] = sum_k L[i][k] RT[i][k]  (R stored transposed, both rows contiguous)
 };
 
-template <typename T>
-__global__ __launch_bounds__(256) void diag_prod_kernel(const DiagItem<T>* __restrict__ items) {
-    const DiagItem<T> it = items[blockIdx.y];
+template <typename TL, typename TR = TL>
+__global__ __launch_bounds__(256) void diag_prod_kernel(const DiagItem<TL, TR>* __restrict__ items) {
+    const DiagItem<TL, TR> it = items[blockIdx.y];
     const int n = it.n;
     const int ii = threadIdx.x >> 4;   // 0..15
     const int kk = threadIdx.x & 15;   // k lane

@@ -302,92 +302,110 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* _
 }
 
 // ---------------------------------------------------------------------------------------
-// FP32 chain product with direct global->LDS loads (gfx950 global_load_lds_dwordx4):
+// Chain product with direct global->LDS loads (gfx950 global_load_lds_dwordx4), FP32 or FP64:
 //     C = POST(A * PRE(B)),  A real and SYMMETRIC (shared by all items), B complex planes.
 // Because A = A^T, the k-rows of A are its columns, so both operand slabs are plain copies
-// of row segments: slab rows k0..k0+31 of A (columns m0..) and of B (columns n0..).  The
+// of row segments: slab rows k0..k0+BK-1 of A (columns m0..) and of B (columns n0..).  The
 // copies go straight into LDS (no staging registers), three slabs in flight.  In LDS a row
-// of 4-float chunks is XOR-swizzled (chunk ^ 4 on odd rows) so the four k-rows read by one
-// MFMA step fall into different banks; the swizzle is applied by choosing each lane's
-// source chunk, since a wave's direct load writes 64 consecutive 16-byte chunks.
-// Requirements (checked by the launcher): k % 32 == 0, m, n, lda, ldb, ldc % 4 == 0.
-constexpr int RCD_BK = 32;
-
+// of 16-byte chunks is XOR-swizzled (chunk ^ 4 on odd rows) so the k-rows read by one MFMA
+// step fall into different banks; the swizzle is applied by choosing each lane's source
+// chunk, since a wave's direct load writes 64 consecutive 16-byte chunks.
+// Requirements (the caller vouches): k % BK == 0; m, n, lda, ldb, ldc multiples of 4;
+// 16-byte aligned operands and phase vectors.
 constexpr int RCD_STAGES = 3;
 
-template <int BM, int BN>
+template <typename T, int BM, int BN, int BK>
 struct RcdLayout {
-    static constexpr int A = RCD_BK * BM;        // floats
-    static constexpr int B = RCD_BK * BN;        // per plane
-    static constexpr int PRE = 4 * 2 * RCD_BK;   // one copy of pre (re[32], im[32]) per wave
+    static constexpr int EPC = 16 / sizeof(T);   // elements per 16-byte chunk
+    static constexpr int A = BK * BM;            // elements
+    static constexpr int B = BK * BN;            // per plane
+    static constexpr int PRE = 4 * 2 * BK;       // one copy of pre (re[BK], im[BK]) per wave
     static constexpr int STAGE = A + 2 * B + PRE;
-    static constexpr size_t BYTES = sizeof(float) * STAGE * RCD_STAGES;
+    static constexpr size_t BYTES = sizeof(T) * STAGE * RCD_STAGES;
 };
 
-__device__ __forceinline__ int rcd_swz(int k, int m) { return ((((m >> 2) ^ ((k & 1) << 2))) << 2) | (m & 3); }
+template <int EPC>
+__device__ __forceinline__ int rcd_swz(int k, int m) {
+    return (((m / EPC) ^ ((k & 1) << 2)) * EPC) + (m % EPC);
+}
 
-template <int POST, bool PRE, int BM, int BN>
-__global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<float>* __restrict__ items,
+// Issues this wave's direct copies of slab `slab` (A rows, B re/im rows, and with PRE the
+// phase factors) into stage slab % RCD_STAGES of the LDS ring.
+template <typename T, bool PRE, int BM, int BN, int BK>
+__device__ __forceinline__ void rcd_issue(const GemmItem<T>& it, T* lds, int slab, int m0, int n0, int wave,
+                                          int lane) {
+    constexpr int EPC = 16 / sizeof(T);
+    constexpr int AQ = BK * BM / EPC / 256, BQ = BK * BN / EPC / 256;
+    constexpr int OFF_BR = BK * BM, OFF_BI = BK * (BM + BN), OFF_PRE = BK * (BM + 2 * BN);
+    constexpr int STAGE_E = OFF_PRE + 4 * 2 * BK;
+    constexpr int CH = 64 * EPC;  // elements written by one wave instruction
+    constexpr int PC = BK / EPC;  // chunks of one phase plane
+    T* st = lds + (slab % RCD_STAGES) * STAGE_E;
+    const int k0 = slab * BK;
+#pragma unroll
+    for (int q = 0; q < AQ; ++q) {
+        const int c = (wave * AQ + q) * 64 + lane;  // chunk slot of this lane
+        const int k = c / (BM / EPC), slot = c % (BM / EPC);
+        int col = m0 + EPC * (slot ^ ((k & 1) << 2));
+        if (col >= it.m) col = 0;  // only feeds rows >= m of C
+        __builtin_amdgcn_global_load_lds(it.a_re + (long)(k0 + k) * it.lda + col, st + (wave * AQ + q) * CH, 16, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) {
+        const int c = (wave * BQ + q) * 64 + lane;
+        const int k = c / (BN / EPC), slot = c % (BN / EPC);
+        int col = n0 + EPC * (slot ^ ((k & 1) << 2));
+        if (col >= it.n) col = 0;  // only feeds columns >= n of C
+        const long off = (long)(k0 + k) * it.ldb + col;
+        __builtin_amdgcn_global_load_lds(it.b_re + off, st + OFF_BR + (wave * BQ + q) * CH, 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(it.b_im + off, st + OFF_BI + (wave * BQ + q) * CH, 16, 0, 0);
+    }
+    if (PRE && lane < 2 * PC) {  // every wave copies pre[k0..k0+BK) (re then im) into its own 2*BK elements
+        const T* src = (lane < PC ? it.pre_re : it.pre_im) + k0 + EPC * (lane % PC);
+        __builtin_amdgcn_global_load_lds(src, st + OFF_PRE + wave * 2 * BK, 16, 0, 0);
+    }
+}
+
+template <typename T, int POST, bool PRE, int BM, int BN, int BK>
+__global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items,
                                                                 int tiles_n_max) {
-    typedef RcdLayout<BM, BN> LY;
+    typedef RcdLayout<T, BM, BN, BK> LY;
+    constexpr int EPC = 16 / sizeof(T);
     constexpr int MI = BM / 32, NJ = BN / 32;
-    constexpr int ACH = RCD_BK * BM / 4, BCH = RCD_BK * BN / 4;  // 16-byte chunks per slab
+    constexpr int ACH = BK * BM / EPC, BCH = BK * BN / EPC;  // 16-byte chunks per slab
     static_assert(ACH % 256 == 0 && BCH % 256 == 0, "slab chunks must cover whole waves");
-    constexpr int AQ = ACH / 256, BQ = BCH / 256;                 // direct loads per wave per plane
-    extern __shared__ float rcd_lds[];
+    static_assert((BM / EPC) % 8 == 0 && (BN / EPC) % 8 == 0, "swizzle needs multiples of 8 chunks per row");
+    static_assert(2 * BK / EPC <= 64 && BK % 16 == 0, "pre copy must fit one wave instruction");
+    constexpr int AQ = ACH / 256, BQ = BCH / 256;  // direct loads per wave per plane
+    constexpr bool BLOCKED = sizeof(T) == 4;       // FP32: each 16 k summed in a fresh accumulator
+    typedef typename Acc4<T>::type V4;
+    extern __shared__ __align__(16) unsigned char rcd_raw[];
+    T* rcd_lds = reinterpret_cast<T*>(rcd_raw);
 
     int tile, item;
     xcd_map(gridDim.x, gridDim.y, tile, item);
-    const GemmItem<float> it = items[item];
+    const GemmItem<T> it = items[item];
     const int m0 = (tile / tiles_n_max) * BM, n0 = (tile % tiles_n_max) * BN;
     if (m0 >= it.m || n0 >= it.n) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
     const int lr = lane & 15, lk = lane >> 4;
-    const int nslab = it.k / RCD_BK;
+    const int nslab = it.k / BK;
 
-    // (offsets as literal constants: the slab lambda must not name the layout's static members)
-    constexpr int OFF_BR = RCD_BK * BM, OFF_BI = RCD_BK * (BM + BN), OFF_PRE = RCD_BK * (BM + 2 * BN);
-    constexpr int STAGE_F = OFF_PRE + 4 * 2 * RCD_BK;
-    static_assert(STAGE_F == LY::STAGE, "layout");
-    auto issue = [&](int slab) {
-        float* st = rcd_lds + (slab % RCD_STAGES) * STAGE_F;
-        const int k0 = slab * RCD_BK;
-#pragma unroll
-        for (int q = 0; q < AQ; ++q) {
-            const int c = (wave * AQ + q) * 64 + lane;           // chunk slot of this lane
-            const int k = c / (BM / 4), slot = c % (BM / 4);
-            int col = m0 + 4 * (slot ^ ((k & 1) << 2));
-            if (col >= it.m) col = 0;                              // only feeds rows >= m of C
-            __builtin_amdgcn_global_load_lds(it.a_re + (long)(k0 + k) * it.lda + col,
-                                             st + (wave * AQ + q) * 256, 16, 0, 0);
-        }
-#pragma unroll
-        for (int q = 0; q < BQ; ++q) {
-            const int c = (wave * BQ + q) * 64 + lane;
-            const int k = c / (BN / 4), slot = c % (BN / 4);
-            int col = n0 + 4 * (slot ^ ((k & 1) << 2));
-            if (col >= it.n) col = 0;                              // only feeds columns >= n of C
-            const long off = (long)(k0 + k) * it.ldb + col;
-            __builtin_amdgcn_global_load_lds(it.b_re + off, st + OFF_BR + (wave * BQ + q) * 256, 16, 0, 0);
-            __builtin_amdgcn_global_load_lds(it.b_im + off, st + OFF_BI + (wave * BQ + q) * 256, 16, 0, 0);
-        }
-        if (PRE && lane < 16) {  // every wave copies pre[k0..k0+31] (re then im) into its own 64 floats
-            const float* src = (lane < 8 ? it.pre_re : it.pre_im) + k0 + 4 * (lane & 7);
-            __builtin_amdgcn_global_load_lds(src, st + OFF_PRE + wave * 64, 16, 0, 0);
-        }
-    };
-    // direct loads issued per wave and slab (the pre copy is one instruction of 16 lanes)
+    constexpr int OFF_BR = BK * BM, OFF_BI = BK * (BM + BN), OFF_PRE = BK * (BM + 2 * BN);
+    constexpr int STAGE_E = OFF_PRE + 4 * 2 * BK;
+    static_assert(STAGE_E == LY::STAGE, "layout");
+    // direct loads issued per wave and slab (the pre copy is one instruction)
     constexpr int PER_SLAB = AQ + 2 * BQ + (PRE ? 1 : 0);
 
-    flt4 acc_re[MI][NJ], acc_im[MI][NJ];
+    V4 acc_re[MI][NJ], acc_im[MI][NJ];
 #pragma unroll
     for (int i = 0; i < MI; ++i)
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) acc_re[i][j] = acc_im[i][j] = flt4{0, 0, 0, 0};
+        for (int j = 0; j < NJ; ++j) acc_re[i][j] = acc_im[i][j] = V4{0, 0, 0, 0};
 
-    issue(0);
-    if (nslab > 1) issue(1);
+    rcd_issue<T, PRE, BM, BN, BK>(it, rcd_lds, 0, m0, n0, wave, lane);
+    if (nslab > 1) rcd_issue<T, PRE, BM, BN, BK>(it, rcd_lds, 1, m0, n0, wave, lane);
     for (int s = 0; s < nslab; ++s) {
         // wait for this wave's copies of slab s (slab s+1 may stay in flight), then for all
         // waves.  A plain s_barrier: __syncthreads() would also drain the copies in flight.
@@ -395,36 +413,38 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<f
             asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(PER_SLAB) : "memory");
         else
             asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
-        if (s + 2 < nslab) issue(s + 2);  // into the buffer every wave finished reading
-        const float* st = rcd_lds + (s % RCD_STAGES) * LY::STAGE;
-        const float* As = st;
-        const float* Br = st + LY::A;
-        const float* Bi = st + LY::A + LY::B;
-        const float* pw = st + LY::A + 2 * LY::B + wave * 64;
+        if (s + 2 < nslab) rcd_issue<T, PRE, BM, BN, BK>(it, rcd_lds, s + 2, m0, n0, wave, lane);  // freed buffer
+        const T* st = rcd_lds + (s % RCD_STAGES) * STAGE_E;
+        const T* As = st;
+        const T* Br = st + OFF_BR;
+        const T* Bi = st + OFF_BI;
+        const T* pw = st + OFF_PRE + wave * 2 * BK;
 #pragma unroll
-        for (int kg = 0; kg < RCD_BK; kg += 16) {
-            flt4 pr_[MI][NJ], pi_[MI][NJ];  // blocked summation: each 16 k in a fresh accumulator
+        for (int kg = 0; kg < BK; kg += 16) {
+            V4 pr_[BLOCKED ? MI : 1][BLOCKED ? NJ : 1], pi_[BLOCKED ? MI : 1][BLOCKED ? NJ : 1];
+            if constexpr (BLOCKED) {
 #pragma unroll
-            for (int i = 0; i < MI; ++i)
+                for (int i = 0; i < MI; ++i)
 #pragma unroll
-                for (int j = 0; j < NJ; ++j) pr_[i][j] = pi_[i][j] = flt4{0, 0, 0, 0};
+                    for (int j = 0; j < NJ; ++j) pr_[i][j] = pi_[i][j] = V4{0, 0, 0, 0};
+            }
 #pragma unroll
             for (int ks = 0; ks < 4; ++ks) {
                 const int kk = kg + ks * 4 + lk;
-                float a[MI], br[NJ], bi[NJ];
+                T a[MI], br[NJ], bi[NJ];
 #pragma unroll
-                for (int i = 0; i < MI; ++i) a[i] = As[kk * BM + rcd_swz(kk, wm + i * 16 + lr)];
-                float cr = 1.0f, ci = 0.0f;
+                for (int i = 0; i < MI; ++i) a[i] = As[kk * BM + rcd_swz<EPC>(kk, wm + i * 16 + lr)];
+                T cr = 1, ci = 0;
                 if (PRE) {
                     cr = pw[kk];
-                    ci = pw[32 + kk];
+                    ci = pw[BK + kk];
                 }
 #pragma unroll
                 for (int j = 0; j < NJ; ++j) {
-                    const int o = kk * BN + rcd_swz(kk, wn + j * 16 + lr);
-                    float vr = Br[o], vi = Bi[o];
+                    const int o = kk * BN + rcd_swz<EPC>(kk, wn + j * 16 + lr);
+                    T vr = Br[o], vi = Bi[o];
                     if (PRE) {
-                        const float t = vr * cr - vi * ci;
+                        const T t = vr * cr - vi * ci;
                         vi = vr * ci + vi * cr;
                         vr = t;
                     }
@@ -435,17 +455,21 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<f
                 for (int i = 0; i < MI; ++i)
 #pragma unroll
                     for (int j = 0; j < NJ; ++j) {
-                        pr_[i][j] = mfma16(a[i], br[j], pr_[i][j]);
-                        pi_[i][j] = mfma16(a[i], bi[j], pi_[i][j]);
+                        V4& xr = BLOCKED ? pr_[BLOCKED ? i : 0][BLOCKED ? j : 0] : acc_re[i][j];
+                        V4& xi = BLOCKED ? pi_[BLOCKED ? i : 0][BLOCKED ? j : 0] : acc_im[i][j];
+                        xr = mfma16(a[i], br[j], xr);
+                        xi = mfma16(a[i], bi[j], xi);
                     }
             }
+            if constexpr (BLOCKED) {
 #pragma unroll
-            for (int i = 0; i < MI; ++i)
+                for (int i = 0; i < MI; ++i)
 #pragma unroll
-                for (int j = 0; j < NJ; ++j) {
-                    acc_re[i][j] += pr_[i][j];
-                    acc_im[i][j] += pi_[i][j];
-                }
+                    for (int j = 0; j < NJ; ++j) {
+                        acc_re[i][j] += pr_[i][j];
+                        acc_im[i][j] += pi_[i][j];
+                    }
+            }
         }
     }
 #pragma unroll
@@ -454,14 +478,14 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<f
         for (int j = 0; j < NJ; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm + i * 16 + acc_row<float>(lk, r);
+                const int row = m0 + wm + i * 16 + acc_row<T>(lk, r);
                 const int col = n0 + wn + j * 16 + lr;
                 if (row < it.m && col < it.n) {
-                    float vr = acc_re[i][j][r], vi = acc_im[i][j][r];
+                    T vr = acc_re[i][j][r], vi = acc_im[i][j][r];
                     if (POST != SCALE_NONE) {
                         const int di = POST == SCALE_ROW ? row : col;
-                        const float pr = it.post_re[di], pi = it.post_im[di];
-                        const float t = vr * pr - vi * pi;
+                        const T pr = it.post_re[di], pi = it.post_im[di];
+                        const T t = vr * pr - vi * pi;
                         vi = vr * pi + vi * pr;
                         vr = t;
                     }
@@ -475,14 +499,14 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<f
 // Tile shapes the launcher can pick (the plan's autotuner times them per step):
 //   0: 64x64, K slab 16   1: 32x32, K slab 32   2: 32x32, K slab 64   3: 64x64, K slab 32
 //   4: 96x96, K slab 32   5: 96x96, K slab 16   (96 = 288 / 3: no partial tiles at Nx = 288)
-//   6: 64x64 and 7: 96x96 direct-to-LDS (gemm_rcd_kernel): FP32, real symmetric A times a
-//      complex B only; other shapes/types report "does not fit"
+//   6: 64x64 and 7: 96x96 direct-to-LDS (gemm_rcd_kernel; K slab 32 in FP32, 16 in FP64):
+//      real symmetric A times a complex B only; other operand kinds report "does not fit"
 enum : int { N_TILES = 8 };
 
-template <int POST, bool PRE, int BM, int BN>
-inline bool launch_gemm_rcd(const GemmItem<float>* d_items, int nitems, int max_m, int max_n, hipStream_t s) {
-    const size_t lds = RcdLayout<BM, BN>::BYTES;
-    auto kern = gemm_rcd_kernel<POST, PRE, BM, BN>;
+template <typename T, int POST, bool PRE, int BM, int BN, int BK>
+inline bool launch_gemm_rcd(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s) {
+    const size_t lds = RcdLayout<T, BM, BN, BK>::BYTES;
+    auto kern = gemm_rcd_kernel<T, POST, PRE, BM, BN, BK>;
     ISLE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds));
     const int tm = (max_m + BM - 1) / BM, tn = (max_n + BN - 1) / BN;
@@ -511,17 +535,19 @@ inline bool launch_gemm_tile(const GemmItem<T>* d_items, int nitems, int max_m, 
     }
 }
 
-// direct tiles need: FP32, A real, square and symmetric, B complex, no A pre-scaling,
-// k % 32 == 0, every dimension and leading dimension a multiple of 4 and 16-byte aligned
-// operand / phase pointers (the caller vouches for all of it with `direct_ok`)
+// Direct tiles (6, 7) need: A real, square and symmetric, B complex, no A pre-scaling,
+// k a multiple of the K slab, every dimension and leading dimension a multiple of 4 and
+// 16-byte aligned operand / phase pointers (the caller vouches for all of it with
+// `direct_ok`).
 template <typename T, bool AC, bool BC, bool PRE, int POST>
 inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s,
                         int tile = 0, bool direct_ok = false) {
     if (tile >= 6) {
-        if constexpr (std::is_same<T, float>::value && !AC && BC && POST != SUBTRACT) {
+        if constexpr (!AC && BC && POST != SUBTRACT) {
             if (!direct_ok) return false;
-            return tile == 6 ? launch_gemm_rcd<POST, PRE, 64, 64>(d_items, nitems, max_m, max_n, s)
-                             : launch_gemm_rcd<POST, PRE, 96, 96>(d_items, nitems, max_m, max_n, s);
+            constexpr int BK = sizeof(T) == 4 ? 32 : 16;
+            return tile == 6 ? launch_gemm_rcd<T, POST, PRE, 64, 64, BK>(d_items, nitems, max_m, max_n, s)
+                             : launch_gemm_rcd<T, POST, PRE, 96, 96, BK>(d_items, nitems, max_m, max_n, s);
         } else {
             return false;
         }

@@ -306,13 +306,17 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
             hipLaunchKernelGGL(phase_kernel, dim3(ew_blocks((long)n * op.nt), st.count), dim3(EW_THREADS), 0, s,
                                (const PhaseItem*)st.items, phi, op.phi_stride);
             break;
+        // every real x complex chain step multiplies a symmetric slice matrix (A_s, E_p) into
+        // square n x n planes: the direct-to-LDS tiles apply when n % 32 == 0
         case K::GEMM64_RC_PRE:
             if (!launch_gemm<double, false, true, true, SCALE_NONE>((const GemmItem<double>*)st.items, st.count,
-                                                               st.max_m, st.max_n, s, st.tile)) return false;
+                                                               st.max_m, st.max_n, s, st.tile, n % 32 == 0))
+                return false;
             break;
         case K::GEMM64_RC_ROW:
             if (!launch_gemm<double, false, true, false, SCALE_ROW>((const GemmItem<double>*)st.items, st.count,
-                                                               st.max_m, st.max_n, s, st.tile)) return false;
+                                                               st.max_m, st.max_n, s, st.tile, n % 32 == 0))
+                return false;
             break;
         case K::GEMM64_CR_PRE:
             if (!launch_gemm<double, true, false, true, SCALE_NONE>((const GemmItem<double>*)st.items, st.count,
@@ -322,8 +326,6 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
             if (!launch_gemm<double, true, true, false, SCALE_NONE>((const GemmItem<double>*)st.items, st.count,
                                                                st.max_m, st.max_n, s, st.tile)) return false;
             break;
-        // the FP32 chain steps multiply the symmetric slice matrix A_s (exp(-kappa~ +- mu~) or
-        // K_s) into square n x n planes: the direct-to-LDS tiles apply when n % 32 == 0
         case K::GEMM32_RC_ROW:
             if (!launch_gemm<float, false, true, false, SCALE_ROW>((const GemmItem<float>*)st.items, st.count, st.max_m,
                                                               st.max_n, s, st.tile, n % 32 == 0)) return false;
@@ -357,6 +359,10 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
         case K::DIAG32:
             hipLaunchKernelGGL(diag_prod_kernel<float>, dim3(cdiv(n, 16), st.count), dim3(256), 0, s,
                                (const DiagItem<float>*)st.items);
+            break;
+        case K::DIAG64X32:
+            hipLaunchKernelGGL((diag_prod_kernel<double, float>), dim3(cdiv(n, 16), st.count), dim3(256), 0, s,
+                               (const DiagItem<double, float>*)st.items);
             break;
         case K::COPY64:
             hipLaunchKernelGGL((copy_items_kernel<double, double>), dim3(ew_blocks(st.max_m), st.count),
@@ -482,9 +488,11 @@ OpCache& Plan::get_force_plain(int nt, int batch, int nspecies, int spin) {
 
 // ---------------------------------------------------------------------------
 // Mixed-precision DIRECT_SINGLE force.  Slices are split into ng = nt/g groups
-// [s_j, e_j].  Per group (FP32, MFMA, all groups/species/fields in one launch):
+// [s_j, e_j].  Per group (FP64, MFMA, all groups/species/fields in one launch):
 //   Suf_{e_j} = X_{e_j},  Suf_t = X_t Suf_{t+1}  (Suf_{s_j} = P_j, the group product)
-// Top level (FP64, group products promoted exactly):
+// (FP32 here would put its rounding into P_j, which the top level amplifies by the
+// conditioning of 1 + A^{-1}: 1e-4 relative force error near zeros of det M at Nx=288.)
+// Top level (FP64):
 //   GL_{ng-1} = I, GL_j = P_{j+1} GL_{j+1};  Mid = (1 + P_0 GL_0)^{-1}
 //   RR_0 = Mid, RR_j = RR_{j-1} P_{j-1};     K_j = GL_j RR_j
 // Per group again (FP32):  Rt_{s_j} = K_j X_{s_j}, Rt_tau = Rt_{tau-1} X_tau, carried
@@ -509,9 +517,9 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
     const int nb = batch * nspecies;
     const int ng = nt / g;
     float* ph = b.d32((size_t)nb * 2 * nt * n);
-    float* Suf = b.d32((size_t)nb * nt * 2 * nn);
+    double* ph64 = b.d64((size_t)nb * 2 * nt * n);
+    double* Suf = b.d64((size_t)nb * nt * 2 * nn);
     float* Rt = b.d32((size_t)nb * nt * 2 * nn);
-    double* P64 = b.d64((size_t)nb * ng * 2 * nn);
     double* GL = b.d64((size_t)nb * ng * 2 * nn);
     double* RR = b.d64((size_t)nb * ng * 2 * nn);
     double* KK = b.d64((size_t)nb * ng * 2 * nn);
@@ -521,57 +529,60 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
     op->parts = b.d64((size_t)nb * nt * n * 2);
     auto D = [&](int item, int t) { return CMat<float>{ph + (long)item * 2 * nt * n + (long)t * n,
                                                        ph + (long)item * 2 * nt * n + (long)nt * n + (long)t * n}; };
+    auto D64 = [&](int item, int t) {
+        return CMat<double>{ph64 + (long)item * 2 * nt * n + (long)t * n,
+                            ph64 + (long)item * 2 * nt * n + (long)nt * n + (long)t * n};
+    };
     auto mS = [&](int item, int t) { return cmat(Suf, (long)item * nt + t, nn); };
     auto mRt = [&](int item, int t) { return cmat(Rt, (long)item * nt + t, nn); };
     auto m64 = [&](double* base, int item, int j) { return cmat(base, (long)item * ng + j, nn); };
+    auto mP = [&](int item, int j) { return mS(item, j * g); };  // group product P_j = Suf_{s_j}
     auto mK32 = [&](int item, int j) { return cmat(K32, (long)item * ng + j, nn); };
     auto spec = [&](int item) { return item % nspecies; };
     const float* A32[2] = {a32_[0]->as<float>(), a32_[1]->as<float>()};
+    const double* A64[2] = {a64_[0]->as<double>(), a64_[1]->as<double>()};
 
     std::vector<PhaseItem> phs;
     for (int item = 0; item < nb; ++item)
-        phs.push_back({item / nspecies, spin, sigma_of(spec(item)), nullptr, nullptr, D(item, 0).re, D(item, 0).im, n, nt});
+        phs.push_back({item / nspecies, spin, sigma_of(spec(item)), D64(item, 0).re, D64(item, 0).im, D(item, 0).re,
+                       D(item, 0).im, n, nt});
     b.add(K::PHASE, phs);
+    // ---- local suffix chains in FP64: their first elements are the group products P_j,
+    //      whose rounding the top level would amplify by the conditioning of 1 + A^{-1}
     {
-        std::vector<ScaleItem<float>> sc;
+        std::vector<ScaleItem<double>> sc;
         for (int item = 0; item < nb; ++item)
             for (int j = 0; j < ng; ++j) {
                 const int e = j * g + g - 1;
-                sc.push_back({A32[spec(item)], D(item, e).re, D(item, e).im, mS(item, e).re, mS(item, e).im, n,
+                sc.push_back({A64[spec(item)], D64(item, e).re, D64(item, e).im, mS(item, e).re, mS(item, e).im, n,
                               SCALE_ROW});
             }
-        b.add(K::SCALE32, sc);
+        b.add(K::SCALE64, sc);
     }
     for (int q = 1; q < g; ++q) {
-        std::vector<GemmItem<float>> gi;
+        std::vector<GemmItem<double>> gi;
         for (int item = 0; item < nb; ++item)
             for (int j = 0; j < ng; ++j) {
                 const int t = j * g + g - 1 - q;
-                const CMat<float> src = mS(item, t + 1);
-                gi.push_back(gitem<float>(A32[spec(item)], nullptr, src.re, src.im, mS(item, t), nullptr, nullptr,
-                                          D(item, t).re, D(item, t).im, n));
+                const CMat<double> src = mS(item, t + 1);
+                gi.push_back(gitem<double>(A64[spec(item)], nullptr, src.re, src.im, mS(item, t), nullptr, nullptr,
+                                           D64(item, t).re, D64(item, t).im, n));
             }
-        b.add(K::GEMM32_RC_ROW, gi, n, n);
-    }
-    {
-        std::vector<CopyItem<float, double>> cv;
-        for (int item = 0; item < nb; ++item)
-            for (int j = 0; j < ng; ++j) cv.push_back({mS(item, j * g).re, m64(P64, item, j).re, 2 * nn});
-        b.add(K::CVT32TO64, cv, (int)(2 * nn));
+        b.add(K::GEMM64_RC_ROW, gi, n, n);
     }
     // ---- top level (FP64)
     if (ng == 1) {
         std::vector<CopyItem<double, double>> cp;
-        for (int item = 0; item < nb; ++item) cp.push_back({m64(P64, item, 0).re, LUb + (long)item * 2 * nn, 2 * nn});
+        for (int item = 0; item < nb; ++item) cp.push_back({mP(item, 0).re, LUb + (long)item * 2 * nn, 2 * nn});
         b.add(K::COPY64, cp, (int)(2 * nn));
     } else {
         std::vector<CopyItem<double, double>> cp;
-        for (int item = 0; item < nb; ++item) cp.push_back({m64(P64, item, ng - 1).re, m64(GL, item, ng - 2).re, 2 * nn});
+        for (int item = 0; item < nb; ++item) cp.push_back({mP(item, ng - 1).re, m64(GL, item, ng - 2).re, 2 * nn});
         b.add(K::COPY64, cp, (int)(2 * nn));
         for (int j = ng - 3; j >= 0; --j) {
             std::vector<GemmItem<double>> gi;
             for (int item = 0; item < nb; ++item) {
-                const CMat<double> a = m64(P64, item, j + 1), bb = m64(GL, item, j + 1);
+                const CMat<double> a = mP(item, j + 1), bb = m64(GL, item, j + 1);
                 gi.push_back(gitem<double>(a.re, a.im, bb.re, bb.im, m64(GL, item, j), nullptr, nullptr, nullptr,
                                            nullptr, n));
             }
@@ -579,7 +590,7 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
         }
         std::vector<GemmItem<double>> gi;
         for (int item = 0; item < nb; ++item) {
-            const CMat<double> a = m64(P64, item, 0), bb = m64(GL, item, 0);
+            const CMat<double> a = mP(item, 0), bb = m64(GL, item, 0);
             gi.push_back(gitem<double>(a.re, a.im, bb.re, bb.im, cmat(LUb, item, nn), nullptr, nullptr, nullptr,
                                        nullptr, n));
         }
@@ -613,7 +624,7 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
     for (int j = 1; j < ng; ++j) {
         std::vector<GemmItem<double>> gi;
         for (int item = 0; item < nb; ++item) {
-            const CMat<double> a = m64(RR, item, j - 1), bb = m64(P64, item, j - 1);
+            const CMat<double> a = m64(RR, item, j - 1), bb = mP(item, j - 1);
             gi.push_back(gitem<double>(a.re, a.im, bb.re, bb.im, m64(RR, item, j), nullptr, nullptr, nullptr, nullptr, n));
         }
         b.add(K::GEMM64_CC, gi, n, n);
@@ -650,15 +661,15 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
         b.add(K::GEMM32_RC_PRE, gi, n, n);
     }
     {
-        std::vector<DiagItem<float>> dg;
+        std::vector<DiagItem<double, float>> dg;
         for (int item = 0; item < nb; ++item)
             for (int tau = 0; tau < nt; ++tau) {
                 const bool last = (tau % g) == g - 1;
-                const CMat<float> l = last ? CMat<float>{nullptr, nullptr} : mS(item, tau + 1);
+                const CMat<double> l = last ? CMat<double>{nullptr, nullptr} : mS(item, tau + 1);
                 const CMat<float> r = mRt(item, tau);
                 dg.push_back({l.re, l.im, r.re, r.im, op->parts + ((long)item * nt + tau) * 2 * n, n, 1});
             }
-        b.add(K::DIAG32, dg);
+        b.add(K::DIAG64X32, dg);
     }
     b.finish();
     tune(*op);

@@ -51,6 +51,7 @@ enum class K : int {
     COPY64, CVT32TO64, CVT64TO32,
     PHASE,              // phase vectors from the runtime phi
     TCVT64TO32,         // transposed copy with conversion
+    DIAG64X32,          // diag(L R), FP64 left factor, FP32 right factor
 };
 
 struct Step {
