@@ -312,9 +312,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* _
 // chunk, since a wave's direct load writes 64 consecutive 16-byte chunks.
 // Requirements (the caller vouches): k % BK == 0; m, n, lda, ldb, ldc multiples of 4;
 // 16-byte aligned operands and phase vectors.
-constexpr int RCD_STAGES = 3;
-
-template <typename T, int BM, int BN, int BK>
+template <typename T, int BM, int BN, int BK, int RCD_STAGES = 3>
 struct RcdLayout {
     static constexpr int EPC = 16 / sizeof(T);   // elements per 16-byte chunk
     static constexpr int A = BK * BM;            // elements
@@ -331,7 +329,7 @@ __device__ __forceinline__ int rcd_swz(int k, int m) {
 
 // Issues this wave's direct copies of slab `slab` (A rows, B re/im rows, and with PRE the
 // phase factors) into stage slab % RCD_STAGES of the LDS ring.
-template <typename T, bool PRE, int BM, int BN, int BK>
+template <typename T, bool PRE, int BM, int BN, int BK, int RCD_STAGES>
 __device__ __forceinline__ void rcd_issue(const GemmItem<T>& it, T* lds, int slab, int m0, int n0, int wave,
                                           int lane) {
     constexpr int EPC = 16 / sizeof(T);
@@ -366,16 +364,17 @@ __device__ __forceinline__ void rcd_issue(const GemmItem<T>& it, T* lds, int sla
     }
 }
 
-template <typename T, int POST, bool PRE, int BM, int BN, int BK>
+template <typename T, int POST, bool PRE, int BM, int BN, int BK, int RCD_STAGES>
 __global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items,
                                                                 int tiles_n_max) {
-    typedef RcdLayout<T, BM, BN, BK> LY;
+    typedef RcdLayout<T, BM, BN, BK, RCD_STAGES> LY;
     constexpr int EPC = 16 / sizeof(T);
     constexpr int MI = BM / 32, NJ = BN / 32;
     constexpr int ACH = BK * BM / EPC, BCH = BK * BN / EPC;  // 16-byte chunks per slab
     static_assert(ACH % 256 == 0 && BCH % 256 == 0, "slab chunks must cover whole waves");
     static_assert((BM / EPC) % 8 == 0 && (BN / EPC) % 8 == 0, "swizzle needs multiples of 8 chunks per row");
     static_assert(2 * BK / EPC <= 64 && BK % 16 == 0, "pre copy must fit one wave instruction");
+    static_assert(RCD_STAGES >= 3 && RCD_STAGES <= 4, "two or three slabs in flight");
     constexpr int AQ = ACH / 256, BQ = BCH / 256;  // direct loads per wave per plane
     constexpr bool BLOCKED = sizeof(T) == 4;       // FP32: each 16 k summed in a fresh accumulator
     typedef typename Acc4<T>::type V4;
@@ -404,16 +403,22 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<T
 #pragma unroll
         for (int j = 0; j < NJ; ++j) acc_re[i][j] = acc_im[i][j] = V4{0, 0, 0, 0};
 
-    rcd_issue<T, PRE, BM, BN, BK>(it, rcd_lds, 0, m0, n0, wave, lane);
-    if (nslab > 1) rcd_issue<T, PRE, BM, BN, BK>(it, rcd_lds, 1, m0, n0, wave, lane);
+    // slabs are issued RCD_STAGES-1 ahead of the one being computed
+    for (int j = 0; j < RCD_STAGES - 1 && j < nslab; ++j)
+        rcd_issue<T, PRE, BM, BN, BK, RCD_STAGES>(it, rcd_lds, j, m0, n0, wave, lane);
     for (int s = 0; s < nslab; ++s) {
-        // wait for this wave's copies of slab s (slab s+1 may stay in flight), then for all
-        // waves.  A plain s_barrier: __syncthreads() would also drain the copies in flight.
-        if (s + 1 < nslab)
+        // wait for this wave's copies of slab s (the later slabs already issued stay in
+        // flight), then for all waves.  A plain s_barrier: __syncthreads() would also drain
+        // the copies in flight.
+        const int ahead = min(RCD_STAGES - 2, nslab - 1 - s);
+        if (ahead >= 2)
+            asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(2 * PER_SLAB) : "memory");
+        else if (ahead == 1)
             asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(PER_SLAB) : "memory");
         else
             asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
-        if (s + 2 < nslab) rcd_issue<T, PRE, BM, BN, BK>(it, rcd_lds, s + 2, m0, n0, wave, lane);  // freed buffer
+        if (s + RCD_STAGES - 1 < nslab)  // into the buffer every wave finished reading
+            rcd_issue<T, PRE, BM, BN, BK, RCD_STAGES>(it, rcd_lds, s + RCD_STAGES - 1, m0, n0, wave, lane);
         const T* st = rcd_lds + (s % RCD_STAGES) * STAGE_E;
         const T* As = st;
         const T* Br = st + OFF_BR;
@@ -499,14 +504,16 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<T
 // Tile shapes the launcher can pick (the plan's autotuner times them per step):
 //   0: 64x64, K slab 16   1: 32x32, K slab 32   2: 32x32, K slab 64   3: 64x64, K slab 32
 //   4: 96x96, K slab 32   5: 96x96, K slab 16   (96 = 288 / 3: no partial tiles at Nx = 288)
-//   6: 64x64 and 7: 96x96 direct-to-LDS (gemm_rcd_kernel; K slab 32 in FP32, 16 in FP64):
-//      real symmetric A times a complex B only; other operand kinds report "does not fit"
-enum : int { N_TILES = 8 };
+//   direct-to-LDS (gemm_rcd_kernel; real symmetric A times a complex B only, other operand
+//   kinds report "does not fit"), K slab S = 32 in FP32, 16 in FP64:
+//   6: 64x64 slab S   7: 96x96 slab S   8: 32x32 slab S   9: 64x64 slab 2S   10: 64x64 slab S, 4 stages
+enum : int { N_TILES = 11 };
 
-template <typename T, int POST, bool PRE, int BM, int BN, int BK>
+template <typename T, int POST, bool PRE, int BM, int BN, int BK, int ST = 3>
 inline bool launch_gemm_rcd(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s) {
-    const size_t lds = RcdLayout<T, BM, BN, BK>::BYTES;
-    auto kern = gemm_rcd_kernel<T, POST, PRE, BM, BN, BK>;
+    const size_t lds = RcdLayout<T, BM, BN, BK, ST>::BYTES;
+    if (lds > 160 * 1024) return false;
+    auto kern = gemm_rcd_kernel<T, POST, PRE, BM, BN, BK, ST>;
     ISLE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds));
     const int tm = (max_m + BM - 1) / BM, tn = (max_n + BN - 1) / BN;
@@ -535,19 +542,26 @@ inline bool launch_gemm_tile(const GemmItem<T>* d_items, int nitems, int max_m, 
     }
 }
 
-// Direct tiles (6, 7) need: A real, square and symmetric, B complex, no A pre-scaling,
-// k a multiple of the K slab, every dimension and leading dimension a multiple of 4 and
-// 16-byte aligned operand / phase pointers (the caller vouches for all of it with
-// `direct_ok`).
+// Direct tiles (6..10) need: A real, square and symmetric, B complex, no A pre-scaling,
+// every dimension and leading dimension a multiple of 4 and 16-byte aligned operand /
+// phase pointers (the caller vouches for all of it by passing direct_k, the K dimension
+// shared by every item, or 0 when not eligible); K must be a multiple of the tile's K slab,
+// which is checked here.
 template <typename T, bool AC, bool BC, bool PRE, int POST>
 inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s,
-                        int tile = 0, bool direct_ok = false) {
+                        int tile = 0, int direct_k = 0) {
     if (tile >= 6) {
         if constexpr (!AC && BC && POST != SUBTRACT) {
-            if (!direct_ok) return false;
             constexpr int BK = sizeof(T) == 4 ? 32 : 16;
-            return tile == 6 ? launch_gemm_rcd<T, POST, PRE, 64, 64, BK>(d_items, nitems, max_m, max_n, s)
-                             : launch_gemm_rcd<T, POST, PRE, 96, 96, BK>(d_items, nitems, max_m, max_n, s);
+            const int slab = tile == 9 ? 2 * BK : BK;
+            if (direct_k <= 0 || direct_k % slab != 0) return false;
+            switch (tile) {
+            case 6: return launch_gemm_rcd<T, POST, PRE, 64, 64, BK>(d_items, nitems, max_m, max_n, s);
+            case 7: return launch_gemm_rcd<T, POST, PRE, 96, 96, BK>(d_items, nitems, max_m, max_n, s);
+            case 8: return launch_gemm_rcd<T, POST, PRE, 32, 32, BK>(d_items, nitems, max_m, max_n, s);
+            case 9: return launch_gemm_rcd<T, POST, PRE, 64, 64, 2 * BK>(d_items, nitems, max_m, max_n, s);
+            default: return launch_gemm_rcd<T, POST, PRE, 64, 64, BK, 4>(d_items, nitems, max_m, max_n, s);
+            }
         } else {
             return false;
         }

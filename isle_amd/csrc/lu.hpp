@@ -74,89 +74,22 @@ __device__ inline double block_sum(double v, double* s) {
     return t;
 }
 
-// Panel LDS used by the column steps.
-template <int NB>
-struct PanelShared {
-    unsigned long long* pkey;  // [NB] pivot keys, one per column
-    double (*xrow)[2 * NB];
-    int* xorig;
-};
-
 // Pivot key: the bits of |re|+|im| (a non-negative double, so its bit pattern orders like
-// its value) with the low 9 mantissa bits replaced by 511 - row.  The maximal key is the
-// largest candidate, ties (and values equal to 2^-43 relative) going to the first row,
-// as izamax.  Rows must be < 512.
-__device__ __forceinline__ unsigned long long pivot_key(double v, int row) {
+// its value) with the low 9 mantissa bits replaced by 511 - position.  The maximal key is
+// the largest candidate, ties (and values equal to 2^-43 relative) going to the first
+// position, as izamax.  Positions must be < 512.
+__device__ __forceinline__ unsigned long long pivot_key(double v, int pos) {
     return (static_cast<unsigned long long>(__double_as_longlong(v)) & ~0x1FFull) |
-           static_cast<unsigned long long>(511 - row);
-}
-
-// Column KK of the panel factorisation (then KK+1, ...): pivot search, row exchange
-// through LDS, scaling and rank-1 update of this thread's row.  Templated on the
-// column so every register index is static.
-template <int NB, int KK>
-__device__ __forceinline__ void panel_column(const PanelShared<NB>& sh, double (&ar)[NB], double (&ai)[NB], int& orig,
-                                             int kb, bool own, int tid, int* piv, int k0) {
-    if (KK < kb) {
-        if (own && tid >= KK) atomicMax(&sh.pkey[KK], pivot_key(cabs1(ar[KK], ai[KK]), tid));
-        __syncthreads();
-        const int p = 511 - static_cast<int>(sh.pkey[KK] & 0x1FFull);
-        if (tid == p) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) {
-                sh.xrow[0][2 * c] = ar[c];
-                sh.xrow[0][2 * c + 1] = ai[c];
-            }
-            sh.xorig[0] = orig;
-        }
-        if (tid == KK && p != KK) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) {
-                sh.xrow[1][2 * c] = ar[c];
-                sh.xrow[1][2 * c + 1] = ai[c];
-            }
-            sh.xorig[1] = orig;
-        }
-        __syncthreads();
-        if (p != KK && (tid == p || tid == KK)) {
-            const int src = tid == p ? 1 : 0;
-#pragma unroll
-            for (int c = 0; c < NB; ++c) {
-                ar[c] = sh.xrow[src][2 * c];
-                ai[c] = sh.xrow[src][2 * c + 1];
-            }
-            orig = sh.xorig[src];
-        }
-        if (tid == 0) piv[KK] = k0 + p;
-        if (own && tid > KK) {
-            const double ur = sh.xrow[0][2 * KK], ui = sh.xrow[0][2 * KK + 1];
-            double lr = ar[KK], li = ai[KK];
-            if (ur != 0.0 || ui != 0.0) {
-                const double den = ur * ur + ui * ui;
-                const double ir = ur / den, ii = -ui / den;
-                const double t = lr * ir - li * ii;
-                li = lr * ii + li * ir;
-                lr = t;
-            }
-            ar[KK] = lr;
-            ai[KK] = li;
-#pragma unroll
-            for (int c = KK + 1; c < NB; ++c) {
-                const double br = sh.xrow[0][2 * c], bi = sh.xrow[0][2 * c + 1];
-                ar[c] -= lr * br - li * bi;
-                ai[c] -= lr * bi + li * br;
-            }
-        }
-    }
-    if constexpr (KK + 1 < NB) panel_column<NB, KK + 1>(sh, ar, ai, orig, kb, own, tid, piv, k0);
+           static_cast<unsigned long long>(511 - pos);
 }
 
 template <int NB>
 __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __restrict__ items, int k0,
                                                             int add_identity) {
     __shared__ unsigned long long pkey[NB];
-    __shared__ double xrow[2][2 * NB];  // [0]: the pivot row, [1]: the displaced row kk (re, im interleaved)
-    __shared__ int xorig[2];
+    __shared__ int pvt[NB];  // pivot position of each column
+    __shared__ double xrow[2 * NB];  // the pivot row (re, im interleaved); a column's writes and reads
+                                     // are separated by that column's barriers
     __shared__ double lt_re[NB][NB + 1], lt_im[NB][NB + 1];  // L11, inverted in place
     __shared__ double ut_re[NB][NB + 1], ut_im[NB][NB + 1];  // U11, inverted in place
     __shared__ double tl_re[NB][NB + 1], tl_im[NB][NB + 1];  // doubling temporaries
@@ -181,7 +114,7 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
     }
 
     double ar[NB], ai[NB];
-    int orig = tid;
+    int pos = tid;  // current position of this thread's original panel row
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
         double vr = 0, vi = 0;
@@ -195,40 +128,106 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
     }
 
     __syncthreads();
-    PanelShared<NB> sh{pkey, xrow, xorig};
-    panel_column<NB, 0>(sh, ar, ai, orig, kb, own, tid, it.piv + k0, k0);
+    // Column loop.  Rows never move between threads: thread t keeps original panel row t in
+    // registers and `pos` is its current position (LAPACK's interchange of rows KK and p is
+    // the exchange of two positions).  The register row is rotated by one slot per column:
+    // slot 0 always holds column KK, and the slot freed at the end takes the multiplier
+    // (rows still to be pivoted) or the departing column (rows already pivoted), so after
+    // NB rotations every row is back in column order, L multipliers then U entries.  This
+    // keeps all register indices static inside a real loop (a fully unrolled column loop
+    // does not fit the instruction cache).
+    for (int kk = 0; kk < NB; ++kk) {
+        double nr = ar[0], ni = ai[0];  // enters the last slot: the multiplier or the departing value
+        if (kk < kb) {
+            {  // max over the wave first (lanes to one LDS address serialise), then one atomic per wave
+                unsigned long long key = (own && pos >= kk) ? pivot_key(cabs1(ar[0], ai[0]), pos) : 0ull;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const unsigned long long o = __shfl_xor(key, off);
+                    key = o > key ? o : key;
+                }
+                if ((tid & 63) == 0 && key != 0ull) atomicMax(&pkey[kk], key);
+            }
+            __syncthreads();
+            const int p = 511 - static_cast<int>(pkey[kk] & 0x1FFull);  // pivot position
+            if (own && pos == p) {
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    xrow[2 * j] = ar[j];
+                    xrow[2 * j + 1] = ai[j];
+                }
+            }
+            if (own) {  // the interchange of positions kk and p
+                if (pos == p)
+                    pos = kk;
+                else if (pos == kk)
+                    pos = p;
+            }
+            if (tid == 0) pvt[kk] = p;  // global stores here would stall every LDS wait (FLAT)
+            __syncthreads();
+            if (own && pos > kk) {  // rows not yet pivoted: multiplier and rank-1 update
+                const double ur = xrow[0], ui = xrow[1];
+                if (ur != 0.0 || ui != 0.0) {
+                    const double rden = 1.0 / (ur * ur + ui * ui);
+                    const double ir = ur * rden, ii = -ui * rden;
+                    const double t = nr * ir - ni * ii;
+                    ni = nr * ii + ni * ir;
+                    nr = t;
+                }
+#pragma unroll
+                for (int j = 1; j < NB; ++j) {
+                    if (j < NB - kk) {  // slots NB-kk.. hold this row's earlier multipliers
+                        const double br = xrow[2 * j], bi = xrow[2 * j + 1];
+                        ar[j] -= nr * br - ni * bi;
+                        ai[j] -= nr * bi + ni * br;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NB - 1; ++j) {
+            ar[j] = ar[j + 1];
+            ai[j] = ai[j + 1];
+        }
+        ar[NB - 1] = nr;
+        ai[NB - 1] = ni;
+    }
+    if (tid < kb) it.piv[k0 + tid] = k0 + pvt[tid];
     if (own) {
 #pragma unroll
         for (int c = 0; c < NB; ++c) {
             if (c < kb) {
-                re[(long)(k0 + tid) * ld + k0 + c] = ar[c];
-                im[(long)(k0 + tid) * ld + k0 + c] = ai[c];
+                re[(long)(k0 + pos) * ld + k0 + c] = ar[c];
+                im[(long)(k0 + pos) * ld + k0 + c] = ai[c];
             }
         }
-        if (orig != tid) {  // at most 2*NB positions change: the panel rows and their partners
+        if (pos != tid) {  // at most 2*NB positions change: the panel rows and their partners
             const int sl = atomicAdd(&slot_cnt, 1);
             if (sl < 2 * NB) {
-                slot_dst[sl] = tid;
-                slot_src[sl] = orig;
+                slot_dst[sl] = pos;
+                slot_src[sl] = tid;
             }
         }
+        if (pos < NB) orig_of[pos] = tid;
     }
-    if (tid < NB) {
-        orig_of[tid] = orig;
-        const bool in = tid < kb;
+    // L11 and U11 (diagonal inverted) into LDS from the rows now at positions 0..kb-1;
+    // positions kb..NB-1 of a partial last panel are padded with the identity
+    if ((own && pos < NB) || (!own && tid < NB)) {
+        const int r = own ? pos : tid;
+        const bool in = own;
 #pragma unroll
         for (int c = 0; c < NB; ++c) {
-            lt_re[tid][c] = (in && c < tid) ? ar[c] : (c == tid ? 1.0 : 0.0);
-            lt_im[tid][c] = (in && c < tid) ? ai[c] : 0.0;
-            const bool u = in && c >= tid && c < kb;
-            double vr = u ? ar[c] : (c == tid ? 1.0 : 0.0), vi = u ? ai[c] : 0.0;
-            if (c == tid) {  // the diagonal of U11^{-1} is 1/u_ii
+            lt_re[r][c] = (in && c < r) ? ar[c] : (c == r ? 1.0 : 0.0);
+            lt_im[r][c] = (in && c < r) ? ai[c] : 0.0;
+            const bool u = in && c >= r && c < kb;
+            double vr = u ? ar[c] : (c == r ? 1.0 : 0.0), vi = u ? ai[c] : 0.0;
+            if (c == r) {  // the diagonal of U11^{-1} is 1/u_ii
                 const double den = vr * vr + vi * vi;
                 vr = vr / den;
                 vi = -vi / den;
             }
-            ut_re[tid][c] = vr;
-            ut_im[tid][c] = vi;
+            ut_re[r][c] = vr;
+            ut_im[r][c] = vi;
         }
     }
     __syncthreads();
@@ -302,26 +301,52 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
 
     const int nrt = n - k0 - NB;  // columns right of the panel
     if (nrt > 0) {
-        // staged A12 = the new rows k0..k0+NB-1 of the columns right of the panel
+        // One column per thread, all row reads of a column in flight at once:
+        //   staged A12 = the new rows k0..k0+NB-1 (positions q read original row orig_of[q]);
+        //   then the displaced rows below the panel take their new content, which is always an
+        //   original panel row (rows k0..k0+NB-1 are not written here, A12 goes to the stage).
+        // A column's reads precede its writes in the same thread, so no barrier is needed.
         double* st_re = lu_stage(it);
         double* st_im = st_re + (size_t)NB * n;
-        for (int e = tid; e < NB * nrt; e += nth) {
-            const int q = e / nrt, c = e - q * nrt;
-            const long src = (long)(k0 + orig_of[q]) * ld + k0 + NB + c;
-            st_re[(long)q * n + c] = re[src];
-            st_im[(long)q * n + c] = im[src];
-        }
-        __syncthreads();  // every staged read precedes the displaced-row writes below
-        // displaced rows below the panel take their new content (always an original panel
-        // row, which this kernel does not write right of the panel)
-        for (int e = tid; e < cnt * nrt; e += nth) {
-            const int q = e / nrt, c = e - q * nrt;
-            const int dst = slot_dst[q];
-            if (dst < NB) continue;
-            const long so = (long)(k0 + slot_src[q]) * ld + k0 + NB + c;
-            const long d = (long)(k0 + dst) * ld + k0 + NB + c;
-            re[d] = re[so];
-            im[d] = im[so];
+        constexpr int PG = 16;  // rows copied per batch (loads of a batch are all in flight)
+        for (int c = tid; c < nrt; c += nth) {
+            const long col = k0 + NB + c;
+#pragma unroll
+            for (int q0 = 0; q0 < NB; q0 += PG) {
+                double vr[PG], vi[PG];
+#pragma unroll
+                for (int j = 0; j < PG; ++j) {
+                    const long src = (long)(k0 + orig_of[q0 + j]) * ld + col;
+                    vr[j] = re[src];
+                    vi[j] = im[src];
+                }
+#pragma unroll
+                for (int j = 0; j < PG; ++j) {
+                    st_re[(long)(q0 + j) * n + c] = vr[j];
+                    st_im[(long)(q0 + j) * n + c] = vi[j];
+                }
+            }
+            for (int q0 = 0; q0 < cnt; q0 += PG) {
+                double wr[PG], wi[PG];
+#pragma unroll
+                for (int j = 0; j < PG; ++j) {
+                    const int q = q0 + j;
+                    if (q < cnt && slot_dst[q] >= NB) {
+                        const long so = (long)(k0 + slot_src[q]) * ld + col;
+                        wr[j] = re[so];
+                        wi[j] = im[so];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < PG; ++j) {
+                    const int q = q0 + j;
+                    if (q < cnt && slot_dst[q] >= NB) {
+                        const long d = (long)(k0 + slot_dst[q]) * ld + col;
+                        re[d] = wr[j];
+                        im[d] = wi[j];
+                    }
+                }
+            }
         }
     } else if (it.logdet) {
         // the item's last panel: every U_ii and pivot is final

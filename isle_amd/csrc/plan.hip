@@ -310,12 +310,12 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
         // square n x n planes: the direct-to-LDS tiles apply when n % 32 == 0
         case K::GEMM64_RC_PRE:
             if (!launch_gemm<double, false, true, true, SCALE_NONE>((const GemmItem<double>*)st.items, st.count,
-                                                               st.max_m, st.max_n, s, st.tile, n % 32 == 0))
+                                                               st.max_m, st.max_n, s, st.tile, n % 4 == 0 ? n : 0))
                 return false;
             break;
         case K::GEMM64_RC_ROW:
             if (!launch_gemm<double, false, true, false, SCALE_ROW>((const GemmItem<double>*)st.items, st.count,
-                                                               st.max_m, st.max_n, s, st.tile, n % 32 == 0))
+                                                               st.max_m, st.max_n, s, st.tile, n % 4 == 0 ? n : 0))
                 return false;
             break;
         case K::GEMM64_CR_PRE:
@@ -328,11 +328,11 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
             break;
         case K::GEMM32_RC_ROW:
             if (!launch_gemm<float, false, true, false, SCALE_ROW>((const GemmItem<float>*)st.items, st.count, st.max_m,
-                                                              st.max_n, s, st.tile, n % 32 == 0)) return false;
+                                                              st.max_n, s, st.tile, n % 4 == 0 ? n : 0)) return false;
             break;
         case K::GEMM32_RC_PRE:
             if (!launch_gemm<float, false, true, true, SCALE_NONE>((const GemmItem<float>*)st.items, st.count, st.max_m,
-                                                              st.max_n, s, st.tile, n % 32 == 0)) return false;
+                                                              st.max_n, s, st.tile, n % 4 == 0 ? n : 0)) return false;
             break;
         case K::TCVT64TO32:
             hipLaunchKernelGGL((transpose_items_kernel<double, float>), dim3(64, 2, st.count), dim3(256), 0, s,
