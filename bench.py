@@ -36,10 +36,52 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch the integration eagerly, not from a HIP graph")
+    ap.add_argument("--config4", action="store_true",
+                    help="the 256-chain ensemble of uneven lattices (L in {3,4,6,8}, Nt in {16,32,64}, beta=Nt/8, "
+                         "U=2, 20 steps): prints chain-trajectories/s instead of the config-3 line")
     ap.add_argument("--force-only", action="store_true",
                     help="profiling aid: run --steps batched mixed forces and one 1 GiB device copy "
                          "(the calibration kernel for HBM byte counters), print timings, no HMC")
     return ap.parse_args()
+
+
+def config4(args):
+    """256 independent chains, L drawn from {3,4,6,8} (Nx = 2L^2) and Nt from {16,32,64}
+    with a seeded RNG, beta = Nt/8, U = 2, mu = 0, EXP, mixed force, HMC length 1 with 20
+    steps; chain c starts from N(0, U beta/Nt) drawn from NumpyRNG(1000 + c).  One step =
+    one trajectory of every chain (integration of all buckets replayed from one HIP graph)."""
+    import torch
+    import isle_amd as isle
+    from isle_amd.ensemble import Chain, EnsembleLeapfrog
+    from isle_amd.evolver import NumpyRNG
+    pick = np.random.RandomState(4)
+    n_chains = 256
+    ls = pick.choice([3, 4, 6, 8], n_chains)
+    nts = pick.choice([16, 32, 64], n_chains)
+    lats = {l: isle.honeycomb(int(l)).hopping() for l in (3, 4, 6, 8)}
+    chains = [Chain(lats[int(l)], int(nt), nt / 8.0, 2.0, 0.0, -1, NumpyRNG(1000 + c))
+              for c, (l, nt) in enumerate(zip(ls, nts))]
+    ens = EnsembleLeapfrog(chains, 1.0, 20, precision="mixed", group=32, graph=not args.no_graph)
+    for _ in range(args.warmup):
+        ens.evolve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ens.evolve()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    pts = np.array(ens.trajPoints[args.warmup:], dtype=float)
+    dh = np.array(ens.deltaH[args.warmup:], dtype=float)
+    print(json.dumps({
+        "metric": "chain-trajectories/s, 256 uneven chains (config 4)", "value": n_chains * args.steps / dt,
+        "unit": "chain-traj/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 right chains + f64 suffix/top level/action", "data": "synthetic",
+        "config": {"workload": "256 honeycomb tori, L in {3,4,6,8}, Nt in {16,32,64}, beta=Nt/8, U=2, mu=0, EXP, "
+                               "particle-hole, sigmaKappa=-1, HMC length 1 with 20 steps",
+                   "buckets": len(ens.buckets), "parallelism": "buckets on separate streams, one HIP graph"},
+        "acceptance": float(pts.mean()) if pts.size else None,
+        "median_dH": float(np.median(dh)) if dh.size else None}))
 
 
 def force_only(args):
@@ -109,6 +151,9 @@ def main():
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     if args.force_only:
         force_only(args)
+        return
+    if args.config4:
+        config4(args)
         return
     import torch
     import isle_amd as isle

@@ -8,13 +8,15 @@ Drop-in names for the reference's ``isle`` / ``isle.action`` surface on this pat
     leapfrog, action.{Action, SumAction, HubbardGaugeAction, HubbardFermiAction,
     makeHubbardFermiAction, HFAHopping, HFABasis, HFAAlgorithm},
     evolver.{ConstStepLeapfrog, BinarySelector, EvolutionStage}, random.NumpyRNG,
-    and autograd.{action_value, logdetM_value} (the action and log det M as
-    torch.autograd functions whose backward is the force sweep).
+    autograd.{action_value, logdetM_value} (the action and log det M as
+    torch.autograd functions whose backward is the force sweep), and
+    ensemble.{Chain, EnsembleLeapfrog} (many chains of uneven Nx and Nt advanced
+    together, one HIP graph per trajectory).
 
 Every numerical routine runs in the hand-written HIP kernels of libisle_hfa.so; there
 is no CPU fallback.
 """
-from . import action, autograd, evolver
+from . import action, autograd, ensemble, evolver
 from .core import (CDVector, HubbardFermiMatrixDia, HubbardFermiMatrixExp, Species, Vector, as_field, invert,
                    logdet, logdetM, solveM, to_first_log_branch)
 from .integrator import leapfrog, leapfrog_batch
@@ -24,6 +26,7 @@ random = evolver  # isle.random.NumpyRNG lives in isle_amd.evolver
 
 __all__ = [
     "CDVector", "HubbardFermiMatrixDia", "HubbardFermiMatrixExp", "Lattice", "Species", "Vector", "action", "autograd",
+    "ensemble",
     "as_field", "evolver", "honeycomb", "invert", "isBipartite", "leapfrog", "leapfrog_batch", "loadLattice",
     "logdet", "logdetM", "random", "solveM", "to_first_log_branch",
 ]

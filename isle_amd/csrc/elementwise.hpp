@@ -222,10 +222,11 @@ __global__ void combine_force_kernel(const double* __restrict__ fp, const double
 // ---------------------------------------------------------------------------
 // Per-vector sums of phi (complex) and of phi*phi (bilinear), one workgroup per vector.
 // out[4*b + 0..3] = (Re sum phi, Im sum phi, Re sum phi^2, Im sum phi^2)
+// Field b starts `stride` complex elements (2*stride doubles) after field b-1.
 __global__ __launch_bounds__(256) void sums_kernel(const double* __restrict__ phi, long n, long stride,
                                                    double* __restrict__ out) {
     __shared__ double s[4][256];
-    const double* p = phi + blockIdx.x * stride;
+    const double* p = phi + 2 * blockIdx.x * stride;
     double a = 0, b = 0, c = 0, d = 0;
     for (long e = threadIdx.x; e < n; e += blockDim.x) {
         const double r = p[2 * e], i = p[2 * e + 1];

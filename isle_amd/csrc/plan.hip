@@ -840,7 +840,7 @@ void Plan::logdetM(const double* phi, int nt, int batch, int mode, int group, in
     run(op, phi);
     // sums of phi per field
     DevBuf sums((size_t)batch * 4 * 8);
-    hipLaunchKernelGGL(sums_kernel, dim3(batch), dim3(256), 0, stream_, phi, (long)nx_ * nt, 2L * nx_ * nt,
+    hipLaunchKernelGGL(sums_kernel, dim3(batch), dim3(256), 0, stream_, phi, (long)nx_ * nt, (long)nx_ * nt,
                        sums.as<double>());
     ISLE_HIP(hipGetLastError());
     std::vector<double> ld((size_t)batch * nspecies * 2), sm((size_t)batch * 4);
