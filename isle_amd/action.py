@@ -11,6 +11,10 @@ Besides the reference's single-field ``eval``/``force``, HubbardFermiAction offe
 * ``"fp64"``  - the reference DIRECT_SINGLE algorithm, every product in FP64;
 * ``"mixed"`` - force from FP32 chains over groups of ``group`` time slices on MFMA,
   combined in FP64; eval (the Metropolis action) from FP64 group products.
+* ``"stable"`` - for ill-conditioned products (large beta, spin basis, mu != 0, where the
+  product of all time slices spans more decades than FP64 holds): eval and force from FP64
+  group products combined through the structured pivoted elimination of their block-cyclic
+  matrix, never through the product of all groups.
 """
 import enum
 
@@ -153,8 +157,8 @@ class HubbardFermiAction(Action):
                  allowShortcut=False, precision="fp64", group=32):
         if HFAAlgorithm(algorithm) != HFAAlgorithm.DIRECT_SINGLE:
             raise NotImplementedError("HFAAlgorithm.DIRECT_SQUARE is not implemented on the GPU")
-        if precision not in ("fp64", "mixed"):
-            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if precision not in ("fp64", "mixed", "stable"):
+            raise ValueError("precision must be 'fp64', 'mixed' or 'stable'")
         kappa = kappaTilde.hopping() if isinstance(kappaTilde, Lattice) else np.asarray(kappaTilde, dtype=np.float64)
         self.hopping = HFAHopping(hopping)
         self.basis = HFABasis(basis)
@@ -176,15 +180,16 @@ class HubbardFermiAction(Action):
                                                                          int(sigmaKappa))
 
     def _mode(self):
-        return 0 if self.precision == "fp64" else 1
+        return {"fp64": 0, "mixed": 1, "stable": 2}[self.precision]
 
     def _group(self, nt):
-        """Slices per group of the mixed force: the largest divisor of nt not above
-        ``group`` whose FP32 right chain keeps the slice-conditioning bound
-        g * spread <= 12 (cond <= e^12 over one group)."""
+        """Slices per group: the largest divisor of nt not above ``group`` that keeps the
+        slice-conditioning bound g * spread <= 12 (cond <= e^12 over one group, for the
+        FP32 right chains of the mixed force) or <= 24 (FP64 chains of "stable")."""
         g = min(self.group, nt)
+        bound = 24.0 if self.precision == "stable" else 12.0
         if self._slice_spread > 0:
-            g = max(1, min(g, int(12.0 / self._slice_spread)))
+            g = max(1, min(g, int(bound / self._slice_spread)))
         while nt % g:
             g -= 1
         return g
@@ -262,7 +267,7 @@ def makeHubbardFermiAction(lat_or_kappa, *args, **kwargs):
                            algorithm=DIRECT_SINGLE, allowShortcut=False)
     makeHubbardFermiAction(lat, beta, muTilde, sigmaKappa, hopping=DIA, ...)  with
     kappaTilde = lat.hopping() * beta / lat.nt() (bind_action.cpp:206-221).
-    Extra keyword arguments: precision ('fp64' | 'mixed'), group (slices per FP32 group).
+    Extra keyword arguments: precision ('fp64' | 'mixed' | 'stable'), group (slices per group).
     """
     if isinstance(lat_or_kappa, Lattice):
         beta, mu, sk, *rest = args

@@ -18,6 +18,14 @@
 //   lu_update_kernel (32x32 tiles, every CU): U12 = L11^{-1} A12 and
 //                     A22 -= L21 U12 on FP64 MFMA, plus the panel's interchanges on the
 //                     columns left of it.
+//
+// Besides square matrices the same kernels factorise the first kf columns of an m x ncols
+// matrix whose rows [k0+NB, nt) are, in the columns not yet factorised, known to be rows of
+// the identity (a block row [I  X  Y] stacked on a general block row, as in one step of the
+// structured elimination of a block-cyclic matrix): such rows can neither be pivots nor
+// receive an update (their multipliers are zero), so the candidates of the panel at k0 are
+// its own rows k0..k0+NB-1 and the rows nt..m-1.  Panel positions p index those candidates,
+// lu_row() maps them to matrix rows.
 #pragma once
 
 #include "common.hpp"
@@ -31,14 +39,27 @@ constexpr int LU_TILE = 32;        // trailing-update tile
 constexpr int LU_SWAP_COLS = 64;   // columns per left-interchange workgroup
 
 struct LuItem {
-    double* re;      // n x n, row-major, row stride ld (factorised in place)
+    double* re;      // n x n (or m x ncols), row-major, row stride ld (factorised in place)
     double* im;
-    int* piv;        // length n (0-based row index swapped with row k)
-    double* logdet;  // 2 doubles (re, im) or null
-    double* work;    // lu_work_doubles(n) doubles of scratch
-    int n;
+    int* piv;        // length kf (0-based row index swapped with row k)
+    double* logdet;  // 2 doubles (re, im) or null: sum over the kf factorised columns
+    double* work;    // lu_work_doubles(kf, ncols) doubles of scratch
+    int n;           // order of a square matrix (m = ncols = kf = n when the fields below are 0)
     int ld;
+    int m = 0, ncols = 0, kf = 0;  // rows, columns, columns to factorise (kf % LU_NB == 0 if nt > 0)
+    int nt = 0;      // > 0: rows [k0 + LU_NB, nt) are identity rows, never candidates (see above)
 };
+__host__ __device__ inline int lu_m(const LuItem& it) { return it.m ? it.m : it.n; }
+__host__ __device__ inline int lu_ncols(const LuItem& it) { return it.ncols ? it.ncols : it.n; }
+__host__ __device__ inline int lu_kf(const LuItem& it) { return it.kf ? it.kf : it.n; }
+// Candidate rows of the panel at k0 and the matrix row of panel position p.
+__host__ __device__ inline bool lu_structured(int k0, int nt) { return nt > 0 && k0 + LU_NB < nt; }
+__host__ __device__ inline int lu_rows(int k0, int m, int nt) {
+    return lu_structured(k0, nt) ? LU_NB + m - nt : m - k0;
+}
+__host__ __device__ inline int lu_row(int k0, int nt, int p) {
+    return (lu_structured(k0, nt) && p >= LU_NB) ? nt + p - LU_NB : k0 + p;
+}
 
 // Work layout: for every panel p, L11^{-1} and U11^{-1} of that panel ([NB][NB] re and im
 // planes each; a partial last panel is padded with the identity), then the staged A12
@@ -47,16 +68,17 @@ struct LuItem {
 // factorisation and are used by the triangular solve.
 constexpr int LU_DINV = 4 * LU_NB * LU_NB;  // doubles per panel: L11^{-1}, U11^{-1} (re, im)
 __host__ __device__ inline int lu_panels(int n) { return (n + LU_NB - 1) / LU_NB; }
-__host__ __device__ inline size_t lu_work_doubles(int n) {
-    return (size_t)LU_DINV * lu_panels(n) + 2 * (size_t)LU_NB * n + 2 * LU_NB + 2;
+__host__ __device__ inline size_t lu_work_doubles(int kf, int ncols) {
+    return (size_t)LU_DINV * lu_panels(kf) + 2 * (size_t)LU_NB * ncols + 2 * LU_NB + 2;
 }
+__host__ __device__ inline size_t lu_work_doubles(int n) { return lu_work_doubles(n, n); }
 __device__ inline double* lu_linv(const LuItem& it, int panel) { return it.work + (size_t)LU_DINV * panel; }
 __device__ inline double* lu_uinv(const LuItem& it, int panel) {
     return it.work + (size_t)LU_DINV * panel + 2 * LU_NB * LU_NB;
 }
-__device__ inline double* lu_stage(const LuItem& it) { return it.work + (size_t)LU_DINV * lu_panels(it.n); }
+__device__ inline double* lu_stage(const LuItem& it) { return it.work + (size_t)LU_DINV * lu_panels(lu_kf(it)); }
 __device__ inline int* lu_perm(const LuItem& it) {
-    return reinterpret_cast<int*>(lu_stage(it) + 2 * (size_t)LU_NB * it.n);
+    return reinterpret_cast<int*>(lu_stage(it) + 2 * (size_t)LU_NB * lu_ncols(it));
 }
 
 __device__ __forceinline__ double cabs1(double r, double i) { return fabs(r) + fabs(i); }
@@ -99,10 +121,10 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
     __shared__ double rsum[LU_MAX_N / 64];
 
     const LuItem it = items[blockIdx.x];
-    const int n = it.n;
-    if (k0 >= n) return;
+    const int n = it.n, m = lu_m(it), ncols = lu_ncols(it), kf = lu_kf(it), ntop = it.nt;
+    if (k0 >= kf) return;
     const int ld = it.ld, tid = threadIdx.x, nth = blockDim.x;
-    const int rows = n - k0, kb = min(NB, rows);
+    const int rows = lu_rows(k0, m, ntop), kb = min(NB, kf - k0);
     const int panel = k0 / NB;
     double* re = it.re;
     double* im = it.im;
@@ -119,8 +141,8 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
     for (int c = 0; c < NB; ++c) {
         double vr = 0, vi = 0;
         if (own && c < kb) {
-            vr = re[(long)(k0 + tid) * ld + k0 + c];
-            vi = im[(long)(k0 + tid) * ld + k0 + c];
+            vr = re[(long)lu_row(k0, ntop, tid) * ld + k0 + c];
+            vi = im[(long)lu_row(k0, ntop, tid) * ld + k0 + c];
             if (add_identity && c == tid) vr += 1.0;
         }
         ar[c] = vr;
@@ -192,13 +214,14 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
         ar[NB - 1] = nr;
         ai[NB - 1] = ni;
     }
-    if (tid < kb) it.piv[k0 + tid] = k0 + pvt[tid];
+    if (tid < kb) it.piv[k0 + tid] = lu_row(k0, ntop, pvt[tid]);
     if (own) {
+        const long orow = lu_row(k0, ntop, pos);
 #pragma unroll
         for (int c = 0; c < NB; ++c) {
             if (c < kb) {
-                re[(long)(k0 + pos) * ld + k0 + c] = ar[c];
-                im[(long)(k0 + pos) * ld + k0 + c] = ai[c];
+                re[orow * ld + k0 + c] = ar[c];
+                im[orow * ld + k0 + c] = ai[c];
             }
         }
         if (pos != tid) {  // at most 2*NB positions change: the panel rows and their partners
@@ -299,7 +322,7 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
         }
     }
 
-    const int nrt = n - k0 - NB;  // columns right of the panel
+    const int nrt = ncols - k0 - NB;  // columns right of the panel
     if (nrt > 0) {
         // One column per thread, all row reads of a column in flight at once:
         //   staged A12 = the new rows k0..k0+NB-1 (positions q read original row orig_of[q]);
@@ -307,7 +330,7 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
         //   original panel row (rows k0..k0+NB-1 are not written here, A12 goes to the stage).
         // A column's reads precede its writes in the same thread, so no barrier is needed.
         double* st_re = lu_stage(it);
-        double* st_im = st_re + (size_t)NB * n;
+        double* st_im = st_re + (size_t)NB * ncols;
         constexpr int PG = 16;  // rows copied per batch (loads of a batch are all in flight)
         for (int c = tid; c < nrt; c += nth) {
             const long col = k0 + NB + c;
@@ -316,14 +339,14 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
                 double vr[PG], vi[PG];
 #pragma unroll
                 for (int j = 0; j < PG; ++j) {
-                    const long src = (long)(k0 + orig_of[q0 + j]) * ld + col;
+                    const long src = (long)lu_row(k0, ntop, orig_of[q0 + j]) * ld + col;
                     vr[j] = re[src];
                     vi[j] = im[src];
                 }
 #pragma unroll
                 for (int j = 0; j < PG; ++j) {
-                    st_re[(long)(q0 + j) * n + c] = vr[j];
-                    st_im[(long)(q0 + j) * n + c] = vi[j];
+                    st_re[(long)(q0 + j) * ncols + c] = vr[j];
+                    st_im[(long)(q0 + j) * ncols + c] = vi[j];
                 }
             }
             for (int q0 = 0; q0 < cnt; q0 += PG) {
@@ -332,7 +355,7 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
                 for (int j = 0; j < PG; ++j) {
                     const int q = q0 + j;
                     if (q < cnt && slot_dst[q] >= NB) {
-                        const long so = (long)(k0 + slot_src[q]) * ld + col;
+                        const long so = (long)lu_row(k0, ntop, slot_src[q]) * ld + col;
                         wr[j] = re[so];
                         wi[j] = im[so];
                     }
@@ -341,17 +364,19 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
                 for (int j = 0; j < PG; ++j) {
                     const int q = q0 + j;
                     if (q < cnt && slot_dst[q] >= NB) {
-                        const long d = (long)(k0 + slot_dst[q]) * ld + col;
+                        const long d = (long)lu_row(k0, ntop, slot_dst[q]) * ld + col;
                         re[d] = wr[j];
                         im[d] = wi[j];
                     }
                 }
             }
         }
-    } else if (it.logdet) {
-        // the item's last panel: every U_ii and pivot is final
+    }
+    if (it.logdet && k0 + NB >= kf) {
+        // the item's last panel: every U_ii and pivot of the kf columns is final
+        __syncthreads();
         double lr_ = 0, la = 0, pc = 0;
-        for (int i = tid; i < n; i += nth) {
+        for (int i = tid; i < kf; i += nth) {
             const double dr = re[(long)i * ld + i], di = im[(long)i * ld + i];
             lr_ += log(hypot(dr, di));
             la += atan2(di, dr);
@@ -371,17 +396,18 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
 // grid.x: tm*tm trailing tiles (tm = tiles per dimension of the largest item's A22), then
 // swap_wgs workgroups of LU_SWAP_COLS left columns each; grid.y: item.
 __global__ __launch_bounds__(LU_UPD_THREADS) void lu_update_kernel(const LuItem* __restrict__ items, int k0,
-                                                                    int tm) {
+                                                                    int tmr, int tmc) {
     constexpr int T = LU_TILE, NB = LU_NB;
     static_assert(T == 32 && NB == 32, "tile code assumes 32");
     const LuItem it = items[blockIdx.y];
-    const int n = it.n, ld = it.ld, tid = threadIdx.x;
-    if (k0 >= n) return;
+    const int m = lu_m(it), ncols = lu_ncols(it), kf = lu_kf(it), ntop = it.nt;
+    const int ld = it.ld, tid = threadIdx.x;
+    if (k0 >= kf) return;
     double* re = it.re;
     double* im = it.im;
-    if ((int)blockIdx.x >= tm * tm) {
+    if ((int)blockIdx.x >= tmr * tmc) {
         // ---- interchanges of this panel on LU_SWAP_COLS columns left of it
-        const int c0 = ((int)blockIdx.x - tm * tm) * LU_SWAP_COLS;
+        const int c0 = ((int)blockIdx.x - tmr * tmc) * LU_SWAP_COLS;
         if (c0 >= k0) return;
         const int* perm = lu_perm(it);
         const int cnt = perm[0];
@@ -391,7 +417,7 @@ __global__ __launch_bounds__(LU_UPD_THREADS) void lu_update_kernel(const LuItem*
         for (int j = 0; j < PER; ++j) {
             const int e = tid + j * LU_UPD_THREADS, q = e / LU_SWAP_COLS, c = c0 + e % LU_SWAP_COLS;
             if (q < cnt && c < k0) {
-                const long o = (long)(k0 + perm[1 + 2 * NB + q]) * ld + c;
+                const long o = (long)lu_row(k0, ntop, perm[1 + 2 * NB + q]) * ld + c;
                 vr[j] = re[o];
                 vi[j] = im[o];
             }
@@ -401,34 +427,35 @@ __global__ __launch_bounds__(LU_UPD_THREADS) void lu_update_kernel(const LuItem*
         for (int j = 0; j < PER; ++j) {
             const int e = tid + j * LU_UPD_THREADS, q = e / LU_SWAP_COLS, c = c0 + e % LU_SWAP_COLS;
             if (q < cnt && c < k0) {
-                const long o = (long)(k0 + perm[1 + q]) * ld + c;
+                const long o = (long)lu_row(k0, ntop, perm[1 + q]) * ld + c;
                 re[o] = vr[j];
                 im[o] = vi[j];
             }
         }
         return;
     }
-    // ---- trailing tile (I, J) of A22 = rows/cols c0.. of the matrix
-    const int c0 = k0 + NB, nt = n - c0;
-    const int I = blockIdx.x / tm, J = blockIdx.x % tm;
-    if (I * T >= nt || J * T >= nt) return;
+    // ---- trailing tile (I, J) of A22 = panel positions NB.. x columns c0.. of the matrix
+    const int c0 = k0 + NB, ntr = lu_rows(k0, m, ntop) - NB, ntc = ncols - c0;
+    const int I = blockIdx.x / tmc, J = blockIdx.x % tmc;
+    if (I * T >= ntr || J * T >= ntc) return;
     __shared__ double sa_re[T][T + 1], sa_im[T][T + 1];  // L11^{-1}
     __shared__ double sl_re[T][T + 1], sl_im[T][T + 1];  // L21 tile
     __shared__ double sb_re[T][T + 1], sb_im[T][T + 1];  // staged A12 tile, then U12 tile
     const double* li_re = lu_linv(it, k0 / NB);
     const double* li_im = li_re + NB * NB;
     const double* st_re = lu_stage(it);
-    const double* st_im = st_re + (size_t)NB * n;
+    const double* st_im = st_re + (size_t)NB * ncols;
     for (int e = tid; e < T * T; e += LU_UPD_THREADS) {
         const int r = e / T, c = e % T;
         sa_re[r][c] = li_re[e];
         sa_im[r][c] = li_im[e];
         const int col = J * T + c;
-        sb_re[r][c] = col < nt ? st_re[(long)r * n + col] : 0.0;
-        sb_im[r][c] = col < nt ? st_im[(long)r * n + col] : 0.0;
+        sb_re[r][c] = col < ntc ? st_re[(long)r * ncols + col] : 0.0;
+        sb_im[r][c] = col < ntc ? st_im[(long)r * ncols + col] : 0.0;
         const int row = I * T + r;
-        sl_re[r][c] = row < nt ? re[(long)(c0 + row) * ld + k0 + c] : 0.0;
-        sl_im[r][c] = row < nt ? im[(long)(c0 + row) * ld + k0 + c] : 0.0;
+        const long mrow = lu_row(k0, ntop, NB + row);
+        sl_re[r][c] = row < ntr ? re[mrow * ld + k0 + c] : 0.0;
+        sl_im[r][c] = row < ntr ? im[mrow * ld + k0 + c] : 0.0;
     }
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
@@ -440,8 +467,8 @@ __global__ __launch_bounds__(LU_UPD_THREADS) void lu_update_kernel(const LuItem*
     for (int r = 0; r < 4; ++r) {
         const int crow = I * T + wm + lk + 4 * r;
         cr[r] = ci[r] = 0.0;
-        if (crow < nt && ccol < nt) {
-            const long o = (long)(c0 + crow) * ld + c0 + ccol;
+        if (crow < ntr && ccol < ntc) {
+            const long o = (long)lu_row(k0, ntop, NB + crow) * ld + c0 + ccol;
             cr[r] = re[o];
             ci[r] = im[o];
         }
@@ -468,7 +495,7 @@ __global__ __launch_bounds__(LU_UPD_THREADS) void lu_update_kernel(const LuItem*
     if (I == 0) {  // U12 into rows k0.. of the matrix (read by no other workgroup of this launch)
         for (int e = tid; e < T * T; e += LU_UPD_THREADS) {
             const int r = e / T, c = e % T, col = J * T + c;
-            if (col < nt) {
+            if (col < ntc) {
                 re[(long)(k0 + r) * ld + c0 + col] = sb_re[r][c];
                 im[(long)(k0 + r) * ld + c0 + col] = sb_im[r][c];
             }
@@ -491,37 +518,52 @@ __global__ __launch_bounds__(LU_UPD_THREADS) void lu_update_kernel(const LuItem*
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int crow = I * T + wm + lk + 4 * r;
-        if (crow < nt && ccol < nt) {
-            const long o = (long)(c0 + crow) * ld + c0 + ccol;
+        if (crow < ntr && ccol < ntc) {
+            const long o = (long)lu_row(k0, ntop, NB + crow) * ld + c0 + ccol;
             re[o] = cr[r] - (rr[r] - ii[r]);
             im[o] = ci[r] - (ri[r] + ir[r]);
         }
     }
 }
 
-// Factorise nitems matrices (n <= max_n <= LU_MAX_N each); add_identity: factorise 1 + A.
-inline void launch_lu(const LuItem* d_items, int nitems, int max_n, int add_identity, hipStream_t s) {
-    if (max_n > LU_MAX_N) throw std::invalid_argument("matrix too large for the LU kernel");
-    for (int k0 = 0; k0 < max_n; k0 += LU_NB) {
-        const int threads = ((max_n - k0 + 63) / 64) * 64;
+// Factorise the first kf columns of nitems m x ncols matrices of one shape (identity rows
+// above nt as described at the top; nt = 0 for a general matrix).  Candidate rows of every
+// panel must fit one workgroup: lu_rows(k0, m, nt) <= LU_MAX_N.
+inline void launch_lu_shape(const LuItem* d_items, int nitems, int m, int ncols, int kf, int nt, int add_identity,
+                            hipStream_t s) {
+    if (kf > ncols || kf > m || (nt > 0 && (kf % LU_NB != 0 || nt > m)))
+        throw std::invalid_argument("inconsistent LU shape");
+    for (int k0 = 0; k0 < kf; k0 += LU_NB)
+        if (lu_rows(k0, m, nt) > LU_MAX_N) throw std::invalid_argument("matrix too large for the LU kernel");
+    for (int k0 = 0; k0 < kf; k0 += LU_NB) {
+        const int rows = lu_rows(k0, m, nt);
+        const int threads = ((rows + 63) / 64) * 64;
         hipLaunchKernelGGL(lu_panel_kernel<LU_NB>, dim3(nitems), dim3(threads), 0, s, d_items, k0,
                            k0 == 0 ? add_identity : 0);
         ISLE_HIP(hipGetLastError());
-        const int ntr = max_n - k0 - LU_NB;
-        const int tm = ntr > 0 ? (ntr + LU_TILE - 1) / LU_TILE : 0;
+        const int ntr = rows - LU_NB, ntc = ncols - k0 - LU_NB;
+        const int tmr = (ntr > 0 && ntc > 0) ? (ntr + LU_TILE - 1) / LU_TILE : 0;
+        const int tmc = (ntr > 0 && ntc > 0) ? (ntc + LU_TILE - 1) / LU_TILE : 0;
         const int swap_wgs = (k0 + LU_SWAP_COLS - 1) / LU_SWAP_COLS;
-        if (tm * tm + swap_wgs > 0) {
-            hipLaunchKernelGGL(lu_update_kernel, dim3(tm * tm + swap_wgs, nitems), dim3(LU_UPD_THREADS), 0, s,
-                               d_items, k0, tm);
+        if (tmr * tmc + swap_wgs > 0) {
+            hipLaunchKernelGGL(lu_update_kernel, dim3(tmr * tmc + swap_wgs, nitems), dim3(LU_UPD_THREADS), 0, s,
+                               d_items, k0, tmr, tmc);
             ISLE_HIP(hipGetLastError());
         }
     }
 }
 
+// Factorise nitems square matrices (n <= max_n <= LU_MAX_N each); add_identity: factorise 1 + A.
+inline void launch_lu(const LuItem* d_items, int nitems, int max_n, int add_identity, hipStream_t s) {
+    launch_lu_shape(d_items, nitems, max_n, max_n, max_n, 0, add_identity, s);
+}
+
 // X = A^{-1} B for the LU factors of A.  Element (row r, rhs c) of B is read from
 // b_re[c*bsc + r*bsr] / b_im[...] and X is written to x_re/x_im with its own strides
 // (so interleaved complex and transposed layouts need no copy).  If b_re is null,
-// B is the identity.  16 right-hand sides per workgroup, X staged in LDS.
+// B is the identity.  16 right-hand sides per workgroup, X staged in LDS.  `mode` selects
+// the parts: SOLVE_PERM (row interchanges piv), SOLVE_L (unit lower L), SOLVE_U (upper U);
+// B and X may be the same array (each workgroup reads its columns before writing them).
 struct SolveItem {
     const double* lu_re;
     const double* lu_im;
@@ -534,7 +576,9 @@ struct SolveItem {
     double* x_im;
     long xsr, xsc;
     int n, ld, nrhs;
+    int mode = 7;
 };
+enum : int { SOLVE_PERM = 1, SOLVE_L = 2, SOLVE_U = 4 };
 
 constexpr int SOLVE_COLS = 16;
 constexpr int SOLVE_THREADS = 256;
@@ -571,15 +615,20 @@ __global__ __launch_bounds__(SOLVE_THREADS) void lu_solve_kernel(const SolveItem
 
     // row permutation: original row r ends at position pos(r) after the swaps piv[0..n-1];
     // sigma[pos(r)] = r, so (P B)[i] = B[sigma[i]]
-    for (int i = tid; i < n; i += SOLVE_THREADS) pv[i] = it.piv[i];
-    __syncthreads();
-    for (int r = tid; r < n; r += SOLVE_THREADS) {
-        int pos = r;
-        for (int k = 0; k < n; ++k) {
-            const int p = pv[k];
-            pos = pos == k ? p : (pos == p ? k : pos);
+    const int mode = it.mode;
+    if (mode & SOLVE_PERM) {
+        for (int i = tid; i < n; i += SOLVE_THREADS) pv[i] = it.piv[i];
+        __syncthreads();
+        for (int r = tid; r < n; r += SOLVE_THREADS) {
+            int pos = r;
+            for (int k = 0; k < n; ++k) {
+                const int p = pv[k];
+                pos = pos == k ? p : (pos == p ? k : pos);
+            }
+            sigma[pos] = r;
         }
-        sigma[pos] = r;
+    } else {
+        for (int r = tid; r < n; r += SOLVE_THREADS) sigma[r] = r;
     }
     __syncthreads();
     for (int e = tid; e < np * SOLVE_COLS; e += SOLVE_THREADS) {
@@ -601,6 +650,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void lu_solve_kernel(const SolveItem
 
     const int ei = tid >> 4, ec = tid & 15;  // this thread's element of a 16x16 block
     for (int sweep = 0; sweep < 2; ++sweep) {  // 0: L (unit lower), 1: U (upper)
+        if (!(mode & (sweep == 0 ? SOLVE_L : SOLVE_U))) continue;
         for (int bi = 0; bi < nb; ++bi) {
             const int b = sweep == 0 ? bi : nb - 1 - bi;
             const int r0 = b * 16;

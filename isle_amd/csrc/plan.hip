@@ -19,7 +19,7 @@ struct Builder {
     struct Pending {
         K kind;
         size_t off;
-        int count, mm, mn, flag;
+        int count, mm, mn, flag, aux;
         double flops;
     };
     std::vector<Pending> pend;
@@ -27,18 +27,18 @@ struct Builder {
     explicit Builder(OpCache& o) : op(o) {}
 
     template <typename It>
-    void add(K kind, const std::vector<It>& items, int mm = 0, int mn = 0, int flag = 0) {
+    void add(K kind, const std::vector<It>& items, int mm = 0, int mn = 0, int flag = 0, int aux = 0) {
         if (items.empty()) return;
         size_t off = (blob.size() + 63) & ~size_t(63);
         blob.resize(off + items.size() * sizeof(It));
         std::memcpy(blob.data() + off, items.data(), items.size() * sizeof(It));
-        pend.push_back({kind, off, (int)items.size(), mm, mn, flag, flops_of(kind, items)});
+        pend.push_back({kind, off, (int)items.size(), mm, mn, flag, aux, flops_of(kind, items)});
     }
     template <typename It>
     static double flops_of(K, const std::vector<It>&) { return 0.0; }
     template <typename T>
     static double flops_of(K kind, const std::vector<GemmItem<T>>& items) {
-        const double reals = (kind == K::GEMM64_CC) ? 4.0 : 2.0;  // real GEMMs per complex product
+        const double reals = (kind == K::GEMM64_CC || kind == K::GEMM64_CC_SUB) ? 4.0 : 2.0;  // real GEMMs per product
         double f = 0;
         for (const auto& g : items) f += reals * 2.0 * g.m * (double)g.n * g.k;
         return f;
@@ -59,7 +59,7 @@ struct Builder {
         op.bufs.emplace_back(new DevBuf(blob.size() ? blob.size() : 64));
         char* dev = op.bufs.back()->as<char>();
         if (!blob.empty()) ISLE_HIP(hipMemcpy(dev, blob.data(), blob.size(), hipMemcpyHostToDevice));
-        for (auto& p : pend) op.steps.push_back({p.kind, dev + p.off, p.count, p.mm, p.mn, p.flag, p.flops, 0});
+        for (auto& p : pend) op.steps.push_back({p.kind, dev + p.off, p.count, p.mm, p.mn, p.flag, p.flops, 0, p.aux});
     }
 };
 
@@ -223,7 +223,7 @@ void Plan::run(OpCache& op, const double* phi) {
 
 static bool is_gemm(K k) {
     return k == K::GEMM64_RC_PRE || k == K::GEMM64_RC_ROW || k == K::GEMM64_CR_PRE || k == K::GEMM64_CC ||
-           k == K::GEMM32_RC_ROW || k == K::GEMM32_RC_PRE;
+           k == K::GEMM32_RC_ROW || k == K::GEMM32_RC_PRE || k == K::GEMM64_CC_SUB;
 }
 
 void Plan::tune(OpCache& op) {
@@ -266,7 +266,9 @@ void Plan::profile(const double* phi, int opk, int nt, int batch, int group, int
     OpCache& op = opk == 0   ? get_force_plain(nt, batch, nspecies, spin)
                   : opk == 1 ? get_force_mixed(nt, batch, group, nspecies, spin)
                   : opk == 2 ? get_logdet_plain(nt, batch, nspecies, spin)
-                             : get_logdet_grouped(nt, batch, group, nspecies, spin);
+                  : opk == 3 ? get_logdet_grouped(nt, batch, group, nspecies, spin)
+                  : opk == 4 ? get_force_stable(nt, batch, group, nspecies, spin)
+                             : get_logdet_stable(nt, batch, group, nspecies, spin);
     run(op, phi);  // warm up
     const size_t ns = op.steps.size();
     std::vector<hipEvent_t> ev(ns + 1);
@@ -363,6 +365,27 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
         case K::DIAG64X32:
             hipLaunchKernelGGL((diag_prod_kernel<double, float>), dim3(cdiv(n, 16), st.count), dim3(256), 0, s,
                                (const DiagItem<double, float>*)st.items);
+            break;
+        case K::GEMM64_CC_SUB:
+            if (!launch_gemm<double, true, true, false, SUBTRACT>((const GemmItem<double>*)st.items, st.count,
+                                                               st.max_m, st.max_n, s, st.tile)) return false;
+            break;
+        case K::LU_SHAPE:
+            launch_lu_shape((const LuItem*)st.items, st.count, st.max_m, st.max_n, st.flag, st.aux, 0, s);
+            break;
+        case K::BLOCK:
+            hipLaunchKernelGGL(block_fill_kernel, dim3(ew_blocks(st.aux), st.count), dim3(256), 0, s,
+                               (const BlockItem*)st.items);
+            break;
+        case K::SIGMA:
+            hipLaunchKernelGGL(sigma_kernel, dim3(st.count), dim3(256), 0, s, (const SigmaItem*)st.items);
+            break;
+        case K::GATHER:
+            hipLaunchKernelGGL(stack_gather_kernel, dim3(cdiv(st.aux, 256), st.max_m, st.count), dim3(256), 0, s,
+                               (const GatherItem*)st.items);
+            break;
+        case K::TBLOCK:
+            hipLaunchKernelGGL(tblock_kernel, dim3(64, 2, st.count), dim3(256), 0, s, (const TBlockItem*)st.items);
             break;
         case K::COPY64:
             hipLaunchKernelGGL((copy_items_kernel<double, double>), dim3(ew_blocks(st.max_m), st.count),
@@ -832,18 +855,413 @@ OpCache& Plan::get_logdet_grouped(int nt, int batch, int g, int nspecies, int sp
 }
 
 // ---------------------------------------------------------------------------
+// Structured elimination of the block-cyclic matrix R of the group products (I on the
+// diagonal, -P_j right of it, P_{ng-1} bottom left; det R = det(1 + P_0 ... P_{ng-1})).
+// GEPP of R only ever chooses pivots in the current block row and the last one, so step k
+// factorises the 2n x 3n matrix W_k = [[I, -P_k, 0], [B_k, 0, C_k]] (columns k, k+1, ng-1 of
+// R) over its first n columns, and the Schur complement of its bottom rows is [B_{k+1}, C_{k+1}];
+// the last step factorises [[I, -P_{ng-2}], [B, C]] completely.  The pivots are those of GEPP
+// on R, and no product of more than one group is ever formed, so the factorisation is as
+// stable as the group products are accurate.  Requires ng >= 2, n % 32 == 0, n + 32 <= 512.
+namespace {
+struct Cyclic {
+    int n, ng, nb;
+    double* W;     // [nb][ng-1] step matrices, re plane then im plane of 2n x 3n each
+    double* work;  // [nb][ng-1] LU scratch
+    int* piv;      // [nb][ng-1][2n]
+    size_t wstride, lwork;
+    int ldw(int k) const { return k < ng - 2 ? 3 * n : 2 * n; }
+    int kf(int k) const { return k < ng - 2 ? n : 2 * n; }
+    double* wre(int item, int k) const { return W + ((size_t)item * (ng - 1) + k) * wstride; }
+    double* wim(int item, int k) const { return wre(item, k) + wstride / 2; }
+    double* wk(int item, int k) const { return work + ((size_t)item * (ng - 1) + k) * lwork; }
+    int* pv(int item, int k) const { return piv + ((size_t)item * (ng - 1) + k) * 2 * n; }
+};
+
+// P(item, j) -> CMat<double> of group product j; logdet slots [nb][ng-1][2] (or null)
+template <typename PF>
+Cyclic add_cyclic_factor(Builder& b, int n, int ng, int nb, PF P, double* logdet) {
+    if (ng < 2 || n % LU_NB != 0 || n + LU_NB > LU_MAX_N || 2 * n > 1024)
+        throw std::invalid_argument("block-cyclic factorisation needs nx % 32 == 0 and nx <= 480");
+    Cyclic c;
+    c.n = n;
+    c.ng = ng;
+    c.nb = nb;
+    c.wstride = 2 * (size_t)(2 * n) * (3 * n);
+    c.lwork = std::max(lu_work_doubles(n, 3 * n), lu_work_doubles(2 * n, 2 * n));
+    c.W = b.d64((size_t)nb * (ng - 1) * c.wstride);
+    c.work = b.d64((size_t)nb * (ng - 1) * c.lwork);
+    c.piv = b.di((size_t)nb * (ng - 1) * 2 * n);
+    const long nn = (long)n * n;
+    auto blk = [&](std::vector<BlockItem>& v, int item, int k, int r0, int c0, const double* sre, const double* sim,
+                   long lds, int mode) {
+        const int ld = c.ldw(k);
+        v.push_back({c.wre(item, k) + (long)r0 * ld + c0, c.wim(item, k) + (long)r0 * ld + c0, sre, sim, ld, lds, n, n,
+                     mode});
+    };
+    // top block rows of every step: [I, -P_k, 0]
+    {
+        std::vector<BlockItem> v;
+        for (int item = 0; item < nb; ++item)
+            for (int k = 0; k < ng - 1; ++k) {
+                blk(v, item, k, 0, 0, nullptr, nullptr, 0, 1);
+                const CMat<double> p = P(item, k);
+                blk(v, item, k, 0, n, p.re, p.im, n, 3);
+                if (k < ng - 2) blk(v, item, k, 0, 2 * n, nullptr, nullptr, 0, 0);
+            }
+        b.add(K::BLOCK, v, 0, 0, 0, (int)nn);
+    }
+    for (int k = 0; k < ng - 1; ++k) {
+        std::vector<BlockItem> v;
+        for (int item = 0; item < nb; ++item) {
+            const bool last = k == ng - 2;
+            if (k == 0) {
+                const CMat<double> p = P(item, ng - 1);
+                blk(v, item, k, n, 0, p.re, p.im, n, 2);
+                if (last) {
+                    blk(v, item, k, n, n, nullptr, nullptr, 0, 1);
+                } else {
+                    blk(v, item, k, n, n, nullptr, nullptr, 0, 0);
+                    blk(v, item, k, n, 2 * n, nullptr, nullptr, 0, 1);
+                }
+            } else {  // Schur complement of the previous step's bottom rows
+                const int lp = c.ldw(k - 1);
+                const double* sre = c.wre(item, k - 1) + (long)n * lp;
+                const double* sim = c.wim(item, k - 1) + (long)n * lp;
+                blk(v, item, k, n, 0, sre + n, sim + n, lp, 2);
+                if (last) {
+                    blk(v, item, k, n, n, sre + 2 * n, sim + 2 * n, lp, 2);
+                } else {
+                    blk(v, item, k, n, n, nullptr, nullptr, 0, 0);
+                    blk(v, item, k, n, 2 * n, sre + 2 * n, sim + 2 * n, lp, 2);
+                }
+            }
+        }
+        b.add(K::BLOCK, v, 0, 0, 0, (int)nn);
+        std::vector<LuItem> lu;
+        for (int item = 0; item < nb; ++item) {
+            LuItem it{c.wre(item, k), c.wim(item, k), c.pv(item, k),
+                      logdet ? logdet + ((long)item * (ng - 1) + k) * 2 : nullptr, c.wk(item, k), c.kf(k), c.ldw(k)};
+            it.m = 2 * n;
+            it.ncols = c.ldw(k);
+            it.kf = c.kf(k);
+            it.nt = n;
+            lu.push_back(it);
+        }
+        b.add(K::LU_SHAPE, lu, 2 * n, c.ldw(k), c.kf(k), n);
+    }
+    return c;
+}
+}  // namespace
+
+// Stable log det M: FP64 group products (as in the grouped form), then the structured
+// factorisation of their block-cyclic matrix; per-step log-determinants are summed on the host.
+OpCache& Plan::get_logdet_stable(int nt, int batch, int g, int nspecies, int spin) {
+    if (g <= 0 || nt % g != 0) throw std::invalid_argument("group size must divide nt");
+    const int ng = nt / g;
+    if (ng == 1) return get_logdet_grouped(nt, batch, g, nspecies, spin);
+    auto key = std::make_tuple(5, nt, batch, g, nspecies, spin);
+    auto found = cache_.find(key);
+    if (found != cache_.end()) return *found->second;
+    auto op = std::make_unique<OpCache>();
+    op->nt = nt;
+    op->batch = batch;
+    op->nspecies = nspecies;
+    op->phi_stride = 2L * nx_ * nt;
+    Builder b(*op);
+    const int n = nx_;
+    const long nn = (long)n * n;
+    const int nb = batch * nspecies;
+    double* ph = b.d64((size_t)nb * 2 * nt * n);
+    double* Wg = b.d64((size_t)nb * ng * 2 * 2 * nn);
+    op->logdet = b.d64((size_t)nb * (ng - 1) * 2);
+    auto D = [&](int item, int t) { return CMat<double>{ph + (long)item * 2 * nt * n + (long)t * n,
+                                                        ph + (long)item * 2 * nt * n + (long)nt * n + (long)t * n}; };
+    auto mW = [&](int item, int j, int k) { return cmat(Wg, ((long)item * ng + j) * 2 + k, nn); };
+    auto spec = [&](int item) { return item % nspecies; };
+    std::vector<PhaseItem> phs;
+    for (int item = 0; item < nb; ++item)
+        phs.push_back({item / nspecies, spin, sigma_of(spec(item)), D(item, 0).re, D(item, 0).im, nullptr, nullptr, n, nt});
+    b.add(K::PHASE, phs);
+    std::vector<ScaleItem<double>> sc;
+    for (int item = 0; item < nb; ++item)
+        for (int j = 0; j < ng; ++j) {
+            const int e = j * g + g - 1;
+            sc.push_back({a64_[spec(item)]->as<double>(), D(item, e).re, D(item, e).im, mW(item, j, 0).re,
+                          mW(item, j, 0).im, n, SCALE_ROW});
+        }
+    b.add(K::SCALE64, sc);
+    for (int q = 1; q < g; ++q) {
+        std::vector<GemmItem<double>> gi;
+        for (int item = 0; item < nb; ++item)
+            for (int j = 0; j < ng; ++j) {
+                const int t = j * g + g - 1 - q;
+                const CMat<double> src = mW(item, j, (q - 1) & 1);
+                gi.push_back(gitem<double>(a64_[spec(item)]->as<double>(), nullptr, src.re, src.im, mW(item, j, q & 1),
+                                           nullptr, nullptr, D(item, t).re, D(item, t).im, n));
+            }
+        b.add(K::GEMM64_RC_ROW, gi, n, n);
+    }
+    add_cyclic_factor(b, n, ng, nb, [&](int item, int j) { return mW(item, j, (g - 1) & 1); }, op->logdet);
+    b.finish();
+    tune(*op);
+    auto& ref = *op;
+    cache_[key] = std::move(op);
+    return ref;
+}
+
+// Stable DIRECT_SINGLE force.  Local FP64 suffix chains per group give P_j; the top level
+// needs K_j = lefts (1 + A^{-1})^{-1} rights restricted to group j, which is a block of R^{-1}:
+// K_j = -Y[j+1, j] (j < ng-1), K_{ng-1} = Y[0, ng-1].  Those blocks come from the structured
+// factorisation by block forward/back substitution on the identity, columns ordered
+// (ng-1, 0, 1, ..., ng-2) so that every step works on a prefix of the columns; then the FP64
+// right chains Rt_tau = A_s (D_tau Rt_{tau-1}) from Rt_{s_j - 1} = K_j^T and
+// f[tau] = diag(Suf_{tau+1} R_tau) as in the other forms.
+OpCache& Plan::get_force_stable(int nt, int batch, int g, int nspecies, int spin) {
+    if (g <= 0 || nt % g != 0) throw std::invalid_argument("group size must divide nt");
+    auto key = std::make_tuple(4, nt, batch, g, nspecies, spin);
+    auto found = cache_.find(key);
+    if (found != cache_.end()) return *found->second;
+    auto op = std::make_unique<OpCache>();
+    op->nt = nt;
+    op->batch = batch;
+    op->nspecies = nspecies;
+    op->phi_stride = 2L * nx_ * nt;
+    Builder b(*op);
+    const int n = nx_;
+    const long nn = (long)n * n;
+    const int nb = batch * nspecies;
+    const int ng = nt / g;
+    const long wd = (long)ng * n;  // columns of the block solves
+    double* ph = b.d64((size_t)nb * 2 * nt * n);
+    double* Suf = b.d64((size_t)nb * nt * 2 * nn);
+    double* Rt = b.d64((size_t)nb * nt * 2 * nn);
+    double* Kt = b.d64((size_t)nb * ng * 2 * nn);
+    double* Tb = b.d64((size_t)nb * ng * 2 * n * wd);
+    op->parts = b.d64((size_t)nb * nt * n * 2);
+    auto D = [&](int item, int t) {
+        return CMat<double>{ph + (long)item * 2 * nt * n + (long)t * n,
+                            ph + (long)item * 2 * nt * n + (long)nt * n + (long)t * n};
+    };
+    auto mS = [&](int item, int t) { return cmat(Suf, (long)item * nt + t, nn); };
+    auto mRt = [&](int item, int t) { return cmat(Rt, (long)item * nt + t, nn); };
+    auto mK = [&](int item, int j) { return cmat(Kt, (long)item * ng + j, nn); };
+    auto mT = [&](int item, int k) { return cmat(Tb, (long)item * ng + k, (long)n * wd); };
+    auto spec = [&](int item) { return item % nspecies; };
+    const double* A64[2] = {a64_[0]->as<double>(), a64_[1]->as<double>()};
+
+    std::vector<PhaseItem> phs;
+    for (int item = 0; item < nb; ++item)
+        phs.push_back({item / nspecies, spin, sigma_of(spec(item)), D(item, 0).re, D(item, 0).im, nullptr, nullptr, n, nt});
+    b.add(K::PHASE, phs);
+    {
+        std::vector<ScaleItem<double>> sc;
+        for (int item = 0; item < nb; ++item)
+            for (int j = 0; j < ng; ++j) {
+                const int e = j * g + g - 1;
+                sc.push_back({A64[spec(item)], D(item, e).re, D(item, e).im, mS(item, e).re, mS(item, e).im, n,
+                              SCALE_ROW});
+            }
+        b.add(K::SCALE64, sc);
+    }
+    for (int q = 1; q < g; ++q) {
+        std::vector<GemmItem<double>> gi;
+        for (int item = 0; item < nb; ++item)
+            for (int j = 0; j < ng; ++j) {
+                const int t = j * g + g - 1 - q;
+                const CMat<double> src = mS(item, t + 1);
+                gi.push_back(gitem<double>(A64[spec(item)], nullptr, src.re, src.im, mS(item, t), nullptr, nullptr,
+                                           D(item, t).re, D(item, t).im, n));
+            }
+        b.add(K::GEMM64_RC_ROW, gi, n, n);
+    }
+    auto mP = [&](int item, int j) { return mS(item, j * g); };
+    auto gsub = [&](std::vector<GemmItem<double>>& v, const double* are, const double* aim, long lda, const double* bre,
+                    const double* bim, double* cre, double* cim, int cols) {
+        GemmItem<double> gm{};
+        gm.a_re = are;
+        gm.a_im = aim;
+        gm.b_re = bre;
+        gm.b_im = bim;
+        gm.c_re = cre;
+        gm.c_im = cim;
+        gm.m = n;
+        gm.n = cols;
+        gm.k = n;
+        gm.lda = (int)lda;
+        gm.ldb = gm.ldc = (int)wd;
+        v.push_back(gm);
+    };
+    auto solve = [&](std::vector<SolveItem>& v, const double* lre, const double* lim, int ld, const double* dinv,
+                     const double* bre, const double* bim, double* xre, double* xim, int cols, int mode) {
+        SolveItem si{};
+        si.lu_re = lre;
+        si.lu_im = lim;
+        si.dinv = dinv;
+        si.b_re = bre;
+        si.b_im = bim;
+        si.bsr = wd;
+        si.bsc = 1;
+        si.x_re = xre;
+        si.x_im = xim;
+        si.xsr = wd;
+        si.xsc = 1;
+        si.n = n;
+        si.ld = ld;
+        si.nrhs = cols;
+        si.mode = mode;
+        v.push_back(si);
+    };
+    if (ng == 1) {
+        // K_0 = (1 + P_0)^{-1}
+        double* LUb = b.d64((size_t)nb * 2 * nn);
+        int* piv = b.di((size_t)nb * n);
+        double* luw = b.d64(nb * lu_work_doubles(n));
+        std::vector<CopyItem<double, double>> cp;
+        for (int item = 0; item < nb; ++item) cp.push_back({mP(item, 0).re, LUb + (long)item * 2 * nn, 2 * nn});
+        b.add(K::COPY64, cp, (int)(2 * nn));
+        std::vector<LuItem> lu;
+        std::vector<SolveItem> sv;
+        for (int item = 0; item < nb; ++item) {
+            lu.push_back({LUb + (long)item * 2 * nn, LUb + (long)item * 2 * nn + nn, piv + (long)item * n, nullptr,
+                          luw + item * lu_work_doubles(n), n, n});
+            SolveItem si{};
+            si.lu_re = LUb + (long)item * 2 * nn;
+            si.lu_im = si.lu_re + nn;
+            si.piv = piv + (long)item * n;
+            si.dinv = luw + item * lu_work_doubles(n);
+            si.x_re = mK(item, 0).re;
+            si.x_im = mK(item, 0).im;
+            si.xsr = 1;  // written transposed: K^T
+            si.xsc = n;
+            si.n = n;
+            si.ld = n;
+            si.nrhs = n;
+            sv.push_back(si);
+        }
+        b.add(K::LU, lu, n, 0, 1);
+        b.add(K::SOLVE, sv, n, n);
+    } else {
+        const Cyclic c = add_cyclic_factor(b, n, ng, nb, mP, nullptr);
+        double* Bm = b.d64((size_t)nb * 2 * 2 * n * wd);
+        int* sig = b.di((size_t)nb * (ng - 1) * 2 * n);
+        auto mB = [&](int item, int k) { return cmat(Bm, (long)item * 2 + k, (long)n * wd); };
+        auto sg = [&](int item, int k) { return sig + ((long)item * (ng - 1) + k) * 2 * n; };
+        {  // bottom right-hand side: identity at columns [0, n) (block ng-1), zero elsewhere
+            std::vector<BlockItem> v;
+            for (int item = 0; item < nb; ++item) {
+                v.push_back({mB(item, 0).re, mB(item, 0).im, nullptr, nullptr, wd, 0, n, n, 1});
+                v.push_back({mB(item, 0).re + n, mB(item, 0).im + n, nullptr, nullptr, wd, 0, n, (int)(wd - n), 0});
+                v.push_back({mB(item, 1).re, mB(item, 1).im, nullptr, nullptr, wd, 0, n, (int)wd, 0});
+            }
+            b.add(K::BLOCK, v, 0, 0, 0, (int)(n * wd));
+            std::vector<SigmaItem> sv;
+            for (int item = 0; item < nb; ++item)
+                for (int k = 0; k < ng - 1; ++k) sv.push_back({c.pv(item, k), sg(item, k), 2 * n, c.kf(k)});
+            b.add(K::SIGMA, sv);
+        }
+        for (int k = 0; k < ng - 1; ++k) {
+            const int w = (k + 2) * n;
+            const int ld = c.ldw(k);
+            std::vector<GatherItem> ga;
+            std::vector<SolveItem> sl;
+            std::vector<GemmItem<double>> gs;
+            for (int item = 0; item < nb; ++item) {
+                const CMat<double> bi = mB(item, k & 1), bo = mB(item, (k + 1) & 1), t = mT(item, k);
+                ga.push_back({sg(item, k), bi.re, bi.im, t.re, t.im, bo.re, bo.im, wd, wd, n, w, (k + 1) * n});
+                solve(sl, c.wre(item, k), c.wim(item, k), ld, c.wk(item, k), t.re, t.im, t.re, t.im, w, SOLVE_L);
+                gsub(gs, c.wre(item, k) + (long)n * ld, c.wim(item, k) + (long)n * ld, ld, t.re, t.im, bo.re, bo.im, w);
+            }
+            b.add(K::GATHER, ga, 2 * n, 0, 0, w);
+            b.add(K::SOLVE, sl, n, w);
+            b.add(K::GEMM64_CC_SUB, gs, n, w);
+        }
+        {  // last step: x_{ng-1} from the bottom-right LU, then x_{ng-2}
+            const int k = ng - 2, ld = 2 * n;
+            std::vector<SolveItem> s1, s2;
+            std::vector<GemmItem<double>> gs;
+            for (int item = 0; item < nb; ++item) {
+                const CMat<double> bo = mB(item, (k + 1) & 1), xl = mT(item, ng - 1), x2 = mT(item, ng - 2);
+                solve(s1, c.wre(item, k) + (long)n * ld + n, c.wim(item, k) + (long)n * ld + n, ld,
+                      c.wk(item, k) + (size_t)LU_DINV * (n / LU_NB), bo.re, bo.im, xl.re, xl.im, (int)wd,
+                      SOLVE_L | SOLVE_U);
+                gsub(gs, c.wre(item, k) + n, c.wim(item, k) + n, ld, xl.re, xl.im, x2.re, x2.im, (int)wd);
+                solve(s2, c.wre(item, k), c.wim(item, k), ld, c.wk(item, k), x2.re, x2.im, x2.re, x2.im, (int)wd,
+                      SOLVE_U);
+            }
+            b.add(K::SOLVE, s1, n, (int)wd);
+            b.add(K::GEMM64_CC_SUB, gs, n, (int)wd);
+            b.add(K::SOLVE, s2, n, (int)wd);
+        }
+        for (int k = ng - 3; k >= 0; --k) {  // back substitution on the columns still needed
+            const int w = (k + 1) * n, ld = c.ldw(k);
+            std::vector<GemmItem<double>> g1, g2;
+            std::vector<SolveItem> sl;
+            for (int item = 0; item < nb; ++item) {
+                const CMat<double> x = mT(item, k), x1 = mT(item, k + 1), xl = mT(item, ng - 1);
+                gsub(g1, c.wre(item, k) + n, c.wim(item, k) + n, ld, x1.re, x1.im, x.re, x.im, w);
+                gsub(g2, c.wre(item, k) + 2 * n, c.wim(item, k) + 2 * n, ld, xl.re, xl.im, x.re, x.im, w);
+                solve(sl, c.wre(item, k), c.wim(item, k), ld, c.wk(item, k), x.re, x.im, x.re, x.im, w, SOLVE_U);
+            }
+            b.add(K::GEMM64_CC_SUB, g1, n, w);
+            b.add(K::GEMM64_CC_SUB, g2, n, w);
+            b.add(K::SOLVE, sl, n, w);
+        }
+        std::vector<TBlockItem> tb;
+        for (int item = 0; item < nb; ++item)
+            for (int j = 0; j < ng; ++j) {
+                const CMat<double> x = j < ng - 1 ? mT(item, j + 1) : mT(item, 0);
+                const long col = j < ng - 1 ? (long)(j + 1) * n : 0;
+                tb.push_back({x.re + col, x.im + col, mK(item, j).re, wd, n, j < ng - 1 ? -1.0 : 1.0});
+            }
+        b.add(K::TBLOCK, tb);
+    }
+    // ---- local right chains (FP64), transposed
+    for (int q = 0; q < g; ++q) {
+        std::vector<GemmItem<double>> gi;
+        for (int item = 0; item < nb; ++item)
+            for (int j = 0; j < ng; ++j) {
+                const int tau = j * g + q;
+                const CMat<double> prev = q == 0 ? mK(item, j) : mRt(item, tau - 1);
+                gi.push_back(gitem<double>(A64[spec(item)], nullptr, prev.re, prev.im, mRt(item, tau), D(item, tau).re,
+                                           D(item, tau).im, nullptr, nullptr, n));
+            }
+        b.add(K::GEMM64_RC_PRE, gi, n, n);
+    }
+    {
+        std::vector<DiagItem<double>> dg;
+        for (int item = 0; item < nb; ++item)
+            for (int tau = 0; tau < nt; ++tau) {
+                const bool last = (tau % g) == g - 1;
+                const CMat<double> l = last ? CMat<double>{nullptr, nullptr} : mS(item, tau + 1);
+                const CMat<double> r = mRt(item, tau);
+                dg.push_back({l.re, l.im, r.re, r.im, op->parts + ((long)item * nt + tau) * 2 * n, n, 1});
+            }
+        b.add(K::DIAG64, dg);
+    }
+    b.finish();
+    tune(*op);
+    auto& ref = *op;
+    cache_[key] = std::move(op);
+    return ref;
+}
+
+// ---------------------------------------------------------------------------
 void Plan::logdetM(const double* phi, int nt, int batch, int mode, int group, int nspecies, int spin,
                    double* out_host) {
     if (nt < 1) throw std::invalid_argument("nt must be positive");
-    OpCache& op = mode == 0 ? get_logdet_plain(nt, batch, nspecies, spin)
-                            : get_logdet_grouped(nt, batch, group, nspecies, spin);
+    OpCache& op = mode == 0   ? get_logdet_plain(nt, batch, nspecies, spin)
+                  : mode == 1 ? get_logdet_grouped(nt, batch, group, nspecies, spin)
+                              : get_logdet_stable(nt, batch, group, nspecies, spin);
+    // log-determinant slots per (field, species): one, or one per elimination step
+    const int per = (mode == 2 && nt / group > 1) ? nt / group - 1 : 1;
     run(op, phi);
     // sums of phi per field
     DevBuf sums((size_t)batch * 4 * 8);
     hipLaunchKernelGGL(sums_kernel, dim3(batch), dim3(256), 0, stream_, phi, (long)nx_ * nt, (long)nx_ * nt,
                        sums.as<double>());
     ISLE_HIP(hipGetLastError());
-    std::vector<double> ld((size_t)batch * nspecies * 2), sm((size_t)batch * 4);
+    std::vector<double> ld((size_t)batch * nspecies * 2 * per), sm((size_t)batch * 4);
     ISLE_HIP(hipMemcpyAsync(ld.data(), op.logdet, ld.size() * 8, hipMemcpyDeviceToHost, stream_));
     ISLE_HIP(hipMemcpyAsync(sm.data(), sums.p, sm.size() * 8, hipMemcpyDeviceToHost, stream_));
     ISLE_HIP(hipStreamSynchronize(stream_));
@@ -853,7 +1271,8 @@ void Plan::logdetM(const double* phi, int nt, int batch, int mode, int group, in
         if (spin) sphi = -I * sphi;  // sum of aux = -i phi
         for (int s = 0; s < nspecies; ++s) {
             const int item = bi * nspecies + s;
-            const std::complex<double> l(ld[2 * item], ld[2 * item + 1]);
+            std::complex<double> l(0, 0);
+            for (int k = 0; k < per; ++k) l += std::complex<double>(ld[2 * (item * per + k)], ld[2 * (item * per + k) + 1]);
             std::complex<double> v;
             if (mode == 0 && hop_ == EXP && s == PARTICLE) {
                 v = l;
@@ -872,8 +1291,9 @@ void Plan::logdetM(const double* phi, int nt, int batch, int mode, int group, in
 void Plan::force_parts(const double* phi, int nt, int batch, int mode, int group, int nspecies, int spin,
                        double* out_dev) {
     if (nt < 2) throw std::invalid_argument("nt < 2 in HubbardFermiAction algorithm DIRECT_SINGLE not supported");
-    OpCache& op = mode == 0 ? get_force_plain(nt, batch, nspecies, spin)
-                            : get_force_mixed(nt, batch, group, nspecies, spin);
+    OpCache& op = mode == 0   ? get_force_plain(nt, batch, nspecies, spin)
+                  : mode == 1 ? get_force_mixed(nt, batch, group, nspecies, spin)
+                              : get_force_stable(nt, batch, group, nspecies, spin);
     run(op, phi);
     const size_t bytes = (size_t)batch * nspecies * nt * nx_ * 2 * 8;
     ISLE_HIP(hipMemcpyAsync(out_dev, op.parts, bytes, hipMemcpyDeviceToDevice, stream_));

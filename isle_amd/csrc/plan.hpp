@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "cyclic.hpp"
 #include "elementwise.hpp"
 #include "gemm.hpp"
 #include "lu.hpp"
@@ -52,6 +53,13 @@ enum class K : int {
     PHASE,              // phase vectors from the runtime phi
     TCVT64TO32,         // transposed copy with conversion
     DIAG64X32,          // diag(L R), FP64 left factor, FP32 right factor
+    GEMM64_CC_SUB,      // C -= A B, complex x complex                     (f64)
+    LU_SHAPE,           // LU of the first kf columns of m x ncols matrices (max_m = m, max_n = ncols,
+                        // flag = kf, aux = identity-row bound nt)
+    BLOCK,              // block assembly (zero / identity / copy / negated copy)
+    SIGMA,              // row permutation of stacked right-hand sides
+    GATHER,             // permuted split of stacked right-hand sides
+    TBLOCK,             // signed transposed block extraction
 };
 
 struct Step {
@@ -62,6 +70,7 @@ struct Step {
     int flag;            // LU: add identity
     double flops;        // algorithmic real flops of the launch (GEMMs: 2mnk per real product)
     int tile = 0;        // GEMM tile shape chosen by the autotuner (TILE_64 / TILE_32)
+    int aux = 0;         // LU_SHAPE: nt; BLOCK / GATHER: largest element count per item
 };
 
 // A cached operation: descriptor blob + workspaces + step list.
@@ -88,20 +97,23 @@ public:
     // log det M for a batch of fields (consecutive, nx*nt complex each) and nspecies
     // species (1: particle only, 2: particle and hole).  mode 0 follows the reference
     // algorithm step by step; mode 1 forms FP64 products of groups of `group` slices
-    // in parallel first.  spin != 0 evaluates at aux = -i*phi.  out_host: [batch][nspecies][2].
+    // in parallel first; mode 2 factorises the block-cyclic matrix of those group products
+    // (structured elimination with partial pivoting, no product of all groups is formed).
+    // spin != 0 evaluates at aux = -i*phi.  out_host: [batch][nspecies][2].
     void logdetM(const double* phi, int nt, int batch, int mode, int group, int nspecies, int spin,
                  double* out_host);
     // DIRECT_SINGLE force terms (without the -i prefactor) for each species, written
     // interleaved to out_dev [batch][nspecies][nx*nt].  mode 0: plain FP64 reference
     // algorithm; mode 1: FP32 local chains over groups of `group` slices on MFMA plus
-    // an FP64 combination of the group products.
+    // an FP64 combination of the group products; mode 2: FP64 local chains with the
+    // combination taken from the structured factorisation of the block-cyclic matrix.
     void force_parts(const double* phi, int nt, int batch, int mode, int group, int nspecies, int spin,
                      double* out_dev);
     void solveM(const double* phi, int nt, int species, const double* rhss, int nrhs, double* out);
     void F(int tp, const double* phi, int nt, int species, int inv, double* out);
     void slice_matrix(int species, int inv, double* out_host);
     void sync() { ISLE_HIP(hipStreamSynchronize(stream_)); }
-    // Time every launch of a cached force (op 0/1) or logdet (op 2/3) operation with HIP
+    // Time every launch of a cached force (op 0/1/4) or logdet (op 2/3/5) operation with HIP
     // events, `reps` times.  Per launch kind k (enum K): out[4k..4k+3] =
     // (launches per operation, ms per operation, algorithmic flops per operation, 0).
     void profile(const double* phi, int op, int nt, int batch, int group, int nspecies, int spin, int reps,
@@ -129,6 +141,8 @@ private:
     OpCache& get_force_mixed(int nt, int batch, int group, int nspecies, int spin);
     OpCache& get_logdet_plain(int nt, int batch, int nspecies, int spin);
     OpCache& get_logdet_grouped(int nt, int batch, int group, int nspecies, int spin);
+    OpCache& get_force_stable(int nt, int batch, int group, int nspecies, int spin);
+    OpCache& get_logdet_stable(int nt, int batch, int group, int nspecies, int spin);
     void run(OpCache& op, const double* phi);
     bool launch_step(OpCache& op, const Step& st, const double* phi);
     // Autotuner: time every tile shape of each GEMM step once per (kind, m, n, items) and

@@ -1,0 +1,97 @@
+"""GPU parity of precision="stable" (FP64 group products combined by the structured pivoted
+elimination of their block-cyclic matrix) against the oracle.
+
+The ill-conditioned case is a small version of the nanotube workload (honeycomb cylinder,
+DIA hopping, spin basis, mu != 0, beta = 10): there the reference's own product of all
+time slices loses the particle log-determinant completely (tests assert that it does), so
+the comparison is with the oracle's stabilised restatement (``logdetM_stable``,
+``force_part_stable``: dense GEPP / inverse of the same block-cyclic matrix).
+Tolerances: logdet 1e-10 relative (imaginary parts modulo 2 pi); force parts 1e-9 relative
+(L2) for groups of 16 slices and 1e-6 for groups of 32 (a group of 32 slices has condition
+number ~1e6 here, which FP64 local chains carry with ~1e-10 relative error per group).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import hubbard_ref as ref
+from tests.helpers import mod2pi_diff, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+isle = pytest.importorskip("isle_amd")
+from isle_amd import action as act  # noqa: E402
+
+
+def _cylinder(nt, seed=7, beta=10.0, U=2.5, mu=0.5):
+    kappa = ref.honeycomb(4, 8, periodic_j=False)  # nx = 64
+    nx = kappa.shape[0]
+    kt, mt, ut = kappa * beta / nt, mu * beta / nt, U * beta / nt
+    phi = np.random.RandomState(seed).normal(0, np.sqrt(ut), nx * nt) + 0j
+    return kt, mt, phi
+
+
+@pytest.mark.parametrize("nt,beta,group", [(128, 10.0, 16), (128, 10.0, 32), (64, 5.0, 32)])
+def test_stable_logdet_ill_conditioned(nt, beta, group):
+    kt, mt, phi = _cylinder(nt, beta=beta)
+    hfm = ref.HFMDia(kt, mt, -1)
+    aux = -1j * phi
+    g = act.HubbardFermiAction(kt, mt, -1, act.HFAHopping.DIA, act.HFABasis.SPIN, precision="stable", group=group)
+    assert g._group(nt) == group
+    ld = g.logdets(torch.as_tensor(phi, device="cuda"))[0]
+    for species in (0, 1):
+        want = ref.logdetM_stable(hfm, aux, species, 8)
+        assert mod2pi_diff(ld[species], want) < 1e-10 * abs(want), (species, ld[species], want)
+    if beta == 10.0:  # the reference's own product is far off on this field (the test is not vacuous)
+        plain = ref.logdetM(hfm, aux, 0)
+        assert mod2pi_diff(plain, ref.logdetM_stable(hfm, aux, 0, 8)) > 1e-3 * abs(plain)
+
+
+@pytest.mark.parametrize("nt,beta,group,tol", [(128, 10.0, 16, 1e-9), (128, 10.0, 32, 1e-6), (64, 5.0, 32, 1e-6)])
+def test_stable_force_ill_conditioned(nt, beta, group, tol):
+    kt, mt, phi = _cylinder(nt, beta=beta)
+    hfm = ref.HFMDia(kt, mt, -1)
+    aux = -1j * phi
+    g = act.HubbardFermiAction(kt, mt, -1, act.HFAHopping.DIA, act.HFABasis.SPIN, precision="stable", group=group)
+    parts = g.force_parts(torch.as_tensor(phi, device="cuda"))[0].cpu().numpy()
+    for species in (0, 1):
+        want = ref.force_part_stable(hfm, aux, species, 8)
+        err = rel_l2(parts[species], want)
+        assert err < tol, (species, err)
+    # action-level force of the spin basis: f_h(aux) - f_p(aux)
+    want = ref.force_part_stable(hfm, aux, 1, 8) - ref.force_part_stable(hfm, aux, 0, 8)
+    assert rel_l2(g.force(phi).cpu().numpy(), want) < tol
+
+
+def test_stable_matches_plain_where_plain_is_accurate():
+    """Well-conditioned EXP torus (particle-hole basis, mu = 0): stable == reference algorithm."""
+    kappa = ref.honeycomb(4, 8)  # nx = 64, periodic
+    nt, beta = 64, 4.0
+    kt = kappa * beta / nt
+    phi = np.random.RandomState(11).normal(0, 0.3, 64 * nt) + 0j
+    r = ref.HubbardFermiAction(kt, 0.0, -1, ref.EXP, ref.PARTICLE_HOLE)
+    for group in (8, 16, 32):
+        g = act.HubbardFermiAction(kt, 0.0, -1, act.HFAHopping.EXP, precision="stable", group=group)
+        assert mod2pi_diff(g.eval(phi), r.eval(phi)) < 1e-10 * abs(r.eval(phi)), group
+        assert rel_l2(g.force(phi).cpu().numpy(), r.force(phi)) < 1e-9, group
+
+
+def test_stable_batch_matches_single():
+    kt, mt, phi0 = _cylinder(64, seed=3, beta=5.0)
+    _, _, phi1 = _cylinder(64, seed=4, beta=5.0)
+    g = act.HubbardFermiAction(kt, mt, -1, act.HFAHopping.DIA, act.HFABasis.SPIN, precision="stable", group=16)
+    both = torch.as_tensor(np.stack([phi0, phi1]), device="cuda")
+    ld = g.logdets(both)
+    fp = g.force_parts(both).cpu().numpy()
+    for i, phi in enumerate((phi0, phi1)):
+        one = g.logdets(torch.as_tensor(phi, device="cuda"))[0]
+        assert np.max(np.abs(ld[i] - one)) < 1e-12 * np.max(np.abs(one))
+        f1 = g.force_parts(torch.as_tensor(phi, device="cuda"))[0].cpu().numpy()
+        assert np.max(np.abs(fp[i] - f1)) < 1e-9 * np.max(np.abs(f1))
+
+
+def test_stable_rejects_unsupported_nx():
+    kappa = ref.honeycomb(3)  # nx = 18: not a multiple of 32
+    g = act.HubbardFermiAction(kappa * 0.25, 0.0, -1, act.HFAHopping.EXP, precision="stable", group=4)
+    with pytest.raises(Exception):
+        g.logdets(torch.zeros(18 * 16, dtype=torch.complex128, device="cuda"))
