@@ -22,7 +22,8 @@ L, NT, BETA, U, NSTEP, LENGTH = 12, 128, 8.0, 3.0, 40, 1.0
 PEAK = {"f32": 157.3, "f64": 78.6}  # dense MFMA TFLOP/s, MI355X
 KIND_NAMES = ["gemm64_rc_pre", "gemm64_rc_row", "gemm64_cr_pre", "gemm64_cc", "gemm32_rc_row", "gemm32_rc_pre",
               "scale64", "scale32", "lu", "lu_solve", "diag64", "diag32", "copy64", "cvt32to64", "cvt64to32",
-              "phase", "tcvt64to32", "diag64x32"]
+              "phase", "tcvt64to32", "diag64x32", "gemm64_cc_sub", "lu_shape", "block", "sigma", "gather",
+              "tblock"]
 
 
 def parse():
@@ -39,6 +40,10 @@ def parse():
     ap.add_argument("--config4", action="store_true",
                     help="the 256-chain ensemble of uneven lattices (L in {3,4,6,8}, Nt in {16,32,64}, beta=Nt/8, "
                          "U=2, 20 steps): prints chain-trajectories/s instead of the config-3 line")
+    ap.add_argument("--config5", action="store_true",
+                    help="the nanotube workload (12 x 16 cell honeycomb cylinder, Nx=384, Nt=256, beta=10, U=2.5, "
+                         "mu=0.5, DIA, spin basis, 60 steps) with the stable block-cyclic form: prints "
+                         "chain-trajectories/s for --chains chains instead of the config-3 line")
     ap.add_argument("--force-only", action="store_true",
                     help="profiling aid: run --steps batched mixed forces and one 1 GiB device copy "
                          "(the calibration kernel for HBM byte counters), print timings, no HMC")
@@ -82,6 +87,75 @@ def config4(args):
                    "buckets": len(ens.buckets), "parallelism": "buckets on separate streams, one HIP graph"},
         "acceptance": float(pts.mean()) if pts.size else None,
         "median_dH": float(np.median(dh)) if dh.size else None}))
+
+
+def config5(args):
+    """Nanotube workload: honeycomb cylinder of 12 cells around (periodic) and 16 along
+    (open), Nx=384, Nt=256, beta=10, U=2.5, mu=0.5 (complex action), DIA hopping, spin basis,
+    sigmaKappa=-1, HMC length 1 with 60 steps; chain c starts from N(0, U beta/Nt) drawn from
+    NumpyRNG(7 + c).  The product of all 256 slices spans ~33 decades here, so eval and force
+    use precision="stable" (FP64 groups of 32 slices combined by the pivoted block-cyclic
+    elimination).  One step = one trajectory of every chain.  Checks: log det M of chain 0's
+    start field against the oracle's stabilised restatement (CPU, groups of 16), and the force
+    with groups of 32 against groups of 16 (a different grouping of the same elimination)."""
+    import torch
+    import isle_amd as isle
+    from isle_amd import action as A
+    from isle_amd.evolver import BatchedConstStepLeapfrog, NumpyRNG
+    from oracle import hubbard_ref as ref
+    nt, beta, u, mu, nstep = 256, 10.0, 2.5, 0.5, 60
+    chains = args.chains
+    kappa = ref.honeycomb(12, 16, periodic_j=False)
+    nx = kappa.shape[0]
+    kt, mt, ut = kappa * beta / nt, mu * beta / nt, u * beta / nt
+    ferm = A.HubbardFermiAction(kt, mt, -1, A.HFAHopping.DIA, A.HFABasis.SPIN, precision="stable", group=32)
+    act = A.SumAction(A.HubbardGaugeAction(ut), ferm)
+    rngs = [NumpyRNG(7 + c) for c in range(chains)]
+    phi0 = np.stack([r.normal(0, np.sqrt(ut), nx * nt) + 0j for r in rngs])
+    dev = torch.device("cuda")
+    phis = torch.as_tensor(phi0, device=dev)
+    # accuracy on chain 0's start field
+    ld = ferm.logdets(phis[:1])[0]
+    aux = -1j * phi0[0]
+    hfm = ref.HFMDia(kt, mt, -1)
+    t_cpu = time.perf_counter()
+    want = [ref.logdetM_stable(hfm, aux, s, 16) for s in (0, 1)]
+    t_cpu = time.perf_counter() - t_cpu
+    ld_err = max(abs(complex(d.real, np.remainder(d.imag + np.pi, 2 * np.pi) - np.pi)) / abs(w)
+                 for d, w in ((ld[s] - want[s], want[s]) for s in (0, 1)))
+    plain_err = abs(ref.logdetM(hfm, aux, 0) - want[0]) / abs(want[0])
+    f32 = ferm.force(phis[0]).cpu().numpy()
+    f16 = A.HubbardFermiAction(kt, mt, -1, A.HFAHopping.DIA, A.HFABasis.SPIN, precision="stable",
+                               group=16).force(phis[0]).cpu().numpy()
+    f_err = float(np.linalg.norm(f32 - f16) / np.linalg.norm(f16))
+    acts = np.array([act.eval(phis[b]) for b in range(chains)])
+    lf = BatchedConstStepLeapfrog(act, LENGTH, nstep, rngs, graph=not args.no_graph)
+    for _ in range(args.warmup):
+        phis, acts, _ = lf.evolve(phis, acts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        phis, acts, _ = lf.evolve(phis, acts)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    pts = np.array(lf.trajPoints[args.warmup:], dtype=float)
+    dh = np.array(lf.deltaH[args.warmup:], dtype=float)
+    print(json.dumps({
+        "metric": "chain-trajectories/s, Nx=384 Nt=256 DIA spin-basis cylinder (config 5)",
+        "value": chains * args.steps / dt, "unit": "chain-traj/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "honeycomb cylinder 12 (periodic) x 16 (open) cells, Nx=384, Nt=256, beta=10, U=2.5, "
+                               "mu=0.5, DIA, spin basis, sigmaKappa=-1, HMC length 1 with 60 steps",
+                   "chains": chains, "precision": "stable", "group": ferm._group(nt),
+                   "parallelism": "chains batched in every launch"},
+        "logdet_rel_err_vs_oracle": ld_err, "oracle_logdet_cpu_s": t_cpu,
+        "reference_product_logdet_rel_err": float(plain_err),
+        "force_rel_diff_group32_vs_group16": f_err,
+        "acceptance": float(pts.mean()) if pts.size else None,
+        "mean_dH": float(dh.mean()) if dh.size else None,
+        "mean_exp_minus_dH": float(np.exp(-dh).mean()) if dh.size else None,
+        "mean_action_phase_cos": float(np.mean(np.cos(np.imag(acts))))}))
 
 
 def force_only(args):
@@ -154,6 +228,9 @@ def main():
         return
     if args.config4:
         config4(args)
+        return
+    if args.config5:
+        config5(args)
         return
     import torch
     import isle_amd as isle

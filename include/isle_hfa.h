@@ -62,15 +62,20 @@ int isle_logdetM(isle_hfm* h, const double* phi, int nt, int species, double* ou
 
 /* Batched logdetM for the action's eval (hubbardFermiAction.cpp:230-272).
  * mode 0: reference algorithm in FP64; mode 1: FP64 products of groups of `group` slices
- * formed in parallel first.  nspecies 1 (particle) or 2 (particle, hole); spin != 0
- * evaluates at aux = -i*phi (spin basis).  out_host: [batch][nspecies][2]. */
+ * formed in parallel first; mode 2: those group products combined by the pivoted
+ * elimination of their block-cyclic matrix (stable where the product of all slices is not
+ * representable in FP64; needs nx % 32 == 0 and nx <= 480 when nt/group > 1).
+ * nspecies 1 (particle) or 2 (particle, hole); spin != 0 evaluates at aux = -i*phi (spin
+ * basis).  out_host: [batch][nspecies][2]. */
 int isle_logdetM_batch(isle_hfm* h, const double* phi, int nt, int batch, int mode, int group, int nspecies,
                        int spin, double* out_host);
 
 /* forceDirectSinglePart(hfm, phi, K, species) for each species (without the -i factor)
  *   src/isle/cpp/action/hubbardFermiAction.cpp:16-60.
  * mode 0: plain FP64 reference algorithm; mode 1: FP32 chains over groups of `group`
- * slices on MFMA with an FP64 combination of the group products.
+ * slices on MFMA with an FP64 combination of the group products; mode 2: FP64 chains over
+ * the groups with the combination taken from the elimination of mode 2 of logdetM_batch
+ * (same size limits).
  * out_dev: device [batch][nspecies][nx*nt] complex128. */
 int isle_force_parts(isle_hfm* h, const double* phi, int nt, int batch, int mode, int group, int nspecies,
                      int spin, double* out_dev);
@@ -104,13 +109,16 @@ int isle_cscale(const double* in, double* out, long n, double cr, double ci, voi
 
 /* Instrumentation (no counterpart in the reference, which only has timeit benchmarks,
  * benchmarks/logdet.py:46,85): replays a cached operation `reps` times with a HIP event
- * after every launch.  op: 0 plain force, 1 mixed force, 2 plain logdet, 3 grouped logdet.
+ * after every launch.  op: 0 plain force, 1 mixed force, 2 plain logdet, 3 grouped logdet,
+ * 4 stable force, 5 stable logdet.
  * For launch kind k < nkinds: out_host[4k..4k+3] = (launches per operation, ms per
  * operation, algorithmic real flops per operation, 0).  Kinds: 0 GEMM64 real x complex
  * (pre-scaled), 1 GEMM64 real x complex (row-scaled), 2 GEMM64 complex x real, 3 GEMM64
  * complex x complex, 4 GEMM32 real x complex (row-scaled), 5 GEMM32 real x complex
  * (pre-scaled), 6-7 chain starts, 8 LU, 9 LU solve, 10-11 diagonal extraction, 12-14
- * copies, 15 phases, 16 transposed copy. */
+ * copies, 15 phases, 16 transposed copy, 17 diagonal extraction FP64 x FP32, 18 GEMM64
+ * complex x complex subtracted, 19 LU of a block-cyclic elimination step, 20 block
+ * assembly, 21 row permutation, 22 permuted gather, 23 transposed block extraction. */
 int isle_profile_kernels(isle_hfm* h, const double* phi, int op, int nt, int batch, int group, int nspecies,
                          int spin, int reps, double* out_host, int nkinds);
 

@@ -1539,6 +1539,7 @@ int isle_logdetM_batch(isle_hfm* h, const double* phi, int nt, int batch, int mo
                        int spin, double* out_host) {
     return guarded([&] {
         if (batch < 1 || (nspecies != 1 && nspecies != 2)) throw std::invalid_argument("bad batch/nspecies");
+        if (mode < 0 || mode > 2) throw std::invalid_argument("mode must be 0, 1 or 2");
         P(h)->logdetM(phi, nt, batch, mode, group, nspecies, spin, out_host);
     });
 }
@@ -1547,6 +1548,7 @@ int isle_force_parts(isle_hfm* h, const double* phi, int nt, int batch, int mode
                      double* out_dev) {
     return guarded([&] {
         if (batch < 1 || (nspecies != 1 && nspecies != 2)) throw std::invalid_argument("bad batch/nspecies");
+        if (mode < 0 || mode > 2) throw std::invalid_argument("mode must be 0, 1 or 2");
         P(h)->force_parts(phi, nt, batch, mode, group, nspecies, spin, out_dev);
     });
 }
@@ -1562,7 +1564,7 @@ int isle_solveM(isle_hfm* h, const double* phi, int nt, int species, const doubl
 int isle_profile_kernels(isle_hfm* h, const double* phi, int op, int nt, int batch, int group, int nspecies,
                          int spin, int reps, double* out_host, int nkinds) {
     return guarded([&] {
-        if (op < 0 || op > 3 || reps < 1) throw std::invalid_argument("bad profile request");
+        if (op < 0 || op > 5 || reps < 1) throw std::invalid_argument("bad profile request");
         P(h)->profile(phi, op, nt, batch, group, nspecies, spin, reps, out_host, nkinds);
     });
 }
