@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch the integration eagerly, not from a HIP graph")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="parts of the batch integrated on streams of their own inside the graph")
     ap.add_argument("--config4", action="store_true",
                     help="the 256-chain ensemble of uneven lattices (L in {3,4,6,8}, Nt in {16,32,64}, beta=Nt/8, "
                          "U=2, 20 steps): prints chain-trajectories/s instead of the config-3 line")
@@ -262,7 +264,8 @@ def main():
     errs = np.linalg.norm(f_m - f_p, axis=1) / np.linalg.norm(f_p, axis=1)
     force_err = float(errs.max())
     force_err_max = float((np.abs(f_m - f_p).max(axis=1) / np.abs(f_p).max(axis=1)).max())
-    ev = BatchedConstStepLeapfrog(act_m, LENGTH, NSTEP, rngs, graph=not args.no_graph)
+    split = [make("mixed")[0] for _ in range(args.streams)] if args.streams > 1 else None
+    ev = BatchedConstStepLeapfrog(act_m, LENGTH, NSTEP, rngs, graph=not args.no_graph, split=split)
     for _ in range(args.warmup):
         phis, acts, _ = ev.evolve(phis, acts)
     torch.cuda.synchronize()
@@ -325,7 +328,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 force (MFMA) + f64 action", "data": "synthetic",
         "config": {"workload": "honeycomb 12x12 torus Nx=288 Nt=128 beta=8 U=3 mu=0 EXP particle-hole "
                                "sigmaKappa=-1, HMC length 1 with 40 leapfrog steps",
-                   "nx": nx, "nt": NT, "chains": B, "group": g, "parallelism": "independent chains, 1 GPU"},
+                   "nx": nx, "nt": NT, "chains": B, "group": g, "streams": args.streams if not args.no_graph else 1,
+                   "parallelism": "independent chains, 1 GPU"},
         "plain_fp64_traj_per_s": plain,
         "speedup_vs_plain_fp64": (traj_per_s / plain) if plain else None,
         "force_rel_err": force_err, "force_rel_err_max": force_err_max,

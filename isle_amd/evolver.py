@@ -139,8 +139,9 @@ class BatchedConstStepLeapfrog:
     uniform from rngs[b] exactly as ConstStepLeapfrog would, so each chain reproduces a
     lone run with the same seed (up to the arithmetic of the batched kernels)."""
 
-    def __init__(self, action, length, nstep, rngs, graph=False):
+    def __init__(self, action, length, nstep, rngs, graph=False, split=None):
         self.action = action
+        self.split = split  # LeapfrogGraph's split: actions with workspaces of their own
         self.length = length
         self.nstep = nstep
         self.rngs = list(rngs)
@@ -158,7 +159,7 @@ class BatchedConstStepLeapfrog:
         pi = torch.as_tensor(pi_host, device=phis.device)
         if self.graph:
             if self._lf is None:
-                self._lf = LeapfrogGraph(self.action, batch, n, self.length, self.nstep)
+                self._lf = LeapfrogGraph(self.action, batch, n, self.length, self.nstep, split=self.split)
             phi1, pi1 = self._lf(phis, pi)
             acts1 = _eval_batch(self.action, phi1)
         else:

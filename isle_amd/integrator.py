@@ -104,22 +104,54 @@ class LeapfrogGraph:
     nsteps+1 batched force evaluations become one graph launch.  The action's plans are
     built and tuned by an eager warm-up before the capture; the force path then makes no
     host synchronisation, allocation or memset.  The action value is not part of the
-    graph (its log-determinants are read back to the host)."""
+    graph (its log-determinants are read back to the host).
 
-    def __init__(self, action, batch, n, length, nsteps, direction=1.0):
+    ``split``: optional list of k actions equal to ``action`` but with workspaces of their
+    own.  The batch is then cut into k contiguous parts, part i integrated with split[i] on a
+    stream of its own inside the graph, so that the latency-bound phases of one part (the
+    LU and solve of the top level, the bandwidth-bound diagonals) overlap the MFMA-bound
+    chain products of the others.  The graph keeps references to every action it captured,
+    whose plans must not be destroyed while it can be replayed."""
+
+    def __init__(self, action, batch, n, length, nsteps, direction=1.0, split=None):
         dev = _lib.device()
         self.phi = torch.zeros((batch, n), dtype=torch.complex128, device=dev)
         self.pi = torch.zeros_like(self.phi)
-        side = torch.cuda.Stream()
-        side.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(side):
-            leapfrog_batch(self.phi, self.pi, action, length, nsteps, direction, evaluate=False)
-        torch.cuda.current_stream().wait_stream(side)
+        actions = list(split) if split else [action]
+        # the graph holds device pointers into these actions' plans (descriptor arrays and
+        # workspaces): keep the actions alive as long as the graph can be replayed
+        self._actions = actions
+        k = len(actions)
+        if k > batch:
+            raise ValueError("more parts than chains")
+        bounds = [round(i * batch / k) for i in range(k + 1)]
+        parts = [(actions[i], bounds[i], bounds[i + 1]) for i in range(k)]
+        main = torch.cuda.current_stream()
+        streams = [torch.cuda.Stream() for _ in parts]
+        for s, (act, lo, hi) in zip(streams, parts):  # warm-up: builds and tunes every plan
+            s.wait_stream(main)
+            with torch.cuda.stream(s):
+                leapfrog_batch(self.phi[lo:hi], self.pi[lo:hi], act, length, nsteps, direction, evaluate=False)
+            main.wait_stream(s)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            self.out_phi, self.out_pi, _ = leapfrog_batch(self.phi, self.pi, action, length, nsteps, direction,
-                                                          evaluate=False)
+            if k == 1:
+                self.out_phi, self.out_pi, _ = leapfrog_batch(self.phi, self.pi, action, length, nsteps, direction,
+                                                              evaluate=False)
+            else:
+                cap = torch.cuda.current_stream()
+                outs = []
+                for s, (act, lo, hi) in zip(streams, parts):
+                    s.wait_stream(cap)
+                    with torch.cuda.stream(s):
+                        outs.append(leapfrog_batch(self.phi[lo:hi], self.pi[lo:hi], act, length, nsteps, direction,
+                                                   evaluate=False))
+                for s in streams:
+                    cap.wait_stream(s)
+                self.out_phi = torch.cat([o[0] for o in outs])
+                self.out_pi = torch.cat([o[1] for o in outs])
+        self._streams = streams
 
     def __call__(self, phis, pis):
         """Returns (phis', pis') as new tensors."""

@@ -371,6 +371,36 @@ def test_graph_trajectory_replays(precision):
     np.testing.assert_allclose(runs[0][1], runs[1][1], rtol=1e-9, atol=1e-11)
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-4)])
+def test_graph_split_over_streams(precision, tol):
+    """A batch of 5 chains captured as 2 parts on streams of their own (each part with its
+    own action and workspaces) equals the eager trajectory of the whole batch, on two
+    replays with eager work between them (different plans, so agreement to the rounding of
+    the precision, not bit-for-bit)."""
+    from isle_amd.integrator import LeapfrogGraph, leapfrog_batch
+    kappa = ref.honeycomb(3)
+    nx, nt, beta, u = 18, 16, 4.0, 2.0
+    kt, ut = kappa * beta / nt, u * beta / nt
+
+    def make():
+        return act.SumAction(act.HubbardGaugeAction(ut),
+                             act.HubbardFermiAction(kt, 0.0, -1, act.HFAHopping.EXP, precision=precision))
+    full = make()
+    batch = 5
+    lf = LeapfrogGraph(full, batch, nx * nt, 1.0, 6, split=[make(), make()])
+    rng = np.random.RandomState(12)
+    for rep in range(2):
+        phis = _dev(rng.normal(0, np.sqrt(ut), (batch, nx * nt)) + 0j)
+        pis = _dev(rng.normal(0, 1, (batch, nx * nt)) + 0j)
+        full[1].force_batch(_dev(rng.normal(0, 1, (2, nx * nt)) + 0j))  # unrelated eager work
+        torch.cuda.synchronize()
+        g_phi, g_pi = lf(phis, pis)
+        torch.cuda.synchronize()
+        e_phi, e_pi, _ = leapfrog_batch(phis, pis, full, 1.0, 6, evaluate=False)
+        assert rel_l2(g_phi.cpu().numpy(), e_phi.cpu().numpy()) < tol
+        assert rel_l2(g_pi.cpu().numpy(), e_pi.cpu().numpy()) < tol
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("mixed", 5e-5)])
 def test_ensemble_uneven_chains_match_single_runs(precision, tol):
     """Chains of uneven (Nx, Nt) advanced together (buckets on separate streams, one HIP
