@@ -105,6 +105,32 @@ __device__ __forceinline__ unsigned long long pivot_key(double v, int pos) {
            static_cast<unsigned long long>(511 - pos);
 }
 
+// One DPP step on a 64-bit value: both halves move with the same lane pattern, so the
+// result is always some lane's whole value (or 0 where the pattern has no source lane).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)v, CTRL, ROW_MASK, 0xf, true);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), CTRL, ROW_MASK, 0xf, true);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Maximum of a non-negative 64-bit key over the 64 lanes of a wave, in registers: prefix
+// maxima within each row of 16 lanes (row_shr 1, 2, 4, 8), then row 0's maximum into row 1
+// and row 2's into row 3 (row_bcast:15), then lane 31's into rows 2 and 3 (row_bcast:31);
+// lane 63 ends with the wave's maximum, which every lane reads.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k) {
+    unsigned long long o;
+    o = dpp_u64<0x111, 0xf>(k); k = o > k ? o : k;
+    o = dpp_u64<0x112, 0xf>(k); k = o > k ? o : k;
+    o = dpp_u64<0x114, 0xf>(k); k = o > k ? o : k;
+    o = dpp_u64<0x118, 0xf>(k); k = o > k ? o : k;
+    o = dpp_u64<0x142, 0xa>(k); k = o > k ? o : k;
+    o = dpp_u64<0x143, 0xc>(k); k = o > k ? o : k;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)k, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(k >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 template <int NB>
 __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __restrict__ items, int k0,
                                                             int add_identity) {
@@ -162,21 +188,22 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
         double nr = ar[0], ni = ai[0];  // enters the last slot: the multiplier or the departing value
         if (kk < kb) {
             {  // max over the wave first (lanes to one LDS address serialise), then one atomic per wave
-                unsigned long long key = (own && pos >= kk) ? pivot_key(cabs1(ar[0], ai[0]), pos) : 0ull;
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
-                    const unsigned long long o = __shfl_xor(key, off);
-                    key = o > key ? o : key;
-                }
+                const unsigned long long key =
+                    wave_max_u64((own && pos >= kk) ? pivot_key(cabs1(ar[0], ai[0]), pos) : 0ull);
                 if ((tid & 63) == 0 && key != 0ull) atomicMax(&pkey[kk], key);
             }
             __syncthreads();
-            const int p = 511 - static_cast<int>(pkey[kk] & 0x1FFull);  // pivot position
+            // pivot position; an exactly zero column (no nonzero key) keeps row kk in place
+            const unsigned long long kmax = pkey[kk];
+            const int p = kmax ? 511 - static_cast<int>(kmax & 0x1FFull) : kk;
             if (own && pos == p) {
+                // slots NB-kk.. of the pivot row hold its own earlier multipliers: broadcast
+                // zeros there, so that the update below needs no per-slot branch
 #pragma unroll
                 for (int j = 0; j < NB; ++j) {
-                    xrow[2 * j] = ar[j];
-                    xrow[2 * j + 1] = ai[j];
+                    const bool u = j < NB - kk;
+                    xrow[2 * j] = u ? ar[j] : 0.0;
+                    xrow[2 * j + 1] = u ? ai[j] : 0.0;
                 }
             }
             if (own) {  // the interchange of positions kk and p
@@ -196,12 +223,24 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
                     ni = nr * ii + ni * ir;
                     nr = t;
                 }
+                // slots NB-kk.. (earlier multipliers) subtract 0; the pivot row is read from
+                // LDS in batches of 8 entries so that a batch's reads are in flight together
+                constexpr int RB = 8;
 #pragma unroll
-                for (int j = 1; j < NB; ++j) {
-                    if (j < NB - kk) {  // slots NB-kk.. hold this row's earlier multipliers
-                        const double br = xrow[2 * j], bi = xrow[2 * j + 1];
-                        ar[j] -= nr * br - ni * bi;
-                        ai[j] -= nr * bi + ni * br;
+                for (int j0 = 0; j0 < NB; j0 += RB) {
+                    double br[RB], bi[RB];
+#pragma unroll
+                    for (int q = 0; q < RB; ++q) {
+                        br[q] = xrow[2 * (j0 + q)];
+                        bi[q] = xrow[2 * (j0 + q) + 1];
+                    }
+#pragma unroll
+                    for (int q = 0; q < RB; ++q) {
+                        const int j = j0 + q;
+                        if (j >= 1) {
+                            ar[j] -= nr * br[q] - ni * bi[q];
+                            ai[j] -= nr * bi[q] + ni * br[q];
+                        }
                     }
                 }
             }

@@ -278,14 +278,18 @@ def test_config2_gradient_check():
     idx = rng.choice(nx * nt, 64, replace=False)
     # Central differences: truncation O(h^2) plus round-off |S| * eta / h with eta ~ 1e-13 the
     # relative accuracy of the FP64 action.  h = 1e-6 (the prescribed step) is round-off
-    # limited (~|S| 1e-7); h = 1e-4 gives a tight check.
-    for h, tol in ((1e-6, 5 * s0 * 1e-13 / 1e-6), (1e-4, 2e-6)):
-        for i in idx:
-            e = np.zeros(nx * nt, dtype=complex)
-            e[i] = h
-            d = g.eval(phi + e) - g.eval(phi - e)
-            d = complex(d.real, np.remainder(d.imag + np.pi, 2 * np.pi) - np.pi)
-            assert abs(-d / (2 * h) - f[i]) < tol, (h, i, -d / (2 * h), f[i])
+    # limited (~|S| 1e-7).  The tight check extrapolates h = 2e-4 and 1e-4 (Richardson,
+    # truncation O(h^4)), leaving the round-off bound 5 |S| 1e-13 / 1e-4 (1.7e-6 at |S| = 350;
+    # the truncation of the plain h = 1e-4 difference alone reaches 1.7e-6 on this field).
+    def cdiff(i, h):
+        e = np.zeros(nx * nt, dtype=complex)
+        e[i] = h
+        d = g.eval(phi + e) - g.eval(phi - e)
+        return -complex(d.real, np.remainder(d.imag + np.pi, 2 * np.pi) - np.pi) / (2 * h)
+    for i in idx:
+        assert abs(cdiff(i, 1e-6) - f[i]) < 5 * s0 * 1e-13 / 1e-6, (i, cdiff(i, 1e-6), f[i])
+        rich = (4 * cdiff(i, 1e-4) - cdiff(i, 2e-4)) / 3
+        assert abs(rich - f[i]) < 5 * s0 * 1e-13 / 1e-4, (i, rich, f[i])
     for s in range(2000, 2016):
         phi = np.random.RandomState(s).normal(0, np.sqrt(ut), nx * nt) + 0j
         assert rel_l2(g.force(phi).cpu().numpy(), r.force(phi)) < 1e-10
