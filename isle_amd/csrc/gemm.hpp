@@ -374,7 +374,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<T
     static_assert(ACH % 256 == 0 && BCH % 256 == 0, "slab chunks must cover whole waves");
     static_assert((BM / EPC) % 8 == 0 && (BN / EPC) % 8 == 0, "swizzle needs multiples of 8 chunks per row");
     static_assert(2 * BK / EPC <= 64 && BK % 16 == 0, "pre copy must fit one wave instruction");
-    static_assert(RCD_STAGES >= 3 && RCD_STAGES <= 4, "two or three slabs in flight");
+    static_assert(RCD_STAGES >= 2 && RCD_STAGES <= 4, "one to three slabs in flight");
     constexpr int AQ = ACH / 256, BQ = BCH / 256;  // direct loads per wave per plane
     constexpr bool BLOCKED = sizeof(T) == 4;       // FP32: each 16 k summed in a fresh accumulator
     typedef typename Acc4<T>::type V4;
@@ -507,7 +507,11 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<T
 //   direct-to-LDS (gemm_rcd_kernel; real symmetric A times a complex B only, other operand
 //   kinds report "does not fit"), K slab S = 32 in FP32, 16 in FP64:
 //   6: 64x64 slab S   7: 96x96 slab S   8: 32x32 slab S   9: 64x64 slab 2S   10: 64x64 slab S, 4 stages
-enum : int { N_TILES = 11 };
+//   11: 32x32 slab S, 2 stages   12: 32x32 slab 2S, 2 stages   13: 32x32 slab S, 4 stages
+//   14: 32x32 slab 2S  (fewer stages: less LDS per workgroup, more workgroups per CU)
+//   15: 96x32 slab S   16: 96x32 slab S, 2 stages   17: 96x32 slab 2S, 2 stages
+//   (48x16 per wave: 5 LDS fragment reads per 6 MFMAs instead of 3 per 2 at 32x32)
+enum : int { N_TILES = 18 };
 
 template <typename T, int POST, bool PRE, int BM, int BN, int BK, int ST = 3>
 inline bool launch_gemm_rcd(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s) {
@@ -553,13 +557,20 @@ inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int m
     if (tile >= 6) {
         if constexpr (!AC && BC && POST != SUBTRACT) {
             constexpr int BK = sizeof(T) == 4 ? 32 : 16;
-            const int slab = tile == 9 ? 2 * BK : BK;
+            const int slab = (tile == 9 || tile == 12 || tile == 14 || tile == 17) ? 2 * BK : BK;
             if (direct_k <= 0 || direct_k % slab != 0) return false;
             switch (tile) {
             case 6: return launch_gemm_rcd<T, POST, PRE, 64, 64, BK>(d_items, nitems, max_m, max_n, s);
             case 7: return launch_gemm_rcd<T, POST, PRE, 96, 96, BK>(d_items, nitems, max_m, max_n, s);
             case 8: return launch_gemm_rcd<T, POST, PRE, 32, 32, BK>(d_items, nitems, max_m, max_n, s);
             case 9: return launch_gemm_rcd<T, POST, PRE, 64, 64, 2 * BK>(d_items, nitems, max_m, max_n, s);
+            case 11: return launch_gemm_rcd<T, POST, PRE, 32, 32, BK, 2>(d_items, nitems, max_m, max_n, s);
+            case 12: return launch_gemm_rcd<T, POST, PRE, 32, 32, 2 * BK, 2>(d_items, nitems, max_m, max_n, s);
+            case 13: return launch_gemm_rcd<T, POST, PRE, 32, 32, BK, 4>(d_items, nitems, max_m, max_n, s);
+            case 14: return launch_gemm_rcd<T, POST, PRE, 32, 32, 2 * BK>(d_items, nitems, max_m, max_n, s);
+            case 15: return launch_gemm_rcd<T, POST, PRE, 96, 32, BK>(d_items, nitems, max_m, max_n, s);
+            case 16: return launch_gemm_rcd<T, POST, PRE, 96, 32, BK, 2>(d_items, nitems, max_m, max_n, s);
+            case 17: return launch_gemm_rcd<T, POST, PRE, 96, 32, 2 * BK, 2>(d_items, nitems, max_m, max_n, s);
             default: return launch_gemm_rcd<T, POST, PRE, 64, 64, BK, 4>(d_items, nitems, max_m, max_n, s);
             }
         } else {
