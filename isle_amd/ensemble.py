@@ -12,6 +12,7 @@ ConstStepLeapfrog would draw them, so each chain reproduces a single-chain run w
 same seed up to the arithmetic of the batched kernels.
 """
 from collections import OrderedDict
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -73,6 +74,7 @@ class EnsembleLeapfrog:
         self.buckets = [_Bucket([self.chains[i] for i in idx], hopping, precision, group) for idx in self.index]
         self.graph = bool(graph)
         self._graph = None
+        self._pool = ThreadPoolExecutor(max_workers=16)
         self.trajPoints, self.deltaH = [], []
 
     def phis(self):
@@ -107,30 +109,45 @@ class EnsembleLeapfrog:
             for s in self._streams:
                 cap.wait_stream(s)
 
+    def _momenta(self):
+        """Every chain's momenta rng.normal(0, 1, n) (real parts), drawn on a thread pool: the
+        chains' generators are independent and numpy releases the GIL while drawing."""
+        real = [np.empty((len(b.chains), b.n)) for b in self.buckets]
+        jobs = [(bi, j) for bi, b in enumerate(self.buckets) for j in range(len(b.chains))]
+
+        def draw(job):
+            bi, j = job
+            real[bi][j] = self.buckets[bi].chains[j].rng.normal(0, 1, self.buckets[bi].n)
+        list(self._pool.map(draw, jobs))
+        return real
+
     def evolve(self):
-        pis_host = [np.stack([c.rng.normal(0, 1, b.n) + 0j for c in b.chains]) for b in self.buckets]
+        pis_host = self._momenta()
         if self.graph:
             if self._graph is None:
                 self._capture()
             for b, (p, q), ph in zip(self.buckets, self._in, pis_host):
                 p.copy_(b.phi)
-                q.copy_(torch.as_tensor(ph, device=q.device))
+                q.zero_()
+                q.real.copy_(torch.from_numpy(ph))
             self._graph.replay()
             results = [(p1.clone(), q1.clone()) for p1, q1 in self._out]
         else:
             results = []
             for b, ph in zip(self.buckets, pis_host):
-                p1, q1, _ = leapfrog_batch(b.phi, torch.as_tensor(ph, device=b.phi.device), b.action, self.length,
-                                           self.nstep, evaluate=False)
+                q = torch.zeros_like(b.phi)
+                q.real.copy_(torch.from_numpy(ph))
+                p1, q1, _ = leapfrog_batch(b.phi, q, b.action, self.length, self.nstep, evaluate=False)
                 results.append((p1, q1))
         points = [None] * len(self.chains)
         dhs = [None] * len(self.chains)
         for idx, b, (p1, q1), ph in zip(self.index, self.buckets, results, pis_host):
             acts1 = np.asarray(eval_batch(b.action, p1), dtype=complex)
-            q1h = q1.cpu().numpy()
+            kin0 = np.einsum("ij,ij->i", ph, ph)
+            kin1 = (q1.real.square() + q1.imag.square()).sum(dim=1).cpu().numpy()
             for j, i in enumerate(idx):
-                e0 = b.acts[j] + np.linalg.norm(ph[j]) ** 2 / 2
-                e1 = acts1[j] + np.linalg.norm(q1h[j]) ** 2 / 2
+                e0 = b.acts[j] + kin0[j] / 2
+                e1 = acts1[j] + kin1[j] / 2
                 dhs[i] = float(np.real(e1 - e0))
                 tp = b.selectors[j].selectTrajPoint(e0, e1)
                 points[i] = tp

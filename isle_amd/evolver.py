@@ -13,6 +13,8 @@ trajectory loop, with the reference's semantics and random-number consumption.
 ``BatchedConstStepLeapfrog`` advances many independent chains together (one field tensor
 [batch, nx*nt]); each chain has its own NumpyRNG, consumed exactly as a lone chain would.
 """
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import torch
 
@@ -150,13 +152,29 @@ class BatchedConstStepLeapfrog:
         self.deltaH = []
         self.graph = bool(graph)  # replay the integration from a captured HIP graph
         self._lf = None
+        self._pool = None
+
+    def momenta(self, n, device):
+        """Each chain's momenta rng.normal(0, 1, n) + 0j (leapfrog.py:51), drawn on a thread
+        pool (the chains' generators are independent and numpy releases the GIL while
+        drawing); returns the real parts on the host and the complex momenta on the device."""
+        batch = len(self.rngs)
+        if self._pool is None:
+            self._pool = ThreadPoolExecutor(max_workers=min(16, batch))
+        real = np.empty((batch, n))
+
+        def draw(b):
+            real[b] = self.rngs[b].normal(0, 1, n)
+        list(self._pool.map(draw, range(batch)))
+        pi = torch.zeros((batch, n), dtype=torch.complex128, device=device)
+        pi.real.copy_(torch.from_numpy(real))
+        return real, pi
 
     def evolve(self, phis, actVals):
         """phis: [batch, nx*nt] device tensor, actVals: numpy complex [batch].
         Returns (phis', actVals', trajPoints)."""
         batch, n = phis.shape
-        pi_host = np.stack([r.normal(0, 1, n) + 0j for r in self.rngs])
-        pi = torch.as_tensor(pi_host, device=phis.device)
+        real0, pi = self.momenta(n, phis.device)
         if self.graph:
             if self._lf is None:
                 self._lf = LeapfrogGraph(self.action, batch, n, self.length, self.nstep, split=self.split)
@@ -164,14 +182,15 @@ class BatchedConstStepLeapfrog:
             acts1 = _eval_batch(self.action, phi1)
         else:
             phi1, pi1, acts1 = leapfrog_batch(phis, pi, self.action, self.length, self.nstep)
-        pi1h = pi1.cpu().numpy()
+        kin0 = np.einsum("ij,ij->i", real0, real0)  # |pi|^2 of each chain
+        kin1 = (pi1.real.square() + pi1.imag.square()).sum(dim=1).cpu().numpy()
         new_phi = phis.clone()
         new_act = np.array(actVals, dtype=complex)
         points = []
         dh = []
         for b in range(batch):
-            e0 = actVals[b] + np.linalg.norm(pi_host[b]) ** 2 / 2
-            e1 = acts1[b] + np.linalg.norm(pi1h[b]) ** 2 / 2
+            e0 = actVals[b] + kin0[b] / 2
+            e1 = acts1[b] + kin1[b] / 2
             dh.append(np.real(e1 - e0))
             tp = self.selectors[b].selectTrajPoint(e0, e1)
             points.append(tp)
