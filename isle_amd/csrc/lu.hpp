@@ -142,7 +142,6 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
     __shared__ double ut_re[NB][NB + 1], ut_im[NB][NB + 1];  // U11, inverted in place
     __shared__ double tl_re[NB][NB + 1], tl_im[NB][NB + 1];  // doubling temporaries
     __shared__ double tu_re[NB][NB + 1], tu_im[NB][NB + 1];
-    __shared__ int orig_of[NB];
     __shared__ int slot_cnt, slot_dst[2 * NB], slot_src[2 * NB];
     __shared__ double rsum[LU_MAX_N / 64];
 
@@ -270,7 +269,6 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
                 slot_src[sl] = tid;
             }
         }
-        if (pos < NB) orig_of[pos] = tid;
     }
     // L11 and U11 (diagonal inverted) into LDS from the rows now at positions 0..kb-1;
     // positions kb..NB-1 of a partial last panel are padded with the identity
@@ -361,56 +359,6 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
         }
     }
 
-    const int nrt = ncols - k0 - NB;  // columns right of the panel
-    if (nrt > 0) {
-        // One column per thread, all row reads of a column in flight at once:
-        //   staged A12 = the new rows k0..k0+NB-1 (positions q read original row orig_of[q]);
-        //   then the displaced rows below the panel take their new content, which is always an
-        //   original panel row (rows k0..k0+NB-1 are not written here, A12 goes to the stage).
-        // A column's reads precede its writes in the same thread, so no barrier is needed.
-        double* st_re = lu_stage(it);
-        double* st_im = st_re + (size_t)NB * ncols;
-        constexpr int PG = 16;  // rows copied per batch (loads of a batch are all in flight)
-        for (int c = tid; c < nrt; c += nth) {
-            const long col = k0 + NB + c;
-#pragma unroll
-            for (int q0 = 0; q0 < NB; q0 += PG) {
-                double vr[PG], vi[PG];
-#pragma unroll
-                for (int j = 0; j < PG; ++j) {
-                    const long src = (long)lu_row(k0, ntop, orig_of[q0 + j]) * ld + col;
-                    vr[j] = re[src];
-                    vi[j] = im[src];
-                }
-#pragma unroll
-                for (int j = 0; j < PG; ++j) {
-                    st_re[(long)(q0 + j) * ncols + c] = vr[j];
-                    st_im[(long)(q0 + j) * ncols + c] = vi[j];
-                }
-            }
-            for (int q0 = 0; q0 < cnt; q0 += PG) {
-                double wr[PG], wi[PG];
-#pragma unroll
-                for (int j = 0; j < PG; ++j) {
-                    const int q = q0 + j;
-                    if (q < cnt && slot_dst[q] >= NB) {
-                        const long so = (long)lu_row(k0, ntop, slot_src[q]) * ld + col;
-                        wr[j] = re[so];
-                        wi[j] = im[so];
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < PG; ++j) {
-                    const int q = q0 + j;
-                    if (q < cnt && slot_dst[q] >= NB) {
-                        const long d = (long)lu_row(k0, ntop, slot_dst[q]) * ld + col;
-                        re[d] = wr[j];
-                        im[d] = wi[j];
-                    }
-                }
-            }
-        }
-    }
     if (it.logdet && k0 + NB >= kf) {
         // the item's last panel: every U_ii and pivot of the kf columns is final
         __syncthreads();
@@ -428,6 +376,81 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
             if (((long)pc) & 1) la += M_PI;
             it.logdet[0] = lr_;
             it.logdet[1] = remainder(la, 2.0 * M_PI);
+        }
+    }
+}
+
+// The panel's interchanges on the columns right of it, on every CU: the new rows
+// k0..k0+NB-1 (A12) go to the stage (position q reads the original row of the row now at q);
+// then the rows displaced below the panel take their new content, which is always an
+// original panel row (rows k0..k0+NB-1 are not written here).  A staged read of a row
+// below the panel precedes the displaced write to that row: both are in the same column,
+// and every column belongs to one workgroup (64 columns, 4 threads per column, a barrier
+// between the two phases).  grid.x: column blocks; grid.y: item.
+__global__ __launch_bounds__(256) void lu_rswap_kernel(const LuItem* __restrict__ items, int k0) {
+    constexpr int NB = LU_NB, CW = 64, PARTS = 256 / CW, QP = NB / PARTS;
+    const LuItem it = items[blockIdx.y];
+    const int kf = lu_kf(it), ncols = lu_ncols(it), ntop = it.nt, ld = it.ld;
+    if (k0 >= kf) return;
+    const int nrt = ncols - k0 - NB;
+    const int c0 = blockIdx.x * CW;
+    if (c0 >= nrt) return;
+    __shared__ int orig[NB], dst[2 * NB], src[2 * NB];
+    __shared__ int cnt_s;
+    const int* perm = lu_perm(it);
+    const int tid = threadIdx.x;
+    if (tid == 0) cnt_s = min(perm[0], 2 * NB);
+    if (tid < NB) orig[tid] = tid;
+    __syncthreads();
+    const int cnt = cnt_s;
+    if (tid < cnt) {
+        dst[tid] = perm[1 + tid];
+        src[tid] = perm[1 + 2 * NB + tid];
+    }
+    __syncthreads();
+    if (tid < cnt && dst[tid] < NB) orig[dst[tid]] = src[tid];
+    __syncthreads();
+    double* re = it.re;
+    double* im = it.im;
+    const int c = c0 + (tid % CW), part = tid / CW;
+    const bool ok = c < nrt;
+    const long col = k0 + NB + c;
+    double* st_re = lu_stage(it);
+    double* st_im = st_re + (size_t)NB * ncols;
+    {
+        double vr[QP], vi[QP];
+#pragma unroll
+        for (int j = 0; j < QP; ++j) {
+            const long o = (long)lu_row(k0, ntop, orig[part + PARTS * j]) * ld + col;
+            vr[j] = ok ? re[o] : 0.0;
+            vi[j] = ok ? im[o] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < QP; ++j)
+            if (ok) {
+                st_re[(long)(part + PARTS * j) * ncols + c] = vr[j];
+                st_im[(long)(part + PARTS * j) * ncols + c] = vi[j];
+            }
+    }
+    __syncthreads();
+    constexpr int EP = 2 * NB / PARTS;  // displaced entries per thread, at most
+    double wr[EP], wi[EP];
+#pragma unroll
+    for (int j = 0; j < EP; ++j) {
+        const int e = part + PARTS * j;
+        if (ok && e < cnt && dst[e] >= NB) {
+            const long o = (long)lu_row(k0, ntop, src[e]) * ld + col;
+            wr[j] = re[o];
+            wi[j] = im[o];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < EP; ++j) {
+        const int e = part + PARTS * j;
+        if (ok && e < cnt && dst[e] >= NB) {
+            const long o = (long)lu_row(k0, ntop, dst[e]) * ld + col;
+            re[o] = wr[j];
+            im[o] = wi[j];
         }
     }
 }
@@ -581,6 +604,10 @@ inline void launch_lu_shape(const LuItem* d_items, int nitems, int m, int ncols,
                            k0 == 0 ? add_identity : 0);
         ISLE_HIP(hipGetLastError());
         const int ntr = rows - LU_NB, ntc = ncols - k0 - LU_NB;
+        if (ntc > 0) {
+            hipLaunchKernelGGL(lu_rswap_kernel, dim3((ntc + 63) / 64, nitems), dim3(256), 0, s, d_items, k0);
+            ISLE_HIP(hipGetLastError());
+        }
         const int tmr = (ntr > 0 && ntc > 0) ? (ntr + LU_TILE - 1) / LU_TILE : 0;
         const int tmc = (ntr > 0 && ntc > 0) ? (ntc + LU_TILE - 1) / LU_TILE : 0;
         const int swap_wgs = (k0 + LU_SWAP_COLS - 1) / LU_SWAP_COLS;
