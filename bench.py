@@ -31,7 +31,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="timed trajectories (per chain) of the batched form")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--chains", type=int, default=32, help="independent chains advanced together")
+    ap.add_argument("--chains", type=int, default=64, help="independent chains advanced together")
     ap.add_argument("--group", type=int, default=32, help="time slices per group of the mixed force")
     ap.add_argument("--plain-steps", type=int, default=1, help="timed trajectories of the plain FP64 form")
     ap.add_argument("--no-plain", action="store_true")

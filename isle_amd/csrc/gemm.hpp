@@ -511,7 +511,8 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<T
 //   14: 32x32 slab 2S  (fewer stages: less LDS per workgroup, more workgroups per CU)
 //   15: 96x32 slab S   16: 96x32 slab S, 2 stages   17: 96x32 slab 2S, 2 stages
 //   (48x16 per wave: 5 LDS fragment reads per 6 MFMAs instead of 3 per 2 at 32x32)
-enum : int { N_TILES = 18 };
+//   register-staged again (any operand kinds): 18: 96x32, K slab 32   19: 96x32, K slab 16
+enum : int { N_TILES = 20 };
 
 template <typename T, int POST, bool PRE, int BM, int BN, int BK, int ST = 3>
 inline bool launch_gemm_rcd(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s) {
@@ -554,7 +555,7 @@ inline bool launch_gemm_tile(const GemmItem<T>* d_items, int nitems, int max_m, 
 template <typename T, bool AC, bool BC, bool PRE, int POST>
 inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s,
                         int tile = 0, int direct_k = 0) {
-    if (tile >= 6) {
+    if (tile >= 6 && tile < 18) {
         if constexpr (!AC && BC && POST != SUBTRACT) {
             constexpr int BK = sizeof(T) == 4 ? 32 : 16;
             const int slab = (tile == 9 || tile == 12 || tile == 14 || tile == 17) ? 2 * BK : BK;
@@ -583,6 +584,8 @@ inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int m
     case 3: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 32>(d_items, nitems, max_m, max_n, s);
     case 4: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 96, 32>(d_items, nitems, max_m, max_n, s);
     case 5: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 96, 16>(d_items, nitems, max_m, max_n, s);
+    case 18: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 32, 32>(d_items, nitems, max_m, max_n, s);
+    case 19: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 32, 16>(d_items, nitems, max_m, max_n, s);
     default: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 16>(d_items, nitems, max_m, max_n, s);
     }
 }
