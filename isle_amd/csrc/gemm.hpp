@@ -312,12 +312,12 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* _
 // chunk, since a wave's direct load writes 64 consecutive 16-byte chunks.
 // Requirements (the caller vouches): k % BK == 0; m, n, lda, ldb, ldc multiples of 4;
 // 16-byte aligned operands and phase vectors.
-template <typename T, int BM, int BN, int BK, int RCD_STAGES = 3>
+template <typename T, int BM, int BN, int BK, int RCD_STAGES = 3, int NW = 4>
 struct RcdLayout {
     static constexpr int EPC = 16 / sizeof(T);   // elements per 16-byte chunk
     static constexpr int A = BK * BM;            // elements
     static constexpr int B = BK * BN;            // per plane
-    static constexpr int PRE = 4 * 2 * BK;       // one copy of pre (re[BK], im[BK]) per wave
+    static constexpr int PRE = NW * 2 * BK;      // one copy of pre (re[BK], im[BK]) per wave
     static constexpr int STAGE = A + 2 * B + PRE;
     static constexpr size_t BYTES = sizeof(T) * STAGE * RCD_STAGES;
 };
@@ -329,13 +329,13 @@ __device__ __forceinline__ int rcd_swz(int k, int m) {
 
 // Issues this wave's direct copies of slab `slab` (A rows, B re/im rows, and with PRE the
 // phase factors) into stage slab % RCD_STAGES of the LDS ring.
-template <typename T, bool PRE, int BM, int BN, int BK, int RCD_STAGES>
+template <typename T, bool PRE, int BM, int BN, int BK, int RCD_STAGES, int NW>
 __device__ __forceinline__ void rcd_issue(const GemmItem<T>& it, T* lds, int slab, int m0, int n0, int wave,
                                           int lane) {
     constexpr int EPC = 16 / sizeof(T);
-    constexpr int AQ = BK * BM / EPC / 256, BQ = BK * BN / EPC / 256;
+    constexpr int AQ = BK * BM / EPC / (64 * NW), BQ = BK * BN / EPC / (64 * NW);
     constexpr int OFF_BR = BK * BM, OFF_BI = BK * (BM + BN), OFF_PRE = BK * (BM + 2 * BN);
-    constexpr int STAGE_E = OFF_PRE + 4 * 2 * BK;
+    constexpr int STAGE_E = OFF_PRE + NW * 2 * BK;
     constexpr int CH = 64 * EPC;  // elements written by one wave instruction
     constexpr int PC = BK / EPC;  // chunks of one phase plane
     T* st = lds + (slab % RCD_STAGES) * STAGE_E;
@@ -364,18 +364,28 @@ __device__ __forceinline__ void rcd_issue(const GemmItem<T>& it, T* lds, int sla
     }
 }
 
-template <typename T, int POST, bool PRE, int BM, int BN, int BK, int RCD_STAGES>
-__global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items,
+// Waves per workgroup: WM along m times WN along n, each owning a (BM/WM) x (BN/WN) block.
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+constexpr bool rcd_shape_ok() {
+    constexpr int EPC = 16 / sizeof(T), NW = WM * WN;
+    return (BK * BM / EPC) % (64 * NW) == 0 && (BK * BN / EPC) % (64 * NW) == 0 && (BM / EPC) % 8 == 0 &&
+           (BN / EPC) % 8 == 0 && 2 * BK / EPC <= 64 && BK % 16 == 0 && BM % (16 * WM) == 0 && BN % (16 * WN) == 0;
+}
+
+template <typename T, int POST, bool PRE, int BM, int BN, int BK, int RCD_STAGES, int WM = 2, int WN = 2>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items,
                                                                 int tiles_n_max) {
-    typedef RcdLayout<T, BM, BN, BK, RCD_STAGES> LY;
+    constexpr int NW = WM * WN;
+    typedef RcdLayout<T, BM, BN, BK, RCD_STAGES, NW> LY;
     constexpr int EPC = 16 / sizeof(T);
-    constexpr int MI = BM / 32, NJ = BN / 32;
+    constexpr int MI = BM / (16 * WM), NJ = BN / (16 * WN);
     constexpr int ACH = BK * BM / EPC, BCH = BK * BN / EPC;  // 16-byte chunks per slab
-    static_assert(ACH % 256 == 0 && BCH % 256 == 0, "slab chunks must cover whole waves");
+    static_assert(rcd_shape_ok<T, BM, BN, BK, WM, WN>(), "tile shape");
+    static_assert(ACH % (64 * NW) == 0 && BCH % (64 * NW) == 0, "slab chunks must cover whole waves");
     static_assert((BM / EPC) % 8 == 0 && (BN / EPC) % 8 == 0, "swizzle needs multiples of 8 chunks per row");
     static_assert(2 * BK / EPC <= 64 && BK % 16 == 0, "pre copy must fit one wave instruction");
     static_assert(RCD_STAGES >= 2 && RCD_STAGES <= 4, "one to three slabs in flight");
-    constexpr int AQ = ACH / 256, BQ = BCH / 256;  // direct loads per wave per plane
+    constexpr int AQ = ACH / (64 * NW), BQ = BCH / (64 * NW);  // direct loads per wave per plane
     constexpr bool BLOCKED = sizeof(T) == 4;       // FP32: each 16 k summed in a fresh accumulator
     typedef typename Acc4<T>::type V4;
     extern __shared__ __align__(16) unsigned char rcd_raw[];
@@ -387,12 +397,12 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<T
     const int m0 = (tile / tiles_n_max) * BM, n0 = (tile % tiles_n_max) * BN;
     if (m0 >= it.m || n0 >= it.n) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
+    const int wm = (wave / WN) * (BM / WM), wn = (wave % WN) * (BN / WN);
     const int lr = lane & 15, lk = lane >> 4;
     const int nslab = it.k / BK;
 
     constexpr int OFF_BR = BK * BM, OFF_BI = BK * (BM + BN), OFF_PRE = BK * (BM + 2 * BN);
-    constexpr int STAGE_E = OFF_PRE + 4 * 2 * BK;
+    constexpr int STAGE_E = OFF_PRE + NW * 2 * BK;
     static_assert(STAGE_E == LY::STAGE, "layout");
     // direct loads issued per wave and slab (the pre copy is one instruction)
     constexpr int PER_SLAB = AQ + 2 * BQ + (PRE ? 1 : 0);
@@ -405,7 +415,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<T
 
     // slabs are issued RCD_STAGES-1 ahead of the one being computed
     for (int j = 0; j < RCD_STAGES - 1 && j < nslab; ++j)
-        rcd_issue<T, PRE, BM, BN, BK, RCD_STAGES>(it, rcd_lds, j, m0, n0, wave, lane);
+        rcd_issue<T, PRE, BM, BN, BK, RCD_STAGES, NW>(it, rcd_lds, j, m0, n0, wave, lane);
     for (int s = 0; s < nslab; ++s) {
         // wait for this wave's copies of slab s (the later slabs already issued stay in
         // flight), then for all waves.  A plain s_barrier: __syncthreads() would also drain
@@ -418,7 +428,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<T
         else
             asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
         if (s + RCD_STAGES - 1 < nslab)  // into the buffer every wave finished reading
-            rcd_issue<T, PRE, BM, BN, BK, RCD_STAGES>(it, rcd_lds, s + RCD_STAGES - 1, m0, n0, wave, lane);
+            rcd_issue<T, PRE, BM, BN, BK, RCD_STAGES, NW>(it, rcd_lds, s + RCD_STAGES - 1, m0, n0, wave, lane);
         const T* st = rcd_lds + (s % RCD_STAGES) * STAGE_E;
         const T* As = st;
         const T* Br = st + OFF_BR;
@@ -512,19 +522,26 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_rcd_kernel(const GemmItem<T
 //   15: 96x32 slab S   16: 96x32 slab S, 2 stages   17: 96x32 slab 2S, 2 stages
 //   (48x16 per wave: 5 LDS fragment reads per 6 MFMAs instead of 3 per 2 at 32x32)
 //   register-staged again (any operand kinds): 18: 96x32, K slab 32   19: 96x32, K slab 16
-enum : int { N_TILES = 20 };
+//   direct, more waves sharing one LDS ring (48x16 per wave): 20: 96x96 slab S, 12 waves, 2 stages
+//   21: 96x96 slab S, 12 waves, 3 stages   22: 96x48 slab S, 6 waves, 2 stages (FP64 only)
+//   23: 96x96 slab S, 6 waves of 48x32, 2 stages   24: the same, 3 stages
+enum : int { N_TILES = 25 };
 
-template <typename T, int POST, bool PRE, int BM, int BN, int BK, int ST = 3>
+template <typename T, int POST, bool PRE, int BM, int BN, int BK, int ST = 3, int WM = 2, int WN = 2>
 inline bool launch_gemm_rcd(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s) {
-    const size_t lds = RcdLayout<T, BM, BN, BK, ST>::BYTES;
-    if (lds > 160 * 1024) return false;
-    auto kern = gemm_rcd_kernel<T, POST, PRE, BM, BN, BK, ST>;
-    ISLE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)lds));
-    const int tm = (max_m + BM - 1) / BM, tn = (max_n + BN - 1) / BN;
-    hipLaunchKernelGGL(kern, dim3(tm * tn, nitems), dim3(GEMM_THREADS), lds, s, d_items, tn);
-    ISLE_HIP(hipGetLastError());
-    return true;
+    if constexpr (!rcd_shape_ok<T, BM, BN, BK, WM, WN>()) {
+        return false;
+    } else {
+        const size_t lds = RcdLayout<T, BM, BN, BK, ST, WM * WN>::BYTES;
+        if (lds > 160 * 1024) return false;
+        auto kern = gemm_rcd_kernel<T, POST, PRE, BM, BN, BK, ST, WM, WN>;
+        ISLE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const int tm = (max_m + BM - 1) / BM, tn = (max_n + BN - 1) / BN;
+        hipLaunchKernelGGL(kern, dim3(tm * tn, nitems), dim3(64 * WM * WN), lds, s, d_items, tn);
+        ISLE_HIP(hipGetLastError());
+        return true;
+    }
 }
 
 template <typename T, bool AC, bool BC, int BM, int BN, int BK>
@@ -555,7 +572,7 @@ inline bool launch_gemm_tile(const GemmItem<T>* d_items, int nitems, int max_m, 
 template <typename T, bool AC, bool BC, bool PRE, int POST>
 inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s,
                         int tile = 0, int direct_k = 0) {
-    if (tile >= 6 && tile < 18) {
+    if ((tile >= 6 && tile < 18) || tile >= 20) {
         if constexpr (!AC && BC && POST != SUBTRACT) {
             constexpr int BK = sizeof(T) == 4 ? 32 : 16;
             const int slab = (tile == 9 || tile == 12 || tile == 14 || tile == 17) ? 2 * BK : BK;
@@ -572,6 +589,11 @@ inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int m
             case 15: return launch_gemm_rcd<T, POST, PRE, 96, 32, BK>(d_items, nitems, max_m, max_n, s);
             case 16: return launch_gemm_rcd<T, POST, PRE, 96, 32, BK, 2>(d_items, nitems, max_m, max_n, s);
             case 17: return launch_gemm_rcd<T, POST, PRE, 96, 32, 2 * BK, 2>(d_items, nitems, max_m, max_n, s);
+            case 20: return launch_gemm_rcd<T, POST, PRE, 96, 96, BK, 2, 2, 6>(d_items, nitems, max_m, max_n, s);
+            case 21: return launch_gemm_rcd<T, POST, PRE, 96, 96, BK, 3, 2, 6>(d_items, nitems, max_m, max_n, s);
+            case 22: return launch_gemm_rcd<T, POST, PRE, 96, 48, BK, 2, 2, 3>(d_items, nitems, max_m, max_n, s);
+            case 23: return launch_gemm_rcd<T, POST, PRE, 96, 96, BK, 2, 2, 3>(d_items, nitems, max_m, max_n, s);
+            case 24: return launch_gemm_rcd<T, POST, PRE, 96, 96, BK, 3, 2, 3>(d_items, nitems, max_m, max_n, s);
             default: return launch_gemm_rcd<T, POST, PRE, 64, 64, BK, 4>(d_items, nitems, max_m, max_n, s);
             }
         } else {
