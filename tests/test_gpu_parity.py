@@ -295,6 +295,32 @@ def test_config2_gradient_check():
         assert rel_l2(g.force(phi).cpu().numpy(), r.force(phi)) < 1e-10
 
 
+@pytest.mark.parametrize("precision,tol_phi,tol_pi,tol_s", [("fp64", 1e-7, 1e-6, 1e-8), ("mixed", 1e-6, 1e-5, 1e-7)])
+def test_config3_leapfrog_start_matches_oracle(precision, tol_phi, tol_pi, tol_s):
+    """Nx=288, Nt=128, beta=8, U=3 (the bench workload), chain 0's start field and momenta
+    (seed 1000): the first 4 of the 40 leapfrog steps (eps = 1/40) against the oracle.
+    Measured: FP64 phi 4e-9, pi 5e-8, S 1e-10; mixed phi 4e-8, pi 6e-7, S 1e-9.  (Near a zero
+    of det M two FP64 evaluations of the reference algorithm with different summation orders
+    already differ by ~1e-8 in the force; later in the trajectory the difference grows
+    exponentially -- 3e-3 in pi after 8 steps, O(1) after 24 -- so whole trajectories from
+    these starts are compared through their accept decisions, not their Delta H.)"""
+    from isle_amd.integrator import leapfrog
+    kappa = ref.honeycomb(12)
+    nx, nt, beta, u = 288, 128, 8.0, 3.0
+    kt, ut = kappa * beta / nt, u * beta / nt
+    rng = np.random.RandomState(1000)
+    phi = rng.normal(0, np.sqrt(ut), nx * nt) + 0j
+    pi = rng.normal(0, 1, nx * nt) + 0j
+    r = ref.SumAction(ref.HubbardGaugeAction(ut), ref.HubbardFermiAction(kt, 0.0, -1, ref.EXP))
+    rphi, rpi, rs = ref.leapfrog(phi, pi, r, 4 / 40.0, 4)
+    g = act.SumAction(act.HubbardGaugeAction(ut),
+                      act.HubbardFermiAction(kt, 0.0, -1, act.HFAHopping.EXP, precision=precision))
+    gphi, gpi, gs = leapfrog(_dev(phi), _dev(pi), g, 4 / 40.0, 4)
+    assert rel_l2(gphi.cpu().numpy(), rphi) < tol_phi
+    assert rel_l2(gpi.cpu().numpy(), rpi) < tol_pi
+    assert abs(gs - rs) < tol_s * abs(rs)
+
+
 @pytest.mark.parametrize("hop,basis,mu", [(ref.EXP, ref.PARTICLE_HOLE, 0.0), (ref.EXP, ref.SPIN, 0.3),
                                           (ref.DIA, ref.PARTICLE_HOLE, 0.2), (ref.DIA, ref.SPIN, 0.5)])
 def test_autograd_logdetM_and_action(hop, basis, mu):
