@@ -118,7 +118,8 @@ int isle_cscale(const double* in, double* out, long n, double cr, double ci, voi
  * (pre-scaled), 6-7 chain starts, 8 LU, 9 LU solve, 10-11 diagonal extraction, 12-14
  * copies, 15 phases, 16 transposed copy, 17 diagonal extraction FP64 x FP32, 18 GEMM64
  * complex x complex subtracted, 19 LU of a block-cyclic elimination step, 20 block
- * assembly, 21 row permutation, 22 permuted gather, 23 transposed block extraction. */
+ * assembly, 21 row permutation, 22 permuted gather, 23 transposed block extraction, 24 blocked
+ * triangular solve (GEMM launches). */
 int isle_profile_kernels(isle_hfm* h, const double* phi, int op, int nt, int batch, int group, int nspecies,
                          int spin, int reps, double* out_host, int nkinds);
 

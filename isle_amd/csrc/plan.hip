@@ -38,7 +38,7 @@ struct Builder {
     static double flops_of(K, const std::vector<It>&) { return 0.0; }
     template <typename T>
     static double flops_of(K kind, const std::vector<GemmItem<T>>& items) {
-        const double reals = (kind == K::GEMM64_CC || kind == K::GEMM64_CC_SUB) ? 4.0 : 2.0;  // real GEMMs per product
+        const double reals = (kind == K::GEMM64_CC || kind == K::GEMM64_CC_SUB || kind == K::TRSM) ? 4.0 : 2.0;
         double f = 0;
         for (const auto& g : items) f += reals * 2.0 * g.m * (double)g.n * g.k;
         return f;
@@ -103,6 +103,66 @@ inline std::complex<double> to_first_log_branch(std::complex<double> z) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Blocked triangular solve X = T^{-1} R on GEMMs, T the unit lower (mode 0) or upper (mode 1)
+// triangle of LU factors with the 32x32 diagonal-block inverses the LU stored per panel:
+// per panel p (forward for L, backward for U) X_p = D_p^{-1} R_p, then the rows still to solve
+// R_q -= T_{q,p} X_p.  R is overwritten; X must not overlap R.  n % 32 == 0.
+namespace {
+struct TrsmMat {
+    const double* f_re;
+    const double* f_im;
+    long ldf;
+    const double* dinv;  // the LU's work buffer (per-panel L11^{-1}, U11^{-1})
+    double* r_re;
+    double* r_im;
+    long ldr;
+    double* x_re;
+    double* x_im;
+    long ldx;
+};
+void add_trsm(Builder& b, int mode, int n, int w, const std::vector<TrsmMat>& mats) {
+    if (n % LU_NB != 0) throw std::invalid_argument("blocked solve needs n % 32 == 0");
+    const int np = n / LU_NB, per = (int)mats.size();
+    std::vector<GemmItem<double>> gi((size_t)2 * np * per);
+    for (int p = 0; p < np; ++p)
+        for (int i = 0; i < per; ++i) {
+            const TrsmMat& t = mats[i];
+            GemmItem<double> d{};
+            const double* dv = t.dinv + (size_t)LU_DINV * p + (mode == 0 ? 0 : 2 * LU_NB * LU_NB);
+            d.a_re = dv;
+            d.a_im = dv + LU_NB * LU_NB;
+            d.b_re = t.r_re + (long)p * LU_NB * t.ldr;
+            d.b_im = t.r_im + (long)p * LU_NB * t.ldr;
+            d.c_re = t.x_re + (long)p * LU_NB * t.ldx;
+            d.c_im = t.x_im + (long)p * LU_NB * t.ldx;
+            d.m = LU_NB;
+            d.n = w;
+            d.k = LU_NB;
+            d.lda = LU_NB;
+            d.ldb = (int)t.ldr;
+            d.ldc = (int)t.ldx;
+            gi[(size_t)p * per + i] = d;
+            GemmItem<double> u{};
+            const long r0 = mode == 0 ? (long)(p + 1) * LU_NB : 0;  // first row still to solve
+            u.a_re = t.f_re + r0 * t.ldf + (long)p * LU_NB;
+            u.a_im = t.f_im + r0 * t.ldf + (long)p * LU_NB;
+            u.b_re = d.c_re;
+            u.b_im = d.c_im;
+            u.c_re = t.r_re + r0 * t.ldr;
+            u.c_im = t.r_im + r0 * t.ldr;
+            u.m = mode == 0 ? n - (p + 1) * LU_NB : p * LU_NB;
+            u.n = w;
+            u.k = LU_NB;
+            u.lda = (int)t.ldf;
+            u.ldb = (int)t.ldx;
+            u.ldc = (int)t.ldr;
+            gi[(size_t)(np + p) * per + i] = u;
+        }
+    b.add(K::TRSM, gi, n, w, mode, per);
+}
+}  // namespace
+
 Plan::Plan(int hopping, int nx, const double* kappa, double mu, int sk)
     : hop_(hopping), nx_(nx), mu_(mu), sk_(sk), kappa_(kappa, kappa + (size_t)nx * nx) {
     if (nx <= 0) throw std::invalid_argument("nx must be positive");
@@ -387,6 +447,19 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
         case K::TBLOCK:
             hipLaunchKernelGGL(tblock_kernel, dim3(64, 2, st.count), dim3(256), 0, s, (const TBlockItem*)st.items);
             break;
+        case K::TRSM: {
+            // items: [diagonal-block products | trailing updates][panel][matrix]
+            const auto* gi = (const GemmItem<double>*)st.items;
+            const int per = st.aux, np = st.max_m / LU_NB, w = st.max_n;
+            for (int q = 0; q < np; ++q) {
+                const int p = st.flag == 0 ? q : np - 1 - q;
+                launch_gemm<double, true, true, false, SCALE_NONE>(gi + (size_t)p * per, per, LU_NB, w, s, 1);
+                const int rest = st.flag == 0 ? st.max_m - (p + 1) * LU_NB : p * LU_NB;
+                if (rest > 0)
+                    launch_gemm<double, true, true, false, SUBTRACT>(gi + (size_t)(np + p) * per, per, rest, w, s, 1);
+            }
+            break;
+        }
         case K::COPY64:
             hipLaunchKernelGGL((copy_items_kernel<double, double>), dim3(ew_blocks(st.max_m), st.count),
                                dim3(EW_THREADS), 0, s, (const CopyItem<double, double>*)st.items);
@@ -626,23 +699,46 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
             lu.push_back({LUb + (long)item * 2 * nn, LUb + (long)item * 2 * nn + nn, piv + (long)item * n, nullptr,
                           luw + item * lu_work_doubles(n), n, n});
         b.add(K::LU, lu, n, 0, 1);
-        std::vector<SolveItem> sv;
-        for (int item = 0; item < nb; ++item) {
-            SolveItem si{};
-            si.lu_re = LUb + (long)item * 2 * nn;
-            si.lu_im = si.lu_re + nn;
-            si.piv = piv + (long)item * n;
-            si.dinv = luw + item * lu_work_doubles(n);
-            si.x_re = m64(RR, item, 0).re;
-            si.x_im = m64(RR, item, 0).im;
-            si.xsr = n;
-            si.xsc = 1;
-            si.n = n;
-            si.ld = n;
-            si.nrhs = n;
-            sv.push_back(si);
+        if (n % LU_NB == 0) {
+            // Mid = (1 + P_0 GL_0)^{-1} = U^{-1} L^{-1} P: the row-permuted identity, then two
+            // blocked triangular solves on GEMMs
+            int* sig = b.di((size_t)nb * n);
+            double* S1 = b.d64((size_t)nb * 2 * nn);
+            double* S2 = b.d64((size_t)nb * 2 * nn);
+            std::vector<SigmaItem> sv;
+            std::vector<GatherItem> ga;
+            std::vector<TrsmMat> tl, tu;
+            for (int item = 0; item < nb; ++item) {
+                const CMat<double> s1 = cmat(S1, item, nn), s2 = cmat(S2, item, nn), rr = m64(RR, item, 0);
+                const double* fre = LUb + (long)item * 2 * nn;
+                sv.push_back({piv + (long)item * n, sig + (long)item * n, n, n});
+                ga.push_back({sig + (long)item * n, nullptr, nullptr, s1.re, s1.im, nullptr, nullptr, n, n, n, n, 0});
+                tl.push_back({fre, fre + nn, n, luw + item * lu_work_doubles(n), s1.re, s1.im, n, s2.re, s2.im, n});
+                tu.push_back({fre, fre + nn, n, luw + item * lu_work_doubles(n), s2.re, s2.im, n, rr.re, rr.im, n});
+            }
+            b.add(K::SIGMA, sv);
+            b.add(K::GATHER, ga, n, 0, 0, n);
+            add_trsm(b, 0, n, n, tl);
+            add_trsm(b, 1, n, n, tu);
+        } else {
+            std::vector<SolveItem> sv;
+            for (int item = 0; item < nb; ++item) {
+                SolveItem si{};
+                si.lu_re = LUb + (long)item * 2 * nn;
+                si.lu_im = si.lu_re + nn;
+                si.piv = piv + (long)item * n;
+                si.dinv = luw + item * lu_work_doubles(n);
+                si.x_re = m64(RR, item, 0).re;
+                si.x_im = m64(RR, item, 0).im;
+                si.xsr = n;
+                si.xsc = 1;
+                si.n = n;
+                si.ld = n;
+                si.nrhs = n;
+                sv.push_back(si);
+            }
+            b.add(K::SOLVE, sv, n, n);
         }
-        b.add(K::SOLVE, sv, n, n);
     }
     for (int j = 1; j < ng; ++j) {
         std::vector<GemmItem<double>> gi;
@@ -1092,26 +1188,6 @@ OpCache& Plan::get_force_stable(int nt, int batch, int g, int nspecies, int spin
         gm.ldb = gm.ldc = (int)wd;
         v.push_back(gm);
     };
-    auto solve = [&](std::vector<SolveItem>& v, const double* lre, const double* lim, int ld, const double* dinv,
-                     const double* bre, const double* bim, double* xre, double* xim, int cols, int mode) {
-        SolveItem si{};
-        si.lu_re = lre;
-        si.lu_im = lim;
-        si.dinv = dinv;
-        si.b_re = bre;
-        si.b_im = bim;
-        si.bsr = wd;
-        si.bsc = 1;
-        si.x_re = xre;
-        si.x_im = xim;
-        si.xsr = wd;
-        si.xsc = 1;
-        si.n = n;
-        si.ld = ld;
-        si.nrhs = cols;
-        si.mode = mode;
-        v.push_back(si);
-    };
     if (ng == 1) {
         // K_0 = (1 + P_0)^{-1}
         double* LUb = b.d64((size_t)nb * 2 * nn);
@@ -1160,52 +1236,64 @@ OpCache& Plan::get_force_stable(int nt, int batch, int g, int nspecies, int spin
                 for (int k = 0; k < ng - 1; ++k) sv.push_back({c.pv(item, k), sg(item, k), 2 * n, c.kf(k)});
             b.add(K::SIGMA, sv);
         }
+        // per-matrix scratch for the blocked solves (X must not overlap the right-hand side)
+        double* Sb = b.d64((size_t)nb * 2 * n * wd);
+        auto mS2 = [&](int item) { return cmat(Sb, item, (long)n * wd); };
         for (int k = 0; k < ng - 1; ++k) {
             const int w = (k + 2) * n;
             const int ld = c.ldw(k);
             std::vector<GatherItem> ga;
-            std::vector<SolveItem> sl;
+            std::vector<TrsmMat> tl;
             std::vector<GemmItem<double>> gs;
             for (int item = 0; item < nb; ++item) {
-                const CMat<double> bi = mB(item, k & 1), bo = mB(item, (k + 1) & 1), t = mT(item, k);
-                ga.push_back({sg(item, k), bi.re, bi.im, t.re, t.im, bo.re, bo.im, wd, wd, n, w, (k + 1) * n});
-                solve(sl, c.wre(item, k), c.wim(item, k), ld, c.wk(item, k), t.re, t.im, t.re, t.im, w, SOLVE_L);
+                const CMat<double> bi = mB(item, k & 1), bo = mB(item, (k + 1) & 1), t = mT(item, k), sc = mS2(item);
+                ga.push_back({sg(item, k), bi.re, bi.im, sc.re, sc.im, bo.re, bo.im, wd, wd, n, w, (k + 1) * n});
+                tl.push_back({c.wre(item, k), c.wim(item, k), ld, c.wk(item, k), sc.re, sc.im, wd, t.re, t.im, wd});
                 gsub(gs, c.wre(item, k) + (long)n * ld, c.wim(item, k) + (long)n * ld, ld, t.re, t.im, bo.re, bo.im, w);
             }
             b.add(K::GATHER, ga, 2 * n, 0, 0, w);
-            b.add(K::SOLVE, sl, n, w);
+            add_trsm(b, 0, n, w, tl);
             b.add(K::GEMM64_CC_SUB, gs, n, w);
         }
         {  // last step: x_{ng-1} from the bottom-right LU, then x_{ng-2}
             const int k = ng - 2, ld = 2 * n;
-            std::vector<SolveItem> s1, s2;
+            std::vector<TrsmMat> t1, t2, t3;
             std::vector<GemmItem<double>> gs;
+            std::vector<BlockItem> cp;
             for (int item = 0; item < nb; ++item) {
-                const CMat<double> bo = mB(item, (k + 1) & 1), xl = mT(item, ng - 1), x2 = mT(item, ng - 2);
-                solve(s1, c.wre(item, k) + (long)n * ld + n, c.wim(item, k) + (long)n * ld + n, ld,
-                      c.wk(item, k) + (size_t)LU_DINV * (n / LU_NB), bo.re, bo.im, xl.re, xl.im, (int)wd,
-                      SOLVE_L | SOLVE_U);
+                const CMat<double> bo = mB(item, (k + 1) & 1), xl = mT(item, ng - 1), x2 = mT(item, ng - 2),
+                                   sc = mS2(item);
+                const double* fre = c.wre(item, k) + (long)n * ld + n;
+                const double* fim = c.wim(item, k) + (long)n * ld + n;
+                const double* dv = c.wk(item, k) + (size_t)LU_DINV * (n / LU_NB);
+                t1.push_back({fre, fim, ld, dv, bo.re, bo.im, wd, sc.re, sc.im, wd});
+                t2.push_back({fre, fim, ld, dv, sc.re, sc.im, wd, xl.re, xl.im, wd});
                 gsub(gs, c.wre(item, k) + n, c.wim(item, k) + n, ld, xl.re, xl.im, x2.re, x2.im, (int)wd);
-                solve(s2, c.wre(item, k), c.wim(item, k), ld, c.wk(item, k), x2.re, x2.im, x2.re, x2.im, (int)wd,
-                      SOLVE_U);
+                cp.push_back({sc.re, sc.im, x2.re, x2.im, wd, wd, n, (int)wd, 2});
+                t3.push_back({c.wre(item, k), c.wim(item, k), ld, c.wk(item, k), sc.re, sc.im, wd, x2.re, x2.im, wd});
             }
-            b.add(K::SOLVE, s1, n, (int)wd);
+            add_trsm(b, 0, n, (int)wd, t1);
+            add_trsm(b, 1, n, (int)wd, t2);
             b.add(K::GEMM64_CC_SUB, gs, n, (int)wd);
-            b.add(K::SOLVE, s2, n, (int)wd);
+            b.add(K::BLOCK, cp, 0, 0, 0, (int)(n * wd));
+            add_trsm(b, 1, n, (int)wd, t3);
         }
         for (int k = ng - 3; k >= 0; --k) {  // back substitution on the columns still needed
             const int w = (k + 1) * n, ld = c.ldw(k);
             std::vector<GemmItem<double>> g1, g2;
-            std::vector<SolveItem> sl;
+            std::vector<BlockItem> cp;
+            std::vector<TrsmMat> tu;
             for (int item = 0; item < nb; ++item) {
-                const CMat<double> x = mT(item, k), x1 = mT(item, k + 1), xl = mT(item, ng - 1);
+                const CMat<double> x = mT(item, k), x1 = mT(item, k + 1), xl = mT(item, ng - 1), sc = mS2(item);
                 gsub(g1, c.wre(item, k) + n, c.wim(item, k) + n, ld, x1.re, x1.im, x.re, x.im, w);
                 gsub(g2, c.wre(item, k) + 2 * n, c.wim(item, k) + 2 * n, ld, xl.re, xl.im, x.re, x.im, w);
-                solve(sl, c.wre(item, k), c.wim(item, k), ld, c.wk(item, k), x.re, x.im, x.re, x.im, w, SOLVE_U);
+                cp.push_back({sc.re, sc.im, x.re, x.im, wd, wd, n, w, 2});
+                tu.push_back({c.wre(item, k), c.wim(item, k), ld, c.wk(item, k), sc.re, sc.im, wd, x.re, x.im, wd});
             }
             b.add(K::GEMM64_CC_SUB, g1, n, w);
             b.add(K::GEMM64_CC_SUB, g2, n, w);
-            b.add(K::SOLVE, sl, n, w);
+            b.add(K::BLOCK, cp, 0, 0, 0, n * w);
+            add_trsm(b, 1, n, w, tu);
         }
         std::vector<TBlockItem> tb;
         for (int item = 0; item < nb; ++item)

@@ -60,6 +60,7 @@ enum class K : int {
     SIGMA,              // row permutation of stacked right-hand sides
     GATHER,             // permuted split of stacked right-hand sides
     TBLOCK,             // signed transposed block extraction
+    TRSM,               // blocked triangular solve on GEMMs (flag 0: unit lower, 1: upper; aux: items)
 };
 
 struct Step {
