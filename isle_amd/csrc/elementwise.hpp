@@ -156,6 +156,27 @@ __global__ __launch_bounds__(256) void diag_prod_kernel(const DiagItem<TL, TR>* 
     }
 }
 
+// out[i] (interleaved complex) = sum over s < slots of part[(s * n + i) * 2 + {0, 1}]: the
+// per-16-column partial row sums written by a FUSED_DIAG chain product.
+struct DReduceItem {
+    const double* part;
+    double* out;
+    int n, slots;
+};
+
+__global__ __launch_bounds__(256) void diag_reduce_kernel(const DReduceItem* __restrict__ items) {
+    const DReduceItem it = items[blockIdx.y];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= it.n) return;
+    double sr = 0, si = 0;
+    for (int s = 0; s < it.slots; ++s) {
+        sr += it.part[((long)s * it.n + i) * 2];
+        si += it.part[((long)s * it.n + i) * 2 + 1];
+    }
+    it.out[2 * i] = sr;
+    it.out[2 * i + 1] = si;
+}
+
 // ---------------------------------------------------------------------------
 // precision conversion between split planes (n elements)
 template <typename S, typename D>

@@ -21,7 +21,11 @@
 
 namespace isle {
 
-enum : int { SCALE_NONE = 0, SCALE_ROW = 1, SCALE_COL = 2, SUBTRACT = 3 };  // POST: C = C - A B
+// POST: SUBTRACT: C = C - A B; FUSED_DIAG (direct-to-LDS kernel only): C = A B, and the row
+// sums of the elementwise product of C with a left factor L (or with the identity when L is
+// null, i.e. diag C) are written per 16-column block: dpart[(s * m + i) * 2 + {0, 1}] =
+// sum_{c in block s} L[i][c] C[i][c], s = 0 .. n/16 - 1 (FP64).
+enum : int { SCALE_NONE = 0, SCALE_ROW = 1, SCALE_COL = 2, SUBTRACT = 3, FUSED_DIAG = 4 };
 
 template <typename T>
 struct GemmItem {
@@ -37,9 +41,29 @@ struct GemmItem {
     const T* post_im;
     int m, n, k;
     int lda, ldb, ldc;
+    const double* dl_re = nullptr;  // FUSED_DIAG: left factor L (m x n, row stride n), or null
+    const double* dl_im = nullptr;
+    double* dpart = nullptr;        // FUSED_DIAG: partial row sums [n/16][m] complex
 };
 
 constexpr int GEMM_THREADS = 256;
+
+// Sum over the 16 lanes of a DPP row (the lanes that share lane / 16); lane 15 of the row ends
+// with the total.  Both 32-bit halves of the double move with the same pattern.
+template <int SHR>
+__device__ __forceinline__ double dpp_shr_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x110 + SHR, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x110 + SHR, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double row16_sum(double v) {
+    v += dpp_shr_f64<1>(v);
+    v += dpp_shr_f64<2>(v);
+    v += dpp_shr_f64<4>(v);
+    v += dpp_shr_f64<8>(v);
+    return v;
+}
 
 // One BM x BN output tile of item `it` (tile index `tile` in a ceil(m/BM) x tiles_n_max
 // grid).  AC/BC: A/B complex.  PRE: scale the (single) complex operand along k.
@@ -328,8 +352,10 @@ __device__ __forceinline__ int rcd_swz(int k, int m) {
 }
 
 // Issues this wave's direct copies of slab `slab` (A rows, B re/im rows, and with PRE the
-// phase factors) into stage slab % RCD_STAGES of the LDS ring.
-template <typename T, bool PRE, int BM, int BN, int BK, int RCD_STAGES, int NW>
+// phase factors) into stage slab % RCD_STAGES of the LDS ring.  POST is not used here: it
+// gives kernels of different POST modes their own instantiation (the host compilation pass
+// rejects one specialization of this device template shared by two of them).
+template <typename T, bool PRE, int BM, int BN, int BK, int RCD_STAGES, int NW, int POST>
 __device__ __forceinline__ void rcd_issue(const GemmItem<T>& it, T* lds, int slab, int m0, int n0, int wave,
                                           int lane) {
     constexpr int EPC = 16 / sizeof(T);
@@ -415,7 +441,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_rcd_kernel(const GemmItem<T
 
     // slabs are issued RCD_STAGES-1 ahead of the one being computed
     for (int j = 0; j < RCD_STAGES - 1 && j < nslab; ++j)
-        rcd_issue<T, PRE, BM, BN, BK, RCD_STAGES, NW>(it, rcd_lds, j, m0, n0, wave, lane);
+        rcd_issue<T, PRE, BM, BN, BK, RCD_STAGES, NW, POST>(it, rcd_lds, j, m0, n0, wave, lane);
     for (int s = 0; s < nslab; ++s) {
         // wait for this wave's copies of slab s (the later slabs already issued stay in
         // flight), then for all waves.  A plain s_barrier: __syncthreads() would also drain
@@ -428,7 +454,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_rcd_kernel(const GemmItem<T
         else
             asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
         if (s + RCD_STAGES - 1 < nslab)  // into the buffer every wave finished reading
-            rcd_issue<T, PRE, BM, BN, BK, RCD_STAGES, NW>(it, rcd_lds, s + RCD_STAGES - 1, m0, n0, wave, lane);
+            rcd_issue<T, PRE, BM, BN, BK, RCD_STAGES, NW, POST>(it, rcd_lds, s + RCD_STAGES - 1, m0, n0, wave, lane);
         const T* st = rcd_lds + (s % RCD_STAGES) * STAGE_E;
         const T* As = st;
         const T* Br = st + OFF_BR;
@@ -495,9 +521,10 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_rcd_kernel(const GemmItem<T
             for (int r = 0; r < 4; ++r) {
                 const int row = m0 + wm + i * 16 + acc_row<T>(lk, r);
                 const int col = n0 + wn + j * 16 + lr;
-                if (row < it.m && col < it.n) {
-                    T vr = acc_re[i][j][r], vi = acc_im[i][j][r];
-                    if (POST != SCALE_NONE) {
+                const bool in = row < it.m && col < it.n;
+                T vr = acc_re[i][j][r], vi = acc_im[i][j][r];
+                if (in) {
+                    if (POST == SCALE_ROW || POST == SCALE_COL) {
                         const int di = POST == SCALE_ROW ? row : col;
                         const T pr = it.post_re[di], pi = it.post_im[di];
                         const T t = vr * pr - vi * pi;
@@ -507,6 +534,24 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_rcd_kernel(const GemmItem<T
                     const long off = (long)row * it.ldc + col;
                     it.c_re[off] = vr;
                     it.c_im[off] = vi;
+                }
+                if constexpr (POST == FUSED_DIAG) {  // every lane takes part in the row sums
+                    double lr_ = 0, li_ = 0;
+                    if (in) {
+                        if (it.dl_re) {
+                            lr_ = it.dl_re[(long)row * it.n + col];
+                            li_ = it.dl_im[(long)row * it.n + col];
+                        } else {
+                            lr_ = row == col ? 1.0 : 0.0;
+                        }
+                    }
+                    const double sr = row16_sum(lr_ * (double)vr - li_ * (double)vi);
+                    const double si = row16_sum(lr_ * (double)vi + li_ * (double)vr);
+                    const int slot = (n0 + wn + j * 16) / 16;
+                    if (lr == 15 && row < it.m && slot < it.n / 16) {
+                        it.dpart[((long)slot * it.m + row) * 2] = sr;
+                        it.dpart[((long)slot * it.m + row) * 2 + 1] = si;
+                    }
                 }
             }
 }
@@ -600,15 +645,19 @@ inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int m
             return false;
         }
     }
-    switch (tile) {
-    case 1: return launch_gemm_tile<T, AC, BC, PRE, POST, 32, 32, 32>(d_items, nitems, max_m, max_n, s);
-    case 2: return launch_gemm_tile<T, AC, BC, PRE, POST, 32, 32, 64>(d_items, nitems, max_m, max_n, s);
-    case 3: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 32>(d_items, nitems, max_m, max_n, s);
-    case 4: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 96, 32>(d_items, nitems, max_m, max_n, s);
-    case 5: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 96, 16>(d_items, nitems, max_m, max_n, s);
-    case 18: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 32, 32>(d_items, nitems, max_m, max_n, s);
-    case 19: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 32, 16>(d_items, nitems, max_m, max_n, s);
-    default: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 16>(d_items, nitems, max_m, max_n, s);
+    if constexpr (POST == FUSED_DIAG) {
+        return false;  // only the direct-to-LDS tiles fuse the row sums
+    } else {
+        switch (tile) {
+        case 1: return launch_gemm_tile<T, AC, BC, PRE, POST, 32, 32, 32>(d_items, nitems, max_m, max_n, s);
+        case 2: return launch_gemm_tile<T, AC, BC, PRE, POST, 32, 32, 64>(d_items, nitems, max_m, max_n, s);
+        case 3: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 32>(d_items, nitems, max_m, max_n, s);
+        case 4: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 96, 32>(d_items, nitems, max_m, max_n, s);
+        case 5: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 96, 16>(d_items, nitems, max_m, max_n, s);
+        case 18: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 32, 32>(d_items, nitems, max_m, max_n, s);
+        case 19: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 32, 16>(d_items, nitems, max_m, max_n, s);
+        default: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 16>(d_items, nitems, max_m, max_n, s);
+        }
     }
 }
 

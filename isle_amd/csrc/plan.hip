@@ -283,7 +283,7 @@ void Plan::run(OpCache& op, const double* phi) {
 
 static bool is_gemm(K k) {
     return k == K::GEMM64_RC_PRE || k == K::GEMM64_RC_ROW || k == K::GEMM64_CR_PRE || k == K::GEMM64_CC ||
-           k == K::GEMM32_RC_ROW || k == K::GEMM32_RC_PRE || k == K::GEMM64_CC_SUB;
+           k == K::GEMM32_RC_ROW || k == K::GEMM32_RC_PRE || k == K::GEMM64_CC_SUB || k == K::GEMM32_RC_PRE_DIAG;
 }
 
 void Plan::tune(OpCache& op) {
@@ -446,6 +446,14 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
             break;
         case K::TBLOCK:
             hipLaunchKernelGGL(tblock_kernel, dim3(64, 2, st.count), dim3(256), 0, s, (const TBlockItem*)st.items);
+            break;
+        case K::GEMM32_RC_PRE_DIAG:
+            if (!launch_gemm<float, false, true, true, FUSED_DIAG>((const GemmItem<float>*)st.items, st.count, st.max_m,
+                                                              st.max_n, s, st.tile, n % 4 == 0 ? n : 0)) return false;
+            break;
+        case K::DREDUCE:
+            hipLaunchKernelGGL(diag_reduce_kernel, dim3(cdiv(n, 256), st.count), dim3(256), 0, s,
+                               (const DReduceItem*)st.items);
             break;
         case K::TRSM: {
             // items: [diagonal-block products | trailing updates][panel][matrix]
@@ -767,19 +775,39 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
             }
         b.add(K::TCVT64TO32, cv);
     }
-    // ---- local right chains (FP32), transposed
+    // ---- local right chains (FP32), transposed.  With n % 32 == 0 the chain products also
+    //      produce the force diagonal f[tau] = rowdot(Suf_{tau+1}, Rt_tau) (diag Rt at a group's
+    //      last slice) as partial row sums per 16 columns, summed by one reduction at the end.
+    const bool fuse = n % 32 == 0;
+    const int slots = n / 16;
+    double* part = fuse ? b.d64((size_t)nb * nt * slots * n * 2) : nullptr;
     for (int q = 0; q < g; ++q) {
         std::vector<GemmItem<float>> gi;
         for (int item = 0; item < nb; ++item)
             for (int j = 0; j < ng; ++j) {
                 const int tau = j * g + q;
                 const CMat<float> prev = q == 0 ? mK32(item, j) : mRt(item, tau - 1);
-                gi.push_back(gitem<float>(A32[spec(item)], nullptr, prev.re, prev.im, mRt(item, tau), D(item, tau).re,
-                                          D(item, tau).im, nullptr, nullptr, n));
+                GemmItem<float> it = gitem<float>(A32[spec(item)], nullptr, prev.re, prev.im, mRt(item, tau),
+                                                  D(item, tau).re, D(item, tau).im, nullptr, nullptr, n);
+                if (fuse) {
+                    if (q < g - 1) {
+                        it.dl_re = mS(item, tau + 1).re;
+                        it.dl_im = mS(item, tau + 1).im;
+                    }
+                    it.dpart = part + ((long)item * nt + tau) * slots * n * 2;
+                }
+                gi.push_back(it);
             }
-        b.add(K::GEMM32_RC_PRE, gi, n, n);
+        b.add(fuse ? K::GEMM32_RC_PRE_DIAG : K::GEMM32_RC_PRE, gi, n, n);
     }
-    {
+    if (fuse) {
+        std::vector<DReduceItem> dr;
+        for (int item = 0; item < nb; ++item)
+            for (int tau = 0; tau < nt; ++tau)
+                dr.push_back({part + ((long)item * nt + tau) * slots * n * 2, op->parts + ((long)item * nt + tau) * 2 * n,
+                              n, slots});
+        b.add(K::DREDUCE, dr);
+    } else {
         std::vector<DiagItem<double, float>> dg;
         for (int item = 0; item < nb; ++item)
             for (int tau = 0; tau < nt; ++tau) {

@@ -61,6 +61,8 @@ enum class K : int {
     GATHER,             // permuted split of stacked right-hand sides
     TBLOCK,             // signed transposed block extraction
     TRSM,               // blocked triangular solve on GEMMs (flag 0: unit lower, 1: upper; aux: items)
+    GEMM32_RC_PRE_DIAG, // GEMM32_RC_PRE with the force diagonal's partial row sums fused (f32)
+    DREDUCE,            // sum of those partial row sums
 };
 
 struct Step {
