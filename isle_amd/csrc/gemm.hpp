@@ -399,7 +399,8 @@ constexpr bool rcd_shape_ok() {
 }
 
 template <typename T, int POST, bool PRE, int BM, int BN, int BK, int RCD_STAGES, int WM = 2, int WN = 2>
-__global__ __launch_bounds__(64 * WM * WN) void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items,
+__global__ __launch_bounds__(64 * WM * WN, (WM * WN >= 12 && sizeof(T) == 8 && POST == SCALE_ROW ? 6 : 1)) void gemm_rcd_kernel(
+    const GemmItem<T>* __restrict__ items,
                                                                 int tiles_n_max) {
     constexpr int NW = WM * WN;
     typedef RcdLayout<T, BM, BN, BK, RCD_STAGES, NW> LY;
