@@ -398,10 +398,12 @@ constexpr bool rcd_shape_ok() {
            (BN / EPC) % 8 == 0 && 2 * BK / EPC <= 64 && BK % 16 == 0 && BM % (16 * WM) == 0 && BN % (16 * WN) == 0;
 }
 
+// The 12-wave tiles ask for 6 waves per SIMD (at most 80 VGPRs): two workgroups per CU, whose
+// 2-stage LDS rings (74 KB in FP64, 80 KB in FP32) fit the 160 KB together.  The FP64
+// kernels other than the row-scaled one would spill at that bound and keep the default.
 template <typename T, int POST, bool PRE, int BM, int BN, int BK, int RCD_STAGES, int WM = 2, int WN = 2>
-__global__ __launch_bounds__(64 * WM * WN, (WM * WN >= 12 && sizeof(T) == 8 && POST == SCALE_ROW ? 6 : 1)) void gemm_rcd_kernel(
-    const GemmItem<T>* __restrict__ items,
-                                                                int tiles_n_max) {
+__global__ __launch_bounds__(64 * WM * WN, (WM * WN >= 12 && (sizeof(T) == 4 || POST == SCALE_ROW) ? 6 : 1))
+void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items, int tiles_n_max) {
     constexpr int NW = WM * WN;
     typedef RcdLayout<T, BM, BN, BK, RCD_STAGES, NW> LY;
     constexpr int EPC = 16 / sizeof(T);
@@ -413,7 +415,10 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN >= 12 && sizeof(T) == 8 && P
     static_assert(2 * BK / EPC <= 64 && BK % 16 == 0, "pre copy must fit one wave instruction");
     static_assert(RCD_STAGES >= 2 && RCD_STAGES <= 4, "one to three slabs in flight");
     constexpr int AQ = ACH / (64 * NW), BQ = BCH / (64 * NW);  // direct loads per wave per plane
-    constexpr bool BLOCKED = sizeof(T) == 4;       // FP32: each 16 k summed in a fresh accumulator
+    // FP32 sums each group of 16 k in a fresh accumulator (blocked summation: about half the
+    // rounding error of one running sum over K = 288), except in the 12-wave tiles, where the
+    // second set of accumulators would cost the second workgroup per CU
+    constexpr bool BLOCKED = sizeof(T) == 4 && NW < 12;
     typedef typename Acc4<T>::type V4;
     extern __shared__ __align__(16) unsigned char rcd_raw[];
     T* rcd_lds = reinterpret_cast<T*>(rcd_raw);
