@@ -519,8 +519,55 @@ void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items, int tiles_n_max) {
             }
         }
     }
+    // Epilogue, one block of 16 rows at a time.  The memory operands of a block (row or column
+    // factors, the left factor of the row sums) are all loaded, branch-free, before the first
+    // of them is used: one memory latency per block instead of one per element.  Elements
+    // outside the matrix load element 0 and discard it.
+    constexpr bool SCALED = POST == SCALE_ROW || POST == SCALE_COL;
 #pragma unroll
-    for (int i = 0; i < MI; ++i)
+    for (int i = 0; i < MI; ++i) {
+        T fr[NJ][4], fi[NJ][4];
+        double lre[NJ][4], lim[NJ][4];
+        if constexpr (SCALED) {
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = m0 + wm + i * 16 + acc_row<T>(lk, r);
+                    const int col = n0 + wn + j * 16 + lr;
+                    const bool in = row < it.m && col < it.n;
+                    const int di = in ? (POST == SCALE_ROW ? row : col) : 0;
+                    fr[j][r] = it.post_re[di];
+                    fi[j][r] = it.post_im[di];
+                }
+        }
+        if constexpr (POST == FUSED_DIAG) {
+            const bool identity = it.dl_re == nullptr;  // uniform over the workgroup
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = m0 + wm + i * 16 + acc_row<T>(lk, r);
+                    const int col = n0 + wn + j * 16 + lr;
+                    const bool in = row < it.m && col < it.n;
+                    lre[j][r] = in && row == col ? 1.0 : 0.0;
+                    lim[j][r] = 0;
+                }
+            if (!identity) {
+#pragma unroll
+                for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = m0 + wm + i * 16 + acc_row<T>(lk, r);
+                        const int col = n0 + wn + j * 16 + lr;
+                        const bool in = row < it.m && col < it.n;
+                        const long o = in ? (long)row * it.n + col : 0;
+                        const double a = it.dl_re[o], b = it.dl_im[o];
+                        lre[j][r] = in ? a : 0.0;
+                        lim[j][r] = in ? b : 0.0;
+                    }
+            }
+        }
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
 #pragma unroll
@@ -529,30 +576,19 @@ void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items, int tiles_n_max) {
                 const int col = n0 + wn + j * 16 + lr;
                 const bool in = row < it.m && col < it.n;
                 T vr = acc_re[i][j][r], vi = acc_im[i][j][r];
+                if (SCALED) {
+                    const T t = vr * fr[j][r] - vi * fi[j][r];
+                    vi = vr * fi[j][r] + vi * fr[j][r];
+                    vr = t;
+                }
                 if (in) {
-                    if (POST == SCALE_ROW || POST == SCALE_COL) {
-                        const int di = POST == SCALE_ROW ? row : col;
-                        const T pr = it.post_re[di], pi = it.post_im[di];
-                        const T t = vr * pr - vi * pi;
-                        vi = vr * pi + vi * pr;
-                        vr = t;
-                    }
                     const long off = (long)row * it.ldc + col;
                     it.c_re[off] = vr;
                     it.c_im[off] = vi;
                 }
                 if constexpr (POST == FUSED_DIAG) {  // every lane takes part in the row sums
-                    double lr_ = 0, li_ = 0;
-                    if (in) {
-                        if (it.dl_re) {
-                            lr_ = it.dl_re[(long)row * it.n + col];
-                            li_ = it.dl_im[(long)row * it.n + col];
-                        } else {
-                            lr_ = row == col ? 1.0 : 0.0;
-                        }
-                    }
-                    const double sr = row16_sum(lr_ * (double)vr - li_ * (double)vi);
-                    const double si = row16_sum(lr_ * (double)vi + li_ * (double)vr);
+                    const double sr = row16_sum(lre[j][r] * (double)vr - lim[j][r] * (double)vi);
+                    const double si = row16_sum(lre[j][r] * (double)vi + lim[j][r] * (double)vr);
                     const int slot = (n0 + wn + j * 16) / 16;
                     if (lr == 15 && row < it.m && slot < it.n / 16) {
                         it.dpart[((long)slot * it.m + row) * 2] = sr;
@@ -560,6 +596,7 @@ void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items, int tiles_n_max) {
                     }
                 }
             }
+    }
 }
 
 // Tile shapes the launcher can pick (the plan's autotuner times them per step):
