@@ -7,7 +7,9 @@ Drop-in names for the reference's ``isle`` / ``isle.action`` surface on this pat
     HubbardFermiMatrixExp, HubbardFermiMatrixDia, logdetM, solveM, logdet, invert,
     leapfrog, action.{Action, SumAction, HubbardGaugeAction, HubbardFermiAction,
     makeHubbardFermiAction, HFAHopping, HFABasis, HFAAlgorithm},
-    evolver.{ConstStepLeapfrog, BinarySelector, EvolutionStage}, random.NumpyRNG,
+    evolver.{ConstStepLeapfrog, LinearStepLeapfrog, BinarySelector, EvolutionStage},
+    random.NumpyRNG, drivers.hmc.{newRun, continueRun, HMC} (run files with checkpoints,
+    isle_amd.runfile), util.parameters,
     autograd.{action_value, logdetM_value} (the action and log det M as
     torch.autograd functions whose backward is the force sweep), and
     ensemble.{Chain, EnsembleLeapfrog} (many chains of uneven Nx and Nt advanced
@@ -16,7 +18,7 @@ Drop-in names for the reference's ``isle`` / ``isle.action`` surface on this pat
 Every numerical routine runs in the hand-written HIP kernels of libisle_hfa.so; there
 is no CPU fallback.
 """
-from . import action, autograd, ensemble, evolver
+from . import action, autograd, drivers, ensemble, evolver, runfile, util
 from .core import (CDVector, HubbardFermiMatrixDia, HubbardFermiMatrixExp, Species, Vector, as_field, invert,
                    logdet, logdetM, solveM, to_first_log_branch)
 from .integrator import leapfrog, leapfrog_batch
@@ -26,7 +28,7 @@ random = evolver  # isle.random.NumpyRNG lives in isle_amd.evolver
 
 __all__ = [
     "CDVector", "HubbardFermiMatrixDia", "HubbardFermiMatrixExp", "Lattice", "Species", "Vector", "action", "autograd",
-    "ensemble",
+    "drivers", "ensemble", "runfile", "util",
     "as_field", "evolver", "honeycomb", "invert", "isBipartite", "leapfrog", "leapfrog_batch", "loadLattice",
     "logdet", "logdetM", "random", "solveM", "to_first_log_branch",
 ]

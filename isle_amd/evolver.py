@@ -1,12 +1,16 @@
 """HMC evolution: EvolutionStage, BinarySelector, ConstStepLeapfrog, NumpyRNG and the
 trajectory loop, with the reference's semantics and random-number consumption.
 
-* ``NumpyRNG`` wraps numpy's MT19937 RandomState (src/isle/random.py:80-124);
-* ``EvolutionStage`` (src/isle/evolver/stage.py:10-117, persistence omitted);
+* ``NumpyRNG`` wraps numpy's MT19937 RandomState (src/isle/random.py:80-185), its state
+  stored in a run-file group by ``writeH5`` / ``readH5`` / ``readStateH5``;
+* ``EvolutionStage`` (src/isle/evolver/stage.py:10-117) with ``save`` / ``fromH5``;
 * ``BinarySelector.selectTrajPoint``: accept iff dE < 0 or exp(-dE) > uniform, the
   uniform drawn only when dE >= 0 (src/isle/evolver/selector.py:20-30);
 * ``ConstStepLeapfrog.evolve``: pi = rng.normal(0, 1, N) + 0j, leapfrog, Metropolis with
   E = sum(logWeights) + |pi|^2/2 (src/isle/evolver/leapfrog.py:38-66);
+* ``LinearStepLeapfrog``: the same with length and step count moving linearly between
+  two values over ``ninterp`` trajectories (leapfrog.py:105-229); both evolvers store
+  their state in checkpoints (``save`` / ``fromH5``);
 * ``hmc``: the HMC loop of src/isle/drivers/hmc.py:41-84 without HDF5 output; a start
   configuration that is not a stage is evaluated first (hmc.py:64-66).
 
@@ -52,6 +56,42 @@ class NumpyRNG:
     def set_state(self, state):
         self._state.set_state(state)
 
+    def writeH5(self, group):
+        """Store the generator state in a run-file group (random.py:140-154)."""
+        kind, keys, pos, has_gauss, cached_gaussian = self._state.get_state()
+        assert kind == self.NAME.rsplit(".")[-1]
+        group["name"] = self.NAME
+        group["keys"] = keys
+        group.attrs["pos"] = pos
+        group.attrs["has_gauss"] = has_gauss
+        group.attrs["cached_gaussian"] = cached_gaussian
+
+    def readH5(self, group):
+        """Restore the generator state written by writeH5 (random.py:156-166)."""
+        name = str(group["name"][()])
+        if name != self.NAME:
+            raise RuntimeError(f"Wrong kind of RNG. Expected {self.NAME}, got {name}")
+        self._state.set_state((self.NAME.rsplit(".")[-1], np.asarray(group["keys"][()], dtype=np.uint32),
+                               int(group.attrs["pos"]), int(group.attrs["has_gauss"]),
+                               float(group.attrs["cached_gaussian"])))
+
+
+def readStateH5(group):
+    """A generator with the state stored in a run-file group (random.py:168-185)."""
+    name = str(group["name"][()])
+    if name != NumpyRNG.NAME:
+        raise ValueError(f"Unknown RNG in file: {name}")
+    rng = NumpyRNG(0)
+    rng.readH5(group)
+    return rng
+
+
+def _host(x):
+    """A field as a complex128 numpy vector."""
+    if isinstance(x, torch.Tensor):
+        return x.detach().to("cpu", torch.complex128).numpy()
+    return np.asarray(x, dtype=np.complex128)
+
 
 class EvolutionStage:
     """Configuration + log weights at one point of the Markov chain (stage.py:10-68)."""
@@ -82,6 +122,33 @@ class EvolutionStage:
 
     def sumLogWeights(self):
         return sum(self.logWeights.values())
+
+    def save(self, group):
+        """Store phi, actVal, trajPoint and any further log weights and extras in a
+        run-file group (stage.py:74-93)."""
+        group["phi"] = _host(self.phi)
+        if self.phi_RTP is not None:
+            group["phi_RT-P"] = _host(self.phi_RTP)
+        group["actVal"] = complex(self.actVal)
+        group["trajPoint"] = int(self.trajPoint)
+        if len(self.logWeights) > 1:
+            weights = group.create_group("logWeights")
+            for key, val in self.logWeights.items():
+                if key != "actVal":
+                    weights[key] = complex(val)
+        if self.extra:
+            extra = group.create_group("extra")
+            for key, val in self.extra.items():
+                extra[key] = val
+
+    @classmethod
+    def fromH5(cls, group):
+        """The stage stored by save; phi comes back as a numpy vector (stage.py:95-117)."""
+        extra = {k: v[()] for k, v in group["extra"].items()} if "extra" in group else None
+        weights = {k: complex(v[()]) for k, v in group["logWeights"].items()} if "logWeights" in group else None
+        phi_rtp = group["phi_RT-P"][()] if "phi_RT-P" in group else None
+        return cls(group["phi"][()], complex(group["actVal"][()]), int(group["trajPoint"][()]), weights, extra,
+                   phi_rtp)
 
 
 class BinarySelector:
@@ -114,23 +181,101 @@ class ConstStepLeapfrog:
         self.last_energies = None
 
     def evolve(self, stage):
-        phi = as_field(stage.phi)
-        n = phi.numel()
-        pi_host = self.rng.normal(0, 1, n) + 0j
-        pi = torch.as_tensor(pi_host, device=phi.device)
-        phi1, pi1, acts = leapfrog_batch(phi.reshape(1, -1), pi.reshape(1, -1), self.action, self.length,
-                                         self.nstep)
-        actVal1 = complex(acts[0])
-        energy0 = stage.sumLogWeights() + np.linalg.norm(pi_host) ** 2 / 2
-        energy1 = actVal1 + 0 + np.linalg.norm(pi1.cpu().numpy()) ** 2 / 2
-        self.last_energies = (energy0, energy1)
-        trajPoint = self.selector.selectTrajPoint(energy0, energy1)
-        self.trajPoints.append(trajPoint)
-        return stage.accept(phi1[0], actVal1) if trajPoint == 1 else stage.reject()
+        return _leapfrog_trajectory(self, stage, self.length, self.nstep)
+
+    def save(self, group, manager):
+        """Store length and nstep in a checkpoint (leapfrog.py:68-77)."""
+        group["length"] = self.length
+        group["nstep"] = self.nstep
+
+    @classmethod
+    def fromH5(cls, group, manager, action, lattice, rng):
+        return cls(action, float(group["length"][()]), int(group["nstep"][()]), rng)
 
     def report(self):
         return f"""<ConstStepLeapfrog> (0x{id(self):x})
   length = {self.length}, nstep = {self.nstep}
+  acceptance rate = {np.mean(self.trajPoints)}"""
+
+
+def _leapfrog_trajectory(ev, stage, length, nstep):
+    """One HMC trajectory of evolver ``ev``: momenta from ev.rng, leapfrog, Metropolis."""
+    phi = as_field(stage.phi)
+    n = phi.numel()
+    pi_host = ev.rng.normal(0, 1, n) + 0j
+    pi = torch.as_tensor(pi_host, device=phi.device)
+    phi1, pi1, acts = leapfrog_batch(phi.reshape(1, -1), pi.reshape(1, -1), ev.action, length, nstep)
+    actVal1 = complex(acts[0])
+    energy0 = stage.sumLogWeights() + np.linalg.norm(pi_host) ** 2 / 2
+    energy1 = actVal1 + 0 + np.linalg.norm(pi1.cpu().numpy()) ** 2 / 2
+    ev.last_energies = (energy0, energy1)
+    trajPoint = ev.selector.selectTrajPoint(energy0, energy1)
+    ev.trajPoints.append(trajPoint)
+    return stage.accept(phi1[0], actVal1) if trajPoint == 1 else stage.reject()
+
+
+def _hinge_range(start, end, step):
+    """start, start + step, ... while short of end (the sum accumulated step by step),
+    then end for ever (src/isle/collection.py:10-30)."""
+    cur = start
+    while (step > 0 and cur < end) or (step < 0 and cur > end):
+        yield cur
+        cur += step
+    while True:
+        yield end
+
+
+class LinearStepLeapfrog:
+    """Leapfrog whose trajectory length and step count move linearly from the first to the
+    second value of their ranges over ``ninterp`` trajectories and then stay at the second
+    (leapfrog.py:105-229).  ``startPoint`` skips that many trajectories of the schedule.
+
+    The position in the schedule that a checkpoint stores counts every trajectory since
+    the schedule began, including those before a resumption (the reference restarts that
+    count at zero when it resumes, so a second resumption would rewind the schedule)."""
+
+    def __init__(self, action, lengthRange, nstepRange, ninterp, rng, startPoint=0, transform=None):
+        if transform is not None:
+            raise NotImplementedError("transforms are not supported")
+        self.action = action
+        self.lengthRange = tuple(lengthRange)
+        self.nstepRange = tuple(nstepRange)
+        self.ninterp = ninterp
+        self.rng = rng
+        self.selector = BinarySelector(rng)
+        self.transform = None
+        self.trajPoints = []
+        self.last_energies = None
+        self._lengths = _hinge_range(*self.lengthRange, (self.lengthRange[1] - self.lengthRange[0]) / ninterp)
+        self._nsteps = _hinge_range(*self.nstepRange, (self.nstepRange[1] - self.nstepRange[0]) / ninterp)
+        self._current = 0
+        for _ in range(int(startPoint)):
+            next(self._lengths)
+            next(self._nsteps)
+            self._current += 1
+
+    def evolve(self, stage):
+        self._current += 1
+        return _leapfrog_trajectory(self, stage, next(self._lengths), int(next(self._nsteps)))
+
+    def save(self, group, manager):
+        """Store the ranges and the position in the schedule (leapfrog.py:183-196)."""
+        group["minLength"] = self.lengthRange[0]
+        group["maxLength"] = self.lengthRange[1]
+        group["minNstep"] = self.nstepRange[0]
+        group["maxNstep"] = self.nstepRange[1]
+        group["ninterp"] = self.ninterp
+        group["current"] = self._current
+
+    @classmethod
+    def fromH5(cls, group, manager, action, lattice, rng):
+        return cls(action, (group["minLength"][()], group["maxLength"][()]),
+                   (group["minNstep"][()], group["maxNstep"][()]), int(group["ninterp"][()]), rng,
+                   startPoint=int(group["current"][()]))
+
+    def report(self):
+        return f"""<LinearStepLeapfrog> (0x{id(self):x})
+  lengthRange = {self.lengthRange}, nstepRange = {self.nstepRange}, ninterp = {self.ninterp}
   acceptance rate = {np.mean(self.trajPoints)}"""
 
 

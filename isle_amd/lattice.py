@@ -15,6 +15,7 @@ class Lattice:
     def __init__(self, nt, nx, name="", comment=""):
         self._nt = int(nt)
         self._hopping = np.zeros((int(nx), int(nx)))
+        self._positions = np.zeros((int(nx), 3))
         self.name = name
         self.comment = comment
 
@@ -44,6 +45,13 @@ class Lattice:
 
     def getNeighbors(self, i):
         return [(int(j), float(self._hopping[i, j])) for j in np.nonzero(self._hopping[i])[0]]
+
+    def position(self, i, x=None, y=None, z=0.0):
+        """Position of site i as (x, y, z); sets it first when x and y are given
+        (lattice.hpp: position(i) / position(i, x, y, z))."""
+        if x is not None:
+            self._positions[i] = (x, y, z)
+        return tuple(float(v) for v in self._positions[i])
 
     @classmethod
     def from_hopping(cls, hopping, nt=0, name="", comment=""):
@@ -107,10 +115,41 @@ def _construct_lattice(loader, node):
     lat = Lattice(d.get("nt", 0), len(d["positions"]), d.get("name", ""), d.get("comment", ""))
     for (i, j), hop in zip(adjacency, hopping):
         lat.setNeighbor(i, j, hop)
+    for i, pos in enumerate(d["positions"]):
+        if len(pos) not in (2, 3):
+            raise RuntimeError(f"Lattice site positions given with {len(pos)} coordinates; "
+                               "only 2D and 3D positions are supported")
+        lat.position(i, *pos)
     return lat
 
 
 _LatticeLoader.add_constructor("!lattice", _construct_lattice)
+
+
+class _LatticeDumper(yaml.SafeDumper):
+    pass
+
+
+def _represent_lattice(dumper, lat):
+    adjacency, hopping = [], []
+    for i in range(lat.nx()):
+        for j, strength in lat.getNeighbors(i):
+            if j > i:
+                adjacency.append([i, j])
+                hopping.append(strength)
+    return dumper.represent_mapping("!lattice", {
+        "name": lat.name, "comment": lat.comment, "nt": lat.nt(), "adjacency": adjacency,
+        "hopping": hopping, "positions": [list(lat.position(i)) for i in range(lat.nx())]},
+        flow_style=False)
+
+
+_LatticeDumper.add_representer(Lattice, _represent_lattice)
+
+
+def dumpLattice(lat):
+    """The ``!lattice`` YAML document of a lattice (each bond once, i < j; the inverse of
+    loadLattice; src/isle/yamlio.py:55-75)."""
+    return yaml.dump(lat, Dumper=_LatticeDumper)
 
 
 def loadLattice(text_or_file):

@@ -1,0 +1,145 @@
+"""GPU: checkpoint / resume of HMC runs with the fermion action.
+
+Workload: honeycomb 3x3 torus (Nx=18), Nt=16, beta=4, U=2, mu=0, EXP hopping,
+particle-hole basis, gauge + fermion action, leapfrog of length 1.  A run that crashes
+(an exception from the evolver) and is resumed from its last checkpoint must write the
+configurations of the uninterrupted run.  Both runs use one action object (one process),
+so the same kernels and tiles run and the comparison is bit for bit.  The chains of a
+batched run are checkpointed one run directory each; resuming them together is again
+bit-identical, and resuming one chain alone with the single-chain driver agrees to
+1e-12 (the single-chain and batched launches may differ in summation order).
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+isle = pytest.importorskip("isle_amd")
+from isle_amd import runfile  # noqa: E402
+from isle_amd.action import HFABasis, HFAHopping  # noqa: E402
+from isle_amd.drivers import hmc  # noqa: E402
+from isle_amd.evolver import BatchedConstStepLeapfrog, ConstStepLeapfrog, LinearStepLeapfrog, NumpyRNG  # noqa: E402
+from isle_amd.integrator import eval_batch  # noqa: E402
+from isle_amd.lattice import honeycomb  # noqa: E402
+from isle_amd.util import parameters  # noqa: E402
+
+PARAMS = parameters(beta=4.0, U=2.0, mu=0.0, sigmaKappa=-1, hopping=HFAHopping.EXP, basis=HFABasis.PARTICLE_HOLE)
+_ACTIONS = {}
+
+
+def make_action(lattice, params):
+    key = (lattice.nx(), lattice.nt())
+    if key not in _ACTIONS:
+        gauge = isle.action.HubbardGaugeAction(params.tilde("U", lattice))
+        ferm = isle.action.makeHubbardFermiAction(lattice, params.beta, params.tilde("mu", lattice),
+                                                  params.sigmaKappa, params.hopping, params.basis)
+        _ACTIONS[key] = gauge + ferm
+    return _ACTIONS[key]
+
+
+class Crash(RuntimeError):
+    pass
+
+
+def _crashing(cls):
+    class Crashing(cls):
+        crash_at = None
+
+        def evolve(self, stage):
+            if len(self.trajPoints) + 1 == self.crash_at:
+                raise Crash("node lost")
+            return super().evolve(stage)
+    Crashing.__name__ = "Crashing" + cls.__name__
+    return Crashing
+
+
+CrashingConst = _crashing(ConstStepLeapfrog)
+CrashingLinear = _crashing(LinearStepLeapfrog)
+DEFS = {"CrashingConstStepLeapfrog": ConstStepLeapfrog, "CrashingLinearStepLeapfrog": LinearStepLeapfrog}
+
+
+def _lattice():
+    return honeycomb(3, nt=16)
+
+
+def _configs(run):
+    out = {}
+    for i in runfile.indices(run, "configuration"):
+        g = runfile.readTrajectory(run, i)
+        out[i] = (g["phi"][()], complex(g["actVal"][()]), int(g["trajPoint"][()]))
+    return out
+
+
+def _evolver(kind, action, rng, crash_at=None):
+    if kind == "const":
+        ev = CrashingConst(action, 1.0, 8, rng)
+    else:
+        ev = CrashingLinear(action, (1.0, 0.5), (12, 6), 4, rng)
+    ev.crash_at = crash_at
+    return ev
+
+
+@pytest.mark.parametrize("kind", ["const", "linear"])
+def test_hmc_crash_resume_bit_identical(tmp_path, kind):
+    print(f"hmc resume, {kind} leapfrog, Nx=18 Nt=16", flush=True)
+    lat = _lattice()
+
+    def start(run):
+        rng = NumpyRNG(1075)
+        phi = rng.normal(0, PARAMS.tilde("U", lat) ** 0.5, lat.lattSize()) + 0j
+        drv = hmc.newRun(lat, PARAMS, rng, make_action, tmp_path / run, False, DEFS)
+        return drv, phi
+
+    drv, phi = start("a")
+    drv(phi, _evolver(kind, drv.action, drv.rng), 6, saveFreq=1, checkpointFreq=2)
+    ref = _configs(tmp_path / "a")
+    assert sorted(ref) == list(range(1, 7))
+
+    drv, phi = start("b")
+    with pytest.raises(Crash):
+        drv(phi, _evolver(kind, drv.action, drv.rng, crash_at=4), 6, saveFreq=1, checkpointFreq=2)
+    drv2, stage, ev, sf, cf = hmc.continueRun(tmp_path / "b", None, -1, True, DEFS, makeAction=make_action)
+    assert drv2.trajIdx == 2 and (sf, cf) == (1, None)  # one checkpoint (2) before the crash
+    drv2(stage, ev, 4, saveFreq=1, checkpointFreq=2)
+    got = _configs(tmp_path / "b")
+    assert sorted(got) == sorted(ref)
+    for i in ref:
+        assert np.array_equal(got[i][0], ref[i][0]) and got[i][1:] == ref[i][1:], i
+
+
+def test_batched_hmc_resume(tmp_path):
+    print("batched hmc resume, 3 chains, Nx=18 Nt=16", flush=True)
+    lat = _lattice()
+    n = lat.lattSize()
+
+    def start(tag):
+        rngs = [NumpyRNG(100 + b) for b in range(3)]
+        phis = np.stack([r.normal(0, PARAMS.tilde("U", lat) ** 0.5, n) + 0j for r in rngs])
+        files = [tmp_path / f"{tag}{b}" for b in range(3)]
+        drv = hmc.newBatchedRun(lat, PARAMS, rngs, make_action, files, False)
+        phis = torch.as_tensor(phis, device="cuda")
+        return drv, phis, eval_batch(drv.action, phis), files
+
+    drv, phis, acts, files_a = start("a")
+    drv(phis, acts, BatchedConstStepLeapfrog(drv.action, 1.0, 8, drv.rngs), 4, saveFreq=1, checkpointFreq=2)
+
+    drv, phis, acts, files_b = start("b")
+    drv(phis, acts, BatchedConstStepLeapfrog(drv.action, 1.0, 8, drv.rngs), 3, saveFreq=1, checkpointFreq=2)
+    drv2, phis2, acts2, ev = hmc.continueBatchedRun(files_b, -1, True, makeAction=make_action)
+    assert drv2.trajIdx == 2 and ev.length == 1.0 and ev.nstep == 8
+    drv2(torch.as_tensor(phis2, device="cuda"), acts2, ev, 2, saveFreq=1, checkpointFreq=2)
+    for fa, fb in zip(files_a, files_b):
+        ref, got = _configs(fa), _configs(fb)
+        assert sorted(got) == sorted(ref) == [1, 2, 3, 4]
+        for i in ref:
+            assert np.array_equal(got[i][0], ref[i][0]) and got[i][1:] == ref[i][1:], (fb, i)
+
+    # one chain resumed alone with the single-chain driver and a ConstStepLeapfrog
+    drv1, stage, ev1, _, _ = hmc.continueRun(files_a[1], tmp_path / "single", 2, False, makeAction=make_action)
+    assert isinstance(ev1, ConstStepLeapfrog) and ev1.nstep == 8
+    drv1(stage, ev1, 2, saveFreq=1, checkpointFreq=2)
+    ref, got = _configs(files_a[1]), _configs(tmp_path / "single")
+    for i in (3, 4):
+        assert got[i][2] == ref[i][2]
+        assert np.max(np.abs(got[i][0] - ref[i][0])) < 1e-12 * np.max(np.abs(ref[i][0]))
