@@ -46,6 +46,9 @@ def parse():
                     help="the nanotube workload (12 x 16 cell honeycomb cylinder, Nx=384, Nt=256, beta=10, U=2.5, "
                          "mu=0.5, DIA, spin basis, 60 steps) with the stable block-cyclic form: prints "
                          "chain-trajectories/s for --chains chains instead of the config-3 line")
+    ap.add_argument("--propagator", action="store_true",
+                    help="the all-to-all propagator measurement (M^-1 of one species, every spacetime source) on "
+                         "the Nx=288, Nt=128 torus: prints propagators/s instead of the HMC line")
     ap.add_argument("--force-only", action="store_true",
                     help="profiling aid: run --steps batched mixed forces and one 1 GiB device copy "
                          "(the calibration kernel for HBM byte counters), print timings, no HMC")
@@ -89,6 +92,80 @@ def config4(args):
                    "buckets": len(ens.buckets), "parallelism": "buckets on separate streams, one HIP graph"},
         "acceptance": float(pts.mean()) if pts.size else None,
         "median_dH": float(np.median(dh)) if dh.size else None}))
+
+
+def propagator(args):
+    """All-to-all propagator of the particles on the config-3 torus (Nx=288, Nt=128, beta=8,
+    U=3, EXP) for a field N(0, U beta/Nt) from NumpyRNG(42): meas.AllToAll with iterative
+    refinement (the exact solve to rounding) and without (solveM as the reference
+    computes it).  One step = one propagator.  Algorithmic work per propagator: solveM
+    against N = Nx Nt sources, Nt slice GEMMs of 8 Nx^2 N flops and Nt pairs of triangular
+    solves of 8 Nx^2 N flops = 16 N^2 Nx real flops (refinement adds about twice that,
+    not counted).  Accuracy: the largest entry of M G - 1 over the first 1024 sources
+    (applyM).  CPU baseline: the oracle's solveM against 288 of the sources."""
+    import torch
+    import isle_amd as isle
+    from isle_amd import meas
+    from isle_amd.evolver import EvolutionStage, NumpyRNG
+    from oracle import hubbard_ref as ref
+    kappa = ref.honeycomb(L)
+    nx = kappa.shape[0]
+    kt, ut = kappa * BETA / NT, U * BETA / NT
+    n = nx * NT
+    phi = NumpyRNG(42).normal(0, np.sqrt(ut), n) + 0j
+    hfm = isle.HubbardFermiMatrixExp(kt, 0.0, -1)
+    stage = EvolutionStage(phi, 0.0, 1)
+    res = {}
+    for refine in (8, 0):
+        a2a = meas.AllToAll(hfm, isle.Species.PARTICLE, refine=refine)
+        for w in range(args.warmup):
+            a2a(stage, -1 - w)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            g = a2a(stage, k)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        gm = g.permute(1, 0, 3, 2).reshape(n, n)
+        cols = gm[:, :1024].T.contiguous()  # rows: M^-1 applied to sources 0..1023
+        eye = torch.zeros_like(cols)
+        eye[torch.arange(1024), torch.arange(1024)] = 1
+        resid = (isle.applyM(hfm, phi, 0, cols) - eye).abs().max().item()
+        res[refine] = (dt, resid)
+        del g, gm, cols, a2a
+    flops = 16.0 * n * n * nx
+    dt, resid = res[8]
+    line = {
+        "metric": "all-to-all propagators/s at Nx=288 Nt=128 (M^-1, every spacetime source)",
+        "value": 1.0 / dt, "unit": "propagators/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "honeycomb 12x12 torus Nx=288 Nt=128 beta=8 U=3 mu=0 EXP, particles, N = 36864 "
+                               "sources in chunks", "refine": "until the residual stops falling (<= 8 steps)"},
+        "max_abs_MG_minus_1_first_1024_sources": resid,
+        "unrefined": {"ms_per_step": 1e3 * res[0][0], "max_abs_MG_minus_1_first_1024_sources": res[0][1]},
+        "roofline": {"bound": "mfma", "achieved": flops / res[0][0] / 1e12, "peak": PEAK["f64"], "unit": "TFLOP/s",
+                     "frac": flops / res[0][0] / 1e12 / PEAK["f64"], "traffic": None,
+                     "kernel": "whole unrefined propagator (16 N^2 Nx flops)"},
+    }
+    if not args.no_cpu:
+        # fixed cost (partial products, LUs) + per-source cost, from two sample sizes
+        rh = ref.HFMExp(kt, 0.0, -1)
+        times = {}
+        for k in (288, 1152):
+            sample = np.zeros((k, n))
+            sample[np.arange(k), np.arange(k)] = 1
+            t0 = time.perf_counter()
+            ref.solveM(rh, phi, 0, sample)
+            times[k] = time.perf_counter() - t0
+        per = (times[1152] - times[288]) / (1152 - 288)
+        t_cpu = max(times[288] - 288 * per, 0.0) + n * per
+        line["cpu_baseline"] = {"value": 1.0 / t_cpu, "unit": "propagators/s", "cores": os.cpu_count(),
+                                "kind": "port",
+                                "sample": f"oracle solveM (numpy/OpenBLAS, unrefined) against 288 and 1152 of the {n} "
+                                          f"sources ({times[288]:.2f} s, {times[1152]:.2f} s); fixed + per-source "
+                                          f"cost extrapolated to all sources: {t_cpu:.1f} s"}
+    print(json.dumps(line))
 
 
 def config5(args):
@@ -233,6 +310,9 @@ def main():
         return
     if args.config5:
         config5(args)
+        return
+    if args.propagator:
+        propagator(args)
         return
     import torch
     import isle_amd as isle

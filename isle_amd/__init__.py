@@ -9,17 +9,19 @@ Drop-in names for the reference's ``isle`` / ``isle.action`` surface on this pat
     makeHubbardFermiAction, HFAHopping, HFABasis, HFAAlgorithm},
     evolver.{ConstStepLeapfrog, LinearStepLeapfrog, BinarySelector, EvolutionStage},
     random.NumpyRNG, drivers.hmc.{newRun, continueRun, HMC} (run files with checkpoints,
-    isle_amd.runfile), util.parameters,
+    isle_amd.runfile), util.parameters, meas.{AllToAll, ChiralCondensate, Logdet}
+    (measurements on solveM / logdetM, refined to the exact solve), applyM (M times rows),
     autograd.{action_value, logdetM_value} (the action and log det M as
     torch.autograd functions whose backward is the force sweep), and
     ensemble.{Chain, EnsembleLeapfrog} (many chains of uneven Nx and Nt advanced
     together, one HIP graph per trajectory).
 
-Every numerical routine runs in the hand-written HIP kernels of libisle_hfa.so; there
-is no CPU fallback.
+Every routine of the fermion-action path runs in the hand-written HIP kernels of
+libisle_hfa.so; applyM (the residual of the measurements' refinement) is one complex
+GEMM of the library kind (torch on the device).  There is no CPU fallback.
 """
-from . import action, autograd, drivers, ensemble, evolver, runfile, util
-from .core import (CDVector, HubbardFermiMatrixDia, HubbardFermiMatrixExp, Species, Vector, as_field, invert,
+from . import action, autograd, drivers, ensemble, evolver, meas, runfile, util
+from .core import (CDVector, HubbardFermiMatrixDia, HubbardFermiMatrixExp, Species, Vector, applyM, as_field, invert,
                    logdet, logdetM, solveM, to_first_log_branch)
 from .integrator import leapfrog, leapfrog_batch
 from .lattice import Lattice, honeycomb, isBipartite, loadLattice
@@ -28,7 +30,7 @@ random = evolver  # isle.random.NumpyRNG lives in isle_amd.evolver
 
 __all__ = [
     "CDVector", "HubbardFermiMatrixDia", "HubbardFermiMatrixExp", "Lattice", "Species", "Vector", "action", "autograd",
-    "drivers", "ensemble", "runfile", "util",
+    "applyM", "drivers", "ensemble", "meas", "runfile", "util",
     "as_field", "evolver", "honeycomb", "invert", "isBipartite", "leapfrog", "leapfrog_batch", "loadLattice",
     "logdet", "logdetM", "random", "solveM", "to_first_log_branch",
 ]

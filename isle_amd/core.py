@@ -226,6 +226,27 @@ def logdetM(hfm, phi, species):
     return complex(out[0], out[1])
 
 
+def applyM(hfm, phi, species, x):
+    """M(phi) applied to every row of x (NRHS x NxNt, time-major), in row form: with v a
+    row split into time slices, (M v)_t = K v_t - F_t v_{t-1} for t >= 1 and
+    (M v)_0 = K v_0 + F_0 v_{nt-1} (hubbardFermiMatrixExp.cpp:150-166, K = 1;
+    hubbardFermiMatrixDia.cpp:140-160, F diagonal).  One complex GEMM of all slices on the
+    device; the slice matrices are the plan's own."""
+    nx = hfm.nx()
+    phi = as_field(phi, nx)
+    nt = phi.shape[-1] // nx
+    x = torch.as_tensor(np.asarray(x) if not isinstance(x, torch.Tensor) else x)
+    x = x.to(device=phi.device, dtype=torch.complex128)
+    rows = x.reshape(-1, nt, nx)
+    sign = 1j if Species(species) == Species.PARTICLE else -1j  # F(tp) = ... e^{sign i phi_{tp-1}}
+    ph = torch.exp(sign * torch.roll(phi.reshape(nt, nx), 1, 0))
+    y = torch.roll(rows, 1, 1) * ph
+    y[:, 0] = -y[:, 0]  # + F_0 v_{nt-1}: the antiperiodic wrap
+    s = torch.as_tensor(hfm._slice(species, False), device=phi.device).to(torch.complex128).T.contiguous()
+    out = rows - torch.matmul(y, s) if hfm.HOPPING == 1 else torch.matmul(rows, s) - y
+    return out.reshape(x.shape)
+
+
 def solveM(hfm, phi, species, rhss):
     """Solve M x = b for every row b of rhss (NRHS x NxNt); returns the same shape
     (bind_hubbardFermiMatrix.cpp:53-55, hubbardFermiMatrixExp.cpp:645-694)."""

@@ -110,6 +110,29 @@ __global__ __launch_bounds__(256) void stack_gather_kernel(const GatherItem* __r
     }
 }
 
+// d[i, :] = s[sigma[i], :] over the n rows of a split-plane block of `cols` columns (row
+// strides lds and ldd): the row interchanges of an LU applied to its right-hand sides.
+struct RowGatherItem {
+    const int* sigma;
+    const double* s_re;
+    const double* s_im;
+    double* d_re;
+    double* d_im;
+    long lds, ldd;
+    int n, cols;
+};
+
+__global__ __launch_bounds__(256) void row_gather_kernel(const RowGatherItem* __restrict__ items) {
+    const RowGatherItem it = items[blockIdx.z];
+    const int i = blockIdx.y;
+    if (i >= it.n) return;
+    const long src = (long)it.sigma[i] * it.lds, dst = (long)i * it.ldd;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < it.cols; c += gridDim.x * 256) {
+        it.d_re[dst + c] = it.s_re[src + c];
+        it.d_im[dst + c] = it.s_im[src + c];
+    }
+}
+
 // dst (n x n planes, re then im) = sign * src^T, src an n x n block with row stride lds.
 struct TBlockItem {
     const double* s_re;

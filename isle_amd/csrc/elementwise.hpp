@@ -379,6 +379,22 @@ __global__ void rhs_transpose_kernel(const double* __restrict__ rhss, double* __
     }
 }
 
+// inverse of rhs_transpose_kernel: blocks [t][2][n][nrhs] back to nrhs interleaved rows of nt*n
+__global__ void rhs_untranspose_kernel(const double* __restrict__ in, double* __restrict__ rhss, int n, int nt,
+                                       int nrhs) {
+    const long total = (long)nt * n * nrhs;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(e % nrhs);
+        const long rest = e / nrhs;
+        const int k = (int)(rest % n);
+        const int t = (int)(rest / n);
+        const long dst = 2 * ((long)c * nt * n + (long)t * n + k);
+        const double* blk = in + (long)t * 2 * n * nrhs;
+        rhss[dst] = blk[(long)k * nrhs + c];
+        rhss[dst + 1] = blk[(long)n * nrhs + (long)k * nrhs + c];
+    }
+}
+
 // z_t += z_{t-1} along t (blocks of 2*n*nrhs doubles)
 __global__ void prefix_sum_t_kernel(double* __restrict__ z, long blk, int nt) {
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < blk; e += (long)gridDim.x * blockDim.x) {

@@ -455,6 +455,10 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
             hipLaunchKernelGGL(diag_reduce_kernel, dim3(cdiv(n, 256), st.count), dim3(256), 0, s,
                                (const DReduceItem*)st.items);
             break;
+        case K::RGATHER:
+            hipLaunchKernelGGL(row_gather_kernel, dim3(std::min<int>(cdiv(st.aux, 256), 64), st.max_m, st.count), dim3(256),
+                               0, s, (const RowGatherItem*)st.items);
+            break;
         case K::TRSM: {
             // items: [diagonal-block products | trailing updates][panel][matrix]
             const auto* gi = (const GemmItem<double>*)st.items;
@@ -1415,6 +1419,15 @@ void Plan::force_parts(const double* phi, int nt, int batch, int mode, int group
     ISLE_HIP(hipMemcpyAsync(out_dev, op.parts, bytes, hipMemcpyDeviceToDevice, stream_));
 }
 
+double* Plan::solve_workspace(size_t doubles) {
+    if (doubles > solve_ws_doubles_) {
+        solve_ws_.reset();  // free before allocating the larger buffer
+        solve_ws_.reset(new DevBuf(doubles * sizeof(double)));
+        solve_ws_doubles_ = doubles;
+    }
+    return solve_ws_->as<double>();
+}
+
 // solveM (hubbardFermiMatrixExp.cpp:645-694, hubbardFermiMatrixDia.cpp:617-673).
 void Plan::solveM(const double* phi, int nt, int species, const double* rhss, int nrhs, double* out) {
     const int n = nx_;
@@ -1427,9 +1440,12 @@ void Plan::solveM(const double* phi, int nt, int species, const double* rhss, in
     double* ph = b.d64((size_t)2 * nt * n);
     double* part = b.d64((size_t)nt * 2 * nn);
     double* Mz = hop_ == DIA ? b.d64((size_t)nt * 2 * nn) : part;  // matrices multiplying the rhs
-    double* Bt = b.d64((size_t)nt * 2 * n * nrhs);
-    double* Z = b.d64((size_t)nt * 2 * n * nrhs);
-    double* Xl = b.d64((size_t)2 * n * nrhs);
+    // right-hand-side buffers: Bt, Z (nt blocks of 2 n nrhs), Xl and a scratch block
+    const size_t rblk = (size_t)2 * n * nrhs;
+    double* Bt = solve_workspace((2 * (size_t)nt + 2) * rblk);
+    double* Z = Bt + (size_t)nt * rblk;
+    double* Xl = Z + (size_t)nt * rblk;
+    double* Sc = Xl + rblk;
     int* piv = b.di((size_t)nt * n);
     auto D = [&](int t) { return CMat<double>{ph + (long)t * n, ph + (long)nt * n + (long)t * n}; };
     auto mP = [&](int t) { return cmat(part, t, nn); };
@@ -1518,6 +1534,61 @@ void Plan::solveM(const double* phi, int nt, int species, const double* rhss, in
         si.nrhs = nrhs;
         return si;
     };
+    if (n % LU_NB == 0 && nrhs >= 64) {
+        // many right-hand sides: the triangular solves as blocked GEMMs (rows gathered into
+        // each LU's pivot order, then unit-lower and upper solves with the stored inverses
+        // of the 32 x 32 diagonal blocks)
+        auto lu_mat = [&](int t, double* rre, double* rim, double* xre, double* xim) {
+            return TrsmMat{mP(t).re, mP(t).im, n, luw.as<double>() + t * lu_work_doubles(n), rre, rim, nrhs, xre, xim,
+                           nrhs};
+        };
+        auto rows = [&](double* p) { return std::make_pair(p, p + (long)n * nrhs); };
+        int* sig = nullptr;
+        {  // the last slice: x_{nt-1} = (1 + A^-1)^-1 z_{nt-1}
+            OpCache o;
+            o.nt = nt;
+            Builder bl(o);
+            sig = bl.di((size_t)nt * n);
+            std::vector<SigmaItem> sv;
+            for (int t = 0; t < nt; ++t) sv.push_back({piv + (long)t * n, sig + (long)t * n, n, n});
+            bl.add(K::SIGMA, sv);
+            auto z = rows(Z + (nt - 1) * blk), g0 = rows(Bt), g1 = rows(Sc), xl = rows(Xl);
+            std::vector<RowGatherItem> rg{{sig + (long)(nt - 1) * n, z.first, z.second, g0.first, g0.second, nrhs,
+                                           nrhs, n, nrhs}};
+            bl.add(K::RGATHER, rg, n, 0, 0, nrhs);
+            add_trsm(bl, 0, n, nrhs, {lu_mat(nt - 1, g0.first, g0.second, g1.first, g1.second)});
+            add_trsm(bl, 1, n, nrhs, {lu_mat(nt - 1, g1.first, g1.second, xl.first, xl.second)});
+            bl.finish();
+            run(o, phi);
+            if (nt > 1) {
+                hipLaunchKernelGGL(sub_last_kernel, dim3(ew_blocks(blk * (nt - 1))), dim3(EW_THREADS), 0, s, Z, Xl,
+                                   blk, nt - 1);
+                ISLE_HIP(hipGetLastError());
+            }
+            // every slice: x_t = A_t^-1 (z_t - x_{nt-1}) (z_{nt-1} unchanged: solves x_{nt-1} again)
+            OpCache o2;
+            o2.nt = nt;
+            Builder b2(o2);
+            std::vector<RowGatherItem> ra;
+            std::vector<TrsmMat> tl, tu;
+            for (int t = 0; t < nt; ++t) {
+                auto zt = rows(Z + t * blk), bt = rows(Bt + t * blk);
+                ra.push_back({sig + (long)t * n, zt.first, zt.second, bt.first, bt.second, nrhs, nrhs, n, nrhs});
+                tl.push_back(lu_mat(t, bt.first, bt.second, zt.first, zt.second));
+                tu.push_back(lu_mat(t, zt.first, zt.second, bt.first, bt.second));
+            }
+            b2.add(K::RGATHER, ra, n, 0, 0, nrhs);
+            add_trsm(b2, 0, n, nrhs, tl);
+            add_trsm(b2, 1, n, nrhs, tu);
+            b2.finish();
+            run(o2, phi);
+            hipLaunchKernelGGL(rhs_untranspose_kernel, dim3(ew_blocks((long)nt * n * nrhs)), dim3(EW_THREADS), 0, s, Bt,
+                               out, n, nt, nrhs);
+            ISLE_HIP(hipGetLastError());
+            ISLE_HIP(hipStreamSynchronize(s));
+        }
+        return;
+    }
     std::vector<SolveItem> s1{sitem(nt - 1, Z + (nt - 1) * blk, nrhs, 1, Xl, Xl + (long)n * nrhs, nrhs, 1)};
     DevBuf ds1(sizeof(SolveItem));
     ISLE_HIP(hipMemcpy(ds1.p, s1.data(), sizeof(SolveItem), hipMemcpyHostToDevice));

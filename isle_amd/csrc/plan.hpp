@@ -63,6 +63,7 @@ enum class K : int {
     TRSM,               // blocked triangular solve on GEMMs (flag 0: unit lower, 1: upper; aux: items)
     GEMM32_RC_PRE_DIAG, // GEMM32_RC_PRE with the force diagonal's partial row sums fused (f32)
     DREDUCE,            // sum of those partial row sums
+    RGATHER,            // rows of right-hand sides in the pivot order of an LU (max_m = rows, aux = columns)
 };
 
 struct Step {
@@ -138,6 +139,11 @@ private:
     std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<OpCache>> cache_;
 
     std::unique_ptr<DevBuf> ident64_;
+    // solveM's right-hand-side buffers (GBs for many sources), kept between calls and grown
+    // when a call needs more: a hipMalloc of that size costs far more than the solve
+    std::unique_ptr<DevBuf> solve_ws_;
+    size_t solve_ws_doubles_ = 0;
+    double* solve_workspace(size_t doubles);
 
     void build_exp(const std::vector<double>& x, DevBuf& out);
     OpCache& get_force_plain(int nt, int batch, int nspecies, int spin);
