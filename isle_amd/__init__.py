@@ -7,7 +7,8 @@ Drop-in names for the reference's ``isle`` / ``isle.action`` surface on this pat
     HubbardFermiMatrixExp, HubbardFermiMatrixDia, logdetM, solveM, logdet, invert,
     leapfrog, action.{Action, SumAction, HubbardGaugeAction, HubbardFermiAction,
     makeHubbardFermiAction, HFAHopping, HFABasis, HFAAlgorithm},
-    evolver.{ConstStepLeapfrog, LinearStepLeapfrog, BinarySelector, EvolutionStage},
+    evolver.{ConstStepLeapfrog, LinearStepLeapfrog, TwoPiJumps, UniformJump, BinarySelector,
+    EvolutionStage},
     random.NumpyRNG, drivers.hmc.{newRun, continueRun, HMC} (run files with checkpoints,
     isle_amd.runfile), util.parameters, meas.{AllToAll, ChiralCondensate, Logdet}
     (measurements on solveM / logdetM, refined to the exact solve), applyM (M times rows),

@@ -14,6 +14,11 @@ trajectory loop, with the reference's semantics and random-number consumption.
 * ``hmc``: the HMC loop of src/isle/drivers/hmc.py:41-84 without HDF5 output; a start
   configuration that is not a stage is evaluated first (hmc.py:64-66).
 
+``TwoPiJumps`` and ``UniformJump`` (src/isle/evolver/hubbard.py:15-184) shift the field by
+2 pi at random sites or by 2 pi / Nt everywhere; the particle-hole fermion action is
+invariant under both, so with only gauge and such fermion actions the new action value
+is the old one plus the change of the gauge action (no evaluation on the GPU).
+
 ``BatchedConstStepLeapfrog`` advances many independent chains together (one field tensor
 [batch, nx*nt]); each chain has its own NumpyRNG, consumed exactly as a lone chain would.
 """
@@ -277,6 +282,144 @@ class LinearStepLeapfrog:
         return f"""<LinearStepLeapfrog> (0x{id(self):x})
   lengthRange = {self.lengthRange}, nstepRange = {self.nstepRange}, ninterp = {self.ninterp}
   acceptance rate = {np.mean(self.trajPoints)}"""
+
+
+class _HubbardActionShortcut:
+    """Action values after jumps that leave the fermion determinant invariant
+    (src/isle/evolver/hubbard.py:187-287).
+
+    The shortcut applies when the action is a sum of HubbardGaugeActions and particle-hole
+    basis HubbardFermiActions: the fermion matrices hold the field only through e^{+-i phi},
+    so a shift by 2 pi at any site, or by 2 pi / Nt at every site (the product of the Nt
+    slices gains e^{+-2 pi i}), leaves them unchanged, and only the gauge action
+    sum phi^2 / (2 Utilde) moves.  In the spin basis the field enters as e^{+-phi}: no
+    shortcut, the action is evaluated.  (The reference names fermion-action classes that
+    do not exist here, so its shortcut never applies to a fermion action, and its global
+    update divides by the sum of 1 / (2 Utilde) where the local one multiplies.)"""
+
+    def __init__(self, action):
+        utildes = self._find_utildes(action)
+        if utildes is None:
+            self._action = action
+            self._half_inv_u = None
+        else:
+            self._action = None
+            self._half_inv_u = sum(1.0 / u for u in utildes) / 2
+
+    @classmethod
+    def _find_utildes(cls, action):
+        from .action import HFABasis, HubbardFermiAction, HubbardGaugeAction, SumAction
+        if isinstance(action, SumAction):
+            found = []
+            for sub in action:
+                aux = cls._find_utildes(sub)
+                if aux is None:
+                    return None
+                found.extend(aux)
+            return found
+        if isinstance(action, HubbardGaugeAction):
+            return [action.utilde]
+        if isinstance(action, HubbardFermiAction) and action.basis == HFABasis.PARTICLE_HOLE:
+            return []
+        return None
+
+    @property
+    def shortcut(self):
+        return self._action is None
+
+    def evalLocal(self, newPhi, sites, shifts, actVal):
+        """Action at newPhi = old phi shifted by ``shifts`` at ``sites``."""
+        if self._action is None:
+            return actVal + np.dot(2 * newPhi[sites] - shifts, shifts) * self._half_inv_u
+        return complex(self._action.eval(newPhi))
+
+    def evalGlobal(self, newPhi, shift, actVal):
+        """Action at newPhi = old phi + shift at every site."""
+        if self._action is None:
+            return actVal + shift * (2 * np.sum(newPhi) - shift * len(newPhi)) * self._half_inv_u
+        return complex(self._action.eval(newPhi))
+
+
+class TwoPiJumps:
+    """Shift the field by +-2 pi at random sites, momenta untouched (hubbard.py:15-116).
+    Per batch: ``batchSize`` distinct sites rng.choice(N, batchSize, replace=False), signs
+    rng.choice((-2 pi, 2 pi), batchSize), then accept / reject of that batch alone.  The
+    returned stage holds the field after the last accepted batch (the reference returns the
+    last batch's proposal, accepted or not)."""
+
+    def __init__(self, nBatches, batchSize, action, lattice, rng):
+        self.nBatches = int(nBatches)
+        self.batchSize = int(batchSize)
+        self.rng = rng
+        self.latSize = lattice.lattSize()
+        self.selector = BinarySelector(rng)
+        self.acceptedPerBatch = []
+        self._action = _HubbardActionShortcut(action)
+
+    def evolve(self, stage):
+        phi = _host(stage.phi)
+        actVal = complex(stage.actVal)
+        accepted = []
+        for _ in range(self.nBatches):
+            sites = self.rng.choice(self.latSize, self.batchSize, replace=False)
+            shifts = self.rng.choice((-2 * np.pi, +2 * np.pi), self.batchSize, replace=True)
+            newPhi = phi.copy()
+            newPhi[sites] += shifts
+            newActVal = self._action.evalLocal(newPhi, sites, shifts, actVal)
+            if self.selector.selectTrajPoint(actVal, newActVal) == 1:
+                phi, actVal = newPhi, newActVal
+                accepted.append(1)
+            else:
+                accepted.append(0)
+        self.acceptedPerBatch.append(accepted)
+        return stage.accept(phi, actVal) if any(accepted) else stage.reject()
+
+    def save(self, group, manager):
+        group["nBatches"] = self.nBatches
+        group["batchSize"] = self.batchSize
+
+    @classmethod
+    def fromH5(cls, group, manager, action, lattice, rng):
+        return cls(int(group["nBatches"][()]), int(group["batchSize"][()]), action, lattice, rng)
+
+    def report(self):
+        return f"""<TwoPiJumps> (0x{id(self):x})
+  nBatches = {self.nBatches}, batchSize = {self.batchSize}
+  acceptance rate per batch = {np.mean(self.acceptedPerBatch)}"""
+
+
+class UniformJump:
+    """Shift the field by +-2 pi / Nt at every site, momenta untouched (hubbard.py:119-184):
+    sign rng.choice((-1, 1)), then accept / reject."""
+
+    def __init__(self, action, lattice, rng):
+        self.rng = rng
+        self.selector = BinarySelector(rng)
+        self.accepted = []
+        self._action = _HubbardActionShortcut(action)
+        self._absShift = 2 * np.pi / lattice.nt()
+
+    def evolve(self, stage):
+        shift = self.rng.choice((-1, +1)) * self._absShift
+        newPhi = _host(stage.phi) + shift
+        newActVal = self._action.evalGlobal(newPhi, shift, complex(stage.actVal))
+        if self.selector.selectTrajPoint(stage.actVal, newActVal) == 1:
+            self.accepted.append(1)
+            return stage.accept(newPhi, newActVal)
+        self.accepted.append(0)
+        return stage.reject()
+
+    def save(self, group, manager):
+        pass
+
+    @classmethod
+    def fromH5(cls, group, manager, action, lattice, rng):
+        return cls(action, lattice, rng)
+
+    def report(self):
+        return f"""<UniformJump> (0x{id(self):x})
+  shift = 2 pi / Nt = {self._absShift}
+  acceptance rate = {np.mean(self.accepted)}"""
 
 
 class BatchedConstStepLeapfrog:

@@ -237,7 +237,8 @@ class EvolverManager:
 
     def _types(self):
         from . import evolver as ev
-        types = {cls.__name__: cls for cls in (ev.ConstStepLeapfrog, ev.LinearStepLeapfrog)}
+        types = {cls.__name__: cls for cls in (ev.ConstStepLeapfrog, ev.LinearStepLeapfrog, ev.TwoPiJumps,
+                                               ev.UniformJump)}
         types.update(self.definitions)
         return types
 
