@@ -5,7 +5,7 @@ Drop-in names for the reference's ``isle`` / ``isle.action`` surface on this pat
 
     Vector, CDVector, Species, Lattice, isBipartite, loadLattice,
     HubbardFermiMatrixExp, HubbardFermiMatrixDia, logdetM, solveM, logdet, invert,
-    leapfrog, action.{Action, SumAction, HubbardGaugeAction, HubbardFermiAction,
+    leapfrog, rungeKutta4Flow, action.{Action, SumAction, HubbardGaugeAction, HubbardFermiAction,
     makeHubbardFermiAction, HFAHopping, HFABasis, HFAAlgorithm},
     evolver.{ConstStepLeapfrog, LinearStepLeapfrog, TwoPiJumps, UniformJump, BinarySelector,
     EvolutionStage},
@@ -24,7 +24,7 @@ GEMM of the library kind (torch on the device).  There is no CPU fallback.
 from . import action, autograd, drivers, ensemble, evolver, meas, runfile, util
 from .core import (CDVector, HubbardFermiMatrixDia, HubbardFermiMatrixExp, Species, Vector, applyM, as_field, invert,
                    logdet, logdetM, solveM, to_first_log_branch)
-from .integrator import leapfrog, leapfrog_batch
+from .integrator import leapfrog, leapfrog_batch, rungeKutta4Flow
 from .lattice import Lattice, honeycomb, isBipartite, loadLattice
 
 random = evolver  # isle.random.NumpyRNG lives in isle_amd.evolver
@@ -33,5 +33,5 @@ __all__ = [
     "CDVector", "HubbardFermiMatrixDia", "HubbardFermiMatrixExp", "Lattice", "Species", "Vector", "action", "autograd",
     "applyM", "drivers", "ensemble", "meas", "runfile", "util",
     "as_field", "evolver", "honeycomb", "invert", "isBipartite", "leapfrog", "leapfrog_batch", "loadLattice",
-    "logdet", "logdetM", "random", "solveM", "to_first_log_branch",
+    "logdet", "logdetM", "random", "rungeKutta4Flow", "solveM", "to_first_log_branch",
 ]

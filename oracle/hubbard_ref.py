@@ -27,6 +27,7 @@ function                     reference
 ``HubbardGaugeAction``       src/isle/cpp/action/hubbardGaugeAction.cpp:7-13
 ``SumAction``                src/isle/cpp/action/sumAction.cpp:46-58
 ``leapfrog``                 src/isle/cpp/integrator.cpp:9-36
+``rk4_flow``                 src/isle/cpp/integrator.cpp:39-195 (k4 with the direction, 3/8-rule beta21 = 1/3)
 ``binary_select``            src/isle/evolver/selector.py:20-30
 ``const_step_leapfrog``      src/isle/evolver/leapfrog.py:38-66
 ===========================  =====================================================
@@ -533,6 +534,57 @@ def leapfrog(phi, pi, action, length, nsteps, direction=1.0):
         phi_out = phi_out + pi_out * eps
     pi_out = pi_out + np.real(action.force(phi_out)) * (eps / 2)
     return phi_out, pi_out, action.eval(phi_out)
+
+
+# classic RK4 (n = 0) and the 3/8 rule (n = 1): weights, then the stage coefficients
+# beta21, beta31, beta32, beta41, beta42, beta43
+_RK4 = {0: ((1 / 6, 1 / 3, 1 / 3, 1 / 6), (0.5, 0.0, 0.5, 0.0, 0.0, 1.0)),
+        1: ((0.125, 0.375, 0.375, 0.125), (1 / 3, -1 / 3, 1.0, 1.0, -1.0, 1.0))}
+
+
+def _rk4_step(phi, action, eps, direction, n):
+    """integrator.cpp:77-117: k_i = -eps direction conj(force(...))."""
+    w, b = _RK4[n]
+    e = eps * direction
+    k1 = -e * np.conj(action.force(phi))
+    k2 = -e * np.conj(action.force(phi + b[0] * k1))
+    k3 = -e * np.conj(action.force(phi + b[1] * k1 + b[2] * k2))
+    k4 = -e * np.conj(action.force(phi + b[3] * k1 + b[4] * k2 + b[5] * k3))
+    out = phi + w[0] * k1 + w[1] * k2 + w[2] * k3 + w[3] * k4
+    return out, action.eval(out)
+
+
+def rk4_flow(phi, action, flow_time, step, act_val=None, n=0, direction=1.0, attenuation=0.9,
+             threshold=1e-8, min_step=None, tolerance=1e-3):
+    """integrator.cpp:144-193: adaptive RK4 holomorphic flow; a step is accepted when
+    |e^{i (Im S_old - Im S_new)} - 1| <= tolerance, the step size shrinks by
+    attenuation (tol/err)^(1/5) (not below min_step) after a rejection and grows by
+    min(max(attenuation (tol/err)^(1/4), 1), 2) after an error below threshold*tol.
+    Returns (phi, action value, flow time reached)."""
+    phi = np.asarray(phi, dtype=np.complex128)
+    if act_val is None:
+        act_val = action.eval(phi)
+    if min_step is None:
+        min_step = max(step / 1000.0, 1e-12)
+    t = 0.0
+    while t < flow_time:
+        if t + step > flow_time:
+            step = flow_time - t
+            if step < min_step:
+                break
+        new_phi, new_act = _rk4_step(phi, action, step, direction, n)
+        err = abs(np.exp(1j * (np.imag(act_val) - np.imag(new_act))) - 1.0)
+        if err > tolerance:
+            if step == min_step:
+                break
+            step = max(step * attenuation * (tolerance / err) ** 0.2, min_step)
+        else:
+            t += step
+            phi, act_val = new_phi, new_act
+            if err < threshold * tolerance:
+                grow = 2.0 if err == 0 else min(max(attenuation * (tolerance / err) ** 0.25, 1.0), 2.0)
+                step *= grow
+    return phi, act_val, t
 
 
 def binary_select(rng, energy0, energy1):
