@@ -94,6 +94,30 @@ def config4(args):
         "median_dH": float(np.median(dh)) if dh.size else None}))
 
 
+# the dominant kind's kernel as it appears in the rocprofv3 PMC summary of this workload
+PMC_SUMMARY = "profiles/force_mixed_nx288_nt128_chains64_hbm_traffic.md"
+PMC_ROWS = {"gemm64_rc_row": ("gemm_rcd_kernel<double, 1, false, 96, 96, 16, 2, 2, 6>", "3538944")}
+
+
+def _pmc_traffic(kind, chains):
+    """HBM bytes per launch of the dominant kernel (calibrated FETCH + WRITE, bytes) from the
+    committed rocprofv3 PMC summary of the 64-chain force, or None for other workloads
+    (bench.py cannot run the counters itself: they need their own profiler passes)."""
+    if chains != 64 or kind not in PMC_ROWS:
+        return None
+    name, grid = PMC_ROWS[kind]
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), PMC_SUMMARY)
+    try:
+        with open(path) as f:
+            for line in f:
+                cells = [c.strip().strip("`") for c in line.split("|")]
+                if len(cells) > 5 and cells[1] == name and cells[2] == grid:
+                    return (float(cells[4]) + float(cells[5])) * 2 ** 20
+    except (OSError, ValueError):
+        return None
+    return None
+
+
 def propagator(args):
     """All-to-all propagator of the particles on the config-3 torus (Nx=288, Nt=128, beta=8,
     U=3, EXP) for a field N(0, U beta/Nt) from NumpyRNG(42): meas.AllToAll with iterative
@@ -380,8 +404,10 @@ def main():
     avg_ms = ms_op / launches
     achieved = flops_op / launches / (avg_ms * 1e-3) / 1e12
     prec = "f32" if "32" in KIND_NAMES[dom] else "f64"
+    traffic = _pmc_traffic(KIND_NAMES[dom], chains)
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK[prec], "unit": "TFLOP/s",
-                "frac": achieved / PEAK[prec], "traffic": None, "kernel": KIND_NAMES[dom],
+                "frac": achieved / PEAK[prec], "traffic": traffic,
+                "traffic_source": PMC_SUMMARY if traffic is not None else None, "kernel": KIND_NAMES[dom],
                 "avg_launch_ms": avg_ms, "flops_per_launch": flops_op / launches,
                 "force_ms": float(prof[:, 1].sum()),
                 "breakdown_ms": {KIND_NAMES[k]: round(float(prof[k, 1]), 4) for k in range(nk) if prof[k, 0] > 0}}
