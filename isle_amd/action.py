@@ -22,8 +22,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .core import (HubbardFermiMatrixDia, HubbardFermiMatrixExp, Lattice, as_field, isBipartite,
-                   to_first_log_branch)
+from .core import (HubbardFermiMatrixDia, HubbardFermiMatrixExp, Lattice, Species, as_field, invert, isBipartite,
+                   logdet, to_first_log_branch)
 
 
 class HFAHopping(enum.IntEnum):
@@ -148,17 +148,20 @@ def _hole_shortcut_possible(basis, kappa, mu_tilde, sigma_kappa):
 class HubbardFermiAction(Action):
     """Fermion action -log det M(phi) M(-phi) (hubbardFermiAction.hpp:96-142).
 
-    Only the DIRECT_SINGLE algorithm runs on the GPU; DIRECT_SQUARE (a dense
-    (NxNt)^2 inverse of Q) raises NotImplementedError.
+    DIRECT_SQUARE (hubbardFermiAction.cpp:62-108, 230-272) is the reference's small-size
+    cross-check: eval = -log det Q and the force from the blocks of Q^-1 next to the
+    off-diagonal blocks T+ and T- of Q, with Q = M_p M_h^T (NxNt)^2 dense, formed by one
+    complex GEMM of the dense M matrices; log det and inverse by the HIP LU, so
+    Nx*Nt <= 512.  FP64 only; no per-species parts (logdets, force_parts).
     """
 
     def __init__(self, kappaTilde, muTilde, sigmaKappa, hopping=HFAHopping.DIA,
                  basis=HFABasis.PARTICLE_HOLE, algorithm=HFAAlgorithm.DIRECT_SINGLE,
                  allowShortcut=False, precision="fp64", group=32):
-        if HFAAlgorithm(algorithm) != HFAAlgorithm.DIRECT_SINGLE:
-            raise NotImplementedError("HFAAlgorithm.DIRECT_SQUARE is not implemented on the GPU")
         if precision not in ("fp64", "mixed", "stable"):
             raise ValueError("precision must be 'fp64', 'mixed' or 'stable'")
+        if HFAAlgorithm(algorithm) == HFAAlgorithm.DIRECT_SQUARE and precision != "fp64":
+            raise ValueError("HFAAlgorithm.DIRECT_SQUARE runs in FP64 only")
         kappa = kappaTilde.hopping() if isinstance(kappaTilde, Lattice) else np.asarray(kappaTilde, dtype=np.float64)
         self.hopping = HFAHopping(hopping)
         self.basis = HFABasis(basis)
@@ -178,6 +181,73 @@ class HubbardFermiAction(Action):
             self._slice_spread = float(np.log((1.0 + mu + r) / lo)) if lo > 0 else np.inf
         self.shortcut = bool(allowShortcut) and _hole_shortcut_possible(self.basis, kappa, float(muTilde),
                                                                          int(sigmaKappa))
+        if self.algorithm == HFAAlgorithm.DIRECT_SQUARE:
+            self.shortcut = False  # Q holds both species
+
+    # ---- DIRECT_SQUARE (hubbardFermiAction.cpp:62-108): dense Q = M_p M_h^T
+    SQUARE_MAX = 512  # order of the dense LU
+
+    def _field(self, phi):
+        return -1j * phi if self.basis == HFABasis.SPIN else phi
+
+    def _q(self, field):
+        n = field.numel()
+        if n > self.SQUARE_MAX:
+            raise ValueError(f"HFAAlgorithm.DIRECT_SQUARE needs Nx*Nt <= {self.SQUARE_MAX} (dense LU), got {n}")
+        mp = self.hfm.M(field, Species.PARTICLE)
+        mh = self.hfm.M(field, Species.HOLE)
+        return torch.matmul(mp, mh.transpose(0, 1).contiguous())
+
+    def _t_pm(self, field, nt):
+        """T+(tau + 1) and T-(tau) for tau = 0..nt-1, [nt, nx, nx] each (the blocks of Q at
+        (tau+1, tau) and (tau, tau+1); hubbardFermiMatrixExp.cpp:188-208,
+        hubbardFermiMatrixDia.cpp:183-214)."""
+        nx = self.hfm.nx()
+        dev = field.device
+        tplus, tminus = [], []
+        if self.hopping == HFAHopping.EXP:
+            for tau in range(nt):
+                tp = (tau + 1) % nt
+                sp = 1.0 if tp == 0 else -1.0  # -anti, anti = -1 on the wrap-around
+                tplus.append(sp * self.hfm.F(tp, field, Species.PARTICLE, False))
+                sm = 1.0 if tau == nt - 1 else -1.0
+                tminus.append(sm * self.hfm.F(tp, field, Species.HOLE, False).transpose(0, 1))
+            return torch.stack(tplus), torch.stack(tminus)
+        kappa = torch.as_tensor(self.hfm.kappaTilde(), device=dev).to(torch.complex128)
+        mu, sk = self.hfm.muTilde(), self.hfm.sigmaKappa()
+        eye = torch.eye(nx, dtype=torch.complex128, device=dev)
+        a = sk * kappa - (1 - mu) * eye
+        b = kappa - (1 + mu) * eye
+        f = field.reshape(nt, nx)
+        for tau in range(nt):
+            tp = (tau + 1) % nt
+            ap = -1.0 if tp == 0 else 1.0
+            tplus.append((ap * torch.exp(1j * f[tau]))[:, None] * a)  # phases of slice tp - 1 = tau
+            am = -1.0 if tau == nt - 1 else 1.0
+            tminus.append(b * (am * torch.exp(-1j * f[tau]))[None, :])
+        return torch.stack(tplus), torch.stack(tminus)
+
+    def _eval_square(self, phi):
+        return -to_first_log_branch(logdet(self._q(self._field(phi))))
+
+    def _force_square(self, phi):
+        field = self._field(phi)
+        nx = self.hfm.nx()
+        nt = field.numel() // nx
+        blocks = invert(self._q(field)).reshape(nt, nx, nt, nx)
+        tau = torch.arange(nt, device=field.device)
+        up = blocks[tau, :, (tau + 1) % nt, :]    # Q^-1 block (tau, tau+1), [nt, nx, nx]
+        down = blocks[(tau + 1) % nt, :, tau, :]  # Q^-1 block (tau+1, tau)
+        tplus, tminus = self._t_pm(field, nt)
+
+        def diag(a, b):  # diagonal of a b, per tau
+            return (a * b.transpose(1, 2)).sum(dim=2)
+        if self.hopping == HFAHopping.DIA:
+            f = 1j * diag(tplus, up) - 1j * diag(down, tminus)
+        else:
+            f = 1j * diag(up, tplus) - 1j * diag(tminus, down)
+        f = f.reshape(-1)
+        return -1j * f if self.basis == HFABasis.SPIN else f
 
     def _mode(self):
         return {"fp64": 0, "mixed": 1, "stable": 2}[self.precision]
@@ -201,8 +271,13 @@ class HubbardFermiAction(Action):
             phi = phi.reshape(1, -1)
         return phi, phi.shape[0], phi.shape[1] // nx
 
+    def _single_only(self):
+        if self.algorithm == HFAAlgorithm.DIRECT_SQUARE:
+            raise ValueError("per-species log dets and force parts belong to HFAAlgorithm.DIRECT_SINGLE")
+
     def logdets(self, phis):
         """[batch][nspecies] complex log det M of each field (aux = -i phi in the spin basis)."""
+        self._single_only()
         phis, batch, nt = self._shape(phis)
         nsp = 1 if self.shortcut else 2
         out = np.empty((batch, nsp, 2))
@@ -213,6 +288,9 @@ class HubbardFermiAction(Action):
 
     def eval_batch(self, phis):
         """Action of each field in a [batch, nx*nt] tensor (hubbardFermiAction.cpp:230-264)."""
+        if self.algorithm == HFAAlgorithm.DIRECT_SQUARE:
+            phis, batch, _ = self._shape(phis)
+            return np.array([self._eval_square(phis[b]) for b in range(batch)])
         ld = self.logdets(phis)
         res = []
         for row in ld:
@@ -229,6 +307,7 @@ class HubbardFermiAction(Action):
         """forceDirectSinglePart (hubbardFermiAction.cpp:16-60) of each species for each
         field: [batch, nspecies, nx*nt] (one species with the hole shortcut; evaluated at
         aux = -i phi in the spin basis)."""
+        self._single_only()
         phis, batch, nt = self._shape(phis)
         nx = self.hfm.nx()
         nsp = 1 if self.shortcut else 2
@@ -240,6 +319,16 @@ class HubbardFermiAction(Action):
 
     def force_batch(self, phis, out=None, accumulate=False):
         """Force -dS/dphi of each field (hubbardFermiAction.cpp:243-272), [batch, nx*nt]."""
+        if self.algorithm == HFAAlgorithm.DIRECT_SQUARE:
+            phis, batch, _ = self._shape(phis)
+            f = torch.stack([self._force_square(phis[b]) for b in range(batch)])
+            if out is None:
+                return f
+            if accumulate:
+                out += f
+            else:
+                out.copy_(f)
+            return out
         phis, batch, nt = self._shape(phis)
         nx = self.hfm.nx()
         nsp = 1 if self.shortcut else 2

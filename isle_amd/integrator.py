@@ -16,7 +16,7 @@ import math
 import torch
 
 from . import _lib
-from .action import HubbardFermiAction, HubbardGaugeAction, SumAction
+from .action import HFAAlgorithm, HubbardFermiAction, HubbardGaugeAction, SumAction
 from .core import as_field
 
 
@@ -120,6 +120,10 @@ class LeapfrogGraph:
     whose plans must not be destroyed while it can be replayed."""
 
     def __init__(self, action, batch, n, length, nsteps, direction=1.0, split=None):
+        fused = _split_fused(action)
+        if fused is not None and fused[0].algorithm == HFAAlgorithm.DIRECT_SQUARE:
+            raise ValueError("HFAAlgorithm.DIRECT_SQUARE (dense LU with its own allocations) cannot be captured "
+                             "in a graph; integrate it eagerly")
         dev = _lib.device()
         self.phi = torch.zeros((batch, n), dtype=torch.complex128, device=dev)
         self.pi = torch.zeros_like(self.phi)
