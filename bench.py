@@ -23,7 +23,7 @@ PEAK = {"f32": 157.3, "f64": 78.6}  # dense MFMA TFLOP/s, MI355X
 KIND_NAMES = ["gemm64_rc_pre", "gemm64_rc_row", "gemm64_cr_pre", "gemm64_cc", "gemm32_rc_row", "gemm32_rc_pre",
               "scale64", "scale32", "lu", "lu_solve", "diag64", "diag32", "copy64", "cvt32to64", "cvt64to32",
               "phase", "tcvt64to32", "diag64x32", "gemm64_cc_sub", "lu_shape", "block", "sigma", "gather",
-              "tblock", "trsm", "gemm32_rc_pre_diag", "dreduce"]
+              "tblock", "trsm", "gemm32_rc_pre_diag", "dreduce", "rgather"]
 
 
 def parse():

@@ -119,7 +119,8 @@ int isle_cscale(const double* in, double* out, long n, double cr, double ci, voi
  * copies, 15 phases, 16 transposed copy, 17 diagonal extraction FP64 x FP32, 18 GEMM64
  * complex x complex subtracted, 19 LU of a block-cyclic elimination step, 20 block
  * assembly, 21 row permutation, 22 permuted gather, 23 transposed block extraction, 24 blocked
- * triangular solve (GEMM launches). */
+ * triangular solve (GEMM launches), 25 GEMM32 real x complex with the force diagonal's row sums
+ * fused, 26 sum of those row sums, 27 rows of right-hand sides in pivot order (solveM). */
 int isle_profile_kernels(isle_hfm* h, const double* phi, int op, int nt, int batch, int group, int nspecies,
                          int spin, int reps, double* out_host, int nkinds);
 
