@@ -85,6 +85,15 @@ int isle_force_parts(isle_hfm* h, const double* phi, int nt, int batch, int mode
  * rhss, out: device [nrhs][nx*nt] complex128. */
 int isle_solveM(isle_hfm* h, const double* phi, int nt, int species, const double* rhss, int nrhs, double* out);
 
+/* solveM with the factorization kept between calls (no counterpart in the reference: for
+ * callers that solve many right-hand sides against one field, such as iterative refinement).
+ * reuse = 0: as isle_solveM (the handle keeps the factorization: partial products and their
+ * LUs, 2 nt nx^2 complex128).  reuse = 1: the caller promises that phi is the field of the
+ * previous solveM on this handle, unchanged; with the same nt and species the factorization
+ * is reused (otherwise recomputed). */
+int isle_solveM_ex(isle_hfm* h, const double* phi, int nt, int species, const double* rhss, int nrhs, double* out,
+                   int reuse);
+
 /* Combination of the species terms into the fermion force, n complex elements
  *   (hubbardFermiAction.cpp:243-272): mode 0: -i(fp - fh); 1: -i(fp - conj fp); 2: fh - fp. */
 int isle_combine_force(const double* fp, const double* fh, double* out, long n, int mode, int accumulate,

@@ -247,9 +247,12 @@ def applyM(hfm, phi, species, x):
     return out.reshape(x.shape)
 
 
-def solveM(hfm, phi, species, rhss):
+def solveM(hfm, phi, species, rhss, reuse_factor=False):
     """Solve M x = b for every row b of rhss (NRHS x NxNt); returns the same shape
-    (bind_hubbardFermiMatrix.cpp:53-55, hubbardFermiMatrixExp.cpp:645-694)."""
+    (bind_hubbardFermiMatrix.cpp:53-55, hubbardFermiMatrixExp.cpp:645-694).
+
+    ``reuse_factor`` (no counterpart in the reference): phi is the field of the previous
+    solveM on this matrix, unchanged, so its partial products and LUs are reused."""
     nx = hfm.nx()
     phi = as_field(phi, nx)
     nt = phi.shape[-1] // nx
@@ -260,6 +263,6 @@ def solveM(hfm, phi, species, rhss):
     if r.shape[1] != nx * nt:
         raise RuntimeError("Right hand side does not have correct size (spacetime vector)")
     out = torch.empty_like(r)
-    _lib.check(_lib.load().isle_solveM(hfm.handle, _lib.ptr(phi), nt, int(species), _lib.ptr(r), r.shape[0],
-                                       _lib.ptr(out)))
+    _lib.check(_lib.load().isle_solveM_ex(hfm.handle, _lib.ptr(phi), nt, int(species), _lib.ptr(r), r.shape[0],
+                                          _lib.ptr(out), int(bool(reuse_factor))))
     return out

@@ -1429,53 +1429,77 @@ double* Plan::solve_workspace(size_t doubles) {
 }
 
 // solveM (hubbardFermiMatrixExp.cpp:645-694, hubbardFermiMatrixDia.cpp:617-673).
-void Plan::solveM(const double* phi, int nt, int species, const double* rhss, int nrhs, double* out) {
+void Plan::solveM(const double* phi, int nt, int species, const double* rhss, int nrhs, double* out, bool reuse) {
     const int n = nx_;
     const long nn = (long)n * n;
     hipStream_t s = stream_;
-    OpCache op;
-    op.nt = nt;
-    op.phi_stride = 0;
-    Builder b(op);
-    double* ph = b.d64((size_t)2 * nt * n);
-    double* part = b.d64((size_t)nt * 2 * nn);
-    double* Mz = hop_ == DIA ? b.d64((size_t)nt * 2 * nn) : part;  // matrices multiplying the rhs
+    const bool refactor = !(reuse && sf_.nt == nt && sf_.species == species && sf_.part);
+    if (refactor) {
+        sf_ = SolveFactor{};
+        sf_.ph.reset(new DevBuf((size_t)2 * nt * n * sizeof(double)));
+        sf_.part.reset(new DevBuf((size_t)nt * 2 * nn * sizeof(double)));
+        sf_.mz.reset(new DevBuf((size_t)nt * 2 * nn * sizeof(double)));
+        sf_.piv.reset(new DevBuf((size_t)nt * n * sizeof(int)));
+        sf_.luw.reset(new DevBuf((size_t)nt * lu_work_doubles(n) * sizeof(double)));
+    }
+    double* ph = sf_.ph->as<double>();
+    double* part = sf_.part->as<double>();
+    double* Mz = sf_.mz->as<double>();  // matrices multiplying the rhs (EXP: a copy of the partial products)
+    int* piv = sf_.piv->as<int>();
+    double* luw_p = sf_.luw->as<double>();
     // right-hand-side buffers: Bt, Z (nt blocks of 2 n nrhs), Xl and a scratch block
     const size_t rblk = (size_t)2 * n * nrhs;
     double* Bt = solve_workspace((2 * (size_t)nt + 2) * rblk);
     double* Z = Bt + (size_t)nt * rblk;
     double* Xl = Z + (size_t)nt * rblk;
     double* Sc = Xl + rblk;
-    int* piv = b.di((size_t)nt * n);
     auto D = [&](int t) { return CMat<double>{ph + (long)t * n, ph + (long)nt * n + (long)t * n}; };
     auto mP = [&](int t) { return cmat(part, t, nn); };
     auto mM = [&](int t) { return cmat(Mz, t, nn); };
-    std::vector<PhaseItem> phs{{0, 0, sigma_of(species), D(0).re, D(0).im, nullptr, nullptr, n, nt}};
-    b.add(K::PHASE, phs);
-    const double* A = a64_[species]->as<double>();
-    std::vector<ScaleItem<double>> sc{{A, D(0).re, D(0).im, mP(0).re, mP(0).im, n, SCALE_ROW}};
-    if (hop_ == DIA) sc.push_back({ident64_->as<double>(), D(0).re, D(0).im, mM(0).re, mM(0).im, n, SCALE_ROW});
-    b.add(K::SCALE64, sc);
-    for (int t = 1; t < nt; ++t) {
-        std::vector<GemmItem<double>> gi{gitem<double>(mP(t - 1).re, mP(t - 1).im, A, nullptr, mP(t), D(t).re, D(t).im,
-                                                       nullptr, nullptr, n)};
-        b.add(K::GEMM64_CR_PRE, gi, n, n);
-    }
-    b.finish();
-    run(op, phi);
-    if (hop_ == DIA) {
-        // W_t = part_{t-1} D_t (the product without the final K)
-        std::vector<CScaleItem<double>> cs;
-        for (int t = 1; t < nt; ++t)
-            cs.push_back({mP(t - 1).re, mP(t - 1).im, D(t).re, D(t).im, mM(t).re, mM(t).im, n, SCALE_COL});
-        if (!cs.empty()) {
-            DevBuf dcs(cs.size() * sizeof(CScaleItem<double>));
-            ISLE_HIP(hipMemcpy(dcs.p, cs.data(), cs.size() * sizeof(cs[0]), hipMemcpyHostToDevice));
-            hipLaunchKernelGGL(scale_complex_kernel<double>, dim3(ew_blocks(nn), cs.size()), dim3(EW_THREADS), 0, s,
-                               dcs.as<CScaleItem<double>>());
-            ISLE_HIP(hipGetLastError());
-            ISLE_HIP(hipStreamSynchronize(s));
+    if (refactor) {
+        OpCache op;
+        op.nt = nt;
+        op.phi_stride = 0;
+        Builder b(op);
+        std::vector<PhaseItem> phs{{0, 0, sigma_of(species), D(0).re, D(0).im, nullptr, nullptr, n, nt}};
+        b.add(K::PHASE, phs);
+        const double* A = a64_[species]->as<double>();
+        std::vector<ScaleItem<double>> sc{{A, D(0).re, D(0).im, mP(0).re, mP(0).im, n, SCALE_ROW}};
+        if (hop_ == DIA) sc.push_back({ident64_->as<double>(), D(0).re, D(0).im, mM(0).re, mM(0).im, n, SCALE_ROW});
+        b.add(K::SCALE64, sc);
+        for (int t = 1; t < nt; ++t) {
+            std::vector<GemmItem<double>> gi{gitem<double>(mP(t - 1).re, mP(t - 1).im, A, nullptr, mP(t), D(t).re,
+                                                           D(t).im, nullptr, nullptr, n)};
+            b.add(K::GEMM64_CR_PRE, gi, n, n);
         }
+        b.finish();
+        run(op, phi);
+        if (hop_ == DIA) {
+            // W_t = part_{t-1} D_t (the product without the final K)
+            std::vector<CScaleItem<double>> cs;
+            for (int t = 1; t < nt; ++t)
+                cs.push_back({mP(t - 1).re, mP(t - 1).im, D(t).re, D(t).im, mM(t).re, mM(t).im, n, SCALE_COL});
+            if (!cs.empty()) {
+                DevBuf dcs(cs.size() * sizeof(CScaleItem<double>));
+                ISLE_HIP(hipMemcpy(dcs.p, cs.data(), cs.size() * sizeof(cs[0]), hipMemcpyHostToDevice));
+                hipLaunchKernelGGL(scale_complex_kernel<double>, dim3(ew_blocks(nn), cs.size()), dim3(EW_THREADS), 0,
+                                   s, dcs.as<CScaleItem<double>>());
+                ISLE_HIP(hipGetLastError());
+            }
+        } else {
+            ISLE_HIP(hipMemcpyAsync(Mz, part, (size_t)nt * 2 * nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
+        // LU of every partial product (+1 on the last), in place; Mz keeps the products
+        std::vector<LuItem> lu;
+        for (int t = 0; t < nt; ++t)
+            lu.push_back({mP(t).re, mP(t).im, piv + (long)t * n, nullptr, luw_p + t * lu_work_doubles(n), n, n});
+        DevBuf dlu(lu.size() * sizeof(LuItem));
+        ISLE_HIP(hipMemcpy(dlu.p, lu.data(), lu.size() * sizeof(LuItem), hipMemcpyHostToDevice));
+        if (nt > 1) launch_lu(dlu.as<LuItem>(), nt - 1, n, 0, s);
+        launch_lu(dlu.as<LuItem>() + (nt - 1), 1, n, 1, s);
+        ISLE_HIP(hipStreamSynchronize(s));
+        sf_.nt = nt;
+        sf_.species = species;
     }
     const long blk = 2L * n * nrhs;
     hipLaunchKernelGGL(rhs_transpose_kernel, dim3(ew_blocks((long)nt * n * nrhs)), dim3(EW_THREADS), 0, s, rhss, Bt,
@@ -1506,21 +1530,12 @@ void Plan::solveM(const double* phi, int nt, int species, const double* rhss, in
         ISLE_HIP(hipGetLastError());
         ISLE_HIP(hipStreamSynchronize(s));
     }
-    // LU of every partial product (+1 on the last)
-    std::vector<LuItem> lu;
-    DevBuf luw(nt * lu_work_doubles(n) * sizeof(double));
-    for (int t = 0; t < nt; ++t)
-        lu.push_back({mP(t).re, mP(t).im, piv + (long)t * n, nullptr, luw.as<double>() + t * lu_work_doubles(n), n, n});
-    DevBuf dlu(lu.size() * sizeof(LuItem));
-    ISLE_HIP(hipMemcpy(dlu.p, lu.data(), lu.size() * sizeof(LuItem), hipMemcpyHostToDevice));
-    if (nt > 1) launch_lu(dlu.as<LuItem>(), nt - 1, n, 0, s);
-    launch_lu(dlu.as<LuItem>() + (nt - 1), 1, n, 1, s);
     auto sitem = [&](int t, const double* bre, long bsr, long bsc, double* xre, double* xim, long xsr, long xsc) {
         SolveItem si{};
         si.lu_re = mP(t).re;
         si.lu_im = mP(t).im;
         si.piv = piv + (long)t * n;
-        si.dinv = luw.as<double>() + t * lu_work_doubles(n);
+        si.dinv = luw_p + t * lu_work_doubles(n);
         si.b_re = bre;
         si.b_im = bre + (long)n * nrhs;
         si.bsr = bsr;
@@ -1539,7 +1554,7 @@ void Plan::solveM(const double* phi, int nt, int species, const double* rhss, in
         // each LU's pivot order, then unit-lower and upper solves with the stored inverses
         // of the 32 x 32 diagonal blocks)
         auto lu_mat = [&](int t, double* rre, double* rim, double* xre, double* xim) {
-            return TrsmMat{mP(t).re, mP(t).im, n, luw.as<double>() + t * lu_work_doubles(n), rre, rim, nrhs, xre, xim,
+            return TrsmMat{mP(t).re, mP(t).im, n, luw_p + t * lu_work_doubles(n), rre, rim, nrhs, xre, xim,
                            nrhs};
         };
         auto rows = [&](double* p) { return std::make_pair(p, p + (long)n * nrhs); };
@@ -1745,6 +1760,15 @@ int isle_solveM(isle_hfm* h, const double* phi, int nt, int species, const doubl
         check_species(species);
         if (nrhs < 1) throw std::invalid_argument("need at least one right hand side");
         P(h)->solveM(phi, nt, species, rhss, nrhs, out);
+    });
+}
+
+int isle_solveM_ex(isle_hfm* h, const double* phi, int nt, int species, const double* rhss, int nrhs, double* out,
+                   int reuse) {
+    return guarded([&] {
+        check_species(species);
+        if (nrhs < 1) throw std::invalid_argument("need at least one right hand side");
+        P(h)->solveM(phi, nt, species, rhss, nrhs, out, reuse != 0);
     });
 }
 

@@ -113,7 +113,9 @@ public:
     // combination taken from the structured factorisation of the block-cyclic matrix.
     void force_parts(const double* phi, int nt, int batch, int mode, int group, int nspecies, int spin,
                      double* out_dev);
-    void solveM(const double* phi, int nt, int species, const double* rhss, int nrhs, double* out);
+    // reuse: phi, nt and species equal those of the previous call (its factorization is kept)
+    void solveM(const double* phi, int nt, int species, const double* rhss, int nrhs, double* out,
+                bool reuse = false);
     void F(int tp, const double* phi, int nt, int species, int inv, double* out);
     void slice_matrix(int species, int inv, double* out_host);
     void sync() { ISLE_HIP(hipStreamSynchronize(stream_)); }
@@ -142,6 +144,13 @@ private:
     // solveM's right-hand-side buffers (GBs for many sources), kept between calls and grown
     // when a call needs more: a hipMalloc of that size costs far more than the solve
     std::unique_ptr<DevBuf> solve_ws_;
+    // solveM's factorization of the last field: phases, the matrices that multiply the
+    // right-hand sides, the partial products' LUs (in place), pivots and LU work buffers
+    struct SolveFactor {
+        int nt = -1, species = -1;
+        std::unique_ptr<DevBuf> ph, mz, part, piv, luw;
+    };
+    SolveFactor sf_;
     size_t solve_ws_doubles_ = 0;
     double* solve_workspace(size_t doubles);
 

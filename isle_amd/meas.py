@@ -48,11 +48,12 @@ class _Measurement:
         np.savez(path, **{name: np.asarray(vals) for name, vals in self.values.items()})
 
 
-def solve_refined(hfm, phi, species, rhss, refine=8, tol=1e-14):
+def solve_refined(hfm, phi, species, rhss, refine=8, tol=1e-14, reuse_factor=False):
     """solveM followed by iterative refinement: at most ``refine`` steps, ending once the
     largest entry of the residual b - M x is below ``tol`` times the largest entry of b or
-    when a step no longer lowers it (the rounding floor)."""
-    x = solveM(hfm, phi, species, rhss)
+    when a step no longer lowers it (the rounding floor).  ``reuse_factor``: the previous
+    solveM on ``hfm`` had this field and species (its factorization is reused)."""
+    x = solveM(hfm, phi, species, rhss, reuse_factor=reuse_factor)
     if refine <= 0:
         return x
     goal = tol * rhss.abs().max().item()
@@ -61,7 +62,7 @@ def solve_refined(hfm, phi, species, rhss, refine=8, tol=1e-14):
     for _ in range(refine):
         if res <= goal:
             break
-        x_new = x + solveM(hfm, phi, species, r)
+        x_new = x + solveM(hfm, phi, species, r, reuse_factor=True)  # same field: same factorization
         r_new = rhss - applyM(hfm, phi, species, x_new)
         res_new = r_new.abs().max().item()
         if not res_new < res:
@@ -111,7 +112,7 @@ class AllToAll:
             c1 = min(n, c0 + step)
             sources = torch.zeros((c1 - c0, n), dtype=torch.complex128, device=phi.device)
             sources[torch.arange(c1 - c0), torch.arange(c0, c1)] = 1
-            res[c0:c1] = solve_refined(self.hfm, phi, self.species, sources, self.refine)
+            res[c0:c1] = solve_refined(self.hfm, phi, self.species, sources, self.refine, reuse_factor=c0 > 0)
             del sources
         # res[i, j] = (M^-1)[j, i]: transpose so the source is the right index, then split
         # both spacetime indices (time-major) into (site, time)

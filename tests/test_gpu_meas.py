@@ -103,6 +103,26 @@ def test_solveM_paths_nx32(hop, nrhs):
     assert np.abs(_as_matrix(prop, nx, nt).cpu().numpy() - want).max() < 1e-10 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("hop", ["exp", "dia"])
+def test_solveM_reuses_factorization(hop):
+    """solveM(reuse_factor=True) after a solveM with the same field and species gives the
+    fresh solve bit for bit; asked to reuse across a change of species it refactorizes."""
+    hfm, rh, phi, nx, nt = _setup(hop, l=4)
+    rs = np.random.RandomState(3)
+    b1, b2 = (torch.as_tensor(rs.normal(size=(100, nx * nt)) + 1j * rs.normal(size=(100, nx * nt)), device="cuda")
+              for _ in range(2))
+    isle.solveM(hfm, phi, 0, b1)
+    reused = isle.solveM(hfm, phi, 0, b2, reuse_factor=True)
+    fresh = isle.solveM(hfm, phi, 0, b2)
+    assert torch.equal(reused, fresh)
+    other = isle.solveM(hfm, phi, 1, b1, reuse_factor=True)  # species changed: refactorized
+    assert torch.equal(other, isle.solveM(hfm, phi, 1, b1))
+    exact = np.linalg.solve(rh.M(phi, 1), b1.cpu().numpy().T).T
+    alg = ref.solveM(rh, phi, 1, b1.cpu().numpy())
+    scale = np.abs(exact).max()
+    assert np.abs(other.cpu().numpy() - exact).max() <= 10 * np.abs(alg - exact).max() + 1e-12 * scale
+
+
 def test_all_to_all_chunked():
     """Sources solved in chunks of 100 (ragged last chunk: 288 = 2 x 100 + 88) give the
     propagator of one solve."""
