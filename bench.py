@@ -126,7 +126,8 @@ def propagator(args):
     against N = Nx Nt sources, Nt slice GEMMs of 8 Nx^2 N flops and Nt pairs of triangular
     solves of 8 Nx^2 N flops = 16 N^2 Nx real flops (refinement adds about twice that,
     not counted).  Accuracy: the largest entry of M G - 1 over the first 1024 sources
-    (applyM).  CPU baseline: the oracle's solveM against 288 of the sources."""
+    (applyM).  CPU baseline: the oracle's solveM against 288 and 1152 of the sources, its
+    fixed cost (partial products, LUs) and per-source cost extrapolated to all N."""
     import torch
     import isle_amd as isle
     from isle_amd import meas
@@ -322,8 +323,9 @@ def cpu_baseline(kappa_tilde, ut, nx):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on exactly one GPU (the chains of the batch share it); "
-              "multi-GPU runs are not part of this build", file=sys.stderr)
+        print("bench.py runs on exactly one GPU (the chains of the batch share it); independent chains "
+              "would shard across GPUs with no communication, which this benchmark does not launch",
+              file=sys.stderr)
         sys.exit(2)
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     if args.force_only:
