@@ -406,7 +406,7 @@ def main():
     avg_ms = ms_op / launches
     achieved = flops_op / launches / (avg_ms * 1e-3) / 1e12
     prec = "f32" if "32" in KIND_NAMES[dom] else "f64"
-    traffic = _pmc_traffic(KIND_NAMES[dom], chains)
+    traffic = _pmc_traffic(KIND_NAMES[dom], B)
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK[prec], "unit": "TFLOP/s",
                 "frac": achieved / PEAK[prec], "traffic": traffic,
                 "traffic_source": PMC_SUMMARY if traffic is not None else None, "kernel": KIND_NAMES[dom],

@@ -21,18 +21,19 @@ struct Builder {
         size_t off;
         int count, mm, mn, flag, aux;
         double flops;
+        int tile;
     };
     std::vector<Pending> pend;
 
     explicit Builder(OpCache& o) : op(o) {}
 
     template <typename It>
-    void add(K kind, const std::vector<It>& items, int mm = 0, int mn = 0, int flag = 0, int aux = 0) {
+    void add(K kind, const std::vector<It>& items, int mm = 0, int mn = 0, int flag = 0, int aux = 0, int tile = 0) {
         if (items.empty()) return;
         size_t off = (blob.size() + 63) & ~size_t(63);
         blob.resize(off + items.size() * sizeof(It));
         std::memcpy(blob.data() + off, items.data(), items.size() * sizeof(It));
-        pend.push_back({kind, off, (int)items.size(), mm, mn, flag, aux, flops_of(kind, items)});
+        pend.push_back({kind, off, (int)items.size(), mm, mn, flag, aux, flops_of(kind, items), tile});
     }
     template <typename It>
     static double flops_of(K, const std::vector<It>&) { return 0.0; }
@@ -59,7 +60,7 @@ struct Builder {
         op.bufs.emplace_back(new DevBuf(blob.size() ? blob.size() : 64));
         char* dev = op.bufs.back()->as<char>();
         if (!blob.empty()) ISLE_HIP(hipMemcpy(dev, blob.data(), blob.size(), hipMemcpyHostToDevice));
-        for (auto& p : pend) op.steps.push_back({p.kind, dev + p.off, p.count, p.mm, p.mn, p.flag, p.flops, 0, p.aux});
+        for (auto& p : pend) op.steps.push_back({p.kind, dev + p.off, p.count, p.mm, p.mn, p.flag, p.flops, p.tile, p.aux});
     }
 };
 
@@ -121,11 +122,17 @@ struct TrsmMat {
     double* x_im;
     long ldx;
 };
+// Blocked triangular solve X = L^-1 R (mode 0: unit lower, panels first to last) or
+// X = U^-1 R (mode 1: upper, last to first) of `per` matrices at once with w right-hand-side
+// columns, from the LU's stored inverses D_p of its 32 x 32 diagonal blocks.  Panels go in
+// pairs (p, q): x_p = D_p r_p; r_q -= F_qp x_p; x_q = D_q r_q; then the rows beyond the pair
+// take one update with k = 64, half the right-hand-side traffic of two k = 32 updates.
+// Every product is a step of its own (flag 1: C -= A B); R is overwritten.
 void add_trsm(Builder& b, int mode, int n, int w, const std::vector<TrsmMat>& mats) {
     if (n % LU_NB != 0) throw std::invalid_argument("blocked solve needs n % 32 == 0");
     const int np = n / LU_NB, per = (int)mats.size();
-    std::vector<GemmItem<double>> gi((size_t)2 * np * per);
-    for (int p = 0; p < np; ++p)
+    auto diag = [&](int p) {  // x_p = D_p r_p
+        std::vector<GemmItem<double>> gi(per);
         for (int i = 0; i < per; ++i) {
             const TrsmMat& t = mats[i];
             GemmItem<double> d{};
@@ -142,24 +149,55 @@ void add_trsm(Builder& b, int mode, int n, int w, const std::vector<TrsmMat>& ma
             d.lda = LU_NB;
             d.ldb = (int)t.ldr;
             d.ldc = (int)t.ldx;
-            gi[(size_t)p * per + i] = d;
+            gi[i] = d;
+        }
+        b.add(K::TRSM, gi, LU_NB, w, 0, 0, 1);
+    };
+    auto update = [&](int r0, int rows, int c0, int k) {  // r[r0, rows) -= F[r0.., c0..c0+k) x[c0..c0+k)
+        if (rows <= 0) return;
+        std::vector<GemmItem<double>> gi(per);
+        for (int i = 0; i < per; ++i) {
+            const TrsmMat& t = mats[i];
             GemmItem<double> u{};
-            const long r0 = mode == 0 ? (long)(p + 1) * LU_NB : 0;  // first row still to solve
-            u.a_re = t.f_re + r0 * t.ldf + (long)p * LU_NB;
-            u.a_im = t.f_im + r0 * t.ldf + (long)p * LU_NB;
-            u.b_re = d.c_re;
-            u.b_im = d.c_im;
-            u.c_re = t.r_re + r0 * t.ldr;
-            u.c_im = t.r_im + r0 * t.ldr;
-            u.m = mode == 0 ? n - (p + 1) * LU_NB : p * LU_NB;
+            u.a_re = t.f_re + (long)r0 * t.ldf + c0;
+            u.a_im = t.f_im + (long)r0 * t.ldf + c0;
+            u.b_re = t.x_re + (long)c0 * t.ldx;
+            u.b_im = t.x_im + (long)c0 * t.ldx;
+            u.c_re = t.r_re + (long)r0 * t.ldr;
+            u.c_im = t.r_im + (long)r0 * t.ldr;
+            u.m = rows;
             u.n = w;
-            u.k = LU_NB;
+            u.k = k;
             u.lda = (int)t.ldf;
             u.ldb = (int)t.ldx;
             u.ldc = (int)t.ldr;
-            gi[(size_t)(np + p) * per + i] = u;
+            gi[i] = u;
         }
-    b.add(K::TRSM, gi, n, w, mode, per);
+        b.add(K::TRSM, gi, rows, w, 1, 0, rows <= LU_NB ? 1 : 0);
+    };
+    if (mode == 0) {
+        for (int p = 0; p < np; p += 2) {
+            diag(p);
+            if (p + 1 < np) {
+                update((p + 1) * LU_NB, LU_NB, p * LU_NB, LU_NB);
+                diag(p + 1);
+                update((p + 2) * LU_NB, n - (p + 2) * LU_NB, p * LU_NB, 2 * LU_NB);
+            } else {
+                update((p + 1) * LU_NB, n - (p + 1) * LU_NB, p * LU_NB, LU_NB);
+            }
+        }
+    } else {
+        for (int p = np - 1; p >= 0; p -= 2) {
+            diag(p);
+            if (p >= 1) {
+                update((p - 1) * LU_NB, LU_NB, p * LU_NB, LU_NB);
+                diag(p - 1);
+                update(0, (p - 1) * LU_NB, (p - 1) * LU_NB, 2 * LU_NB);
+            } else {
+                update(0, p * LU_NB, p * LU_NB, LU_NB);
+            }
+        }
+    }
 }
 }  // namespace
 
@@ -283,7 +321,8 @@ void Plan::run(OpCache& op, const double* phi) {
 
 static bool is_gemm(K k) {
     return k == K::GEMM64_RC_PRE || k == K::GEMM64_RC_ROW || k == K::GEMM64_CR_PRE || k == K::GEMM64_CC ||
-           k == K::GEMM32_RC_ROW || k == K::GEMM32_RC_PRE || k == K::GEMM64_CC_SUB || k == K::GEMM32_RC_PRE_DIAG;
+           k == K::GEMM32_RC_ROW || k == K::GEMM32_RC_PRE || k == K::GEMM64_CC_SUB || k == K::GEMM32_RC_PRE_DIAG ||
+           k == K::TRSM;
 }
 
 void Plan::tune(OpCache& op) {
@@ -459,19 +498,16 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
             hipLaunchKernelGGL(row_gather_kernel, dim3(std::min<int>(cdiv(st.aux, 256), 64), st.max_m, st.count), dim3(256),
                                0, s, (const RowGatherItem*)st.items);
             break;
-        case K::TRSM: {
-            // items: [diagonal-block products | trailing updates][panel][matrix]
-            const auto* gi = (const GemmItem<double>*)st.items;
-            const int per = st.aux, np = st.max_m / LU_NB, w = st.max_n;
-            for (int q = 0; q < np; ++q) {
-                const int p = st.flag == 0 ? q : np - 1 - q;
-                launch_gemm<double, true, true, false, SCALE_NONE>(gi + (size_t)p * per, per, LU_NB, w, s, 1);
-                const int rest = st.flag == 0 ? st.max_m - (p + 1) * LU_NB : p * LU_NB;
-                if (rest > 0)
-                    launch_gemm<double, true, true, false, SUBTRACT>(gi + (size_t)(np + p) * per, per, rest, w, s, 1);
+        case K::TRSM:  // one product of a blocked triangular solve (flag 1: C -= A B)
+            if (st.flag) {
+                if (!launch_gemm<double, true, true, false, SUBTRACT>((const GemmItem<double>*)st.items, st.count,
+                                                                      st.max_m, st.max_n, s, st.tile))
+                    return false;
+            } else if (!launch_gemm<double, true, true, false, SCALE_NONE>((const GemmItem<double>*)st.items, st.count,
+                                                                           st.max_m, st.max_n, s, st.tile)) {
+                return false;
             }
             break;
-        }
         case K::COPY64:
             hipLaunchKernelGGL((copy_items_kernel<double, double>), dim3(ew_blocks(st.max_m), st.count),
                                dim3(EW_THREADS), 0, s, (const CopyItem<double, double>*)st.items);

@@ -60,7 +60,7 @@ enum class K : int {
     SIGMA,              // row permutation of stacked right-hand sides
     GATHER,             // permuted split of stacked right-hand sides
     TBLOCK,             // signed transposed block extraction
-    TRSM,               // blocked triangular solve on GEMMs (flag 0: unit lower, 1: upper; aux: items)
+    TRSM,               // one product of a blocked triangular solve (flag 1: C -= A B)
     GEMM32_RC_PRE_DIAG, // GEMM32_RC_PRE with the force diagonal's partial row sums fused (f32)
     DREDUCE,            // sum of those partial row sums
     RGATHER,            // rows of right-hand sides in the pivot order of an LU (max_m = rows, aux = columns)
