@@ -1538,10 +1538,13 @@ void Plan::solveM(const double* phi, int nt, int species, const double* rhss, in
         sf_.species = species;
     }
     const long blk = 2L * n * nrhs;
-    hipLaunchKernelGGL(rhs_transpose_kernel, dim3(ew_blocks((long)nt * n * nrhs)), dim3(EW_THREADS), 0, s, rhss, Bt,
-                       n, nt, nrhs);
-    ISLE_HIP(hipGetLastError());
+    // The slice products of the right-hand sides and the blocked triangular solves are launch
+    // steps with tuned tiles (once per shape: the plan caches the choice).  Tuning runs them
+    // on the workspace before it holds this call's data; the factorization is only read.
+    OpCache og;
+    og.nt = nt;
     {
+        Builder bg(og);
         std::vector<GemmItem<double>> gi;
         for (int t = 0; t < nt; ++t) {
             GemmItem<double> g{};
@@ -1559,13 +1562,70 @@ void Plan::solveM(const double* phi, int nt, int species, const double* rhss, in
             g.ldc = nrhs;
             gi.push_back(g);
         }
-        DevBuf dg(gi.size() * sizeof(gi[0]));
-        ISLE_HIP(hipMemcpy(dg.p, gi.data(), gi.size() * sizeof(gi[0]), hipMemcpyHostToDevice));
-        launch_gemm<double, true, true, false, SCALE_NONE>(dg.as<GemmItem<double>>(), nt, n, nrhs, s);
-        hipLaunchKernelGGL(prefix_sum_t_kernel, dim3(ew_blocks(blk)), dim3(EW_THREADS), 0, s, Z, blk, nt);
+        bg.add(K::GEMM64_CC, gi, n, nrhs);
+        bg.finish();
+        tune(og);
+    }
+    // many right-hand sides: the triangular solves as blocked GEMMs (rows gathered into
+    // each LU's pivot order, then unit-lower and upper solves with the stored inverses of
+    // the 32 x 32 diagonal blocks): o1 solves the last slice, o2 every slice
+    const bool blocked = n % LU_NB == 0 && nrhs >= 64;
+    OpCache o1, o2;
+    o1.nt = o2.nt = nt;
+    if (blocked) {
+        auto lu_mat = [&](int t, double* rre, double* rim, double* xre, double* xim) {
+            return TrsmMat{mP(t).re, mP(t).im, n, luw_p + t * lu_work_doubles(n), rre, rim, nrhs, xre, xim, nrhs};
+        };
+        auto rows = [&](double* p) { return std::make_pair(p, p + (long)n * nrhs); };
+        Builder b1(o1);
+        int* sig = b1.di((size_t)nt * n);  // o1 outlives o2, which reads it
+        std::vector<SigmaItem> sv;
+        for (int t = 0; t < nt; ++t) sv.push_back({piv + (long)t * n, sig + (long)t * n, n, n});
+        b1.add(K::SIGMA, sv);
+        auto z = rows(Z + (nt - 1) * blk), g0 = rows(Bt), g1 = rows(Sc), xl = rows(Xl);
+        std::vector<RowGatherItem> rg{{sig + (long)(nt - 1) * n, z.first, z.second, g0.first, g0.second, nrhs, nrhs, n,
+                                       nrhs}};
+        b1.add(K::RGATHER, rg, n, 0, 0, nrhs);
+        add_trsm(b1, 0, n, nrhs, {lu_mat(nt - 1, g0.first, g0.second, g1.first, g1.second)});
+        add_trsm(b1, 1, n, nrhs, {lu_mat(nt - 1, g1.first, g1.second, xl.first, xl.second)});
+        b1.finish();
+        Builder b2(o2);
+        std::vector<RowGatherItem> ra;
+        std::vector<TrsmMat> tl, tu;
+        for (int t = 0; t < nt; ++t) {
+            auto zt = rows(Z + t * blk), bt = rows(Bt + t * blk);
+            ra.push_back({sig + (long)t * n, zt.first, zt.second, bt.first, bt.second, nrhs, nrhs, n, nrhs});
+            tl.push_back(lu_mat(t, bt.first, bt.second, zt.first, zt.second));
+            tu.push_back(lu_mat(t, zt.first, zt.second, bt.first, bt.second));
+        }
+        b2.add(K::RGATHER, ra, n, 0, 0, nrhs);
+        add_trsm(b2, 0, n, nrhs, tl);
+        add_trsm(b2, 1, n, nrhs, tu);
+        b2.finish();
+        tune(o1);
+        tune(o2);
+    }
+    hipLaunchKernelGGL(rhs_transpose_kernel, dim3(ew_blocks((long)nt * n * nrhs)), dim3(EW_THREADS), 0, s, rhss, Bt,
+                       n, nt, nrhs);
+    ISLE_HIP(hipGetLastError());
+    run(og, phi);
+    hipLaunchKernelGGL(prefix_sum_t_kernel, dim3(ew_blocks(blk)), dim3(EW_THREADS), 0, s, Z, blk, nt);
+    ISLE_HIP(hipGetLastError());
+    if (blocked) {
+        run(o1, phi);  // x_{nt-1} = (1 + A^-1)^-1 z_{nt-1}
+        if (nt > 1) {
+            hipLaunchKernelGGL(sub_last_kernel, dim3(ew_blocks(blk * (nt - 1))), dim3(EW_THREADS), 0, s, Z, Xl, blk,
+                               nt - 1);
+            ISLE_HIP(hipGetLastError());
+        }
+        run(o2, phi);  // x_t = A_t^-1 (z_t - x_{nt-1}) (z_{nt-1} unchanged: solves x_{nt-1} again)
+        hipLaunchKernelGGL(rhs_untranspose_kernel, dim3(ew_blocks((long)nt * n * nrhs)), dim3(EW_THREADS), 0, s, Bt,
+                           out, n, nt, nrhs);
         ISLE_HIP(hipGetLastError());
         ISLE_HIP(hipStreamSynchronize(s));
+        return;
     }
+    ISLE_HIP(hipStreamSynchronize(s));
     auto sitem = [&](int t, const double* bre, long bsr, long bsc, double* xre, double* xim, long xsr, long xsc) {
         SolveItem si{};
         si.lu_re = mP(t).re;
@@ -1585,61 +1645,6 @@ void Plan::solveM(const double* phi, int nt, int species, const double* rhss, in
         si.nrhs = nrhs;
         return si;
     };
-    if (n % LU_NB == 0 && nrhs >= 64) {
-        // many right-hand sides: the triangular solves as blocked GEMMs (rows gathered into
-        // each LU's pivot order, then unit-lower and upper solves with the stored inverses
-        // of the 32 x 32 diagonal blocks)
-        auto lu_mat = [&](int t, double* rre, double* rim, double* xre, double* xim) {
-            return TrsmMat{mP(t).re, mP(t).im, n, luw_p + t * lu_work_doubles(n), rre, rim, nrhs, xre, xim,
-                           nrhs};
-        };
-        auto rows = [&](double* p) { return std::make_pair(p, p + (long)n * nrhs); };
-        int* sig = nullptr;
-        {  // the last slice: x_{nt-1} = (1 + A^-1)^-1 z_{nt-1}
-            OpCache o;
-            o.nt = nt;
-            Builder bl(o);
-            sig = bl.di((size_t)nt * n);
-            std::vector<SigmaItem> sv;
-            for (int t = 0; t < nt; ++t) sv.push_back({piv + (long)t * n, sig + (long)t * n, n, n});
-            bl.add(K::SIGMA, sv);
-            auto z = rows(Z + (nt - 1) * blk), g0 = rows(Bt), g1 = rows(Sc), xl = rows(Xl);
-            std::vector<RowGatherItem> rg{{sig + (long)(nt - 1) * n, z.first, z.second, g0.first, g0.second, nrhs,
-                                           nrhs, n, nrhs}};
-            bl.add(K::RGATHER, rg, n, 0, 0, nrhs);
-            add_trsm(bl, 0, n, nrhs, {lu_mat(nt - 1, g0.first, g0.second, g1.first, g1.second)});
-            add_trsm(bl, 1, n, nrhs, {lu_mat(nt - 1, g1.first, g1.second, xl.first, xl.second)});
-            bl.finish();
-            run(o, phi);
-            if (nt > 1) {
-                hipLaunchKernelGGL(sub_last_kernel, dim3(ew_blocks(blk * (nt - 1))), dim3(EW_THREADS), 0, s, Z, Xl,
-                                   blk, nt - 1);
-                ISLE_HIP(hipGetLastError());
-            }
-            // every slice: x_t = A_t^-1 (z_t - x_{nt-1}) (z_{nt-1} unchanged: solves x_{nt-1} again)
-            OpCache o2;
-            o2.nt = nt;
-            Builder b2(o2);
-            std::vector<RowGatherItem> ra;
-            std::vector<TrsmMat> tl, tu;
-            for (int t = 0; t < nt; ++t) {
-                auto zt = rows(Z + t * blk), bt = rows(Bt + t * blk);
-                ra.push_back({sig + (long)t * n, zt.first, zt.second, bt.first, bt.second, nrhs, nrhs, n, nrhs});
-                tl.push_back(lu_mat(t, bt.first, bt.second, zt.first, zt.second));
-                tu.push_back(lu_mat(t, zt.first, zt.second, bt.first, bt.second));
-            }
-            b2.add(K::RGATHER, ra, n, 0, 0, nrhs);
-            add_trsm(b2, 0, n, nrhs, tl);
-            add_trsm(b2, 1, n, nrhs, tu);
-            b2.finish();
-            run(o2, phi);
-            hipLaunchKernelGGL(rhs_untranspose_kernel, dim3(ew_blocks((long)nt * n * nrhs)), dim3(EW_THREADS), 0, s, Bt,
-                               out, n, nt, nrhs);
-            ISLE_HIP(hipGetLastError());
-            ISLE_HIP(hipStreamSynchronize(s));
-        }
-        return;
-    }
     std::vector<SolveItem> s1{sitem(nt - 1, Z + (nt - 1) * blk, nrhs, 1, Xl, Xl + (long)n * nrhs, nrhs, 1)};
     DevBuf ds1(sizeof(SolveItem));
     ISLE_HIP(hipMemcpy(ds1.p, s1.data(), sizeof(SolveItem), hipMemcpyHostToDevice));
