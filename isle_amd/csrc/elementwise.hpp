@@ -363,35 +363,61 @@ __global__ void transpose_items_kernel(const TCopyItem<S, D>* __restrict__ items
     }
 }
 
-// solveM helpers: rhss [nrhs][nt*n] interleaved -> per-slice split planes [t][2][n][nrhs]
-__global__ void rhs_transpose_kernel(const double* __restrict__ rhss, double* __restrict__ out, int n, int nt,
-                                     int nrhs) {
-    const long total = (long)nt * n * nrhs;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(e % nrhs);
-        const long rest = e / nrhs;
-        const int k = (int)(rest % n);
-        const int t = (int)(rest / n);
-        const long src = 2 * ((long)c * nt * n + (long)t * n + k);
-        double* blk = out + (long)t * 2 * n * nrhs;
-        blk[(long)k * nrhs + c] = rhss[src];
-        blk[(long)n * nrhs + (long)k * nrhs + c] = rhss[src + 1];
+// Right-hand sides between the caller's layout (nrhs interleaved rows of nt*n complex) and
+// blocks [t][re/im][n][nrhs]: 32 x 32 tiles through LDS so that both the row reads and the
+// block writes are contiguous.  Grid (ceil(n/32), ceil(nrhs/32), nt), 256 threads.
+__global__ __launch_bounds__(256) void rhs_transpose_kernel(const double* __restrict__ rhss, double* __restrict__ out,
+                                                            int n, int nt, int nrhs) {
+    __shared__ double tr[32][33], ti[32][33];
+    const int kb = blockIdx.x * 32, cb = blockIdx.y * 32, t = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {  // rows c = cb + r, sites k = kb + tx
+        const int c = cb + r, k = kb + tx;
+        double vr = 0, vi = 0;
+        if (c < nrhs && k < n) {
+            const long src = 2 * ((long)c * nt * n + (long)t * n + k);
+            vr = rhss[src];
+            vi = rhss[src + 1];
+        }
+        tr[r][tx] = vr;
+        ti[r][tx] = vi;
+    }
+    __syncthreads();
+    double* blk = out + (long)t * 2 * n * nrhs;
+    for (int r = ty; r < 32; r += 8) {  // sites k = kb + r, rows c = cb + tx
+        const int k = kb + r, c = cb + tx;
+        if (k < n && c < nrhs) {
+            blk[(long)k * nrhs + c] = tr[tx][r];
+            blk[(long)n * nrhs + (long)k * nrhs + c] = ti[tx][r];
+        }
     }
 }
 
-// inverse of rhs_transpose_kernel: blocks [t][2][n][nrhs] back to nrhs interleaved rows of nt*n
-__global__ void rhs_untranspose_kernel(const double* __restrict__ in, double* __restrict__ rhss, int n, int nt,
-                                       int nrhs) {
-    const long total = (long)nt * n * nrhs;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(e % nrhs);
-        const long rest = e / nrhs;
-        const int k = (int)(rest % n);
-        const int t = (int)(rest / n);
-        const long dst = 2 * ((long)c * nt * n + (long)t * n + k);
-        const double* blk = in + (long)t * 2 * n * nrhs;
-        rhss[dst] = blk[(long)k * nrhs + c];
-        rhss[dst + 1] = blk[(long)n * nrhs + (long)k * nrhs + c];
+// inverse of rhs_transpose_kernel (same grid)
+__global__ __launch_bounds__(256) void rhs_untranspose_kernel(const double* __restrict__ in, double* __restrict__ rhss,
+                                                              int n, int nt, int nrhs) {
+    __shared__ double tr[32][33], ti[32][33];
+    const int kb = blockIdx.x * 32, cb = blockIdx.y * 32, t = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const double* blk = in + (long)t * 2 * n * nrhs;
+    for (int r = ty; r < 32; r += 8) {  // sites k = kb + r, rows c = cb + tx
+        const int k = kb + r, c = cb + tx;
+        double vr = 0, vi = 0;
+        if (k < n && c < nrhs) {
+            vr = blk[(long)k * nrhs + c];
+            vi = blk[(long)n * nrhs + (long)k * nrhs + c];
+        }
+        tr[r][tx] = vr;
+        ti[r][tx] = vi;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {  // rows c = cb + r, sites k = kb + tx
+        const int c = cb + r, k = kb + tx;
+        if (c < nrhs && k < n) {
+            const long dst = 2 * ((long)c * nt * n + (long)t * n + k);
+            rhss[dst] = tr[tx][r];
+            rhss[dst + 1] = ti[tx][r];
+        }
     }
 }
 

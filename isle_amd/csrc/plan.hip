@@ -1605,8 +1605,8 @@ void Plan::solveM(const double* phi, int nt, int species, const double* rhss, in
         tune(o1);
         tune(o2);
     }
-    hipLaunchKernelGGL(rhs_transpose_kernel, dim3(ew_blocks((long)nt * n * nrhs)), dim3(EW_THREADS), 0, s, rhss, Bt,
-                       n, nt, nrhs);
+    const dim3 tgrid(cdiv(n, 32), cdiv(nrhs, 32), nt);
+    hipLaunchKernelGGL(rhs_transpose_kernel, tgrid, dim3(256), 0, s, rhss, Bt, n, nt, nrhs);
     ISLE_HIP(hipGetLastError());
     run(og, phi);
     hipLaunchKernelGGL(prefix_sum_t_kernel, dim3(ew_blocks(blk)), dim3(EW_THREADS), 0, s, Z, blk, nt);
@@ -1619,8 +1619,7 @@ void Plan::solveM(const double* phi, int nt, int species, const double* rhss, in
             ISLE_HIP(hipGetLastError());
         }
         run(o2, phi);  // x_t = A_t^-1 (z_t - x_{nt-1}) (z_{nt-1} unchanged: solves x_{nt-1} again)
-        hipLaunchKernelGGL(rhs_untranspose_kernel, dim3(ew_blocks((long)nt * n * nrhs)), dim3(EW_THREADS), 0, s, Bt,
-                           out, n, nt, nrhs);
+        hipLaunchKernelGGL(rhs_untranspose_kernel, tgrid, dim3(256), 0, s, Bt, out, n, nt, nrhs);
         ISLE_HIP(hipGetLastError());
         ISLE_HIP(hipStreamSynchronize(s));
         return;
