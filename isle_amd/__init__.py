@@ -18,8 +18,8 @@ Drop-in names for the reference's ``isle`` / ``isle.action`` surface on this pat
     together, one HIP graph per trajectory).
 
 Every routine of the fermion-action path runs in the hand-written HIP kernels of
-libisle_hfa.so; applyM (the residual of the measurements' refinement) is one complex
-GEMM of the library kind (torch on the device).  There is no CPU fallback.
+libisle_hfa.so; applyM (the residual of the measurements' refinement) is one real GEMM
+of the library kind (torch on the device).  There is no CPU fallback.
 """
 from . import action, autograd, drivers, ensemble, evolver, meas, runfile, util
 from .core import (CDVector, HubbardFermiMatrixDia, HubbardFermiMatrixExp, Species, Vector, applyM, as_field, invert,

@@ -230,8 +230,9 @@ def applyM(hfm, phi, species, x):
     """M(phi) applied to every row of x (NRHS x NxNt, time-major), in row form: with v a
     row split into time slices, (M v)_t = K v_t - F_t v_{t-1} for t >= 1 and
     (M v)_0 = K v_0 + F_0 v_{nt-1} (hubbardFermiMatrixExp.cpp:150-166, K = 1;
-    hubbardFermiMatrixDia.cpp:140-160, F diagonal).  One complex GEMM of all slices on the
-    device; the slice matrices are the plan's own."""
+    hubbardFermiMatrixDia.cpp:140-160, F diagonal).  One real GEMM of all slices' real and
+    imaginary parts with the (real) slice matrix on the device; the slice matrices are the
+    plan's own."""
     nx = hfm.nx()
     phi = as_field(phi, nx)
     nt = phi.shape[-1] // nx
@@ -242,8 +243,14 @@ def applyM(hfm, phi, species, x):
     ph = torch.exp(sign * torch.roll(phi.reshape(nt, nx), 1, 0))
     y = torch.roll(rows, 1, 1) * ph
     y[:, 0] = -y[:, 0]  # + F_0 v_{nt-1}: the antiperiodic wrap
-    s = torch.as_tensor(hfm._slice(species, False), device=phi.device).to(torch.complex128).T.contiguous()
-    out = rows - torch.matmul(y, s) if hfm.HOPPING == 1 else torch.matmul(rows, s) - y
+    s = torch.as_tensor(hfm._slice(species, False), device=phi.device).to(torch.float64).T.contiguous()
+
+    def times_s(v):  # v @ s for complex v and real s: one real 2-D GEMM over the re/im parts
+        shape = v.shape
+        parts = torch.view_as_real(v).transpose(-1, -2).reshape(-1, nx)  # copied: rows (.., re/im)
+        prod = torch.mm(parts, s).reshape(*shape[:-1], 2, nx)
+        return torch.view_as_complex(prod.transpose(-1, -2).contiguous())
+    out = rows - times_s(y) if hfm.HOPPING == 1 else times_s(rows) - y
     return out.reshape(x.shape)
 
 
