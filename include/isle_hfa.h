@@ -98,6 +98,11 @@ int isle_solveM_ex(isle_hfm* h, const double* phi, int nt, int species, const do
  *   (hubbardFermiAction.cpp:243-272): mode 0: -i(fp - fh); 1: -i(fp - conj fp); 2: fh - fp. */
 int isle_combine_force(const double* fp, const double* fh, double* out, long n, int mode, int accumulate,
                        void* stream);
+/* The same for a batch of fields in one launch (no counterpart in the reference, which
+ * evaluates one field at a time): field b's parts start at fp/fh + b*in_stride and its output
+ * at out + b*out_stride (complex elements).  batch <= 65535. */
+int isle_combine_force_batch(const double* fp, const double* fh, double* out, long n, int batch, long in_stride,
+                             long out_stride, int mode, int accumulate, void* stream);
 
 /* HubbardGaugeAction::force = -phi/utilde (hubbardGaugeAction.cpp:11-13). */
 int isle_gauge_force(const double* phi, double* out, long n, double utilde, int accumulate, void* stream);

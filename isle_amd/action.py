@@ -338,11 +338,12 @@ class HubbardFermiAction(Action):
             out = torch.empty((batch, nx * nt), dtype=torch.complex128, device=phis.device)
             accumulate = False
         mode = 2 if self.basis == HFABasis.SPIN else (1 if self.shortcut else 0)
-        for b in range(batch):
-            fp = parts[b, 0]
-            fh = parts[b, 1] if nsp == 2 else parts[b, 0]
-            _lib.check(lib.isle_combine_force(_lib.ptr(fp), _lib.ptr(fh), _lib.ptr(out[b]), nx * nt, mode,
-                                              int(bool(accumulate)), _lib.stream_ptr()))
+        if out.stride(-1) != 1 or (batch > 1 and out.stride(0) < nx * nt):
+            raise ValueError("out must hold the fields' forces as rows of contiguous elements")
+        fh = parts[:, 1] if nsp == 2 else parts[:, 0]  # one launch for the whole batch
+        _lib.check(lib.isle_combine_force_batch(_lib.ptr(parts[:, 0]), _lib.ptr(fh), _lib.ptr(out), nx * nt, batch,
+                                                nsp * nx * nt, out.stride(0) if batch > 1 else 0, mode,
+                                                int(bool(accumulate)), _lib.stream_ptr()))
         return out
 
     def force(self, phi):

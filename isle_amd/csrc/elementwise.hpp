@@ -208,9 +208,15 @@ __global__ void interleaved_to_split_kernel(const double* __restrict__ in, doubl
 //  mode 0 (particle/hole):          out = -i (fp - fh)
 //  mode 1 (particle/hole shortcut): out = -i (fp - conj(fp))
 //  mode 2 (spin):                   out = fh - fp
-// ``accumulate`` adds to out instead of overwriting.
+// ``accumulate`` adds to out instead of overwriting.  blockIdx.y selects one of a batch of
+// fields: its parts start in_stride and its output out_stride complex elements after the
+// previous field's.
 __global__ void combine_force_kernel(const double* __restrict__ fp, const double* __restrict__ fh,
-                                     double* __restrict__ out, long n, int mode, int accumulate) {
+                                     double* __restrict__ out, long n, long in_stride, long out_stride, int mode,
+                                     int accumulate) {
+    fp += 2 * (long)blockIdx.y * in_stride;
+    fh += 2 * (long)blockIdx.y * in_stride;
+    out += 2 * (long)blockIdx.y * out_stride;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
         const double pr = fp[2 * e], pi = fp[2 * e + 1];
         double hr, hi;

@@ -1828,7 +1828,17 @@ int isle_combine_force(const double* fp, const double* fh, double* out, long n, 
                        void* stream) {
     return guarded([&] {
         hipLaunchKernelGGL(isle::combine_force_kernel, dim3(isle::ew_blocks(n)), dim3(256), 0, (hipStream_t)stream,
-                           fp, fh, out, n, mode, accumulate);
+                           fp, fh, out, n, 0L, 0L, mode, accumulate);
+        ISLE_HIP(hipGetLastError());
+    });
+}
+
+int isle_combine_force_batch(const double* fp, const double* fh, double* out, long n, int batch, long in_stride,
+                             long out_stride, int mode, int accumulate, void* stream) {
+    return guarded([&] {
+        if (batch < 1 || batch > 65535) throw std::invalid_argument("batch must be in 1..65535");
+        hipLaunchKernelGGL(isle::combine_force_kernel, dim3(isle::ew_blocks(n), batch), dim3(256), 0,
+                           (hipStream_t)stream, fp, fh, out, n, in_stride, out_stride, mode, accumulate);
         ISLE_HIP(hipGetLastError());
     });
 }
