@@ -31,7 +31,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="timed trajectories (per chain) of the batched form")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--chains", type=int, default=64, help="independent chains advanced together")
+    ap.add_argument("--chains", type=int, default=128, help="independent chains advanced together")
     ap.add_argument("--group", type=int, default=32, help="time slices per group of the mixed force")
     ap.add_argument("--plain-steps", type=int, default=1, help="timed trajectories of the plain FP64 form")
     ap.add_argument("--no-plain", action="store_true")
@@ -95,15 +95,16 @@ def config4(args):
 
 
 # the dominant kind's kernel as it appears in the rocprofv3 PMC summary of this workload
-PMC_SUMMARY = "profiles/force_mixed_nx288_nt128_chains64_hbm_traffic.md"
-PMC_ROWS = {"gemm64_rc_row": ("gemm_rcd_kernel<double, 1, false, 96, 96, 16, 2, 2, 6>", "3538944")}
+PMC_SUMMARY = "profiles/force_mixed_nx288_nt128_chains128_hbm_traffic.md"
+PMC_CHAINS = 128
+PMC_ROWS = {"gemm64_rc_row": ("gemm_rcd_kernel<double, 1, false, 96, 96, 16, 2, 2, 6>", "7077888")}
 
 
 def _pmc_traffic(kind, chains):
     """HBM bytes per launch of the dominant kernel (calibrated FETCH + WRITE, bytes) from the
-    committed rocprofv3 PMC summary of the 64-chain force, or None for other workloads
+    committed rocprofv3 PMC summary of the 128-chain force, or None for other workloads
     (bench.py cannot run the counters itself: they need their own profiler passes)."""
-    if chains != 64 or kind not in PMC_ROWS:
+    if chains != PMC_CHAINS or kind not in PMC_ROWS:
         return None
     name, grid = PMC_ROWS[kind]
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), PMC_SUMMARY)
