@@ -146,6 +146,13 @@ int isle_logdet(const double* a, int n, double* out_host, void* stream);
  * a, out: device n*n complex128 (a is not modified). */
 int isle_invert(const double* a, int n, double* out, void* stream);
 
+/* A^{-1} and log det A (Im in [-pi, pi]) from one LU; no single reference counterpart.  It
+ * serves the QLU recursion of Q = M_p M_h^T, which inverts every diagonal block d_i
+ * (src/isle/cpp/hubbardFermiMatrixExp.cpp:420-500) and later sums -ilogdet(d_i^{-1})
+ * (:526-541); here log det d_i comes from the LU that produced d_i^{-1}.
+ * a, out: device n*n complex128; logdet_host: 2 doubles (required). */
+int isle_invert_logdet(const double* a, int n, double* out, double* logdet_host, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

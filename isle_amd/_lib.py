@@ -39,6 +39,7 @@ SIGNATURES = {
     "isle_profile_kernels": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _dp,
                                       _c_int]),
     "isle_invert": (_c_int, [_vp, _c_int, _vp, _vp]),
+    "isle_invert_logdet": (_c_int, [_vp, _c_int, _vp, _dp, _vp]),
 }
 
 _ERRORS = {1: ValueError, 2: IndexError, 3: RuntimeError, 4: RuntimeError}

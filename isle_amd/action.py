@@ -22,8 +22,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .core import (HubbardFermiMatrixDia, HubbardFermiMatrixExp, Lattice, Species, as_field, invert, isBipartite,
-                   logdet, to_first_log_branch)
+from .core import (HubbardFermiMatrixDia, HubbardFermiMatrixExp, Lattice, as_field, invert, isBipartite, logdetQ,
+                   to_first_log_branch)
 
 
 class HFAHopping(enum.IntEnum):
@@ -149,10 +149,10 @@ class HubbardFermiAction(Action):
     """Fermion action -log det M(phi) M(-phi) (hubbardFermiAction.hpp:96-142).
 
     DIRECT_SQUARE (hubbardFermiAction.cpp:62-108, 230-272) is the reference's small-size
-    cross-check: eval = -log det Q and the force from the blocks of Q^-1 next to the
-    off-diagonal blocks T+ and T- of Q, with Q = M_p M_h^T (NxNt)^2 dense, formed by one
-    complex GEMM of the dense M matrices; log det and inverse by the HIP LU, so
-    Nx*Nt <= 512.  FP64 only; no per-species parts (logdets, force_parts).
+    cross-check: eval = -logdetQ through the block LU of Q = M_p M_h^T (any size), and the
+    force from the blocks of the dense (NxNt)^2 Q^-1 next to the off-diagonal blocks T+ and
+    T- of Q, inverted by the HIP LU, so the force needs Nx*Nt <= 512.  FP64 only; no
+    per-species parts (logdets, force_parts).
     """
 
     def __init__(self, kappaTilde, muTilde, sigmaKappa, hopping=HFAHopping.DIA,
@@ -184,8 +184,8 @@ class HubbardFermiAction(Action):
         if self.algorithm == HFAAlgorithm.DIRECT_SQUARE:
             self.shortcut = False  # Q holds both species
 
-    # ---- DIRECT_SQUARE (hubbardFermiAction.cpp:62-108): dense Q = M_p M_h^T
-    SQUARE_MAX = 512  # order of the dense LU
+    # ---- DIRECT_SQUARE (hubbardFermiAction.cpp:62-108): Q = M_p M_h^T
+    SQUARE_MAX = 512  # order of the dense LU of the force's Q^-1
 
     def _field(self, phi):
         return -1j * phi if self.basis == HFABasis.SPIN else phi
@@ -193,42 +193,21 @@ class HubbardFermiAction(Action):
     def _q(self, field):
         n = field.numel()
         if n > self.SQUARE_MAX:
-            raise ValueError(f"HFAAlgorithm.DIRECT_SQUARE needs Nx*Nt <= {self.SQUARE_MAX} (dense LU), got {n}")
-        mp = self.hfm.M(field, Species.PARTICLE)
-        mh = self.hfm.M(field, Species.HOLE)
-        return torch.matmul(mp, mh.transpose(0, 1).contiguous())
+            raise ValueError(f"HFAAlgorithm.DIRECT_SQUARE force needs Nx*Nt <= {self.SQUARE_MAX} (dense LU), got {n}")
+        return self.hfm.Q(field)
 
     def _t_pm(self, field, nt):
         """T+(tau + 1) and T-(tau) for tau = 0..nt-1, [nt, nx, nx] each (the blocks of Q at
-        (tau+1, tau) and (tau, tau+1); hubbardFermiMatrixExp.cpp:188-208,
-        hubbardFermiMatrixDia.cpp:183-214)."""
-        nx = self.hfm.nx()
-        dev = field.device
-        tplus, tminus = [], []
-        if self.hopping == HFAHopping.EXP:
-            for tau in range(nt):
-                tp = (tau + 1) % nt
-                sp = 1.0 if tp == 0 else -1.0  # -anti, anti = -1 on the wrap-around
-                tplus.append(sp * self.hfm.F(tp, field, Species.PARTICLE, False))
-                sm = 1.0 if tau == nt - 1 else -1.0
-                tminus.append(sm * self.hfm.F(tp, field, Species.HOLE, False).transpose(0, 1))
-            return torch.stack(tplus), torch.stack(tminus)
-        kappa = torch.as_tensor(self.hfm.kappaTilde(), device=dev).to(torch.complex128)
-        mu, sk = self.hfm.muTilde(), self.hfm.sigmaKappa()
-        eye = torch.eye(nx, dtype=torch.complex128, device=dev)
-        a = sk * kappa - (1 - mu) * eye
-        b = kappa - (1 + mu) * eye
-        f = field.reshape(nt, nx)
-        for tau in range(nt):
-            tp = (tau + 1) % nt
-            ap = -1.0 if tp == 0 else 1.0
-            tplus.append((ap * torch.exp(1j * f[tau]))[:, None] * a)  # phases of slice tp - 1 = tau
-            am = -1.0 if tau == nt - 1 else 1.0
-            tminus.append(b * (am * torch.exp(-1j * f[tau]))[None, :])
+        (tau+1, tau) and (tau, tau+1); hubbardFermiMatrixExp.cpp:188-215,
+        hubbardFermiMatrixDia.cpp:183-220)."""
+        tplus = [self.hfm.Tplus((tau + 1) % nt, field) for tau in range(nt)]
+        tminus = [self.hfm.Tminus(tau, field) for tau in range(nt)]
         return torch.stack(tplus), torch.stack(tminus)
 
     def _eval_square(self, phi):
-        return -to_first_log_branch(logdet(self._q(self._field(phi))))
+        # -logdetQ through the block LU of Q, as the reference evaluates it
+        # (hubbardFermiAction.cpp:204-221, 275-292); no size limit
+        return -logdetQ(self.hfm, self._field(phi))
 
     def _force_square(self, phi):
         field = self._field(phi)

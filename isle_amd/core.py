@@ -159,6 +159,21 @@ class _HFMBase:
             m[tp * nx:(tp + 1) * nx, (tp - 1) * nx:tp * nx] -= self.F(tp, phi, species)
         return m
 
+    def Q(self, phi):
+        """Dense (NxNt)^2 Q = M_p M_h^T from its blocks: P on the diagonal, Tplus(tp) at
+        (tp, tp-1), Tminus(tp) at (tp, tp+1), cyclic (hubbardFermiMatrixExp.cpp:217-240,
+        hubbardFermiMatrixDia.cpp:222-245).  Intended for tests and small sizes."""
+        nx = self.nx()
+        phi = as_field(phi, nx)
+        nt = phi.shape[-1] // nx
+        q = torch.zeros((nt, nx, nt, nx), dtype=torch.complex128, device=phi.device)
+        p = torch.as_tensor(self.P(), device=phi.device).to(torch.complex128)
+        for tp in range(nt):
+            q[tp, :, tp] += p
+            q[tp, :, (tp + nt - 1) % nt] += self.Tplus(tp, phi)
+            q[tp, :, (tp + 1) % nt] += self.Tminus(tp, phi)
+        return q.reshape(nx * nt, nx * nt)
+
 
 class HubbardFermiMatrixExp(_HFMBase):
     """Exponential hopping discretisation (hubbardFermiMatrixExp.hpp:46-366)."""
@@ -178,6 +193,29 @@ class HubbardFermiMatrixExp(_HFMBase):
     def logdetKinv(self, species):
         return 0j
 
+    def P(self):
+        """Diagonal block of Q: 2 I for sigmaKappa = -1 (the exponentials are inverses of each
+        other), else I + exp(kappa - mu) exp(sigmaKappa kappa + mu)
+        (hubbardFermiMatrixExp.cpp:176-186); host float64."""
+        nx = self.nx()
+        if self.sigmaKappa() == -1:
+            return 2 * np.eye(nx)
+        return np.eye(nx) + self.expKappa(Species.PARTICLE, False) @ self.expKappa(Species.HOLE, False)
+
+    def Tplus(self, tp, phi):
+        """Block (tp, tp-1) of Q: -F_p(tp), +F_p(0) on the antiperiodic wrap
+        (hubbardFermiMatrixExp.cpp:188-200); device complex128."""
+        sign = 1.0 if tp == 0 else -1.0
+        return sign * self.F(tp, phi, Species.PARTICLE, False)
+
+    def Tminus(self, tp, phi):
+        """Block (tp, tp+1) of Q: -F_h(tp+1)^T, + on the wrap tp = nt-1
+        (hubbardFermiMatrixExp.cpp:202-215); device complex128."""
+        phi = as_field(phi, self.nx())
+        nt = phi.shape[-1] // self.nx()
+        sign = 1.0 if tp == nt - 1 else -1.0
+        return sign * self.F((tp + 1) % nt, phi, Species.HOLE, False).transpose(0, 1).contiguous()
+
 
 class HubbardFermiMatrixDia(_HFMBase):
     """Diagonal hopping discretisation (hubbardFermiMatrixDia.hpp:46-344)."""
@@ -192,6 +230,37 @@ class HubbardFermiMatrixDia(_HFMBase):
         """K^{-1}, inverted on the device by LU (hubbardFermiMatrixDia.cpp:25-42)."""
         k = torch.as_tensor(self.K(species), device=_lib.device()).to(torch.complex128)
         return invert(k).real.cpu().numpy()
+
+    def P(self):
+        """Diagonal block of Q = K_p K_h^T + I: (2 - mu^2) I - (sigmaKappa (1 + mu) + 1 - mu) kappa
+        + sigmaKappa kappa^2 (hubbardFermiMatrixDia.cpp:170-181); host float64."""
+        k = np.asarray(self.kappaTilde(), dtype=np.float64)
+        mu, sk = self.muTilde(), self.sigmaKappa()
+        return (2 - mu * mu) * np.eye(self.nx()) - (sk * (1 + mu) + 1 - mu) * k + sk * (k @ k)
+
+    def _phases(self, phi, t, sign):
+        phi = as_field(phi, self.nx())
+        return torch.exp(sign * 1j * phi[t * self.nx():(t + 1) * self.nx()])
+
+    def Tplus(self, tp, phi):
+        """Block (tp, tp-1) of Q: diag(e^{i phi_{tp-1}}) (sigmaKappa kappa - (1 - mu) I), negated
+        on the antiperiodic wrap tp = 0 (hubbardFermiMatrixDia.cpp:183-201); device complex128."""
+        phi = as_field(phi, self.nx())
+        nt = phi.shape[-1] // self.nx()
+        k = torch.as_tensor(self.kappaTilde(), device=phi.device).to(torch.complex128)
+        t = self.sigmaKappa() * k - (1 - self.muTilde()) * torch.eye(self.nx(), dtype=k.dtype, device=k.device)
+        sign = -1.0 if tp == 0 else 1.0
+        return (sign * self._phases(phi, (tp + nt - 1) % nt, 1))[:, None] * t
+
+    def Tminus(self, tp, phi):
+        """Block (tp, tp+1) of Q: (kappa - (1 + mu) I) diag(e^{-i phi_tp}), negated on the wrap
+        tp = nt-1 (hubbardFermiMatrixDia.cpp:203-220); device complex128."""
+        phi = as_field(phi, self.nx())
+        nt = phi.shape[-1] // self.nx()
+        k = torch.as_tensor(self.kappaTilde(), device=phi.device).to(torch.complex128)
+        t = k - (1 + self.muTilde()) * torch.eye(self.nx(), dtype=k.dtype, device=k.device)
+        sign = -1.0 if tp == nt - 1 else 1.0
+        return t * (sign * self._phases(phi, tp, -1))[None, :]
 
 
 def logdet(matrix):
@@ -273,3 +342,172 @@ def solveM(hfm, phi, species, rhss, reuse_factor=False):
     _lib.check(_lib.load().isle_solveM_ex(hfm.handle, _lib.ptr(phi), nt, int(species), _lib.ptr(r), r.shape[0],
                                           _lib.ptr(out), int(bool(reuse_factor))))
     return out
+
+
+def _invert_logdet(a, out):
+    """out = a^{-1} and log det a from one HIP LU (a, out: contiguous device nx x nx)."""
+    ld = np.empty(2)
+    _lib.check(_lib.load().isle_invert_logdet(_lib.ptr(a), a.shape[0], _lib.ptr(out), _lib.dptr(ld),
+                                              _lib.stream_ptr()))
+    return complex(ld[0], ld[1])
+
+
+class QLU:
+    """Block LU of the block-cyclic tridiagonal Q = M_p M_h^T (HubbardFermiMatrixExp::QLU,
+    hubbardFermiMatrixExp.hpp; hubbardFermiMatrixExp.cpp:282-360).
+
+    Device complex128 stacks: ``dinv`` [nt, nx, nx] (d_i^{-1}), ``u``, ``l`` [nt-1, nx, nx],
+    ``v``, ``h`` [nt-2, nx, nx]; ``logdet_d`` holds log det d_i from the LU that inverted d_i."""
+
+    def __init__(self, dinv, u, l, v, h, logdet_d):
+        self.dinv, self.u, self.l, self.v, self.h = dinv, u, l, v, h
+        self.logdet_d = list(logdet_d)
+
+    def nt(self):
+        return self.dinv.shape[0]
+
+    def isConsistent(self):
+        """Block counts fit nt (hubbardFermiMatrixExp.cpp:290-303)."""
+        nt = self.dinv.shape[0]
+        if nt == 0:
+            return False
+        if self.u.shape[0] != nt - 1 or self.l.shape[0] != nt - 1:
+            return False
+        return nt < 2 or (self.v.shape[0] == nt - 2 and self.h.shape[0] == nt - 2)
+
+    def reconstruct(self):
+        """L U multiplied out into the dense (NxNt)^2 Q (hubbardFermiMatrixExp.cpp:305-360).
+        Needs nt >= 3: the reference's formula reads l[nt-3] and h[0]."""
+        nt, nx = self.dinv.shape[0], self.dinv.shape[1]
+        if nt < 3:
+            raise ValueError("Reconstruction of LU needs nt >= 3")
+        if not self.isConsistent():
+            raise RuntimeError("Components of LU not initialized properly")
+        d = torch.stack([invert(x) for x in self.dinv])
+        l, u, v, h = self.l, self.u, self.v, self.h
+        q = torch.zeros((nt, nx, nt, nx), dtype=torch.complex128, device=d.device)
+        q[0, :, 0] = d[0]
+        q[0, :, 1] = u[0]
+        q[0, :, nt - 1] = v[0]
+        for i in range(1, nt - 2):
+            q[i, :, i] = d[i] + l[i - 1] @ u[i - 1]
+            q[i, :, i - 1] = l[i - 1] @ d[i - 1]
+            q[i, :, i + 1] = u[i]
+            q[i, :, nt - 1] = l[i - 1] @ v[i - 1] + v[i]
+        q[nt - 2, :, nt - 2] = d[nt - 2] + l[nt - 3] @ u[nt - 3]
+        q[nt - 2, :, nt - 3] = l[nt - 3] @ d[nt - 3]
+        q[nt - 2, :, nt - 1] = l[nt - 3] @ v[nt - 3] + u[nt - 2]
+        q[nt - 1, :, 0] = h[0] @ d[0]
+        for i in range(1, nt - 2):
+            q[nt - 1, :, i] = h[i - 1] @ u[i - 1] + h[i] @ d[i]
+        q[nt - 1, :, nt - 2] = h[nt - 3] @ u[nt - 3] + l[nt - 2] @ d[nt - 2]
+        q[nt - 1, :, nt - 1] = d[nt - 1] + l[nt - 2] @ u[nt - 2] + _hcat(h) @ _vcat(v)
+        return q.reshape(nx * nt, nx * nt)
+
+
+def _hcat(h):
+    """[h_0 h_1 ...] as one nx x (k nx) matrix."""
+    k, nx = h.shape[0], h.shape[1]
+    return h.permute(1, 0, 2).reshape(nx, k * nx)
+
+
+def _vcat(v):
+    """[v_0; v_1; ...] as one (k nx) x nx matrix."""
+    return v.reshape(-1, v.shape[2])
+
+
+def getQLU(hfm, phi):
+    """LU of Q(phi) in blocks (getQLU, hubbardFermiMatrixExp.cpp:482-492; nt == 1 :368-383,
+    nt == 2 :386-416, nt > 2 :419-479; the Dia file repeats it).
+
+    The recursion is sequential in the time slice: each diagonal block
+    d_i = P - l_{i-1} u_{i-1} is inverted by the HIP LU (which also gives log det d_i)
+    before l_i and h_i can be formed; the block products run as complex GEMMs on the
+    device and sum_i h_i v_i is one GEMM over the stacked blocks."""
+    nx = hfm.nx()
+    phi = as_field(phi, nx)
+    if phi.dim() != 1:
+        raise ValueError("getQLU takes one field")
+    nt = phi.shape[0] // nx
+    dev = phi.device
+    c128 = torch.complex128
+    p = torch.as_tensor(hfm.P(), device=dev).to(c128)
+    dinv = torch.empty((nt, nx, nx), dtype=c128, device=dev)
+    u = torch.empty((max(nt - 1, 0), nx, nx), dtype=c128, device=dev)
+    l = torch.empty_like(u)
+    v = torch.empty((max(nt - 2, 0), nx, nx), dtype=c128, device=dev)
+    h = torch.empty_like(v)
+    lds = []
+
+    def inv(i, d):
+        lds.append(_invert_logdet(d.contiguous(), dinv[i]))
+
+    if nt == 1:
+        inv(0, p + hfm.Tplus(0, phi) + hfm.Tminus(0, phi))
+        return QLU(dinv, u, l, v, h, lds)
+    inv(0, p)
+    if nt == 2:
+        l[0] = (hfm.Tplus(1, phi) + hfm.Tminus(1, phi)) @ dinv[0]
+        u[0] = hfm.Tplus(0, phi) + hfm.Tminus(0, phi)
+        inv(1, p - l[0] @ u[0])
+        return QLU(dinv, u, l, v, h, lds)
+    u[0] = hfm.Tminus(0, phi)
+    l[0] = hfm.Tplus(1, phi) @ dinv[0]
+    v[0] = hfm.Tplus(0, phi)
+    h[0] = hfm.Tminus(nt - 1, phi) @ dinv[0]
+    for i in range(1, nt - 2):
+        inv(i, p - l[i - 1] @ u[i - 1])
+        l[i] = hfm.Tplus(i + 1, phi) @ dinv[i]
+        h[i] = -(h[i - 1] @ u[i - 1]) @ dinv[i]
+        v[i] = -(l[i - 1] @ v[i - 1])
+        u[i] = hfm.Tminus(i, phi)
+    inv(nt - 2, p - l[nt - 3] @ u[nt - 3])
+    u[nt - 2] = hfm.Tminus(nt - 2, phi) - l[nt - 3] @ v[nt - 3]
+    l[nt - 2] = (hfm.Tplus(nt - 1, phi) - h[nt - 3] @ u[nt - 3]) @ dinv[nt - 2]
+    inv(nt - 1, p - l[nt - 2] @ u[nt - 2] - _hcat(h) @ _vcat(v))
+    return QLU(dinv, u, l, v, h, lds)
+
+
+def logdetQ(hfm, phi=None):
+    """log det Q(phi) = sum_i log det d_i, Im in [-pi, pi] (logdetQ,
+    hubbardFermiMatrixExp.cpp:540-570, bind_hubbardFermiMatrix.cpp:44-46).
+    ``logdetQ(qlu)`` reuses a factorisation from getQLU."""
+    lu = hfm if isinstance(hfm, QLU) else getQLU(hfm, phi)
+    if not lu.isConsistent():
+        raise RuntimeError("Components of LU not initialized properly")
+    return to_first_log_branch(sum(lu.logdet_d))
+
+
+def solveQ(hfm, phi, rhs=None):
+    """x with Q(phi) x = rhs (solveQ, hubbardFermiMatrixExp.cpp:494-537,
+    bind_hubbardFermiMatrix.cpp:47-49): forward substitution with l and h, then backward
+    with dinv, u and v.  ``solveQ(qlu, rhs)`` reuses a factorisation.  rhs: a spacetime
+    vector (NxNt); a 2-D rhs solves every row.  Returns a device tensor of rhs's shape."""
+    if isinstance(hfm, QLU):
+        lu, rhs = hfm, phi
+    else:
+        lu = getQLU(hfm, phi)
+    if not lu.isConsistent():
+        raise RuntimeError("Components of LU not initialized properly")
+    nt, nx = lu.dinv.shape[0], lu.dinv.shape[1]
+    b = as_field(rhs)
+    shape = b.shape
+    if b.dim() not in (1, 2) or shape[-1] != nt * nx:
+        raise RuntimeError("Right hand side does not have correct size (spacetime vector)")
+    # columns are right-hand sides: b[t] is nx x nrhs
+    b = b.reshape(-1, nt, nx).permute(1, 2, 0).contiguous()
+    y = torch.empty_like(b)
+    y[0] = b[0]
+    for i in range(1, nt):
+        y[i] = b[i] - lu.l[i - 1] @ y[i - 1]
+    if nt > 2:
+        y[nt - 1] -= _hcat(lu.h) @ y[:nt - 2].reshape((nt - 2) * nx, -1)
+    x = torch.empty_like(b)
+    x[nt - 1] = lu.dinv[nt - 1] @ y[nt - 1]
+    if nt > 1:
+        x[nt - 2] = lu.dinv[nt - 2] @ (y[nt - 2] - lu.u[nt - 2] @ x[nt - 1])
+    if nt > 2:
+        vx = torch.matmul(lu.v, x[nt - 1])  # v_i x_{nt-1} for every i at once
+        for i in range(nt - 3, -1, -1):
+            x[i] = lu.dinv[i] @ (y[i] - lu.u[i] @ x[i + 1] - vx[i])
+    return x.permute(2, 0, 1).reshape(shape)

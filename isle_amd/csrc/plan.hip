@@ -1906,17 +1906,18 @@ int isle_logdet(const double* a, int n, double* out_host, void* stream) {
     });
 }
 
-int isle_invert(const double* a, int n, double* out, void* stream) {
+// A^{-1} by one LU with partial pivoting; with logdet_host set, also log det A from the same LU.
+static int invert_impl(const double* a, int n, double* out, double* logdet_host, void* stream) {
     return guarded([&] {
         if (n < 1) throw std::invalid_argument("empty matrix");
         hipStream_t s = (hipStream_t)stream;
         const long nn = (long)n * n;
         isle::DevBuf m(2 * nn * 8), piv(n * sizeof(int)), item(sizeof(isle::LuItem)), sitem(sizeof(isle::SolveItem)),
-            w(isle::lu_work_doubles(n) * sizeof(double));
+            w(isle::lu_work_doubles(n) * sizeof(double)), ld(logdet_host ? 2 * 8 : 0);
         hipLaunchKernelGGL(isle::interleaved_to_split_kernel, dim3(isle::ew_blocks(nn)), dim3(256), 0, s, a,
                            m.as<double>(), m.as<double>() + nn, nn);
         ISLE_HIP(hipGetLastError());
-        isle::LuItem it{m.as<double>(), m.as<double>() + nn, piv.as<int>(), nullptr, w.as<double>(), n, n};
+        isle::LuItem it{m.as<double>(), m.as<double>() + nn, piv.as<int>(), ld.as<double>(), w.as<double>(), n, n};
         ISLE_HIP(hipMemcpyAsync(item.p, &it, sizeof(it), hipMemcpyHostToDevice, s));
         isle::launch_lu(item.as<isle::LuItem>(), 1, n, 0, s);
         isle::SolveItem si{};
@@ -1933,8 +1934,18 @@ int isle_invert(const double* a, int n, double* out, void* stream) {
         si.nrhs = n;
         ISLE_HIP(hipMemcpyAsync(sitem.p, &si, sizeof(si), hipMemcpyHostToDevice, s));
         isle::launch_lu_solve(sitem.as<isle::SolveItem>(), 1, n, n, s);
+        if (logdet_host) ISLE_HIP(hipMemcpyAsync(logdet_host, ld.p, 16, hipMemcpyDeviceToHost, s));
         ISLE_HIP(hipStreamSynchronize(s));
     });
+}
+
+int isle_invert(const double* a, int n, double* out, void* stream) {
+    return invert_impl(a, n, out, nullptr, stream);
+}
+
+int isle_invert_logdet(const double* a, int n, double* out, double* logdet_host, void* stream) {
+    if (!logdet_host) return invert_impl(a, 0, out, nullptr, stream);  // reports the bad argument
+    return invert_impl(a, n, out, logdet_host, stream);
 }
 
 }  // extern "C"

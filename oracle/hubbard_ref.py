@@ -22,6 +22,8 @@ function                     reference
 ``solveM``                   hubbardFermiMatrixExp.cpp:645-694, hubbardFermiMatrixDia.cpp:617-673
 ``force_direct_single_part`` src/isle/cpp/action/hubbardFermiAction.cpp:16-60
 ``force_direct_square``      src/isle/cpp/action/hubbardFermiAction.cpp:62-108
+``qlu`` / ``solve_q``        src/isle/cpp/hubbardFermiMatrixExp.cpp:305-570 (QLU of Q, solveQ, logdetQ)
+``logdet_q``                 (same; hubbardFermiMatrixDia.cpp:304-569 repeats it)
 ``hole_shortcut_possible``   src/isle/cpp/action/hubbardFermiAction.cpp:114-157
 ``HubbardFermiAction``       src/isle/cpp/action/hubbardFermiAction.cpp:159-298
 ``HubbardGaugeAction``       src/isle/cpp/action/hubbardGaugeAction.cpp:7-13
@@ -437,6 +439,112 @@ def force_direct_square(hfm, phi):
 def logdetQ_dense(hfm, phi):
     """-eval of DIRECT_SQUARE; the reference test compares logdetQ to dense logdet(Q)."""
     return ilogdet(hfm.Q(phi))
+
+
+class QLUParts:
+    """Blocks of the LU of the block-cyclic tridiagonal Q (hubbardFermiMatrixExp.hpp QLU):
+    dinv[i] = d_i^{-1} (nt), u, l (nt-1), v, h (nt-2); lists of nx x nx complex arrays."""
+
+    def __init__(self):
+        self.dinv, self.u, self.l, self.v, self.h = [], [], [], [], []
+
+
+def qlu(hfm, phi):
+    """getQLU (hubbardFermiMatrixExp.cpp:482-492): nt == 1 (:368-383), nt == 2 (:386-416), nt > 2
+    (:419-479); the Dia file repeats it (hubbardFermiMatrixDia.cpp:367-491)."""
+    nx = hfm.nx()
+    nt = get_nt(phi, nx)
+    p = hfm.P().astype(np.complex128)
+    lu = QLUParts()
+    if nt == 1:
+        lu.dinv.append(np.linalg.inv(p + hfm.Tplus(0, phi) + hfm.Tminus(0, phi)))
+        return lu
+    if nt == 2:
+        lu.dinv.append(np.linalg.inv(p))
+        lu.l.append((hfm.Tplus(1, phi) + hfm.Tminus(1, phi)) @ lu.dinv[0])
+        lu.u.append(hfm.Tplus(0, phi) + hfm.Tminus(0, phi))
+        lu.dinv.append(np.linalg.inv(p - lu.l[0] @ lu.u[0]))
+        return lu
+    lu.dinv.append(np.linalg.inv(p))
+    lu.u.append(hfm.Tminus(0, phi))
+    lu.l.append(hfm.Tplus(1, phi) @ lu.dinv[0])
+    lu.v.append(hfm.Tplus(0, phi))
+    lu.h.append(hfm.Tminus(nt - 1, phi) @ lu.dinv[0])
+    for i in range(1, nt - 2):
+        lu.dinv.append(np.linalg.inv(p - lu.l[i - 1] @ lu.u[i - 1]))
+        lu.l.append(hfm.Tplus(i + 1, phi) @ lu.dinv[i])
+        lu.h.append(-lu.h[i - 1] @ lu.u[i - 1] @ lu.dinv[i])
+        lu.v.append(-lu.l[i - 1] @ lu.v[i - 1])
+        lu.u.append(hfm.Tminus(i, phi))
+    lu.dinv.append(np.linalg.inv(p - lu.l[nt - 3] @ lu.u[nt - 3]))
+    lu.u.append(hfm.Tminus(nt - 2, phi) - lu.l[nt - 3] @ lu.v[nt - 3])
+    lu.l.append((hfm.Tplus(nt - 1, phi) - lu.h[nt - 3] @ lu.u[nt - 3]) @ lu.dinv[nt - 2])
+    last = p - lu.l[nt - 2] @ lu.u[nt - 2]
+    for i in range(nt - 2):
+        last = last - lu.h[i] @ lu.v[i]
+    lu.dinv.append(np.linalg.inv(last))
+    return lu
+
+
+def qlu_reconstruct(lu):
+    """L U assembled back into Q (hubbardFermiMatrixExp.cpp:305-360), nt >= 3."""
+    nt = len(lu.dinv)
+    if nt < 3:
+        raise ValueError("reconstruction needs nt >= 3")
+    nx = lu.dinv[0].shape[0]
+    q = np.zeros((nx * nt, nx * nt), dtype=np.complex128)
+
+    def blk(r, c):
+        return q[r * nx:(r + 1) * nx, c * nx:(c + 1) * nx]
+    d = [np.linalg.inv(x) for x in lu.dinv]
+    blk(0, 0)[:] = d[0]
+    blk(0, 1)[:] = lu.u[0]
+    blk(0, nt - 1)[:] = lu.v[0]
+    for i in range(1, nt - 2):
+        blk(i, i)[:] = d[i] + lu.l[i - 1] @ lu.u[i - 1]
+        blk(i, i - 1)[:] = lu.l[i - 1] @ d[i - 1]
+        blk(i, i + 1)[:] = lu.u[i]
+        blk(i, nt - 1)[:] = lu.l[i - 1] @ lu.v[i - 1] + lu.v[i]
+    blk(nt - 2, nt - 2)[:] = d[nt - 2] + lu.l[nt - 3] @ lu.u[nt - 3]
+    blk(nt - 2, nt - 3)[:] = lu.l[nt - 3] @ d[nt - 3]
+    blk(nt - 2, nt - 1)[:] = lu.l[nt - 3] @ lu.v[nt - 3] + lu.u[nt - 2]
+    blk(nt - 1, 0)[:] = lu.h[0] @ d[0]
+    for i in range(1, nt - 2):
+        blk(nt - 1, i)[:] = lu.h[i - 1] @ lu.u[i - 1] + lu.h[i] @ d[i]
+    blk(nt - 1, nt - 2)[:] = lu.h[nt - 3] @ lu.u[nt - 3] + lu.l[nt - 2] @ d[nt - 2]
+    last = d[nt - 1] + lu.l[nt - 2] @ lu.u[nt - 2]
+    for i in range(nt - 2):
+        last = last + lu.h[i] @ lu.v[i]
+    blk(nt - 1, nt - 1)[:] = last
+    return q
+
+
+def solve_q(lu, rhs):
+    """Q x = rhs by forward then backward substitution with the QLU blocks
+    (hubbardFermiMatrixExp.cpp:500-537)."""
+    nt = len(lu.dinv)
+    nx = lu.dinv[0].shape[0]
+    b = np.asarray(rhs, dtype=np.complex128).reshape(nt, nx)
+    y = np.empty_like(b)
+    y[0] = b[0]
+    for i in range(1, nt - 1):
+        y[i] = b[i] - lu.l[i - 1] @ y[i - 1]
+    if nt > 1:
+        y[nt - 1] = b[nt - 1] - lu.l[nt - 2] @ y[nt - 2]
+        for j in range(nt - 2):
+            y[nt - 1] -= lu.h[j] @ y[j]
+    x = np.empty_like(b)
+    x[nt - 1] = lu.dinv[nt - 1] @ y[nt - 1]
+    if nt > 1:
+        x[nt - 2] = lu.dinv[nt - 2] @ (y[nt - 2] - lu.u[nt - 2] @ x[nt - 1])
+        for i in range(nt - 3, -1, -1):
+            x[i] = lu.dinv[i] @ (y[i] - lu.u[i] @ x[i + 1] - lu.v[i] @ x[nt - 1])
+    return x.reshape(-1)
+
+
+def logdet_q(lu):
+    """log det Q = -sum_i log det d_i^{-1}, first branch (hubbardFermiMatrixExp.cpp:540-570)."""
+    return to_first_log_branch(-sum(ilogdet(x) for x in lu.dinv))
 
 
 # --------------------------------------------------------------------------- #

@@ -1,4 +1,4 @@
-"""GPU: HFAAlgorithm.DIRECT_SQUARE (dense Q = M_p M_h^T, HIP LU) against the oracle.
+"""GPU: HFAAlgorithm.DIRECT_SQUARE (Q = M_p M_h^T, HIP LU) against the oracle.
 
 The reference's own test compares DIRECT_SINGLE with DIRECT_SQUARE
 (tests/test_hubbardFermiAction.py:52-77, 122-147); here the GPU DIRECT_SQUARE action and
@@ -6,8 +6,9 @@ force equal the oracle's DIRECT_SQUARE restatement (dense Q from its block assem
 numpy inverse; pinned in tests/test_oracle_*.py) and the GPU DIRECT_SINGLE ones, to
 1e-10 relative (action: imaginary part modulo 2 pi), on a honeycomb 3x3 torus (Nx=18),
 Nt=16, beta=4 (N = 288) for EXP / DIA hopping, particle-hole / spin basis, sigmaKappa =
-+-1 and mu = 0 / 0.1, with a complex field.  Fields beyond the dense LU (N > 512) and
-graph capture are refused.
++-1 and mu = 0 / 0.1, with a complex field.  The action goes through the block LU of Q
+(logdetQ, tests/test_gpu_qlu.py); the force's dense Q^{-1} beyond N = 512 and graph
+capture are refused.
 """
 import numpy as np
 import pytest
@@ -45,7 +46,7 @@ def test_direct_square_limits():
     sq = A.HubbardFermiAction(kappa, 0.0, -1, A.HFAHopping.EXP, A.HFABasis.PARTICLE_HOLE,
                               A.HFAAlgorithm.DIRECT_SQUARE)
     with pytest.raises(ValueError):
-        sq.eval(np.zeros(18 * 32, complex))  # N = 576 > 512
+        sq.force(np.zeros(18 * 32, complex))  # N = 576 > 512: the dense Q^{-1} is refused
     with pytest.raises(ValueError):
         sq.logdets(np.zeros(18 * 16, complex))
     with pytest.raises(ValueError):

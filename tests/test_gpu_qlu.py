@@ -1,0 +1,116 @@
+"""GPU: the blocks of Q (P, Tplus, Tminus, Q), its block LU (getQLU), logdetQ and solveQ
+against the oracle (tests/test_oracle_qlu.py pins it against dense Q = M_p M_h^T, LAPACK
+log det and solve).
+
+* P / Tplus / Tminus / Q of both hoppings equal the oracle's to 1e-13 (honeycomb 3x3,
+  Nx=18, Nt=8, sigmaKappa = +-1, mu = 0 / 0.1, complex field);
+* the blocks of getQLU equal the oracle's qlu to 1e-10 and multiply back to Q
+  (reconstruct), logdetQ equals the oracle's and the dense LAPACK log det of Q to 1e-10
+  (imaginary parts modulo 2 pi), solveQ equals numpy.linalg.solve on the dense Q to 1e-10
+  for one right-hand side and for a stack of them; Nt = 1, 2, 3, 4, 8, 32 (the reference
+  test uses 4, 8, 32; 1 and 2 are its special cases) with kappaTilde = kappa / Nt as in
+  the reference test (cond Q <= 2e3: the oracle's exp(kappa) by eigh and the GPU's by a
+  Taylor series differ by rounding, which cond Q amplifies; at beta = 3, Nt = 1 it is 7e7);
+* at Nx=72, Nt=64 (honeycomb 6x6, beta=6), beyond a dense check, log det Q equals
+  log det M_p + log det M_h (det Q = det M_p det M_h) from the GPU logdetM to 1e-9;
+* DIRECT_SQUARE's action goes through logdetQ and so has no size limit: at Nx*Nt = 576
+  it equals DIRECT_SINGLE's.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import hubbard_ref as ref
+from tests.helpers import mod2pi_diff, random_phi
+
+pytestmark = pytest.mark.gpu
+
+isle = pytest.importorskip("isle_amd")
+from isle_amd import action as A  # noqa: E402
+
+HOPS = {"EXP": (isle.HubbardFermiMatrixExp, ref.HFMExp), "DIA": (isle.HubbardFermiMatrixDia, ref.HFMDia)}
+
+
+def _pair(hop, nt, mu, sk, L=3, beta=3.0):
+    kappa = ref.honeycomb(L) * beta / nt
+    gcls, rcls = HOPS[hop]
+    return gcls(kappa, mu * beta / nt, sk), rcls(kappa, mu * beta / nt, sk)
+
+
+def _np(t):
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("hop", ["EXP", "DIA"])
+def test_q_blocks(hop):
+    print(f"Q blocks {hop}, Nx=18 Nt=8", flush=True)
+    rng = np.random.RandomState(40)
+    for mu, sk in itertools.product((0.0, 0.1), (-1, 1)):
+        g, r = _pair(hop, 8, mu, sk)
+        phi = random_phi(rng, 18 * 8)
+        p = r.P()
+        assert np.abs(g.P() - p).max() < 1e-13 * np.abs(p).max()
+        for tp in (0, 3, 7):
+            for name in ("Tplus", "Tminus"):
+                want = getattr(r, name)(tp, phi)
+                got = _np(getattr(g, name)(tp, phi))
+                assert np.abs(got - want).max() < 1e-13 * np.abs(want).max(), (name, tp, mu, sk)
+        q = r.Q(phi)
+        assert np.abs(_np(g.Q(phi)) - q).max() < 1e-13 * np.abs(q).max()
+
+
+@pytest.mark.parametrize("hop", ["EXP", "DIA"])
+@pytest.mark.parametrize("nt", [1, 2, 3, 4, 8, 32])
+def test_qlu_logdet_solve(hop, nt):
+    print(f"QLU {hop} Nx=18 Nt={nt}", flush=True)
+    rng = np.random.RandomState(100 + nt)
+    for mu, sk in ((0.0, -1), (0.0, 1), (0.1, 1)):
+        g, r = _pair(hop, nt, mu, sk, beta=1.0)
+        phi = random_phi(rng, 18 * nt)
+        lu = isle.getQLU(g, phi)
+        want = ref.qlu(r, phi)
+        assert lu.isConsistent()
+        for part in ("dinv", "u", "l", "v", "h"):
+            got, exp = _np(getattr(lu, part)), np.array(getattr(want, part)).reshape(-1, 18, 18)
+            assert got.shape == exp.shape, part
+            if exp.size:
+                assert np.abs(got - exp).max() < 1e-10 * np.abs(exp).max(), (part, mu, sk)
+        q = r.Q(phi)
+        if nt >= 3:
+            assert np.abs(_np(lu.reconstruct()) - q).max() < 1e-10 * np.abs(q).max()
+        dense = ref.ilogdet(q)
+        for got in (isle.logdetQ(g, phi), isle.logdetQ(lu)):
+            assert mod2pi_diff(got, dense) < 1e-10 * max(1.0, abs(dense)), (mu, sk)
+            assert mod2pi_diff(got, ref.logdet_q(want)) < 1e-10 * max(1.0, abs(dense))
+        b = random_phi(rng, 18 * nt)
+        x = np.linalg.solve(q, b)
+        assert np.abs(_np(isle.solveQ(g, phi, b)) - x).max() < 1e-10 * np.abs(x).max()
+        bs = np.stack([random_phi(rng, 18 * nt) for _ in range(3)])
+        xs = np.linalg.solve(q, bs.T).T
+        got = _np(isle.solveQ(lu, bs))
+        assert got.shape == bs.shape and np.abs(got - xs).max() < 1e-10 * np.abs(xs).max()
+
+
+def test_logdetQ_is_sum_of_logdetM():
+    print("logdetQ = logdetM_p + logdetM_h, Nx=72 Nt=64", flush=True)
+    nt, beta, U = 64, 6.0, 3.0
+    kappa = ref.honeycomb(6) * beta / nt
+    phi = np.random.RandomState(16).normal(0, np.sqrt(U * beta / nt), 72 * nt) + 0j
+    for cls in (isle.HubbardFermiMatrixExp, isle.HubbardFermiMatrixDia):
+        hfm = cls(kappa, 0.0, -1)
+        want = isle.logdetM(hfm, phi, isle.Species.PARTICLE) + isle.logdetM(hfm, phi, isle.Species.HOLE)
+        got = isle.logdetQ(hfm, phi)
+        assert mod2pi_diff(got, want) < 1e-9 * abs(want), cls.__name__
+
+
+def test_direct_square_eval_has_no_size_limit():
+    kappa = ref.honeycomb(3) * 4.0 / 32
+    phi = random_phi(np.random.RandomState(3), 18 * 32)
+    for hop in (A.HFAHopping.EXP, A.HFAHopping.DIA):
+        sq = A.HubbardFermiAction(kappa, 0.0, -1, hop, A.HFABasis.PARTICLE_HOLE, A.HFAAlgorithm.DIRECT_SQUARE)
+        single = A.HubbardFermiAction(kappa, 0.0, -1, hop, A.HFABasis.PARTICLE_HOLE, A.HFAAlgorithm.DIRECT_SINGLE)
+        s_sq, s_single = sq.eval(phi), single.eval(phi)
+        assert mod2pi_diff(s_sq, s_single) < 1e-10 * abs(s_single)
+        with pytest.raises(ValueError):
+            sq.force(phi)  # the force needs the dense Q^{-1}
