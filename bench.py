@@ -11,6 +11,7 @@ Prints one JSON line (see the fields below).  Only one GPU is used; --gpus > 1 e
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -49,6 +50,9 @@ def parse():
     ap.add_argument("--propagator", action="store_true",
                     help="the all-to-all propagator measurement (M^-1 of one species, every spacetime source) on "
                          "the Nx=288, Nt=128 torus: prints propagators/s instead of the HMC line")
+    ap.add_argument("--logdetq", action="store_true",
+                    help="log det Q through the block LU of Q = M_p M_h^T (DIRECT_SQUARE's action) on the "
+                         "Nx=288, Nt=128 torus: prints log dets/s instead of the HMC line")
     ap.add_argument("--force-only", action="store_true",
                     help="profiling aid: run --steps batched mixed forces and one 1 GiB device copy "
                          "(the calibration kernel for HBM byte counters), print timings, no HMC")
@@ -191,6 +195,59 @@ def propagator(args):
                                 "sample": f"oracle solveM (numpy/OpenBLAS, unrefined) against 288 and 1152 of the {n} "
                                           f"sources ({times[288]:.2f} s, {times[1152]:.2f} s); fixed + per-source "
                                           f"cost extrapolated to all sources: {t_cpu:.1f} s"}
+    print(json.dumps(line))
+
+
+def logdetq(args):
+    """log det Q(phi) by the block LU of Q = M_p M_h^T (getQLU + logdetQ) on the config-3 torus
+    (Nx=288, Nt=128, beta=8, U=3, EXP, sigmaKappa=-1) for a field N(0, U beta/Nt) from
+    NumpyRNG(42).  One step = one log det Q.  Algorithmic work: the recursion's Nt block
+    inversions and block products, 8 Nx^3 (7 Nt - 12) real flops (each complex Nx x Nx
+    product or inverse counts 8 Nx^3).  Accuracy: against log det M_p + log det M_h from the
+    DIRECT_SINGLE logdetM (det Q = det M_p det M_h).  CPU baseline: the oracle's QLU
+    (numpy/OpenBLAS) on the same field."""
+    import torch
+    import isle_amd as isle
+    from isle_amd.evolver import NumpyRNG
+    from oracle import hubbard_ref as ref
+    kappa = ref.honeycomb(L)
+    nx = kappa.shape[0]
+    kt, ut = kappa * BETA / NT, U * BETA / NT
+    phi = NumpyRNG(42).normal(0, np.sqrt(ut), nx * NT) + 0j
+    hfm = isle.HubbardFermiMatrixExp(kt, 0.0, -1)
+    dphi = isle.as_field(phi)
+    for _ in range(args.warmup):
+        isle.logdetQ(hfm, dphi)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        got = isle.logdetQ(hfm, dphi)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    want = isle.logdetM(hfm, dphi, isle.Species.PARTICLE) + isle.logdetM(hfm, dphi, isle.Species.HOLE)
+    d = complex(got) - complex(want)
+    err = abs(complex(d.real, math.remainder(d.imag, 2 * math.pi))) / abs(want)
+    flops = 8.0 * nx ** 3 * (7 * NT - 12)
+    line = {
+        "metric": "log det Q by the block LU of Q at Nx=288 Nt=128 (DIRECT_SQUARE action)",
+        "value": 1.0 / dt, "unit": "logdets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "honeycomb 12x12 torus Nx=288 Nt=128 beta=8 U=3 mu=0 EXP sigmaKappa=-1"},
+        "rel_err_vs_logdetM_p_plus_h": err,
+        "roofline": {"bound": "mfma", "achieved": flops / dt / 1e12, "peak": PEAK["f64"], "unit": "TFLOP/s",
+                     "frac": flops / dt / 1e12 / PEAK["f64"], "traffic": None,
+                     "kernel": "whole log det Q (8 Nx^3 (7 Nt - 12) flops; a sequential recursion of Nt "
+                               "block inversions)"},
+    }
+    if not args.no_cpu:
+        rh = ref.HFMExp(kt, 0.0, -1)
+        t0 = time.perf_counter()
+        ref.logdet_q(ref.qlu(rh, phi))
+        t_cpu = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": 1.0 / t_cpu, "unit": "logdets/s", "cores": os.cpu_count(), "kind": "port",
+                                "sample": f"oracle qlu + logdet_q (numpy/OpenBLAS) on the same field, once: "
+                                          f"{t_cpu:.2f} s"}
     print(json.dumps(line))
 
 
@@ -340,6 +397,9 @@ def main():
         return
     if args.propagator:
         propagator(args)
+        return
+    if args.logdetq:
+        logdetq(args)
         return
     import torch
     import isle_amd as isle
