@@ -472,6 +472,8 @@ def logdetQ(hfm, phi=None):
     """log det Q(phi) = sum_i log det d_i, Im in [-pi, pi] (logdetQ,
     hubbardFermiMatrixExp.cpp:540-570, bind_hubbardFermiMatrix.cpp:44-46).
     ``logdetQ(qlu)`` reuses a factorisation from getQLU."""
+    if not isinstance(hfm, QLU) and phi is None:
+        raise TypeError("logdetQ takes (hfm, phi) or (qlu)")
     lu = hfm if isinstance(hfm, QLU) else getQLU(hfm, phi)
     if not lu.isConsistent():
         raise RuntimeError("Components of LU not initialized properly")
@@ -485,7 +487,9 @@ def solveQ(hfm, phi, rhs=None):
     vector (NxNt); a 2-D rhs solves every row.  Returns a device tensor of rhs's shape."""
     if isinstance(hfm, QLU):
         lu, rhs = hfm, phi
-    else:
+    if rhs is None:
+        raise TypeError("solveQ takes (hfm, phi, rhs) or (qlu, rhs)")
+    if not isinstance(hfm, QLU):
         lu = getQLU(hfm, phi)
     if not lu.isConsistent():
         raise RuntimeError("Components of LU not initialized properly")
