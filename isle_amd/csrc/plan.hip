@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/isle_hfa.h"
 
@@ -1700,6 +1701,61 @@ void Plan::F(int tp, const double* phi, int nt, int species, int inv, double* ou
 namespace {
 thread_local std::string g_last_error;
 
+// Device scratch of the one-matrix LU entry points (isle_logdet, isle_invert,
+// isle_invert_logdet): the split copy of the matrix, the LU's work buffer, pivots, the two
+// launch descriptors and the log det, in one allocation at 256-byte aligned offsets.  Up to
+// DENSE_CACHE_MAX bytes it is kept per device between calls (a hipMalloc / hipFree pair per
+// call costs more than the LU of a small block, and hipFree waits for the device); a lease
+// holds a lock for the whole call, since every call on a device shares that buffer.  Larger
+// orders allocate for the call.  Every field is written by the call before it is read.
+constexpr size_t DENSE_CACHE_MAX = size_t(64) << 20;
+std::mutex dense_mu;
+std::map<int, std::pair<void*, size_t>> dense_cache;  // device -> (buffer, bytes), kept to exit
+
+struct DenseScratch {
+    double *m, *w, *ld;
+    int* piv;
+    isle::LuItem* item;
+    isle::SolveItem* sitem;
+    explicit DenseScratch(int n) : lock_(dense_mu) {
+        auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+        const size_t nn = (size_t)n * n;
+        size_t off[6], o = 0;
+        off[0] = o; o = al(o + 2 * nn * sizeof(double));
+        off[1] = o; o = al(o + isle::lu_work_doubles(n) * sizeof(double));
+        off[2] = o; o = al(o + (size_t)n * sizeof(int));
+        off[3] = o; o = al(o + sizeof(isle::LuItem));
+        off[4] = o; o = al(o + sizeof(isle::SolveItem));
+        off[5] = o; o = al(o + 2 * sizeof(double));
+        char* base;
+        if (o <= DENSE_CACHE_MAX) {
+            int dev = 0;
+            ISLE_HIP(hipGetDevice(&dev));
+            auto& e = dense_cache[dev];
+            if (e.second < o) {
+                if (e.first) ISLE_HIP(hipFree(e.first));
+                e = {nullptr, 0};
+                ISLE_HIP(hipMalloc(&e.first, o));
+                e.second = o;
+            }
+            base = static_cast<char*>(e.first);
+        } else {
+            own_ = std::make_unique<isle::DevBuf>(o);
+            base = static_cast<char*>(own_->p);
+        }
+        m = reinterpret_cast<double*>(base + off[0]);
+        w = reinterpret_cast<double*>(base + off[1]);
+        piv = reinterpret_cast<int*>(base + off[2]);
+        item = reinterpret_cast<isle::LuItem*>(base + off[3]);
+        sitem = reinterpret_cast<isle::SolveItem*>(base + off[4]);
+        ld = reinterpret_cast<double*>(base + off[5]);
+    }
+
+private:
+    std::unique_lock<std::mutex> lock_;
+    std::unique_ptr<isle::DevBuf> own_;
+};
+
 template <typename F>
 int guarded(F&& f) {
     try {
@@ -1893,15 +1949,14 @@ int isle_logdet(const double* a, int n, double* out_host, void* stream) {
         if (n < 1) throw std::invalid_argument("empty matrix");
         hipStream_t s = (hipStream_t)stream;
         const long nn = (long)n * n;
-        isle::DevBuf m(2 * nn * 8), piv(n * sizeof(int)), ld(2 * 8), item(sizeof(isle::LuItem)),
-            w(isle::lu_work_doubles(n) * sizeof(double));
-        hipLaunchKernelGGL(isle::interleaved_to_split_kernel, dim3(isle::ew_blocks(nn)), dim3(256), 0, s, a,
-                           m.as<double>(), m.as<double>() + nn, nn);
+        DenseScratch sc(n);
+        hipLaunchKernelGGL(isle::interleaved_to_split_kernel, dim3(isle::ew_blocks(nn)), dim3(256), 0, s, a, sc.m,
+                           sc.m + nn, nn);
         ISLE_HIP(hipGetLastError());
-        isle::LuItem it{m.as<double>(), m.as<double>() + nn, piv.as<int>(), ld.as<double>(), w.as<double>(), n, n};
-        ISLE_HIP(hipMemcpyAsync(item.p, &it, sizeof(it), hipMemcpyHostToDevice, s));
-        isle::launch_lu(item.as<isle::LuItem>(), 1, n, 0, s);
-        ISLE_HIP(hipMemcpyAsync(out_host, ld.p, 16, hipMemcpyDeviceToHost, s));
+        isle::LuItem it{sc.m, sc.m + nn, sc.piv, sc.ld, sc.w, n, n};
+        ISLE_HIP(hipMemcpyAsync(sc.item, &it, sizeof(it), hipMemcpyHostToDevice, s));
+        isle::launch_lu(sc.item, 1, n, 0, s);
+        ISLE_HIP(hipMemcpyAsync(out_host, sc.ld, 16, hipMemcpyDeviceToHost, s));
         ISLE_HIP(hipStreamSynchronize(s));
     });
 }
@@ -1912,19 +1967,18 @@ static int invert_impl(const double* a, int n, double* out, double* logdet_host,
         if (n < 1) throw std::invalid_argument("empty matrix");
         hipStream_t s = (hipStream_t)stream;
         const long nn = (long)n * n;
-        isle::DevBuf m(2 * nn * 8), piv(n * sizeof(int)), item(sizeof(isle::LuItem)), sitem(sizeof(isle::SolveItem)),
-            w(isle::lu_work_doubles(n) * sizeof(double)), ld(logdet_host ? 2 * 8 : 0);
-        hipLaunchKernelGGL(isle::interleaved_to_split_kernel, dim3(isle::ew_blocks(nn)), dim3(256), 0, s, a,
-                           m.as<double>(), m.as<double>() + nn, nn);
+        DenseScratch sc(n);
+        hipLaunchKernelGGL(isle::interleaved_to_split_kernel, dim3(isle::ew_blocks(nn)), dim3(256), 0, s, a, sc.m,
+                           sc.m + nn, nn);
         ISLE_HIP(hipGetLastError());
-        isle::LuItem it{m.as<double>(), m.as<double>() + nn, piv.as<int>(), ld.as<double>(), w.as<double>(), n, n};
-        ISLE_HIP(hipMemcpyAsync(item.p, &it, sizeof(it), hipMemcpyHostToDevice, s));
-        isle::launch_lu(item.as<isle::LuItem>(), 1, n, 0, s);
+        isle::LuItem it{sc.m, sc.m + nn, sc.piv, logdet_host ? sc.ld : nullptr, sc.w, n, n};
+        ISLE_HIP(hipMemcpyAsync(sc.item, &it, sizeof(it), hipMemcpyHostToDevice, s));
+        isle::launch_lu(sc.item, 1, n, 0, s);
         isle::SolveItem si{};
-        si.lu_re = m.as<double>();
-        si.lu_im = m.as<double>() + nn;
-        si.piv = piv.as<int>();
-        si.dinv = w.as<double>();
+        si.lu_re = sc.m;
+        si.lu_im = sc.m + nn;
+        si.piv = sc.piv;
+        si.dinv = sc.w;
         si.x_re = out;
         si.x_im = out + 1;
         si.xsr = 2L * n;
@@ -1932,9 +1986,9 @@ static int invert_impl(const double* a, int n, double* out, double* logdet_host,
         si.n = n;
         si.ld = n;
         si.nrhs = n;
-        ISLE_HIP(hipMemcpyAsync(sitem.p, &si, sizeof(si), hipMemcpyHostToDevice, s));
-        isle::launch_lu_solve(sitem.as<isle::SolveItem>(), 1, n, n, s);
-        if (logdet_host) ISLE_HIP(hipMemcpyAsync(logdet_host, ld.p, 16, hipMemcpyDeviceToHost, s));
+        ISLE_HIP(hipMemcpyAsync(sc.sitem, &si, sizeof(si), hipMemcpyHostToDevice, s));
+        isle::launch_lu_solve(sc.sitem, 1, n, n, s);
+        if (logdet_host) ISLE_HIP(hipMemcpyAsync(logdet_host, sc.ld, 16, hipMemcpyDeviceToHost, s));
         ISLE_HIP(hipStreamSynchronize(s));
     });
 }
