@@ -171,28 +171,36 @@ class LeapfrogGraph:
         return self.out_phi.clone(), self.out_pi.clone()
 
 
-# classic RK4 (n = 0) and the 3/8 rule (n = 1): weights (b1..b4) and the stage
-# coefficients beta21, beta31, beta32, beta41, beta42, beta43 (integrator.cpp:39-69)
-_RK4_WEIGHTS = {0: (1 / 6, 1 / 3, 1 / 3, 1 / 6), 1: (1 / 8, 3 / 8, 3 / 8, 1 / 8)}
-_RK4_STAGES = {0: (0.5, 0.0, 0.5, 0.0, 0.0, 1.0), 1: (1 / 3, -1 / 3, 1.0, 1.0, -1.0, 1.0)}
+# classic RK4 (n = 0) and the 3/8 rule (n = 1) as (weights b1..b4, stage coefficients
+# beta21, beta31, beta32, beta41, beta42, beta43), integrator.cpp:39-69.  "reference" keeps
+# the reference's beta21 = 1/30 in the 3/8 rule; "textbook" has the rule's 1/3.
+_RK4_TABLES = {
+    "reference": {0: ((1 / 6, 1 / 3, 1 / 3, 1 / 6), (0.5, 0.0, 0.5, 0.0, 0.0, 1.0)),
+                  1: ((1 / 8, 3 / 8, 3 / 8, 1 / 8), (1 / 30, -1 / 3, 1.0, 1.0, -1.0, 1.0))},
+    "textbook": {0: ((1 / 6, 1 / 3, 1 / 3, 1 / 6), (0.5, 0.0, 0.5, 0.0, 0.0, 1.0)),
+                 1: ((1 / 8, 3 / 8, 3 / 8, 1 / 8), (1 / 3, -1 / 3, 1.0, 1.0, -1.0, 1.0))},
+}
 
 
-def _rk4_step(phi, action, eps, direction, n):
-    w, b = _RK4_WEIGHTS[n], _RK4_STAGES[n]
+def _rk4_step(phi, action, eps, direction, n, variant):
+    w, b = _RK4_TABLES[variant][n]
     e = eps * direction
+    # the reference's fourth stage takes -eps, not -eps * direction (integrator.cpp:96)
+    e4 = eps if variant == "reference" else e
 
-    def k(arg):
-        return (-e) * torch.conj(as_field(action.force(arg)))
-    k1 = k(phi)
-    k2 = k(phi + b[0] * k1)
-    k3 = k(phi + b[1] * k1 + b[2] * k2)
-    k4 = k(phi + b[3] * k1 + b[4] * k2 + b[5] * k3)
+    def k(scale, arg):
+        return (-scale) * torch.conj(as_field(action.force(arg)))
+    k1 = k(e, phi)
+    k2 = k(e, phi + b[0] * k1)
+    k3 = k(e, phi + b[1] * k1 + b[2] * k2)
+    k4 = k(e4, phi + b[3] * k1 + b[4] * k2 + b[5] * k3)
     out = phi + w[0] * k1 + w[1] * k2 + w[2] * k3 + w[3] * k4
     return out, complex(action.eval(out))
 
 
 def rungeKutta4Flow(phi, action, flowTime, stepSize, actVal=complex(math.nan, math.nan), n=0, direction=+1,
-                    adaptAttenuation=0.9, adaptThreshold=1.0e-8, minStepSize=math.nan, imActTolerance=0.001):
+                    adaptAttenuation=0.9, adaptThreshold=1.0e-8, minStepSize=math.nan, imActTolerance=0.001,
+                    *, variant="reference"):
     """Adaptive RK4 integration of the holomorphic flow for ``flowTime``
     (integrator.hpp:45-84, integrator.cpp:144-193).  Returns (phi, action value, flow time
     reached).
@@ -204,10 +212,13 @@ def rungeKutta4Flow(phi, action, flowTime, stepSize, actVal=complex(math.nan, ma
     The last step is shortened to end at flowTime, or skipped if shorter than minStepSize
     (default max(stepSize / 1000, 1e-12)).  n = 0: classic RK4, n = 1: the 3/8 rule.
 
-    Differences from the reference: its fourth stage drops ``direction`` (-epsilon where
-    the other stages use -epsilon direction) and its 3/8 rule has beta21 = 1/30 (the rule's
-    1/3 gives fourth order; 1/30 leaves the method first order).  Both are corrected here,
-    so the results agree with the reference for n = 0 and direction = +1."""
+    ``variant="reference"`` (default) gives the reference's results: its fourth stage
+    drops ``direction`` (-epsilon where the other stages use -epsilon direction) and its
+    3/8 rule has beta21 = 1/30, which leaves that rule first order.  ``variant="textbook"``
+    keeps the direction in every stage and uses beta21 = 1/3 (fourth order for both n and
+    both directions).  The two agree for n = 0 and direction = +1."""
+    if variant not in _RK4_TABLES:
+        raise ValueError(f"variant must be one of {sorted(_RK4_TABLES)}, not {variant!r}")
     if n not in (0, 1):
         raise ValueError("n must be 0 or 1")
     phi = as_field(phi).clone()
@@ -221,7 +232,7 @@ def rungeKutta4Flow(phi, action, flowTime, stepSize, actVal=complex(math.nan, ma
             stepSize = flowTime - t
             if stepSize < minStepSize:
                 break
-        new_phi, new_act = _rk4_step(phi, action, stepSize, direction, n)
+        new_phi, new_act = _rk4_step(phi, action, stepSize, direction, n, variant)
         err = abs(cmath.exp(1j * (actVal.imag - new_act.imag)) - 1.0)
         if err > imActTolerance:
             if stepSize == minStepSize:

@@ -644,31 +644,38 @@ def leapfrog(phi, pi, action, length, nsteps, direction=1.0):
     return phi_out, pi_out, action.eval(phi_out)
 
 
-# classic RK4 (n = 0) and the 3/8 rule (n = 1): weights, then the stage coefficients
-# beta21, beta31, beta32, beta41, beta42, beta43
-_RK4 = {0: ((1 / 6, 1 / 3, 1 / 3, 1 / 6), (0.5, 0.0, 0.5, 0.0, 0.0, 1.0)),
-        1: ((0.125, 0.375, 0.375, 0.125), (1 / 3, -1 / 3, 1.0, 1.0, -1.0, 1.0))}
+# Stage tables of integrator.cpp:39-70: classic RK4 (n = 0) and the 3/8 rule (n = 1):
+# weights, then beta21, beta31, beta32, beta41, beta42, beta43.  The reference's 3/8 rule
+# has beta21 = 1/30 (integrator.cpp:63); the textbook rule has 1/3.
+_RK4 = {"reference": {0: ((1 / 6, 1 / 3, 1 / 3, 1 / 6), (0.5, 0.0, 0.5, 0.0, 0.0, 1.0)),
+                      1: ((0.125, 0.375, 0.375, 0.125), (1 / 30, -1 / 3, 1.0, 1.0, -1.0, 1.0))},
+        "textbook": {0: ((1 / 6, 1 / 3, 1 / 3, 1 / 6), (0.5, 0.0, 0.5, 0.0, 0.0, 1.0)),
+                     1: ((0.125, 0.375, 0.375, 0.125), (1 / 3, -1 / 3, 1.0, 1.0, -1.0, 1.0))}}
 
 
-def _rk4_step(phi, action, eps, direction, n):
-    """integrator.cpp:77-117: k_i = -eps direction conj(force(...))."""
-    w, b = _RK4[n]
+def _rk4_step(phi, action, eps, direction, n, variant="reference"):
+    """integrator.cpp:77-98: k_i = -eps direction conj(force(...)) for stages 1-3; stage 4
+    of the reference uses -eps without the direction (integrator.cpp:96).  The "textbook"
+    variant keeps the direction in stage 4 and uses the 3/8 rule's beta21 = 1/3."""
+    w, b = _RK4[variant][n]
     e = eps * direction
+    e4 = eps if variant == "reference" else e
     k1 = -e * np.conj(action.force(phi))
     k2 = -e * np.conj(action.force(phi + b[0] * k1))
     k3 = -e * np.conj(action.force(phi + b[1] * k1 + b[2] * k2))
-    k4 = -e * np.conj(action.force(phi + b[3] * k1 + b[4] * k2 + b[5] * k3))
+    k4 = -e4 * np.conj(action.force(phi + b[3] * k1 + b[4] * k2 + b[5] * k3))
     out = phi + w[0] * k1 + w[1] * k2 + w[2] * k3 + w[3] * k4
     return out, action.eval(out)
 
 
 def rk4_flow(phi, action, flow_time, step, act_val=None, n=0, direction=1.0, attenuation=0.9,
-             threshold=1e-8, min_step=None, tolerance=1e-3):
+             threshold=1e-8, min_step=None, tolerance=1e-3, variant="reference"):
     """integrator.cpp:144-193: adaptive RK4 holomorphic flow; a step is accepted when
     |e^{i (Im S_old - Im S_new)} - 1| <= tolerance, the step size shrinks by
     attenuation (tol/err)^(1/5) (not below min_step) after a rejection and grows by
     min(max(attenuation (tol/err)^(1/4), 1), 2) after an error below threshold*tol.
-    Returns (phi, action value, flow time reached)."""
+    Returns (phi, action value, flow time reached).  ``variant`` selects the stage
+    coefficients of _rk4_step."""
     phi = np.asarray(phi, dtype=np.complex128)
     if act_val is None:
         act_val = action.eval(phi)
@@ -680,7 +687,7 @@ def rk4_flow(phi, action, flow_time, step, act_val=None, n=0, direction=1.0, att
             step = flow_time - t
             if step < min_step:
                 break
-        new_phi, new_act = _rk4_step(phi, action, step, direction, n)
+        new_phi, new_act = _rk4_step(phi, action, step, direction, n, variant)
         err = abs(np.exp(1j * (np.imag(act_val) - np.imag(new_act))) - 1.0)
         if err > tolerance:
             if step == min_step:

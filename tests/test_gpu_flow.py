@@ -7,7 +7,8 @@ in tests/test_oracle_flow.py.
 
 * Fixed steps (a tolerance no step misses, no growth): five steps of 0.01; the GPU flow
   (force and action from the HIP kernels) and the oracle's end at the same field and
-  action to 1e-9, for the classic and 3/8-rule steps and for the backward direction.  (The
+  action to 1e-9, for the classic and 3/8-rule steps in both directions, with the
+  reference's stage coefficients (the default) and with the textbook ones.  (The
   force itself agrees to 1e-11 at the start field; the 3/8 rule's stages combine
   differences of stage slopes, which raises the rounding difference to ~1e-10.)
 * Adaptive steps (tolerance 1e-3): the step sizes depend continuously on the measured
@@ -44,14 +45,16 @@ def _start(lat):
     return rs.normal(0, 0.5, lat.lattSize()) + 0.05j * rs.normal(0, 1, lat.lattSize())
 
 
-@pytest.mark.parametrize("n,direction", [(0, 1.0), (1, 1.0), (0, -1.0)])
-def test_fixed_step_flow_matches_oracle(n, direction):
-    print(f"rk4 flow, fixed steps, n={n} direction={direction}, Nx=18 Nt=16", flush=True)
+@pytest.mark.parametrize("variant", ["reference", "textbook"])
+@pytest.mark.parametrize("n,direction", [(0, 1.0), (1, 1.0), (0, -1.0), (1, -1.0)])
+def test_fixed_step_flow_matches_oracle(n, direction, variant):
+    print(f"rk4 flow, fixed steps, n={n} direction={direction} variant={variant}, Nx=18 Nt=16", flush=True)
     lat, gpu, cpu = _actions()
     phi = _start(lat)
     phi_g, act_g, t_g = isle.rungeKutta4Flow(phi, gpu, 0.05, 0.01, n=n, direction=direction, imActTolerance=10.0,
-                                             adaptThreshold=0.0)
-    phi_c, act_c, t_c = ref.rk4_flow(phi, cpu, 0.05, 0.01, n=n, direction=direction, tolerance=10.0, threshold=0.0)
+                                             adaptThreshold=0.0, variant=variant)
+    phi_c, act_c, t_c = ref.rk4_flow(phi, cpu, 0.05, 0.01, n=n, direction=direction, tolerance=10.0, threshold=0.0,
+                                     variant=variant)
     assert t_g == pytest.approx(0.05, abs=1e-15) and t_c == pytest.approx(0.05, abs=1e-15)
     phi_g = phi_g.cpu().numpy()
     assert np.abs(phi_g - phi_c).max() < 1e-9 * np.abs(phi_c).max()
