@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <mutex>
 
 #include "../../include/isle_hfa.h"
@@ -1712,12 +1713,16 @@ constexpr size_t DENSE_CACHE_MAX = size_t(64) << 20;
 std::mutex dense_mu;
 std::map<int, std::pair<void*, size_t>> dense_cache;  // device -> (buffer, bytes), kept to exit
 
+// Device scratch of one dense LU / inverse call, held under dense_mu for the call.  The
+// per-device cache is shared by every call: if the call throws after enqueueing work on
+// `s`, the destructor waits for that work before the lock is released, so the next call
+// (on any stream) cannot overlap kernels still writing the buffer.
 struct DenseScratch {
     double *m, *w, *ld;
     int* piv;
     isle::LuItem* item;
     isle::SolveItem* sitem;
-    explicit DenseScratch(int n) : lock_(dense_mu) {
+    DenseScratch(int n, hipStream_t s) : lock_(dense_mu), stream_(s), unwinding_(std::uncaught_exceptions()) {
         auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
         const size_t nn = (size_t)n * n;
         size_t off[6], o = 0;
@@ -1751,8 +1756,14 @@ struct DenseScratch {
         ld = reinterpret_cast<double*>(base + off[5]);
     }
 
+    ~DenseScratch() {
+        if (std::uncaught_exceptions() > unwinding_) (void)hipStreamSynchronize(stream_);
+    }
+
 private:
     std::unique_lock<std::mutex> lock_;
+    hipStream_t stream_;
+    int unwinding_;
     std::unique_ptr<isle::DevBuf> own_;
 };
 
@@ -1947,9 +1958,10 @@ int isle_cscale(const double* in, double* out, long n, double cr, double ci, voi
 int isle_logdet(const double* a, int n, double* out_host, void* stream) {
     return guarded([&] {
         if (n < 1) throw std::invalid_argument("empty matrix");
+        if (n > isle::LU_MAX_N) throw std::invalid_argument("matrix too large for the LU kernel (n > 512)");
         hipStream_t s = (hipStream_t)stream;
         const long nn = (long)n * n;
-        DenseScratch sc(n);
+        DenseScratch sc(n, s);
         hipLaunchKernelGGL(isle::interleaved_to_split_kernel, dim3(isle::ew_blocks(nn)), dim3(256), 0, s, a, sc.m,
                            sc.m + nn, nn);
         ISLE_HIP(hipGetLastError());
@@ -1965,9 +1977,10 @@ int isle_logdet(const double* a, int n, double* out_host, void* stream) {
 static int invert_impl(const double* a, int n, double* out, double* logdet_host, void* stream) {
     return guarded([&] {
         if (n < 1) throw std::invalid_argument("empty matrix");
+        if (n > isle::LU_MAX_N) throw std::invalid_argument("matrix too large for the LU kernel (n > 512)");
         hipStream_t s = (hipStream_t)stream;
         const long nn = (long)n * n;
-        DenseScratch sc(n);
+        DenseScratch sc(n, s);
         hipLaunchKernelGGL(isle::interleaved_to_split_kernel, dim3(isle::ew_blocks(nn)), dim3(256), 0, s, a, sc.m,
                            sc.m + nn, nn);
         ISLE_HIP(hipGetLastError());

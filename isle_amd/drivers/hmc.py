@@ -1,22 +1,23 @@
-"""HMC driver with run files: trajectories, checkpoints and resumption
-(src/isle/drivers/hmc.py:13-350).
+"""HMC runs that write their chain into a run directory and resume from it (the driver of
+src/isle/drivers/hmc.py:13-350; the directory format is described in isle_amd.runfile).
 
-``newRun(lattice, params, rng, makeAction, outfile, overwrite)`` starts a run and
-``continueRun(infile, outfile, startIdx, overwrite)`` resumes one from a checkpoint,
-returning the driver, the stage, the evolver and the save and checkpoint strides found
-in the file.  Calling the driver, ``hmc(stage, evolver, ntr, saveFreq, checkpointFreq)``,
-evolves ``ntr`` trajectories, saving the configuration every ``saveFreq`` trajectories
-and a checkpoint (RNG state, evolver) every ``checkpointFreq`` (a multiple of
-saveFreq; 0 disables either).  A run resumed from checkpoint i continues exactly as the
-uninterrupted run would have from trajectory i: the RNG state, configuration, action
-value and evolver state are those at the end of trajectory i.
+* ``newRun(lattice, params, rng, makeAction, outfile, overwrite)`` creates the run
+  directory and returns an ``HMC`` driver at trajectory 0;
+* ``continueRun(infile, outfile, startIdx, overwrite)`` returns (driver, stage, evolver,
+  save stride, checkpoint stride) restored from checkpoint ``startIdx`` (negative counts
+  back from the newest checkpoint, -1 being the newest);
+* ``driver(stage, evolver, ntr, saveFreq, checkpointFreq)`` evolves ``ntr`` trajectories
+  and writes configuration i when i % saveFreq == 0 and, with it, checkpoint i (generator
+  state, evolver state) when i % checkpointFreq == 0; 0 turns either off.
 
-``BatchedHMC`` does the same for independent chains advanced together by a
-BatchedConstStepLeapfrog, one run directory (and one RNG) per chain.
+A run resumed at checkpoint i continues as the uninterrupted run would have from
+trajectory i, because the checkpoint holds everything the next trajectory draws on.
 
-The run format is described in isle_amd.runfile.
+``BatchedHMC`` drives independent chains advanced together by a BatchedConstStepLeapfrog;
+every chain writes an ordinary single-chain run directory of its own.
 """
 import inspect
+import itertools
 from logging import getLogger
 
 import numpy as np
@@ -24,9 +25,41 @@ import numpy as np
 from .. import runfile
 from ..evolver import BatchedConstStepLeapfrog, ConstStepLeapfrog, EvolutionStage, readStateH5
 
+_log = getLogger(__name__)
+
+
+class _Cadence:
+    """Which trajectory indices get a configuration and which also a checkpoint."""
+
+    def __init__(self, saveFreq, checkpointFreq):
+        self.save_every = int(saveFreq)
+        self.checkpoint_every = int(checkpointFreq)
+        if self.checkpoint_every and (not self.save_every or self.checkpoint_every % self.save_every):
+            raise ValueError(f"a checkpoint every {self.checkpoint_every} trajectories needs the configuration of "
+                             f"that trajectory, but configurations are written every {self.save_every}")
+
+    def at(self, idx):
+        """(write the configuration, write a checkpoint) after trajectory ``idx``."""
+        config = self.save_every != 0 and idx % self.save_every == 0
+        return config, config and self.checkpoint_every != 0 and idx % self.checkpoint_every == 0
+
+
+def _trajectory_budget(ntr, maxNtr):
+    """Iterations of a run: ``ntr`` of them, or with ntr None at most ``maxNtr`` (at least
+    one), or without end when maxNtr is None too."""
+    if ntr is not None:
+        return range(ntr)
+    if maxNtr is None:
+        return itertools.count()
+    return range(max(int(maxNtr), 1))
+
 
 class HMC:
-    """Driver that evolves a configuration and writes it to a run directory (hmc.py:13-146)."""
+    """Evolves a Markov chain and writes it into the run directory ``outfname``.
+
+    ``trajIdx`` is the index of the last trajectory made (``startIdx`` at first); the
+    evolver manager stores evolvers in checkpoints (built from ``definitions`` when not
+    given)."""
 
     def __init__(self, lattice, params, rng, action, outfname, startIdx, definitions={}, evManager=None):
         self.lattice = lattice
@@ -34,56 +67,58 @@ class HMC:
         self.rng = rng
         self.action = action
         self.outfname = str(outfname)
-        self._trajIdx = startIdx
-        self._evManager = evManager if evManager else runfile.EvolverManager(outfname, definitions=definitions)
+        self._index = int(startIdx)
+        self._manager = evManager or runfile.EvolverManager(outfname, definitions=definitions)
+
+    @property
+    def trajIdx(self):
+        return self._index
+
+    def advance(self, amount=1):
+        """Count ``amount`` more trajectories without writing anything."""
+        self._index += amount
+
+    def resetIndex(self, idx=0):
+        self._index = idx
+
+    def save(self, stage):
+        """Write ``stage`` as the configuration of the current trajectory."""
+        runfile.writeTrajectory(self.outfname, self._index, stage)
+
+    def saveFieldAndCheckpoint(self, stage, evolver):
+        """Write the configuration and a checkpoint that refers to it."""
+        cfg = runfile.writeTrajectory(self.outfname, self._index, stage)
+        runfile.writeCheckpoint(self.outfname, self._index, self.rng, cfg, evolver, self._manager)
 
     def __call__(self, stage, evolver, ntr, saveFreq, checkpointFreq, maxNtr=None):
-        """Evolve ``ntr`` trajectories (``None``: until the evolver raises StopIteration or
-        ``maxNtr`` is reached) and return the last stage (hmc.py:41-84)."""
-        if checkpointFreq != 0 and checkpointFreq % saveFreq != 0:
-            getLogger(__name__).error("checkpointFreq must be a multiple of saveFreq."
-                                      " Got %d and %d, resp.", checkpointFreq, saveFreq)
-            raise ValueError("checkpointFreq must be a multiple of saveFreq.")
+        """Evolve ``ntr`` trajectories (ntr None: until the evolver raises StopIteration or
+        ``maxNtr`` are done) and return the final stage.  A field passed as ``stage`` is
+        evaluated with the driver's action first."""
+        cadence = _Cadence(saveFreq, checkpointFreq)
         if not isinstance(stage, EvolutionStage):
             stage = EvolutionStage(stage, self.action.eval(stage), 1)
-        for _ in _iterTrajectories(ntr, maxNtr):
+        budget = _trajectory_budget(ntr, maxNtr)
+        done = 0
+        for _ in budget:
             try:
                 stage = evolver.evolve(stage)
             except StopIteration:
                 break
-            self.advance()
-            self._saveConditionally(stage, evolver, saveFreq, checkpointFreq)
-        getLogger(__name__).info("Evolution stopped, evolver:\n%s", evolver.report())
+            done += 1
+            self._index += 1
+            config, checkpoint = cadence.at(self._index)
+            if checkpoint:
+                self.saveFieldAndCheckpoint(stage, evolver)
+            elif config:
+                self.save(stage)
+        if ntr is None and isinstance(budget, range) and done == len(budget):
+            _log.warning("open-ended HMC run stopped at its limit of %d trajectories", done)
+        _log.info("HMC run at trajectory %d after %d new ones; %s", self._index, done, evolver.report())
         return stage
 
-    def saveFieldAndCheckpoint(self, stage, evolver):
-        """Write the trajectory endpoint and a checkpoint."""
-        name = runfile.writeTrajectory(self.outfname, self._trajIdx, stage)
-        runfile.writeCheckpoint(self.outfname, self._trajIdx, self.rng, name, evolver, self._evManager)
 
-    def save(self, stage):
-        """Write the trajectory endpoint."""
-        runfile.writeTrajectory(self.outfname, self._trajIdx, stage)
-
-    def advance(self, amount=1):
-        """Advance the trajectory counter without saving."""
-        self._trajIdx += amount
-
-    def resetIndex(self, idx=0):
-        """Set the trajectory counter to ``idx``."""
-        self._trajIdx = idx
-
-    @property
-    def trajIdx(self):
-        return self._trajIdx
-
-    def _saveConditionally(self, stage, evolver, saveFreq, checkpointFreq):
-        if saveFreq != 0 and self._trajIdx % saveFreq == 0:
-            if checkpointFreq != 0 and self._trajIdx % checkpointFreq == 0:
-                self.saveFieldAndCheckpoint(stage, evolver)
-            else:
-                self.save(stage)
-
+# ---------------------------------------------------------------------------------------
+# the action of a run: built from makeAction, its source kept in the run's metadata
 
 def _source_of(makeAction):
     if isinstance(makeAction, str):
@@ -105,107 +140,102 @@ def _function_from_source(src):
     return funcs[0]
 
 
+def _build_action(makeAction, lattice, params):
+    build = makeAction if callable(makeAction) else _function_from_source(makeAction)
+    return build(lattice, params)
+
+
+def _stored_run(fname, makeAction):
+    """(lattice, params, action source, action) of an existing run directory."""
+    lattice, params, source, _versions = runfile.readMetadata(fname)
+    if makeAction is None:
+        if not source:
+            raise ValueError(f"run {fname} keeps no source for its action; pass makeAction")
+        makeAction = source
+    return lattice, params, source, _build_action(makeAction, lattice, params)
+
+
+# ---------------------------------------------------------------------------------------
+# starting and resuming
+
 def newRun(lattice, params, rng, makeAction, outfile, overwrite, definitions={}):
-    """Start a run: create the run directory with its metadata and return the driver
-    (hmc.py:149-179).  ``makeAction(lattice, params)`` builds the action; its source is
-    stored so that continueRun can rebuild it."""
+    """Create the run directory ``outfile`` (replacing an existing run only if
+    ``overwrite``) and return a driver at trajectory 0.  ``makeAction(lattice, params)``
+    builds the action; its source goes into the run so that continueRun can rebuild it."""
     if outfile is None:
-        getLogger(__name__).error("No output file given for HMC driver")
-        raise ValueError("No output file")
+        raise ValueError("newRun needs a run directory to write to (outfile is None)")
     runfile.initializeNewFile(outfile, overwrite, lattice, params, _source_of(makeAction))
-    action = makeAction(lattice, params) if callable(makeAction) else \
-        _function_from_source(makeAction)(lattice, params)
-    return HMC(lattice, params, rng, action, outfile, 0, definitions)
+    return HMC(lattice, params, rng, _build_action(makeAction, lattice, params), outfile, 0, definitions)
+
+
+def _last_gap(indices):
+    """Difference of the two newest indices, None with fewer than two."""
+    return indices[-1] - indices[-2] if len(indices) > 1 else None
+
+
+def _checkpoint_index(requested, available):
+    """The checkpoint to resume from; negative ``requested`` counts back from the newest."""
+    if not available:
+        raise ValueError("the run holds no checkpoint to resume from")
+    newest = available[-1]
+    idx = newest + 1 + requested if requested < 0 else requested
+    if idx in available:
+        return idx
+    if 0 <= idx <= newest:
+        reason = f"no checkpoint was written at trajectory {idx}"
+    else:
+        reason = f"trajectory {idx} is outside the checkpointed range 0..{newest}"
+    _log.error("cannot resume: %s", reason)
+    raise ValueError(f"cannot resume: {reason}")
+
+
+def _drop_later_records(fname, start, overwrite):
+    """Checkpoints and configurations after trajectory ``start`` would be rewritten by the
+    resumed run: refuse, or with ``overwrite`` delete them."""
+    for group in ("checkpoint", "configuration"):
+        later = [i for i in runfile.indices(fname, group) if i > start]
+        if not later:
+            continue
+        what = f"{fname} has {len(later)} {group} record(s) after trajectory {start} (newest {later[-1]})"
+        if not overwrite:
+            _log.error("%s", what)
+            raise RuntimeError(f"{what}; resume from the newest checkpoint or pass overwrite=True")
+        _log.warning("%s; deleting them", what)
+        runfile.removeGreaterThan(fname, group, start)
 
 
 def continueRun(infile, outfile, startIdx, overwrite, definitions={}, makeAction=None):
-    """Resume a run from checkpoint ``startIdx`` (negative: counted from the last
-    checkpoint) (hmc.py:182-323).
+    """Resume the run ``infile`` at checkpoint ``startIdx``.
 
-    Returns (driver, stage, evolver, saveFreq, checkpointFreq); the frequencies are the
-    strides of the last two stored configurations / checkpoints, or None.  The action is
-    ``makeAction(lattice, params)`` when given, else it is rebuilt from the source stored
-    in the run's metadata (which executes that source, as the reference does).
+    Returns (driver, stage, evolver, saveFreq, checkpointFreq): the frequencies are the
+    gaps between the two newest configurations / checkpoints (None with fewer than two).
+    The action is ``makeAction(lattice, params)`` when given, else it is rebuilt from the
+    source kept in the run (executing that source, as the reference does).
 
-    With ``outfile`` None or equal to ``infile``, configurations and checkpoints newer than
-    the start are an error unless ``overwrite``, which deletes them; with another
-    ``outfile``, the new run directory is created (replacing an existing run only if
-    ``overwrite``)."""
+    With ``outfile`` None or equal to ``infile`` the run continues in place: records newer
+    than the start are an error, or deleted with ``overwrite``.  With another ``outfile`` a
+    new run directory is created there (replacing an existing run only if ``overwrite``)."""
     if infile is None:
-        getLogger(__name__).error("No input file given for HMC driver in continuation run")
-        raise ValueError("No input file")
-    if outfile is None:
-        outfile = infile
-    lattice, params, makeActionSrc, _versions = runfile.readMetadata(infile)
-    if makeAction is None:
-        if not makeActionSrc:
-            raise ValueError("the run stores no action source; pass makeAction")
-        makeAction = _function_from_source(makeActionSrc)
-    action = makeAction(lattice, params)
-    same = str(outfile) == str(infile)
-    if not same:
-        runfile.initializeNewFile(outfile, overwrite, lattice, params, makeActionSrc)
-
-    configurations = runfile.indices(infile, "configuration")
+        raise ValueError("continueRun needs the run directory to resume (infile is None)")
+    target = infile if outfile is None else outfile
+    in_place = str(target) == str(infile)
+    lattice, params, source, action = _stored_run(infile, makeAction)
+    if not in_place:
+        runfile.initializeNewFile(target, overwrite, lattice, params, source)
+    configs = runfile.indices(infile, "configuration")
     checkpoints = runfile.indices(infile, "checkpoint")
-    evManager = runfile.EvolverManager(infile, definitions=definitions)
-    checkpointIdx, rng, stage, evolver = _loadCheckpoint(infile, startIdx, checkpoints, evManager, action, lattice)
-    if same:
-        _ensureNoNewerConfigs(infile, checkpointIdx, checkpoints, configurations, overwrite)
-    driver = HMC(lattice, params, rng, action, outfile, checkpointIdx, definitions, evManager if same else None)
-    return driver, stage, evolver, _stride(configurations), _stride(checkpoints)
+    manager = runfile.EvolverManager(infile, definitions=definitions)
+    start = _checkpoint_index(startIdx, checkpoints)
+    rng, cfg, evolver = runfile.loadCheckpoint(infile, start, manager, action, lattice)
+    stage = EvolutionStage.fromH5(cfg)
+    if in_place:
+        _drop_later_records(infile, start, overwrite)
+    driver = HMC(lattice, params, rng, action, target, start, definitions, manager if in_place else None)
+    return driver, stage, evolver, _last_gap(configs), _last_gap(checkpoints)
 
 
-def _stride(values):
-    return values[-1] - values[-2] if len(values) >= 2 else None
-
-
-def _resolve_start(startIdx, checkpoints):
-    if not checkpoints:
-        raise ValueError("The run has no checkpoint to continue from")
-    if startIdx < 0:
-        startIdx = checkpoints[-1] + (startIdx + 1)  # -1 is the last checkpoint
-    if startIdx < 0 or startIdx > checkpoints[-1]:
-        getLogger(__name__).error("Start index for HMC continuation is out of range: %d", startIdx)
-        raise ValueError("Start index out of range")
-    if startIdx not in checkpoints:
-        getLogger(__name__).error("There is no checkpoint matching the given start index: %d", startIdx)
-        raise ValueError("No checkpoint matching start index")
-    return startIdx
-
-
-def _loadCheckpoint(fname, startIdx, checkpoints, evManager, action, lattice):
-    startIdx = _resolve_start(startIdx, checkpoints)
-    rng, cfg, evolver = runfile.loadCheckpoint(fname, startIdx, evManager, action, lattice)
-    return startIdx, rng, EvolutionStage.fromH5(cfg), evolver
-
-
-def _ensureNoNewerConfigs(fname, checkpointIdx, checkpoints, configurations, overwrite):
-    """Refuse (or, with ``overwrite``, delete) checkpoints and configurations newer than the
-    start (hmc.py:262-288)."""
-    for group, idxs in (("checkpoint", checkpoints), ("configuration", configurations)):
-        if idxs and idxs[-1] > checkpointIdx:
-            message = (f"Output file {fname} contains {group}s with greater index than HMC starting point.\n"
-                       f"    Greatest index is {idxs[-1]}, start index is {checkpointIdx}.")
-            if not overwrite:
-                getLogger(__name__).error(message)
-                raise RuntimeError("HMC start index is not latest")
-            getLogger(__name__).warning(message + "\n    Overwriting")
-            runfile.removeGreaterThan(fname, group, checkpointIdx)
-
-
-def _iterTrajectories(ntr, maxNtr):
-    if ntr is not None:
-        yield from range(ntr)
-        return
-    count = 0
-    while True:
-        yield count
-        count += 1
-        if maxNtr is not None and count >= maxNtr:
-            getLogger(__name__).warning("Reached maximum number of trajectories for 'unbounded' evolution.")
-            return
-
+# ---------------------------------------------------------------------------------------
+# independent chains advanced together
 
 class BatchedHMC:
     """HMC over independent chains advanced together, chain b written to ``outfnames[b]``
@@ -220,34 +250,35 @@ class BatchedHMC:
         self.outfnames = [str(f) for f in outfnames]
         if len(self.outfnames) != len(self.rngs):
             raise ValueError("one output run per chain")
-        self._trajIdx = startIdx
-        self._evManager = runfile.EvolverManager()
+        self._index = startIdx
+        self._manager = runfile.EvolverManager()
 
     @property
     def trajIdx(self):
-        return self._trajIdx
+        return self._index
+
+    def _write(self, phis, actVals, points, evolver, checkpoint):
+        host = phis.detach().cpu().numpy()
+        for b, fname in enumerate(self.outfnames):
+            cfg = runfile.writeTrajectory(fname, self._index, EvolutionStage(host[b], actVals[b], points[b]))
+            if checkpoint:  # recorded as the ConstStepLeapfrog that would run this chain alone
+                alone = ConstStepLeapfrog(self.action, evolver.length, evolver.nstep, self.rngs[b])
+                runfile.writeCheckpoint(fname, self._index, self.rngs[b], cfg, alone, self._manager)
 
     def __call__(self, phis, actVals, evolver, ntr, saveFreq, checkpointFreq):
         """Evolve the chains ([batch, nx*nt] device fields, their action values) ``ntr``
         trajectories; returns (phis, actVals, trajPoints of the last trajectory)."""
-        if checkpointFreq != 0 and checkpointFreq % saveFreq != 0:
-            raise ValueError("checkpointFreq must be a multiple of saveFreq.")
+        cadence = _Cadence(saveFreq, checkpointFreq)
         if not isinstance(evolver, BatchedConstStepLeapfrog) or evolver.rngs != self.rngs:
             raise ValueError("the evolver must be a BatchedConstStepLeapfrog drawing from this driver's rngs")
         points = [1] * len(self.rngs)
         actVals = np.asarray(actVals, dtype=complex)
         for _ in range(ntr):
             phis, actVals, points = evolver.evolve(phis, actVals)
-            self._trajIdx += 1
-            if saveFreq != 0 and self._trajIdx % saveFreq == 0:
-                host = phis.detach().cpu().numpy()
-                ckpt = checkpointFreq != 0 and self._trajIdx % checkpointFreq == 0
-                for b, fname in enumerate(self.outfnames):
-                    stage = EvolutionStage(host[b], actVals[b], points[b])
-                    name = runfile.writeTrajectory(fname, self._trajIdx, stage)
-                    if ckpt:  # recorded as the ConstStepLeapfrog that would run this chain alone
-                        single = ConstStepLeapfrog(self.action, evolver.length, evolver.nstep, self.rngs[b])
-                        runfile.writeCheckpoint(fname, self._trajIdx, self.rngs[b], name, single, self._evManager)
+            self._index += 1
+            config, checkpoint = cadence.at(self._index)
+            if config:
+                self._write(phis, actVals, points, evolver, checkpoint)
         return phis, actVals, points
 
 
@@ -256,38 +287,31 @@ def newBatchedRun(lattice, params, rngs, makeAction, outfiles, overwrite):
     src = _source_of(makeAction)
     for fname in outfiles:
         runfile.initializeNewFile(fname, overwrite, lattice, params, src)
-    action = makeAction(lattice, params) if callable(makeAction) else _function_from_source(makeAction)(lattice, params)
-    return BatchedHMC(lattice, params, rngs, action, outfiles, 0)
+    return BatchedHMC(lattice, params, rngs, _build_action(makeAction, lattice, params), outfiles, 0)
 
 
 def continueBatchedRun(infiles, startIdx, overwrite, makeAction=None, graph=False, split=None):
     """Resume every chain of a batched run from checkpoint ``startIdx`` (negative: from the
     last common checkpoint).  Returns (driver, phis [batch, nx*nt] numpy, actVals,
     evolver): the evolver is a BatchedConstStepLeapfrog over the restored generators."""
-    lattice, params, makeActionSrc, _ = runfile.readMetadata(infiles[0])
-    if makeAction is None:
-        if not makeActionSrc:
-            raise ValueError("the run stores no action source; pass makeAction")
-        makeAction = _function_from_source(makeActionSrc)
-    action = makeAction(lattice, params)
+    lattice, params, _source, action = _stored_run(infiles[0], makeAction)
     common = sorted(set.intersection(*[set(runfile.indices(f, "checkpoint")) for f in infiles]))
-    start = _resolve_start(startIdx, common)
-    phis, acts, rngs, lengths = [], [], [], set()
+    start = _checkpoint_index(startIdx, common)
+    phis, acts, rngs, settings = [], [], [], set()
     for fname in infiles:
         grp = runfile.readCheckpoint(fname, start)
         rngs.append(readStateH5(grp["rngState"]))
         ev = grp["evolver"]
         if str(ev["__name__"][()]) != "ConstStepLeapfrog":
             raise ValueError("batched runs resume ConstStepLeapfrog checkpoints only")
-        lengths.add((float(ev["length"][()]), int(ev["nstep"][()])))
+        settings.add((float(ev["length"][()]), int(ev["nstep"][()])))
         stage = EvolutionStage.fromH5(runfile.readTrajectory(fname, start))
         phis.append(stage.phi)
         acts.append(stage.actVal)
-        _ensureNoNewerConfigs(fname, start, runfile.indices(fname, "checkpoint"),
-                              runfile.indices(fname, "configuration"), overwrite)
-    if len(lengths) != 1:
+        _drop_later_records(fname, start, overwrite)
+    if len(settings) != 1:
         raise ValueError("the chains' checkpoints disagree on the trajectory length or step count")
-    length, nstep = lengths.pop()
+    length, nstep = settings.pop()
     evolver = BatchedConstStepLeapfrog(action, length, nstep, rngs, graph=graph, split=split)
     driver = BatchedHMC(lattice, params, rngs, action, infiles, start)
     return driver, np.stack(phis), np.array(acts), evolver

@@ -1,92 +1,105 @@
-"""HMC evolution: EvolutionStage, BinarySelector, ConstStepLeapfrog, NumpyRNG and the
-trajectory loop, with the reference's semantics and random-number consumption.
+"""Markov-chain evolution over the GPU action: generators, chain states, the Metropolis
+choice, leapfrog HMC evolvers, 2 pi jump evolvers and batched chains.
 
-* ``NumpyRNG`` wraps numpy's MT19937 RandomState (src/isle/random.py:80-185), its state
-  stored in a run-file group by ``writeH5`` / ``readH5`` / ``readStateH5``;
-* ``EvolutionStage`` (src/isle/evolver/stage.py:10-117) with ``save`` / ``fromH5``;
-* ``BinarySelector.selectTrajPoint``: accept iff dE < 0 or exp(-dE) > uniform, the
-  uniform drawn only when dE >= 0 (src/isle/evolver/selector.py:20-30);
-* ``ConstStepLeapfrog.evolve``: pi = rng.normal(0, 1, N) + 0j, leapfrog, Metropolis with
-  E = sum(logWeights) + |pi|^2/2 (src/isle/evolver/leapfrog.py:38-66);
-* ``LinearStepLeapfrog``: the same with length and step count moving linearly between
-  two values over ``ninterp`` trajectories (leapfrog.py:105-229); both evolvers store
-  their state in checkpoints (``save`` / ``fromH5``);
-* ``hmc``: the HMC loop of src/isle/drivers/hmc.py:41-84 without HDF5 output; a start
-  configuration that is not a stage is evaluated first (hmc.py:64-66).
+Public names follow the reference's evolver package (src/isle/evolver/, src/isle/random.py)
+so that driver code written against it runs unchanged; the random numbers are consumed
+in the reference's order, which fixed-seed parity of the accept decisions needs:
 
-``TwoPiJumps`` and ``UniformJump`` (src/isle/evolver/hubbard.py:15-184) shift the field by
-2 pi at random sites or by 2 pi / Nt everywhere; the particle-hole fermion action is
-invariant under both, so with only gauge and such fermion actions the new action value
-is the old one plus the change of the gauge action (no evaluation on the GPU).
+* one trajectory draws its momenta rng.normal(0, 1, N) (real, as complex with zero
+  imaginary part) and then, only if the energy rises, one rng.uniform(0, 1);
+* a 2 pi jump batch draws its sites (rng.choice without replacement), then its signs,
+  then the Metropolis uniform as above.
 
-``BatchedConstStepLeapfrog`` advances many independent chains together (one field tensor
-[batch, nx*nt]); each chain has its own NumpyRNG, consumed exactly as a lone chain would.
+Fields live on the GPU during a trajectory; stages may hold numpy vectors or device
+tensors.  ``BatchedConstStepLeapfrog`` advances many independent chains in one set of
+launches, each with its own generator consumed exactly as a lone chain would consume it.
 """
+import math
 from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
 
+from . import transform
 from .core import as_field
 from .integrator import LeapfrogGraph, eval_batch as _eval_batch, leapfrog_batch
+from .transform import pull_back, push_forward
 
+
+# ---------------------------------------------------------------------------------------
+# random numbers
 
 class NumpyRNG:
-    """MT19937 via numpy.random.RandomState (src/isle/random.py:80-124)."""
+    """Mersenne Twister from numpy's legacy ``RandomState``: a seed gives the stream of the
+    reference's NumpyRNG (src/isle/random.py:80-185).
+
+    ``uniform`` and ``normal`` take (first parameter, second parameter, size, cmplx); a
+    complex draw is the real parts followed by the imaginary parts, ``size`` numbers each."""
 
     NAME = "np.MT19937"
+    _BITGEN = "MT19937"
 
     def __init__(self, seed):
-        self._state = np.random.RandomState(seed)
+        self._mt = np.random.RandomState(seed)
 
     def seed(self, seed):
-        self._state.seed(seed)
+        self._mt.seed(seed)
+
+    def _sample(self, method, a, b, size, cmplx):
+        draw = getattr(self._mt, method)
+        values = draw(a, b, size)
+        if cmplx:
+            values = values + 1j * draw(a, b, size)
+        return values
 
     def uniform(self, low=0.0, high=1.0, size=None, cmplx=False):
-        if cmplx:
-            return self._state.uniform(low, high, size) + 1j * self._state.uniform(low, high, size)
-        return self._state.uniform(low, high, size)
+        return self._sample("uniform", low, high, size, cmplx)
 
     def normal(self, mean=0.0, std=1.0, size=None, cmplx=False):
-        if cmplx:
-            return self._state.normal(mean, std, size) + 1j * self._state.normal(mean, std, size)
-        return self._state.normal(mean, std, size)
+        return self._sample("normal", mean, std, size, cmplx)
 
     def choice(self, a, size=None, replace=True, p=None):
-        return self._state.choice(a, size, replace, p)
+        return self._mt.choice(a, size=size, replace=replace, p=p)
 
     def get_state(self):
-        return self._state.get_state()
+        return self._mt.get_state()
 
     def set_state(self, state):
-        self._state.set_state(state)
+        self._mt.set_state(state)
 
     def writeH5(self, group):
-        """Store the generator state in a run-file group (random.py:140-154)."""
-        kind, keys, pos, has_gauss, cached_gaussian = self._state.get_state()
-        assert kind == self.NAME.rsplit(".")[-1]
+        """Put the generator into a checkpoint group: datasets ``name`` and ``keys`` (the
+        624 state words), attributes ``pos``, ``has_gauss`` and ``cached_gaussian``."""
+        snap = self._mt.get_state(legacy=False)
         group["name"] = self.NAME
-        group["keys"] = keys
-        group.attrs["pos"] = pos
-        group.attrs["has_gauss"] = has_gauss
-        group.attrs["cached_gaussian"] = cached_gaussian
+        group["keys"] = np.asarray(snap["state"]["key"], dtype=np.uint32)
+        group.attrs["pos"] = int(snap["state"]["pos"])
+        group.attrs["has_gauss"] = int(snap["has_gauss"])
+        group.attrs["cached_gaussian"] = float(snap["gauss"])
 
     def readH5(self, group):
-        """Restore the generator state written by writeH5 (random.py:156-166)."""
-        name = str(group["name"][()])
-        if name != self.NAME:
-            raise RuntimeError(f"Wrong kind of RNG. Expected {self.NAME}, got {name}")
-        self._state.set_state((self.NAME.rsplit(".")[-1], np.asarray(group["keys"][()], dtype=np.uint32),
-                               int(group.attrs["pos"]), int(group.attrs["has_gauss"]),
-                               float(group.attrs["cached_gaussian"])))
+        """Take the state stored by ``writeH5``."""
+        stored = str(group["name"][()])
+        if stored != self.NAME:
+            raise RuntimeError(f"checkpoint holds a {stored!r} generator; {self.NAME} cannot take its state")
+        attrs = group.attrs
+        self._mt.set_state({
+            "bit_generator": self._BITGEN,
+            "state": {"key": np.asarray(group["keys"][()], dtype=np.uint32), "pos": int(attrs["pos"])},
+            "has_gauss": int(attrs["has_gauss"]),
+            "gauss": float(attrs["cached_gaussian"]),
+        })
+
+
+_GENERATORS = {NumpyRNG.NAME: NumpyRNG}
 
 
 def readStateH5(group):
-    """A generator with the state stored in a run-file group (random.py:168-185)."""
-    name = str(group["name"][()])
-    if name != NumpyRNG.NAME:
-        raise ValueError(f"Unknown RNG in file: {name}")
-    rng = NumpyRNG(0)
+    """A new generator in the state a checkpoint group holds."""
+    kind = str(group["name"][()])
+    if kind not in _GENERATORS:
+        raise ValueError(f"no generator type is known by the name {kind!r}")
+    rng = _GENERATORS[kind](0)
     rng.readH5(group)
     return rng
 
@@ -98,125 +111,183 @@ def _host(x):
     return np.asarray(x, dtype=np.complex128)
 
 
-class EvolutionStage:
-    """Configuration + log weights at one point of the Markov chain (stage.py:10-68)."""
+# ---------------------------------------------------------------------------------------
+# chain states and the Metropolis choice
 
-    __slots__ = "phi", "trajPoint", "logWeights", "extra", "phi_RTP"
+class EvolutionStage:
+    """One point of a Markov chain (src/isle/evolver/stage.py:10-117).
+
+    ``phi`` is the field; ``logWeights`` maps names to log weights whose sum enters the
+    Metropolis energy, ``logWeights["actVal"]`` (also ``actVal``) being the action;
+    ``trajPoint`` is 1 if the step that produced the stage moved the chain, else 0;
+    ``extra`` holds any further data and ``phi_RTP`` an optional second field.
+
+    ``accept`` and ``reject`` build new stages; a new stage gets its own copy of the
+    weights, so later changes to one stage's weights never reach another."""
+
+    __slots__ = ("phi", "trajPoint", "logWeights", "extra", "phi_RTP")
 
     def __init__(self, phi, actVal, trajPoint=1, logWeights=None, extra=None, phi_RTP=None):
         self.phi = phi
-        self.phi_RTP = phi_RTP
         self.trajPoint = trajPoint
-        self.logWeights = logWeights if logWeights is not None else dict()
+        self.phi_RTP = phi_RTP
+        self.logWeights = dict(logWeights) if logWeights else {}
         self.logWeights["actVal"] = actVal
-        self.extra = extra if extra is not None else dict()
+        self.extra = dict(extra) if extra else {}
 
     @property
     def actVal(self):
         return self.logWeights["actVal"]
 
     @actVal.setter
-    def actVal(self, actVal):
-        self.logWeights["actVal"] = actVal
+    def actVal(self, value):
+        self.logWeights["actVal"] = value
 
     def accept(self, phi, actVal, logWeights=None, extra=None, phi_RTP=None):
-        return self.__class__(phi, actVal, 1, logWeights, extra, phi_RTP)
+        """The stage after a move to ``phi`` (trajPoint 1); weights other than the action
+        come from ``logWeights`` only."""
+        return type(self)(phi, actVal, 1, logWeights, extra, phi_RTP)
 
     def reject(self, extra=None):
-        return self.__class__(self.phi, self.actVal, 0, self.logWeights, extra, self.phi_RTP)
+        """The stage after a refused move: same field and weights, trajPoint 0."""
+        return type(self)(self.phi, self.actVal, 0, self.logWeights, extra, self.phi_RTP)
 
     def sumLogWeights(self):
         return sum(self.logWeights.values())
 
     def save(self, group):
-        """Store phi, actVal, trajPoint and any further log weights and extras in a
-        run-file group (stage.py:74-93)."""
+        """Write the stage into a run-file group: ``phi``, ``actVal``, ``trajPoint``,
+        ``phi_RT-P`` when set, ``logWeights/<name>`` for each weight besides the action and
+        ``extra/<name>``."""
         group["phi"] = _host(self.phi)
-        if self.phi_RTP is not None:
-            group["phi_RT-P"] = _host(self.phi_RTP)
         group["actVal"] = complex(self.actVal)
         group["trajPoint"] = int(self.trajPoint)
-        if len(self.logWeights) > 1:
-            weights = group.create_group("logWeights")
-            for key, val in self.logWeights.items():
-                if key != "actVal":
-                    weights[key] = complex(val)
-        if self.extra:
-            extra = group.create_group("extra")
-            for key, val in self.extra.items():
-                extra[key] = val
+        if self.phi_RTP is not None:
+            group["phi_RT-P"] = _host(self.phi_RTP)
+        subgroups = (("logWeights", {k: complex(v) for k, v in self.logWeights.items() if k != "actVal"}),
+                     ("extra", self.extra))
+        for title, entries in subgroups:
+            if entries:
+                sub = group.create_group(title)
+                for key, value in entries.items():
+                    sub[key] = value
 
     @classmethod
     def fromH5(cls, group):
-        """The stage stored by save; phi comes back as a numpy vector (stage.py:95-117)."""
-        extra = {k: v[()] for k, v in group["extra"].items()} if "extra" in group else None
-        weights = {k: complex(v[()]) for k, v in group["logWeights"].items()} if "logWeights" in group else None
-        phi_rtp = group["phi_RT-P"][()] if "phi_RT-P" in group else None
-        return cls(group["phi"][()], complex(group["actVal"][()]), int(group["trajPoint"][()]), weights, extra,
-                   phi_rtp)
+        """The stage that ``save`` wrote; fields come back as numpy vectors."""
+        def entries(title, convert):
+            if title not in group:
+                return None
+            return {key: convert(ds[()]) for key, ds in group[title].items()}
+        second = group["phi_RT-P"][()] if "phi_RT-P" in group else None
+        return cls(group["phi"][()], complex(group["actVal"][()]), int(group["trajPoint"][()]),
+                   entries("logWeights", complex), entries("extra", lambda v: v), second)
 
 
 class BinarySelector:
-    """Metropolis accept/reject (selector.py:9-44)."""
+    """Metropolis choice between the start (0) and the end (1) of a proposal
+    (src/isle/evolver/selector.py:9-44): the end is taken if its energy is lower;
+    otherwise one uniform u is drawn and the end is taken if u < exp(-dE), dE the real
+    part of the energy difference.  A downhill proposal draws nothing."""
 
     def __init__(self, rng):
         self.rng = rng
 
     def selectTrajPoint(self, energy0, energy1):
-        deltaE = np.real(energy1 - energy0)
-        return 1 if deltaE < 0 or np.exp(-deltaE) > self.rng.uniform(0, 1) else 0
+        rise = float(np.real(energy1 - energy0))
+        if rise < 0:
+            return 1
+        return int(self.rng.uniform(0, 1) < math.exp(-rise))
 
     def selectTrajectory(self, energy0, data0, energy1, data1):
-        return (energy1, data1, 1) if self.selectTrajPoint(energy0, energy1) == 1 else (energy0, data0, 0)
+        if self.selectTrajPoint(energy0, energy1):
+            return energy1, data1, 1
+        return energy0, data0, 0
 
 
-class ConstStepLeapfrog:
-    """Leapfrog evolver with constant length and step count (leapfrog.py:15-103)."""
+def _acceptance_line(name, ev, settings, decisions):
+    taken = int(np.sum(decisions)) if len(decisions) else 0
+    total = int(np.size(decisions))
+    rate = f"{taken / total:.4f}" if total else "n/a"
+    return f"{name} @0x{id(ev):x}: {settings}; accepted {taken} of {total} ({rate})"
 
-    def __init__(self, action, length, nstep, rng, transform=None):
-        if transform is not None:
-            raise NotImplementedError("transforms are not supported")
+
+# ---------------------------------------------------------------------------------------
+# leapfrog HMC
+
+class _LeapfrogHMC:
+    """One HMC trajectory per ``evolve`` (src/isle/evolver/leapfrog.py:38-66): the stage's
+    field is pulled back by the transform (if any), integrated by the GPU leapfrog with
+    fresh Gaussian momenta, pushed forward, and the end point is accepted or rejected on
+    E = sum of log weights (+ log det J) + |pi|^2 / 2.  Subclasses choose each
+    trajectory's (length, nstep).  ``trajPoints`` records the decisions and
+    ``last_energies`` the last trajectory's (E0, E1)."""
+
+    def __init__(self, action, rng, transform):
         self.action = action
-        self.length = length
-        self.nstep = nstep
         self.rng = rng
+        self.transform = transform
         self.selector = BinarySelector(rng)
-        self.transform = None
         self.trajPoints = []
         self.last_energies = None
 
+    def _trajectory_parameters(self):
+        raise NotImplementedError
+
     def evolve(self, stage):
-        return _leapfrog_trajectory(self, stage, self.length, self.nstep)
+        length, nstep = self._trajectory_parameters()
+        phi_md, logdet0 = pull_back(self.transform, stage)
+        if self.transform is not None:
+            stage.logWeights.setdefault("logdetJ", logdet0)
+        phi_md = as_field(phi_md).reshape(1, -1)
+        momenta = self.rng.normal(0, 1, phi_md.shape[1])
+        pi = torch.as_tensor(momenta + 0j, device=phi_md.device).reshape(1, -1)
+        ends, pi_end, acts = leapfrog_batch(phi_md, pi, self.action, length, nstep)
+        phi1, act1, logdet1 = push_forward(self.transform, ends[0], complex(acts[0]))
+        energy0 = stage.sumLogWeights() + float(np.dot(momenta, momenta)) / 2
+        energy1 = act1 + logdet1 + float((pi_end.real.square() + pi_end.imag.square()).sum()) / 2
+        self.last_energies = (energy0, energy1)
+        moved = self.selector.selectTrajPoint(energy0, energy1)
+        self.trajPoints.append(moved)
+        if not moved:
+            return stage.reject()
+        return stage.accept(phi1, act1, None if self.transform is None else {"logdetJ": logdet1})
+
+    def _save_transform(self, group, manager):
+        if self.transform is not None:
+            manager.save(self.transform, group.create_group("transform"))
+
+    @staticmethod
+    def _load_transform(group, manager, action, lattice, rng):
+        return manager.load(group["transform"], action, lattice, rng) if "transform" in group else None
+
+
+class ConstStepLeapfrog(_LeapfrogHMC):
+    """Every trajectory has length ``length`` in ``nstep`` leapfrog steps
+    (src/isle/evolver/leapfrog.py:15-103).  Checkpoints store ``length``, ``nstep`` and
+    the transform."""
+
+    def __init__(self, action, length, nstep, rng, transform=None):
+        super().__init__(action, rng, transform)
+        self.length = length
+        self.nstep = nstep
+
+    def _trajectory_parameters(self):
+        return self.length, self.nstep
 
     def save(self, group, manager):
-        """Store length and nstep in a checkpoint (leapfrog.py:68-77)."""
         group["length"] = self.length
         group["nstep"] = self.nstep
+        self._save_transform(group, manager)
 
     @classmethod
     def fromH5(cls, group, manager, action, lattice, rng):
-        return cls(action, float(group["length"][()]), int(group["nstep"][()]), rng)
+        return cls(action, float(group["length"][()]), int(group["nstep"][()]), rng,
+                   cls._load_transform(group, manager, action, lattice, rng))
 
     def report(self):
-        return f"""<ConstStepLeapfrog> (0x{id(self):x})
-  length = {self.length}, nstep = {self.nstep}
-  acceptance rate = {np.mean(self.trajPoints)}"""
-
-
-def _leapfrog_trajectory(ev, stage, length, nstep):
-    """One HMC trajectory of evolver ``ev``: momenta from ev.rng, leapfrog, Metropolis."""
-    phi = as_field(stage.phi)
-    n = phi.numel()
-    pi_host = ev.rng.normal(0, 1, n) + 0j
-    pi = torch.as_tensor(pi_host, device=phi.device)
-    phi1, pi1, acts = leapfrog_batch(phi.reshape(1, -1), pi.reshape(1, -1), ev.action, length, nstep)
-    actVal1 = complex(acts[0])
-    energy0 = stage.sumLogWeights() + np.linalg.norm(pi_host) ** 2 / 2
-    energy1 = actVal1 + 0 + np.linalg.norm(pi1.cpu().numpy()) ** 2 / 2
-    ev.last_energies = (energy0, energy1)
-    trajPoint = ev.selector.selectTrajPoint(energy0, energy1)
-    ev.trajPoints.append(trajPoint)
-    return stage.accept(phi1[0], actVal1) if trajPoint == 1 else stage.reject()
+        return _acceptance_line("ConstStepLeapfrog", self, f"length {self.length}, {self.nstep} steps",
+                                self.trajPoints)
 
 
 def _hinge_range(start, end, step):
@@ -230,7 +301,7 @@ def _hinge_range(start, end, step):
         yield end
 
 
-class LinearStepLeapfrog:
+class LinearStepLeapfrog(_LeapfrogHMC):
     """Leapfrog whose trajectory length and step count move linearly from the first to the
     second value of their ranges over ``ninterp`` trajectories and then stay at the second
     (leapfrog.py:105-229).  ``startPoint`` skips that many trajectories of the schedule.
@@ -240,49 +311,44 @@ class LinearStepLeapfrog:
     count at zero when it resumes, so a second resumption would rewind the schedule)."""
 
     def __init__(self, action, lengthRange, nstepRange, ninterp, rng, startPoint=0, transform=None):
-        if transform is not None:
-            raise NotImplementedError("transforms are not supported")
-        self.action = action
+        super().__init__(action, rng, transform)
         self.lengthRange = tuple(lengthRange)
         self.nstepRange = tuple(nstepRange)
         self.ninterp = ninterp
-        self.rng = rng
-        self.selector = BinarySelector(rng)
-        self.transform = None
-        self.trajPoints = []
-        self.last_energies = None
         self._lengths = _hinge_range(*self.lengthRange, (self.lengthRange[1] - self.lengthRange[0]) / ninterp)
         self._nsteps = _hinge_range(*self.nstepRange, (self.nstepRange[1] - self.nstepRange[0]) / ninterp)
         self._current = 0
         for _ in range(int(startPoint)):
-            next(self._lengths)
-            next(self._nsteps)
-            self._current += 1
+            self._trajectory_parameters()
 
-    def evolve(self, stage):
+    def _trajectory_parameters(self):
         self._current += 1
-        return _leapfrog_trajectory(self, stage, next(self._lengths), int(next(self._nsteps)))
+        return next(self._lengths), int(next(self._nsteps))
 
     def save(self, group, manager):
-        """Store the ranges and the position in the schedule (leapfrog.py:183-196)."""
-        group["minLength"] = self.lengthRange[0]
-        group["maxLength"] = self.lengthRange[1]
-        group["minNstep"] = self.nstepRange[0]
-        group["maxNstep"] = self.nstepRange[1]
-        group["ninterp"] = self.ninterp
-        group["current"] = self._current
+        """Ranges and schedule position under the names minLength, maxLength, minNstep,
+        maxNstep, ninterp and current (leapfrog.py:183-196)."""
+        for key, value in (("minLength", self.lengthRange[0]), ("maxLength", self.lengthRange[1]),
+                           ("minNstep", self.nstepRange[0]), ("maxNstep", self.nstepRange[1]),
+                           ("ninterp", self.ninterp), ("current", self._current)):
+            group[key] = value
+        self._save_transform(group, manager)
 
     @classmethod
     def fromH5(cls, group, manager, action, lattice, rng):
         return cls(action, (group["minLength"][()], group["maxLength"][()]),
                    (group["minNstep"][()], group["maxNstep"][()]), int(group["ninterp"][()]), rng,
-                   startPoint=int(group["current"][()]))
+                   startPoint=int(group["current"][()]),
+                   transform=cls._load_transform(group, manager, action, lattice, rng))
 
     def report(self):
-        return f"""<LinearStepLeapfrog> (0x{id(self):x})
-  lengthRange = {self.lengthRange}, nstepRange = {self.nstepRange}, ninterp = {self.ninterp}
-  acceptance rate = {np.mean(self.trajPoints)}"""
+        return _acceptance_line("LinearStepLeapfrog", self,
+                                f"length {self.lengthRange[0]} to {self.lengthRange[1]}, steps {self.nstepRange[0]} "
+                                f"to {self.nstepRange[1]} over {self.ninterp} trajectories", self.trajPoints)
 
+
+# ---------------------------------------------------------------------------------------
+# jumps by 2 pi
 
 class _HubbardActionShortcut:
     """Action values after jumps that leave the fermion determinant invariant
@@ -341,38 +407,43 @@ class _HubbardActionShortcut:
 
 
 class TwoPiJumps:
-    """Shift the field by +-2 pi at random sites, momenta untouched (hubbard.py:15-116).
-    Per batch: ``batchSize`` distinct sites rng.choice(N, batchSize, replace=False), signs
-    rng.choice((-2 pi, 2 pi), batchSize), then accept / reject of that batch alone.  The
-    returned stage holds the field after the last accepted batch (the reference returns the
-    last batch's proposal, accepted or not)."""
+    """Moves the field by +-2 pi at random sites, momenta untouched
+    (src/isle/evolver/hubbard.py:15-116).
+
+    One ``evolve`` makes ``nBatches`` proposals in turn.  A proposal picks ``batchSize``
+    distinct sites (rng.choice without replacement) and a sign for each (rng.choice of
+    -2 pi and +2 pi) and is accepted or rejected by itself on the change of the action.
+    The stage returned holds the field after the last accepted proposal (trajPoint 1 if
+    any was accepted); ``acceptedPerBatch`` records each evolve's decisions."""
 
     def __init__(self, nBatches, batchSize, action, lattice, rng):
         self.nBatches = int(nBatches)
         self.batchSize = int(batchSize)
         self.rng = rng
-        self.latSize = lattice.lattSize()
         self.selector = BinarySelector(rng)
-        self.acceptedPerBatch = []
+        self._nsites = lattice.lattSize()
         self._action = _HubbardActionShortcut(action)
+        self.acceptedPerBatch = []
+
+    def _proposal(self):
+        sites = self.rng.choice(self._nsites, self.batchSize, replace=False)
+        shifts = self.rng.choice((-2 * np.pi, +2 * np.pi), self.batchSize, replace=True)
+        return sites, shifts
 
     def evolve(self, stage):
-        phi = _host(stage.phi)
-        actVal = complex(stage.actVal)
-        accepted = []
+        current, value = _host(stage.phi), complex(stage.actVal)
+        decisions = []
         for _ in range(self.nBatches):
-            sites = self.rng.choice(self.latSize, self.batchSize, replace=False)
-            shifts = self.rng.choice((-2 * np.pi, +2 * np.pi), self.batchSize, replace=True)
-            newPhi = phi.copy()
-            newPhi[sites] += shifts
-            newActVal = self._action.evalLocal(newPhi, sites, shifts, actVal)
-            if self.selector.selectTrajPoint(actVal, newActVal) == 1:
-                phi, actVal = newPhi, newActVal
-                accepted.append(1)
-            else:
-                accepted.append(0)
-        self.acceptedPerBatch.append(accepted)
-        return stage.accept(phi, actVal) if any(accepted) else stage.reject()
+            sites, shifts = self._proposal()
+            trial = current.copy()
+            trial[sites] += shifts
+            trial_value = self._action.evalLocal(trial, sites, shifts, value)
+            took = self.selector.selectTrajPoint(value, trial_value)
+            decisions.append(took)
+            if took:
+                current, value = trial, trial_value
+        self.acceptedPerBatch.append(decisions)
+        return stage.accept(current, value) if any(decisions) else stage.reject()
 
     def save(self, group, manager):
         group["nBatches"] = self.nBatches
@@ -383,44 +454,43 @@ class TwoPiJumps:
         return cls(int(group["nBatches"][()]), int(group["batchSize"][()]), action, lattice, rng)
 
     def report(self):
-        return f"""<TwoPiJumps> (0x{id(self):x})
-  nBatches = {self.nBatches}, batchSize = {self.batchSize}
-  acceptance rate per batch = {np.mean(self.acceptedPerBatch)}"""
+        return _acceptance_line("TwoPiJumps", self, f"{self.nBatches} proposals of {self.batchSize} sites",
+                                self.acceptedPerBatch)
 
 
 class UniformJump:
-    """Shift the field by +-2 pi / Nt at every site, momenta untouched (hubbard.py:119-184):
-    sign rng.choice((-1, 1)), then accept / reject."""
+    """Moves every entry of the field by the same +-2 pi / Nt, momenta untouched
+    (src/isle/evolver/hubbard.py:119-184): the sign is rng.choice((-1, +1)), then the
+    Metropolis choice on the change of the action.  ``accepted`` records the decisions."""
 
     def __init__(self, action, lattice, rng):
         self.rng = rng
         self.selector = BinarySelector(rng)
-        self.accepted = []
         self._action = _HubbardActionShortcut(action)
         self._absShift = 2 * np.pi / lattice.nt()
+        self.accepted = []
 
     def evolve(self, stage):
-        shift = self.rng.choice((-1, +1)) * self._absShift
-        newPhi = _host(stage.phi) + shift
-        newActVal = self._action.evalGlobal(newPhi, shift, complex(stage.actVal))
-        if self.selector.selectTrajPoint(stage.actVal, newActVal) == 1:
-            self.accepted.append(1)
-            return stage.accept(newPhi, newActVal)
-        self.accepted.append(0)
-        return stage.reject()
+        shift = self._absShift * self.rng.choice((-1, +1))
+        trial = _host(stage.phi) + shift
+        trial_value = self._action.evalGlobal(trial, shift, complex(stage.actVal))
+        took = self.selector.selectTrajPoint(stage.actVal, trial_value)
+        self.accepted.append(took)
+        return stage.accept(trial, trial_value) if took else stage.reject()
 
     def save(self, group, manager):
-        pass
+        return None
 
     @classmethod
     def fromH5(cls, group, manager, action, lattice, rng):
         return cls(action, lattice, rng)
 
     def report(self):
-        return f"""<UniformJump> (0x{id(self):x})
-  shift = 2 pi / Nt = {self._absShift}
-  acceptance rate = {np.mean(self.accepted)}"""
+        return _acceptance_line("UniformJump", self, f"shift 2 pi / Nt = {self._absShift:.6g}", self.accepted)
 
+
+# ---------------------------------------------------------------------------------------
+# batched chains
 
 class BatchedConstStepLeapfrog:
     """ConstStepLeapfrog over independent chains advanced together on the GPU.
@@ -491,16 +561,16 @@ class BatchedConstStepLeapfrog:
 
 
 def hmc(stage, evolver, ntr):
-    """HMC loop (src/isle/drivers/hmc.py:41-84) without file output.
-
-    Returns the list of stages after each trajectory."""
+    """Evolve ``ntr`` steps without writing files and return every stage reached; a field
+    given instead of a stage is evaluated first.  Stops early when the evolver raises
+    StopIteration."""
     if not isinstance(stage, EvolutionStage):
         stage = EvolutionStage(stage, evolver.action.eval(stage), 1)
-    out = []
+    visited = []
     for _ in range(ntr):
         try:
             stage = evolver.evolve(stage)
         except StopIteration:
-            break
-        out.append(stage)
-    return out
+            return visited
+        visited.append(stage)
+    return visited

@@ -3,7 +3,6 @@
 Mirrors src/isle/cpp/lattice.hpp:25-163 (Lattice), src/isle/cpp/lattice.cpp:16-84
 (isBipartite) and src/isle/yamlio.py:16-50 (the ``!lattice`` YAML format).
 """
-import sys
 
 import numpy as np
 import yaml
@@ -64,40 +63,37 @@ class Lattice:
 
 
 def isBipartite(lat_or_hopping):
-    """True iff the lattice is bipartite AND even/odd site labels alternate.
+    """True iff the sites split into two sublattices with hopping only between them AND
+    every even site is on the sublattice of site 0 and every odd site on the other
+    (lattice.cpp:16-84).
 
-    Depth-first 2-colouring in increasing neighbour order that does not recurse from
-    the last row, followed by the alternating-label check (lattice.cpp:16-84).
+    The sublattices come from a depth-first walk from site 0 that visits neighbours in
+    increasing index order and does not walk on from the last site (it only labels that
+    site's neighbours), as the reference does; a site left unreached fails the label test.
+    The walk keeps its own stack of neighbour iterators, so lattice size is not bounded by
+    Python's recursion limit.
     """
     h = lat_or_hopping.hopping() if isinstance(lat_or_hopping, Lattice) else np.asarray(lat_or_hopping)
     n = h.shape[0]
     if n == 0:
         return False
-    none, col_a = 0, 1
-    col_b = ~col_a
-    colors = [none] * n
-    colors[0] = col_a
-    neigh = [np.nonzero(h[i])[0].tolist() for i in range(n)]
-    stack_limit = sys.getrecursionlimit()
-    sys.setrecursionlimit(max(stack_limit, 10 * n + 100))
-
-    def descend(cur):
-        c = colors[cur]
-        for nxt in neigh[cur]:
-            if colors[nxt] == none:
-                colors[nxt] = ~c
-                if cur < n - 1 and not descend(nxt):
-                    return False
-            elif colors[nxt] == c:
+    neighbours = [np.flatnonzero(row) for row in h]
+    side = np.zeros(n, dtype=np.int8)  # 0: not reached, +1: sublattice of site 0, -1: the other
+    side[0] = 1
+    walk = [(0, iter(neighbours[0]))]
+    while walk:
+        site, pending = walk[-1]
+        for nxt in pending:
+            if side[nxt] == 0:
+                side[nxt] = -side[site]
+                if site != n - 1:
+                    walk.append((nxt, iter(neighbours[nxt])))
+                    break
+            elif side[nxt] == side[site]:
                 return False
-        return True
-
-    try:
-        bip = descend(0)
-    finally:
-        sys.setrecursionlimit(stack_limit)
-    alternating = all((c == col_a) if i % 2 == 0 else (c == col_b) for i, c in enumerate(colors))
-    return bip and alternating
+        else:
+            walk.pop()
+    return bool(np.array_equal(side, np.where(np.arange(n) % 2 == 0, 1, -1)))
 
 
 class _LatticeLoader(yaml.SafeLoader):

@@ -5,7 +5,10 @@ src/isle/meas/logdet.py:11-46).
 * ``AllToAll(hfm, species, alpha=1)(stage, itr)`` - the all-to-all propagator M^-1 as a
   [nx, nt, nx, nt] device tensor indexed (sink site, sink time, source site, source
   time): solveM against every spacetime source (the identity), transposed so that the
-  source is the right index.  alpha = 0 uses the spin-basis field -i phi.  The last
+  source is the right index.  alpha = 0 uses the spin-basis field -i phi and, as the
+  reference's dense solve np.linalg.solve(M, I^T)^T does, returns (M^-1)^T in the
+  spacetime indices: element [y, f, x, i] is M^-1[(i, x), (f, y)], sink and source
+  exchanged relative to alpha = 1.  The last
   result is reused while (action value, trajectory point, itr) stay the same, as in the
   reference (the field itself is not compared).
 * ``ChiralCondensate(seed, nsamples, hfm, species)(stage, itr)`` - records
@@ -114,9 +117,13 @@ class AllToAll:
             sources[torch.arange(c1 - c0), torch.arange(c0, c1)] = 1
             res[c0:c1] = solve_refined(self.hfm, phi, self.species, sources, self.refine, reuse_factor=c0 > 0)
             del sources
-        # res[i, j] = (M^-1)[j, i]: transpose so the source is the right index, then split
-        # both spacetime indices (time-major) into (site, time)
-        prop = res.transpose(0, 1).reshape(nt, self.nx, nt, self.nx).permute(1, 0, 3, 2)
+        # res[i, j] = (M^-1)[j, i].  alpha = 1: transpose so the source is the right index
+        # (propagator.py:51-59).  alpha = 0: the reference solves densely,
+        # np.linalg.solve(M, I^T)^T = (M^-1)^T (propagator.py:60-61), which is res as it is.
+        if self._alpha == 1:
+            res = res.transpose(0, 1)
+        # split both spacetime indices (time-major) into (site, time)
+        prop = res.reshape(nt, self.nx, nt, self.nx).permute(1, 0, 3, 2)
         self._key, self._value = key, prop
         return prop
 

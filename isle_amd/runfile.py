@@ -224,11 +224,13 @@ def removeGreaterThan(fname, group, maxIdx):
 
 
 class EvolverManager:
-    """Saves and restores evolvers in checkpoints (src/isle/evolver/manager.py).
+    """Saves and restores evolvers and their transforms in checkpoints
+    (src/isle/evolver/manager.py).
 
-    The evolver's type is stored by name in ``__name__`` of its group (attribute
-    ``__object__kind__`` = "Evolver"); the type is looked up among the evolvers of
-    isle_amd.evolver and the classes in ``definitions``.  Source code of a custom evolver
+    The type is stored by name in ``__name__`` of its group (attribute
+    ``__object__kind__`` = "Evolver" or "Transform"); the type is looked up among the
+    evolvers of isle_amd.evolver, the transforms of isle_amd.transform and the classes in
+    ``definitions``.  Source code of a custom evolver
     is never stored or executed: pass the class in ``definitions``."""
 
     def __init__(self, fname=None, definitions=None):
@@ -237,17 +239,20 @@ class EvolverManager:
 
     def _types(self):
         from . import evolver as ev
+        from . import transform as tr
         types = {cls.__name__: cls for cls in (ev.ConstStepLeapfrog, ev.LinearStepLeapfrog, ev.TwoPiJumps,
-                                               ev.UniformJump)}
+                                               ev.UniformJump, tr.Identity, tr.ConstantShift)}
         types.update(self.definitions)
         return types
 
     def save(self, evolver, group):
+        """Store an evolver, or a transform of one, in ``group``."""
+        from .transform import Transform
         name = type(evolver).__name__
         if name not in self._types():
             raise ValueError(f"cannot save an evolver of type {name}: not a built-in evolver and "
                              "not in definitions")
-        group.attrs["__object__kind__"] = "Evolver"
+        group.attrs["__object__kind__"] = "Transform" if isinstance(evolver, Transform) else "Evolver"
         group["__name__"] = name
         evolver.save(group, self)
 

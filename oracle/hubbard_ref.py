@@ -708,18 +708,81 @@ def binary_select(rng, energy0, energy1):
     return 1 if delta < 0 or np.exp(-delta) > rng.uniform(0, 1) else 0
 
 
-def const_step_leapfrog(rng, action, phi, act_val, length, nstep):
-    """One ConstStepLeapfrog.evolve without transform (leapfrog.py:38-66).
+def const_step_leapfrog(rng, action, phi, act_val, length, nstep, transform=None, log_weights=None):
+    """One ConstStepLeapfrog.evolve (leapfrog.py:38-66), with an optional transform
+    (transform.py:73-101): the MD field is transform.backward(phi); log det J of the start
+    is log_weights["logdetJ"] when given, else the one backward returns; after leapfrog,
+    transform.forward gives the Monte Carlo field, its action and log det J.
+    E0 = act_val + sum of the other log weights (+ log det J) + |pi|^2/2,
+    E1 = act1 + log det J1 + |pi1|^2/2.
 
-    Returns (phi_new, act_val_new, traj_point, energy0, energy1)."""
+    Returns (phi_new, act_val_new, traj_point, energy0, energy1, log_weights_new)."""
+    weights = dict(log_weights or {})
+    if transform is None:
+        phi_md = phi
+    else:
+        phi_md, logdet0 = transform.backward(phi)
+        weights.setdefault("logdetJ", logdet0)
     pi = rng.normal(0, 1, len(phi)) + 0j
-    phi1, pi1, act1 = leapfrog(phi, pi, action, length, nstep)
-    energy0 = act_val + np.linalg.norm(pi) ** 2 / 2
-    energy1 = act1 + 0 + np.linalg.norm(pi1) ** 2 / 2
+    phi1, pi1, act1 = leapfrog(phi_md, pi, action, length, nstep)
+    logdet1 = 0
+    if transform is not None:
+        phi1, act1, logdet1 = transform.forward(phi1, act1)
+    energy0 = act_val + sum(weights.values()) + np.linalg.norm(pi) ** 2 / 2
+    energy1 = act1 + logdet1 + np.linalg.norm(pi1) ** 2 / 2
     tp = binary_select(rng, energy0, energy1)
     if tp == 1:
-        return phi1, act1, 1, energy0, energy1
-    return np.asarray(phi), act_val, 0, energy0, energy1
+        return phi1, act1, 1, energy0, energy1, ({} if transform is None else {"logdetJ": logdet1})
+    return np.asarray(phi), act_val, 0, energy0, energy1, weights
+
+
+class ConstantShift:
+    """constantShift.py:39-58: MC field = MD field + shift; the action value carried is
+    the action of the MD field; log det J = 0."""
+
+    def __init__(self, shift, action):
+        self.shift = np.asarray(shift, dtype=np.complex128)
+        self.action = action
+
+    def forward(self, phi, act_val):
+        return phi + self.shift, self.action.eval(phi), 0.0
+
+    def backward(self, phi):
+        return phi - self.shift, 0.0
+
+
+def two_pi_jumps(rng, action, phi, act_val, n_batches, batch_size):
+    """One TwoPiJumps.evolve (hubbard.py:58-80) with the action evaluated in full: per
+    batch, sites = rng.choice(N, batch_size, replace=False), shifts =
+    rng.choice((-2 pi, 2 pi), batch_size, replace=True), then binary_select on the
+    action values.  Returns (phi, act_val, decisions); phi is the field after the last
+    accepted batch (the reference's hubbard.py:75-77 returns the last proposal even when it
+    was rejected; that defect is not restated)."""
+    phi = np.asarray(phi, dtype=np.complex128).copy()
+    decisions = []
+    for _ in range(n_batches):
+        sites = rng.choice(phi.size, batch_size, replace=False)
+        shifts = rng.choice((-2 * np.pi, 2 * np.pi), batch_size, replace=True)
+        new = phi.copy()
+        new[sites] += shifts
+        new_act = action.eval(new)
+        if binary_select(rng, act_val, new_act):
+            phi, act_val = new, new_act
+            decisions.append(1)
+        else:
+            decisions.append(0)
+    return phi, act_val, decisions
+
+
+def uniform_jump(rng, action, phi, act_val, nt):
+    """One UniformJump.evolve (hubbard.py:160-172): shift = rng.choice((-1, 1)) * 2 pi / Nt
+    at every site, then binary_select on the action values (evaluated in full).
+    Returns (phi, act_val, decision)."""
+    new = np.asarray(phi, dtype=np.complex128) + rng.choice((-1, 1)) * 2 * np.pi / nt
+    new_act = action.eval(new)
+    if binary_select(rng, act_val, new_act):
+        return new, new_act, 1
+    return np.asarray(phi, dtype=np.complex128), act_val, 0
 
 
 # --------------------------------------------------------------------------- #

@@ -143,3 +143,52 @@ def test_batched_hmc_resume(tmp_path):
     for i in (3, 4):
         assert got[i][2] == ref[i][2]
         assert np.max(np.abs(got[i][0] - ref[i][0])) < 1e-12 * np.max(np.abs(ref[i][0]))
+
+
+def _oracle_chain(lat, kind, ntr):
+    """The chain the reference's driver makes from NumpyRNG(1075): the start field and then
+    ntr trajectories of oracle.const_step_leapfrog with the evolver's (length, nstep)."""
+    from oracle import hubbard_ref as ref
+    ut = PARAMS.tilde("U", lat)
+    kt = lat.hopping() * PARAMS.beta / lat.nt()
+    action = ref.SumAction(ref.HubbardGaugeAction(ut), ref.HubbardFermiAction(kt, 0.0, -1, ref.EXP))
+    if kind == "const":
+        schedule = [(1.0, 16)] * ntr
+    else:  # lengths 1 -> 0.5 and steps 12 -> 6 over 4 trajectories (collection.py:10-30)
+        schedule = [(1.0, 12), (0.875, 10), (0.75, 9), (0.625, 7)] + [(0.5, 6)] * (ntr - 4)
+    rs = np.random.RandomState(1075)
+    phi = rs.normal(0, ut ** 0.5, lat.lattSize()) + 0j
+    act = action.eval(phi)
+    out = {}
+    for i, (length, nstep) in enumerate(schedule[:ntr], start=1):
+        phi, act, tp, _, _, _ = ref.const_step_leapfrog(rs, action, phi, act, length, nstep)
+        out[i] = (np.asarray(phi), complex(act), tp)
+    return out
+
+
+@pytest.mark.parametrize("kind", ["const", "linear"])
+def test_resumed_run_matches_oracle_chain(tmp_path, kind):
+    """A run that crashes after trajectory 3 and resumes from checkpoint 2 writes the
+    oracle's chain: same accept decisions, fields and actions within 1e-9."""
+    print(f"hmc resume vs oracle chain, {kind} leapfrog, Nx=18 Nt=16", flush=True)
+    lat = _lattice()
+    rng = NumpyRNG(1075)
+    phi = rng.normal(0, PARAMS.tilde("U", lat) ** 0.5, lat.lattSize()) + 0j
+    drv = hmc.newRun(lat, PARAMS, rng, make_action, tmp_path / "run", False, DEFS)
+    if kind == "const":  # 16 steps: this chain has accepted and rejected trajectories
+        ev = CrashingConst(drv.action, 1.0, 16, drv.rng)
+        ev.crash_at = 4
+    else:
+        ev = _evolver(kind, drv.action, drv.rng, crash_at=4)
+    with pytest.raises(Crash):
+        drv(phi, ev, 6, saveFreq=1, checkpointFreq=2)
+    drv2, stage, ev, _, _ = hmc.continueRun(tmp_path / "run", None, 2, True, DEFS, makeAction=make_action)
+    drv2(stage, ev, 4, saveFreq=1, checkpointFreq=2)
+    got = _configs(tmp_path / "run")
+    want = _oracle_chain(lat, kind, 6)
+    assert sorted(got) == sorted(want)
+    assert {tp for _, _, tp in want.values()} == {0, 1}  # the chain has accepted and rejected trajectories
+    for i in want:
+        assert got[i][2] == want[i][2], i
+        assert np.abs(got[i][0] - want[i][0]).max() < 1e-9 * np.abs(want[i][0]).max(), i
+        assert abs(got[i][1] - want[i][1]) < 1e-9 * abs(want[i][1]), i

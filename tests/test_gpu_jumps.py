@@ -69,3 +69,37 @@ def test_spin_basis_is_evaluated():
     gauge_only = act + np.dot(2 * new[sites] - shifts, shifts) / (2 * (2.0 * 4.0 / 16))
     assert _rel(gauge_only, full) > 1e-3  # the fermion action is not invariant in the spin basis
     assert _rel(ev._action.evalLocal(new, sites, shifts, act), full) < 1e-12
+
+
+@pytest.mark.parametrize("basis", [HFABasis.PARTICLE_HOLE, HFABasis.SPIN])
+def test_jump_chain_matches_oracle(basis):
+    """TwoPiJumps then UniformJump, 6 sweeps, against oracle.two_pi_jumps / uniform_jump
+    from the same seed with the oracle's gauge + fermion action evaluated in full (the
+    product uses the shortcut in the particle-hole basis and the GPU action in the spin
+    basis): same decisions, fields and action values."""
+    from oracle import hubbard_ref as ref
+    print(f"jump chain vs oracle, {basis.name}, Nx=18 Nt=16", flush=True)
+    lat, _ = _action(HFAHopping.EXP, basis)
+    ut = 20.0  # a wide gauge action: jumps are accepted and rejected
+    action = HubbardGaugeAction(ut) + makeHubbardFermiAction(lat, 4.0, 0.0, -1, HFAHopping.EXP, basis)
+    kt = lat.hopping() * 4.0 / 16
+    oracle = ref.SumAction(ref.HubbardGaugeAction(ut),
+                           ref.HubbardFermiAction(kt, 0.0, -1, ref.EXP,
+                                                  ref.PARTICLE_HOLE if basis == HFABasis.PARTICLE_HOLE else ref.SPIN))
+    phi = np.random.RandomState(21).normal(0, 0.5, lat.lattSize()) + 0j
+    rng, rs = NumpyRNG(31), np.random.RandomState(31)
+    two_pi, uni = TwoPiJumps(3, 2, action, lat, rng), UniformJump(action, lat, rng)
+    stage = EvolutionStage(phi, complex(action.eval(phi)), 1)
+    want_phi, want_act = phi, oracle.eval(phi)
+    decisions = []
+    for sweep in range(6):
+        stage = two_pi.evolve(stage)
+        want_phi, want_act, batch = ref.two_pi_jumps(rs, oracle, want_phi, want_act, 3, 2)
+        assert two_pi.acceptedPerBatch[-1] == batch, sweep
+        stage = uni.evolve(stage)
+        want_phi, want_act, took = ref.uniform_jump(rs, oracle, want_phi, want_act, 16)
+        assert uni.accepted[-1] == took, sweep
+        decisions += batch + [took]
+        assert np.abs(np.asarray(stage.phi) - want_phi).max() < 1e-12
+        assert _rel(complex(stage.actVal), want_act) < 1e-10
+    assert set(decisions) == {0, 1}

@@ -11,8 +11,10 @@ dense solve with the oracle's M (numpy.linalg.inv / solve, the textbook routine)
 refine=0 (solveM as the reference computes it) they must be off the dense solve by at
 most 10x the oracle's own solveM error (+1e-12): as accurate as the reference's
 algorithm.  applyM itself equals the oracle's dense M times the rows to 1e-13.  The
-explicit element map of the propagator (prop[y, f, x, i] = M^-1[f nx + y, i nx + x])
-is checked, so an exchanged site/time or sink/source index fails.
+explicit element map of the propagator (prop[y, f, x, i] = M^-1[f nx + y, i nx + x] for
+alpha 1; M^-1[i nx + x, f nx + y] for alpha 0, where the reference's dense solve returns
+(M^-1)^T) and the reference's own formulas are checked, so an exchanged site/time or
+sink/source index fails.
 
 Small case: honeycomb 3x3 torus (Nx=18), Nt=16, beta=4, U=2, mu=0.1 (N = 288).  Log dets:
 the oracle's logdetM (alpha 1) and numpy's dense slogdet of M(-i phi) (alpha 0, the
@@ -70,14 +72,24 @@ def test_all_to_all_is_dense_inverse(hop, species, alpha):
     plain = meas.AllToAll(hfm, isle.Species(species), alpha, refine=0)(stage, 0)
     assert tuple(prop.shape) == (nx, nt, nx, nt)
     field = phi if alpha == 1 else -1j * phi
-    want = np.linalg.inv(rh.M(field, species))
+    dense = rh.M(field, species)
+    # the reference's formulas (propagator.py:51-66): alpha 1 transposes solveM's rows,
+    # alpha 0 solves densely and transposes, which leaves (M^-1)^T
+    n = nx * nt
+    res = np.linalg.inv(dense) if alpha == 1 else np.linalg.solve(dense, np.eye(n).T).T
+    ref_prop = res.reshape(nt, nx, nt, nx).transpose(1, 0, 3, 2)
+    want = np.linalg.inv(dense) if alpha == 1 else np.linalg.inv(dense).T
     scale = np.abs(want).max()
     got = prop.cpu().numpy()
+    assert np.abs(got - ref_prop).max() < 1e-10 * scale
     y, f, x, i = 5, 3, 11, 9
-    assert abs(got[y, f, x, i] - want[f * nx + y, i * nx + x]) < 1e-10 * scale
+    inv = np.linalg.inv(dense)
+    sink, source = f * nx + y, i * nx + x
+    assert abs(got[y, f, x, i] - (inv[sink, source] if alpha == 1 else inv[source, sink])) < 1e-10 * scale
     err = np.abs(_as_matrix(prop, nx, nt).cpu().numpy() - want).max() / scale
     err0 = np.abs(_as_matrix(plain, nx, nt).cpu().numpy() - want).max() / scale
-    ref_err = np.abs(ref.solveM(rh, field, species, np.eye(nx * nt)).T - want).max() / scale
+    ref_sol = ref.solveM(rh, field, species, np.eye(nx * nt)).T
+    ref_err = np.abs((ref_sol if alpha == 1 else ref_sol.T) - want).max() / scale
     print(f"  vs dense inverse: refined {err:.2e}, unrefined {err0:.2e} (oracle solveM {ref_err:.2e})", flush=True)
     assert err < 1e-10
     assert err0 <= 10 * ref_err + 1e-12

@@ -249,7 +249,7 @@ def test_hmc_fixed_seed_matches_oracle():
     phi_r, act_r = phi0, r.eval(phi0)
     for _ in range(4):
         stage = ev.evolve(stage)
-        phi_r, act_r, tp, e0, e1 = ref.const_step_leapfrog(rng_r, r, phi_r, act_r, 1.0, 20)
+        phi_r, act_r, tp, e0, e1, _ = ref.const_step_leapfrog(rng_r, r, phi_r, act_r, 1.0, 20)
         assert tp == stage.trajPoint
         dh_g = np.real(ev.last_energies[1] - ev.last_energies[0])
         assert abs(dh_g - np.real(e1 - e0)) < 1e-8

@@ -1,11 +1,11 @@
 """TwoPiJumps / UniformJump on the CPU (gauge actions only, so no GPU is touched).
 
 With only gauge actions the new action value must be the gauge action of the new field,
-sum phi^2 / (2 Utilde) summed over the gauge terms, computed here directly; the random
-numbers must be drawn as the reference draws them (per batch: distinct sites, signs,
-then a uniform only when the action rises), which a replay with a plain RandomState
-checks, including a sweep whose last batch is rejected after an earlier one was accepted
-(the field of the last accepted batch is kept).  The fermion action's invariance, which
+sum phi^2 / (2 Utilde) summed over the gauge terms, computed here directly; the chain must
+be the oracle's (oracle.two_pi_jumps / uniform_jump from a RandomState of the same seed,
+evaluating the action in full): same draws, decisions, fields and action values,
+including a sweep whose last batch is rejected after an earlier one was accepted (the
+field of the last accepted batch is kept).  The fermion action's invariance, which
 the shortcut relies on, is checked on the GPU in tests/test_gpu_jumps.py.
 """
 import numpy as np
@@ -15,6 +15,7 @@ from isle_amd import runfile
 from isle_amd.action import HubbardGaugeAction, SumAction
 from isle_amd.evolver import EvolutionStage, NumpyRNG, TwoPiJumps, UniformJump
 from isle_amd.lattice import honeycomb
+from oracle import hubbard_ref as ref
 
 U1, U2 = 30.0, 120.0  # a 2 pi jump then costs O(1) in action: both outcomes occur
 
@@ -31,26 +32,8 @@ def _start(seed=3, nt=4):
     return lat, phi, action
 
 
-def _replay_two_pi(seed, phi, nbatches, bsize, sweeps):
-    rs = np.random.RandomState(seed)
-    act = _gauge(phi)
-    out = []
-    for _ in range(sweeps):
-        acc = []
-        for _ in range(nbatches):
-            sites = rs.choice(phi.size, bsize, replace=False)
-            shifts = rs.choice((-2 * np.pi, 2 * np.pi), bsize, replace=True)
-            new = phi.copy()
-            new[sites] += shifts
-            new_act = _gauge(new)
-            de = np.real(new_act - act)
-            if de < 0 or np.exp(-de) > rs.uniform(0, 1):
-                phi, act = new, new_act
-                acc.append(1)
-            else:
-                acc.append(0)
-        out.append((phi.copy(), act, acc))
-    return out
+def _oracle_action(us=(U1, U2)):
+    return ref.SumAction(*[ref.HubbardGaugeAction(u) for u in us])
 
 
 def test_two_pi_jumps_values_and_draws():
@@ -58,11 +41,12 @@ def test_two_pi_jumps_values_and_draws():
     ev = TwoPiJumps(4, 2, action, lat, NumpyRNG(11))
     assert ev._action.shortcut
     stage = EvolutionStage(phi, _gauge(phi), 1)
-    want = _replay_two_pi(11, phi, 4, 2, 25)
+    rs, oracle = np.random.RandomState(11), _oracle_action()
+    wphi, wact = phi, _gauge(phi)
     saw_rejected_last = False
     for k in range(25):
         stage = ev.evolve(stage)
-        wphi, wact, wacc = want[k]
+        wphi, wact, wacc = ref.two_pi_jumps(rs, oracle, wphi, wact, 4, 2)
         assert ev.acceptedPerBatch[k] == wacc
         assert np.allclose(stage.phi, wphi, rtol=0, atol=1e-12)
         assert abs(stage.actVal - wact) < 1e-10 * abs(wact)
@@ -76,16 +60,12 @@ def test_uniform_jump_values_and_draws():
     lat, phi, action = _start(nt=8)
     ev = UniformJump(action, lat, NumpyRNG(5))
     stage = EvolutionStage(phi, _gauge(phi), 1)
-    rs = np.random.RandomState(5)
+    rs, oracle = np.random.RandomState(5), _oracle_action()
     cur, act = phi.copy(), _gauge(phi)
     for _ in range(40):
         stage = ev.evolve(stage)
-        shift = rs.choice((-1, 1)) * 2 * np.pi / 8
-        new = cur + shift
-        new_act = _gauge(new)
-        de = np.real(new_act - act)
-        if de < 0 or np.exp(-de) > rs.uniform(0, 1):
-            cur, act = new, new_act
+        cur, act, took = ref.uniform_jump(rs, oracle, cur, act, 8)
+        assert stage.trajPoint == took
         assert np.allclose(stage.phi, cur, rtol=0, atol=1e-12)
         assert abs(stage.actVal - act) < 1e-10 * abs(act)
     assert 0 < np.mean(ev.accepted) < 1

@@ -99,7 +99,7 @@ def _trajectory_checks(step_fn, seed, nx, length, nstep):
     phi0 = np.random.RandomState(seed + 1).normal(0, np.sqrt(U), nx) + 0j
     s0 = g.eval(phi0)
     rng, twin = np.random.RandomState(seed), np.random.RandomState(seed)
-    phi_new, act_new, point, e0, e1 = step_fn(rng, g, phi0, s0, length, nstep)
+    phi_new, act_new, point, e0, e1 = step_fn(rng, g, phi0, s0, length, nstep)[:5]
     pi = twin.normal(0, 1, nx)
     q1, p1 = _harmonic_end(phi0.real, pi, length, nstep)
     h0 = np.sum(phi0.real ** 2) / (2 * U) + np.sum(pi ** 2) / 2
