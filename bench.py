@@ -20,11 +20,13 @@ import numpy as np
 
 METRIC = "HMC traj/s at Nx=288 Nt=128: batched FP32-force vs plain FP64; force rel err"
 L, NT, BETA, U, NSTEP, LENGTH = 12, 128, 8.0, 3.0, 40, 1.0
+THERM_LENGTH, THERM_NSTEP = 0.25, 100  # thermalisation trajectories: step 0.0025 instead of 0.025
+REV_CHAINS = 8
 PEAK = {"f32": 157.3, "f64": 78.6}  # dense MFMA TFLOP/s, MI355X
 KIND_NAMES = ["gemm64_rc_pre", "gemm64_rc_row", "gemm64_cr_pre", "gemm64_cc", "gemm32_rc_row", "gemm32_rc_pre",
               "scale64", "scale32", "lu", "lu_solve", "diag64", "diag32", "copy64", "cvt32to64", "cvt64to32",
               "phase", "tcvt64to32", "diag64x32", "gemm64_cc_sub", "lu_shape", "block", "sigma", "gather",
-              "tblock", "trsm", "gemm32_rc_pre_diag", "dreduce", "rgather"]
+              "tblock", "trsm", "gemm32_rc_pre_diag", "dreduce", "rgather", "gemm32_rc_diag", "tscale64to32"]
 
 
 def parse():
@@ -34,7 +36,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--chains", type=int, default=128, help="independent chains advanced together")
     ap.add_argument("--group", type=int, default=32, help="time slices per group of the mixed force")
-    ap.add_argument("--plain-steps", type=int, default=1, help="timed trajectories of the plain FP64 form")
+    ap.add_argument("--plain-steps", type=int, default=3, help="timed trajectories of the plain FP64 forms")
+    ap.add_argument("--therm", type=int, default=2, help="thermalisation trajectories (length 0.25, 100 steps) "
+                    "before the timed ones")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch the integration eagerly, not from a HIP graph")
@@ -418,42 +422,107 @@ def main():
         ferm = A.HubbardFermiAction(kt, 0.0, -1, A.HFAHopping.EXP, precision=prec, group=args.group)
         return A.SumAction(A.HubbardGaugeAction(ut), ferm), ferm
 
-    # ---------------- batched mixed precision
+    def clone_rngs(rngs):
+        out = []
+        for r in rngs:
+            c = NumpyRNG(0)
+            c.set_state(r.get_state())
+            out.append(c)
+        return out
+
+    def run(ev, phis, acts, n):
+        for _ in range(n):
+            phis, acts, _ = ev.evolve(phis, acts)
+        return phis, acts
+
+    def stats(ev, skip):
+        dh = np.array(ev.deltaH[skip:], dtype=float).ravel()
+        acc = np.array(ev.trajPoints[skip:], dtype=float).ravel()
+        if not dh.size:
+            return {}
+        w = np.exp(-np.clip(dh, -700.0, None))
+        p = acc.mean()
+        return {"trajectories": int(dh.size), "acceptance": float(p),
+                "acceptance_err": float(np.sqrt(max(p * (1 - p), 1e-300) / dh.size)),
+                "mean_dH": float(dh.mean()), "median_dH": float(np.median(dh)),
+                "mean_exp_minus_dH": float(w.mean()), "mean_exp_minus_dH_err": float(w.std(ddof=1) / np.sqrt(w.size))
+                if w.size > 1 else None}
+
+    # ---------------- start fields N(0, Utilde) per chain, then thermalisation: short
+    #                  trajectories with small steps carry the fields away from the
+    #                  random start (where length-1, 40-step trajectories are rejected)
     act_m, ferm_m = make("mixed")
     rngs = [NumpyRNG(1000 + b) for b in range(B)]
     phis = torch.as_tensor(np.stack([r.normal(0, np.sqrt(ut), nx * NT) + 0j for r in rngs]), device=dev)
     acts = np.array([act_m.eval(phis[b]) for b in range(B)])
+    split = [make("mixed")[0] for _ in range(args.streams)] if args.streams > 1 else None
+    t_th = time.perf_counter()
+    if args.therm > 0:
+        ev_th = BatchedConstStepLeapfrog(act_m, THERM_LENGTH, THERM_NSTEP, rngs, graph=False)
+        phis, acts = run(ev_th, phis, acts, args.therm)
+        therm_stats = stats(ev_th, 0)
+    else:
+        therm_stats = {}
+    torch.cuda.synchronize()
+    t_th = time.perf_counter() - t_th
+    start_phis, start_acts, start_rngs = phis.clone(), acts.copy(), clone_rngs(rngs)
 
-    # ---------------- force error of the mixed form against plain FP64 on the start fields
+    # ---------------- force error of the mixed form against plain FP64 on the thermalised fields
     act_p, ferm_p = make("fp64")
     f_m = ferm_m.force_batch(phis).cpu().numpy()
     f_p = ferm_p.force_batch(phis).cpu().numpy()
     errs = np.linalg.norm(f_m - f_p, axis=1) / np.linalg.norm(f_p, axis=1)
     force_err = float(errs.max())
     force_err_max = float((np.abs(f_m - f_p).max(axis=1) / np.abs(f_p).max(axis=1)).max())
-    split = [make("mixed")[0] for _ in range(args.streams)] if args.streams > 1 else None
+    del f_m, f_p
+
+    # ---------------- batched mixed precision: the timed trajectories
     ev = BatchedConstStepLeapfrog(act_m, LENGTH, NSTEP, rngs, graph=not args.no_graph, split=split)
-    for _ in range(args.warmup):
-        phis, acts, _ = ev.evolve(phis, acts)
+    phis, acts = run(ev, phis, acts, args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        phis, acts, _ = ev.evolve(phis, acts)
+    phis, acts = run(ev, phis, acts, args.steps)
     torch.cuda.synchronize()
     t_mixed = time.perf_counter() - t0
     traj_per_s = B * args.steps / t_mixed
-    dh = np.array(ev.deltaH[args.warmup:]).ravel()
-    acc = np.array(ev.trajPoints[args.warmup:]).ravel()
+    mixed_stats = stats(ev, 0)
 
-    # ---------------- reversibility: a forward trajectory then the reversed one (chain 0)
-    pi0 = torch.as_tensor(NumpyRNG(77).normal(0, 1, nx * NT) + 0j, device=dev).reshape(1, -1)
-    p1, q1, _ = isle.leapfrog_batch(phis[0:1], pi0, act_m, LENGTH, NSTEP, 1.0, evaluate=False)
-    p2, _, _ = isle.leapfrog_batch(p1, q1, act_m, LENGTH, NSTEP, -1.0, evaluate=False)
-    reversibility = float(torch.linalg.norm(p2 - phis[0:1]) / torch.linalg.norm(phis[0:1]))
-    # the same forward/backward pair with the plain FP64 force: how chaotic the trajectory itself is
-    p1, q1, _ = isle.leapfrog_batch(phis[0:1], pi0, act_p, LENGTH, NSTEP, 1.0, evaluate=False)
-    p2, _, _ = isle.leapfrog_batch(p1, q1, act_p, LENGTH, NSTEP, -1.0, evaluate=False)
-    reversibility_fp64 = float(torch.linalg.norm(p2 - phis[0:1]) / torch.linalg.norm(phis[0:1]))
+    # ---------------- batched plain FP64 from the same thermalised fields and generator states
+    plain_batched, plain_stats = None, {}
+    if not args.no_plain:
+        ev_pb = BatchedConstStepLeapfrog(act_p, LENGTH, NSTEP, start_rngs, graph=not args.no_graph)
+        pb_phis, pb_acts = run(ev_pb, start_phis.clone(), start_acts.copy(), 1)  # plan build, tuning, capture
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(ev_pb, pb_phis, pb_acts, args.plain_steps)
+        torch.cuda.synchronize()
+        plain_batched = B * args.plain_steps / (time.perf_counter() - t0)
+        plain_stats = stats(ev_pb, 0)
+        del ev_pb, pb_phis
+
+    # ---------------- reversibility: forward then reversed trajectory on thermalised fields
+    #                  (the first REV chains), mixed and plain FP64 forces, the same momenta
+    rev_n = min(REV_CHAINS, B)
+    pi0 = torch.as_tensor(np.stack([NumpyRNG(77 + c).normal(0, 1, nx * NT) for c in range(rev_n)]) + 0j, device=dev)
+    x0 = start_phis[:rev_n]
+    rev = {}
+    for name, action in (("mixed", make("mixed")[0]), ("fp64", make("fp64")[0])):
+        p1, q1, a1 = isle.leapfrog_batch(x0, pi0, action, LENGTH, NSTEP, 1.0)
+        p2, _, _ = isle.leapfrog_batch(p1, q1, action, LENGTH, NSTEP, -1.0, evaluate=False)
+        err = (torch.linalg.norm(p2 - x0, dim=1) / torch.linalg.norm(x0, dim=1)).cpu().numpy()
+        kin0 = (pi0.real ** 2).sum(dim=1).cpu().numpy() / 2
+        kin1 = (q1.real ** 2 + q1.imag ** 2).sum(dim=1).cpu().numpy() / 2
+        dh = np.real(np.asarray(a1) + kin1 - start_acts[:rev_n] - kin0)
+        rev[name] = {"err": err, "dH": dh}
+    regular = np.abs(rev["fp64"]["dH"]) < 10.0  # trajectories that passed no near-zero of det M
+    reversibility = {
+        "chains": rev_n, "regular_chains": int(regular.sum()),
+        "mixed_median": float(np.median(rev["mixed"]["err"])),
+        "fp64_median": float(np.median(rev["fp64"]["err"])),
+        "mixed_max_regular": float(rev["mixed"]["err"][regular].max()) if regular.any() else None,
+        "fp64_max_regular": float(rev["fp64"]["err"][regular].max()) if regular.any() else None,
+        "dH_fp64": [round(float(x), 5) for x in rev["fp64"]["dH"]],
+    }
 
     # ---------------- kernel roofline (mixed force of the whole chain batch)
     nk = len(KIND_NAMES)
@@ -475,13 +544,13 @@ def main():
                 "force_ms": float(prof[:, 1].sum()),
                 "breakdown_ms": {KIND_NAMES[k]: round(float(prof[k, 1]), 4) for k in range(nk) if prof[k, 0] > 0}}
 
-    # ---------------- plain FP64, one chain, unbatched
+    # ---------------- plain FP64, one chain, unbatched (from chain 0's thermalised field)
     plain = None
     if not args.no_plain:
-        rng = NumpyRNG(1000)
-        phi1 = torch.as_tensor((rng.normal(0, np.sqrt(ut), nx * NT) + 0j).reshape(1, -1), device=dev)
-        a1 = np.array([act_p.eval(phi1[0])])
-        ev_p = BatchedConstStepLeapfrog(act_p, LENGTH, NSTEP, [rng], graph=not args.no_graph)
+        rng = clone_rngs(start_rngs[:1])
+        phi1 = start_phis[0:1].clone()
+        a1 = start_acts[0:1].copy()
+        ev_p = BatchedConstStepLeapfrog(act_p, LENGTH, NSTEP, rng, graph=not args.no_graph)
         phi1, a1, _ = ev_p.evolve(phi1, a1)  # warm-up (plan build, tuning, graph capture)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -501,19 +570,28 @@ def main():
                    "parallelism": "independent chains, 1 GPU"},
         "plain_fp64_traj_per_s": plain,
         "speedup_vs_plain_fp64": (traj_per_s / plain) if plain else None,
+        "plain_fp64_batched_traj_per_s": plain_batched,
+        "speedup_vs_plain_fp64_batched": (traj_per_s / plain_batched) if plain_batched else None,
+        "plain_note": f"plain FP64 = the reference's algorithm in FP64 on the same kernels; single chain and "
+                      f"{B} batched chains, {args.plain_steps} timed trajectories each after one warm-up",
         "force_rel_err": force_err, "force_rel_err_max": force_err_max,
-        "force_rel_err_note": "max over the chains' start fields of |F_mixed - F_fp64|_2 / |F_fp64|_2 "
+        "force_rel_err_note": "max over the chains' thermalised fields of |F_mixed - F_fp64|_2 / |F_fp64|_2 "
                               "(and of the max-norm ratio)",
         "force_evals_per_s": traj_per_s * (NSTEP + 1),
-        "acceptance": float(acc.mean()) if acc.size else None,
-        "mean_dH": float(dh.mean()) if dh.size else None,
-        "median_dH": float(np.median(dh)) if dh.size else None,
-        "mean_exp_minus_dH": float(np.exp(-dh).mean()) if dh.size else None,
-        "reversibility_err": reversibility,
-        "reversibility_err_plain_fp64": reversibility_fp64,
-        "reversibility_note": "|phi(fwd then reversed) - phi0| / |phi0| for chain 0, mixed and plain FP64 forces on "
-                              "the same start; from random starts at these parameters both are O(1) because the "
-                              "trajectory itself is chaotic (Delta H ~ 1e4..1e7)",
+        "thermalisation": {"trajectories": args.therm, "length": THERM_LENGTH, "nstep": THERM_NSTEP,
+                           "seconds": t_th, **{k: v for k, v in therm_stats.items() if k in ("acceptance",
+                                                                                             "median_dH")}},
+        "hmc_mixed": mixed_stats, "hmc_fp64_batched": plain_stats,
+        "acceptance": mixed_stats.get("acceptance"),
+        "mean_dH": mixed_stats.get("mean_dH"),
+        "median_dH": mixed_stats.get("median_dH"),
+        "mean_exp_minus_dH": mixed_stats.get("mean_exp_minus_dH"),
+        "reversibility": reversibility,
+        "reversibility_err": reversibility["mixed_max_regular"],
+        "reversibility_err_plain_fp64": reversibility["fp64_max_regular"],
+        "reversibility_note": "|phi(fwd then reversed) - phi0| / |phi0| per chain on thermalised fields; max over "
+                              "the chains whose FP64 trajectory has |dH| < 10 (the others pass near a zero of det M, "
+                              "where the trajectory itself is chaotic), medians over all",
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -134,7 +134,9 @@ int isle_cscale(const double* in, double* out, long n, double cr, double ci, voi
  * complex x complex subtracted, 19 LU of a block-cyclic elimination step, 20 block
  * assembly, 21 row permutation, 22 permuted gather, 23 transposed block extraction, 24 blocked
  * triangular solve (GEMM launches), 25 GEMM32 real x complex with the force diagonal's row sums
- * fused, 26 sum of those row sums, 27 rows of right-hand sides in pivot order (solveM). */
+ * fused, 26 sum of those row sums, 27 rows of right-hand sides in pivot order (solveM), 28 GEMM32
+ * real x complex with the row sums fused and the stored rows scaled by the next slice's phase,
+ * 29 transposed FP64 -> FP32 copy with complex row scaling. */
 int isle_profile_kernels(isle_hfm* h, const double* phi, int op, int nt, int batch, int group, int nspecies,
                          int spin, int reps, double* out_host, int nkinds);
 

@@ -369,6 +369,45 @@ __global__ void transpose_items_kernel(const TCopyItem<S, D>* __restrict__ items
     }
 }
 
+// dst = diag(d) src^T for split-plane complex n x n matrices: FP64 src (re plane, then the
+// im plane n*n further), FP32 dst, d complex (d_re, d_im; FP64).  The product is formed in
+// FP64 and rounded once.  32 x 32 tiles of both planes through LDS; grid (64, 1, items).
+struct TScaleItem {
+    const double* src;
+    float* dst;
+    const double* d_re;
+    const double* d_im;
+    int n;
+};
+
+__global__ __launch_bounds__(256) void transpose_scale_kernel(const TScaleItem* __restrict__ items) {
+    __shared__ double tr[32][33], ti[32][33];
+    const TScaleItem it = items[blockIdx.z];
+    const int n = it.n;
+    const int tiles = (n + 31) / 32;
+    const long nn = (long)n * n;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+    for (int tb = blockIdx.x; tb < tiles * tiles; tb += gridDim.x) {
+        const int bi = tb / tiles, bj = tb % tiles;
+        for (int r = ty; r < 32; r += 8) {
+            const int i = bi * 32 + r, j = bj * 32 + tx;
+            const bool in = i < n && j < n;
+            tr[r][tx] = in ? it.src[(long)i * n + j] : 0.0;
+            ti[r][tx] = in ? it.src[nn + (long)i * n + j] : 0.0;
+        }
+        __syncthreads();
+        for (int r = ty; r < 32; r += 8) {
+            const int i = bj * 32 + r, j = bi * 32 + tx;  // dst[i][j] = d[i] src[j][i]
+            if (i < n && j < n) {
+                const double vr = tr[tx][r], vi = ti[tx][r], dr = it.d_re[i], di = it.d_im[i];
+                it.dst[(long)i * n + j] = (float)(vr * dr - vi * di);
+                it.dst[nn + (long)i * n + j] = (float)(vr * di + vi * dr);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Right-hand sides between the caller's layout (nrhs interleaved rows of nt*n complex) and
 // blocks [t][re/im][n][nrhs]: 32 x 32 tiles through LDS so that both the row reads and the
 // block writes are contiguous.  Grid (ceil(n/32), ceil(nrhs/32), nt), 256 threads.

@@ -21,10 +21,11 @@
 
 namespace isle {
 
-// POST: SUBTRACT: C = C - A B; FUSED_DIAG (direct-to-LDS kernel only): C = A B, and the row
-// sums of the elementwise product of C with a left factor L (or with the identity when L is
-// null, i.e. diag C) are written per 16-column block: dpart[(s * m + i) * 2 + {0, 1}] =
-// sum_{c in block s} L[i][c] C[i][c], s = 0 .. n/16 - 1 (FP64).
+// POST: SUBTRACT: C = C - A B; FUSED_DIAG (direct-to-LDS kernel only): with P = A B, the row
+// sums of the elementwise product of P with a left factor L (or with the identity when L is
+// null, i.e. diag P) are written per 16-column block: dpart[(s * m + i) * 2 + {0, 1}] =
+// sum_{c in block s} L[i][c] P[i][c], s = 0 .. n/16 - 1 (FP64), and C = diag(post) P when
+// post_re is set (the rows scaled by a complex vector after the sums), else C = P.
 enum : int { SCALE_NONE = 0, SCALE_ROW = 1, SCALE_COL = 2, SUBTRACT = 3, FUSED_DIAG = 4 };
 
 template <typename T>
@@ -524,10 +525,23 @@ void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items, int tiles_n_max) {
     // of them is used: one memory latency per block instead of one per element.  Elements
     // outside the matrix load element 0 and discard it.
     constexpr bool SCALED = POST == SCALE_ROW || POST == SCALE_COL;
+    // FUSED_DIAG with a row factor (uniform over the workgroup): the stored rows are scaled
+    const bool diag_rows = POST == FUSED_DIAG && it.post_re != nullptr;
 #pragma unroll
     for (int i = 0; i < MI; ++i) {
         T fr[NJ][4], fi[NJ][4];
         double lre[NJ][4], lim[NJ][4];
+        if (POST == FUSED_DIAG && diag_rows) {
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = m0 + wm + i * 16 + acc_row<T>(lk, r);
+                    const int di = row < it.m ? row : 0;
+                    fr[j][r] = it.post_re[di];
+                    fi[j][r] = it.post_im[di];
+                }
+        }
         if constexpr (SCALED) {
 #pragma unroll
             for (int j = 0; j < NJ; ++j)
@@ -576,16 +590,6 @@ void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items, int tiles_n_max) {
                 const int col = n0 + wn + j * 16 + lr;
                 const bool in = row < it.m && col < it.n;
                 T vr = acc_re[i][j][r], vi = acc_im[i][j][r];
-                if (SCALED) {
-                    const T t = vr * fr[j][r] - vi * fi[j][r];
-                    vi = vr * fi[j][r] + vi * fr[j][r];
-                    vr = t;
-                }
-                if (in) {
-                    const long off = (long)row * it.ldc + col;
-                    it.c_re[off] = vr;
-                    it.c_im[off] = vi;
-                }
                 if constexpr (POST == FUSED_DIAG) {  // every lane takes part in the row sums
                     const double sr = row16_sum(lre[j][r] * (double)vr - lim[j][r] * (double)vi);
                     const double si = row16_sum(lre[j][r] * (double)vi + lim[j][r] * (double)vr);
@@ -594,6 +598,16 @@ void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items, int tiles_n_max) {
                         it.dpart[((long)slot * it.m + row) * 2] = sr;
                         it.dpart[((long)slot * it.m + row) * 2 + 1] = si;
                     }
+                }
+                if (SCALED || diag_rows) {
+                    const T t = vr * fr[j][r] - vi * fi[j][r];
+                    vi = vr * fi[j][r] + vi * fr[j][r];
+                    vr = t;
+                }
+                if (in) {
+                    const long off = (long)row * it.ldc + col;
+                    it.c_re[off] = vr;
+                    it.c_im[off] = vi;
                 }
             }
     }

@@ -324,7 +324,7 @@ void Plan::run(OpCache& op, const double* phi) {
 static bool is_gemm(K k) {
     return k == K::GEMM64_RC_PRE || k == K::GEMM64_RC_ROW || k == K::GEMM64_CR_PRE || k == K::GEMM64_CC ||
            k == K::GEMM32_RC_ROW || k == K::GEMM32_RC_PRE || k == K::GEMM64_CC_SUB || k == K::GEMM32_RC_PRE_DIAG ||
-           k == K::TRSM;
+           k == K::TRSM || k == K::GEMM32_RC_DIAG;
 }
 
 void Plan::tune(OpCache& op) {
@@ -491,6 +491,15 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
         case K::GEMM32_RC_PRE_DIAG:
             if (!launch_gemm<float, false, true, true, FUSED_DIAG>((const GemmItem<float>*)st.items, st.count, st.max_m,
                                                               st.max_n, s, st.tile, n % 4 == 0 ? n : 0)) return false;
+            break;
+        case K::GEMM32_RC_DIAG:
+            if (!launch_gemm<float, false, true, false, FUSED_DIAG>((const GemmItem<float>*)st.items, st.count,
+                                                               st.max_m, st.max_n, s, st.tile, n % 4 == 0 ? n : 0))
+                return false;
+            break;
+        case K::TSCALE64TO32:
+            hipLaunchKernelGGL(transpose_scale_kernel, dim3(64, 1, st.count), dim3(256), 0, s,
+                               (const TScaleItem*)st.items);
             break;
         case K::DREDUCE:
             hipLaunchKernelGGL(diag_reduce_kernel, dim3(cdiv(n, 256), st.count), dim3(256), 0, s,
@@ -808,7 +817,25 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
             }
         b.add(K::GEMM64_CC, gi, n, n);
     }
-    {
+    // ---- local right chains (FP32), transposed.  With n % 32 == 0 the chain products also
+    //      produce the force diagonal f[tau] = rowdot(Suf_{tau+1}, Rt_tau) (diag Rt at a group's
+    //      last slice) as partial row sums per 16 columns, summed by one reduction at the end,
+    //      and every phase is applied outside the MFMA loop: the group's first phase D_{s_j}
+    //      in the conversion of K_j (K32 = D_{s_j} K_j^T), the next slice's D_{tau+1} to the
+    //      rows a chain product stores (its row sums use the product before that scaling).
+    //      Otherwise the phase scales the complex operand along k as it is read.
+    const bool fuse = n % 32 == 0;
+    const int slots = n / 16;
+    double* part = fuse ? b.d64((size_t)nb * nt * slots * n * 2) : nullptr;
+    if (fuse) {
+        std::vector<TScaleItem> cv;
+        for (int item = 0; item < nb; ++item)
+            for (int j = 0; j < ng; ++j) {
+                const double* src = (j < ng - 1) ? m64(KK, item, j).re : m64(RR, item, ng - 1).re;
+                cv.push_back({src, mK32(item, j).re, D64(item, j * g).re, D64(item, j * g).im, n});
+            }
+        b.add(K::TSCALE64TO32, cv);
+    } else {
         std::vector<TCopyItem<double, float>> cv;
         for (int item = 0; item < nb; ++item)
             for (int j = 0; j < ng; ++j) {
@@ -817,30 +844,29 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
             }
         b.add(K::TCVT64TO32, cv);
     }
-    // ---- local right chains (FP32), transposed.  With n % 32 == 0 the chain products also
-    //      produce the force diagonal f[tau] = rowdot(Suf_{tau+1}, Rt_tau) (diag Rt at a group's
-    //      last slice) as partial row sums per 16 columns, summed by one reduction at the end.
-    const bool fuse = n % 32 == 0;
-    const int slots = n / 16;
-    double* part = fuse ? b.d64((size_t)nb * nt * slots * n * 2) : nullptr;
     for (int q = 0; q < g; ++q) {
         std::vector<GemmItem<float>> gi;
         for (int item = 0; item < nb; ++item)
             for (int j = 0; j < ng; ++j) {
                 const int tau = j * g + q;
                 const CMat<float> prev = q == 0 ? mK32(item, j) : mRt(item, tau - 1);
-                GemmItem<float> it = gitem<float>(A32[spec(item)], nullptr, prev.re, prev.im, mRt(item, tau),
-                                                  D(item, tau).re, D(item, tau).im, nullptr, nullptr, n);
+                GemmItem<float> it;
                 if (fuse) {
-                    if (q < g - 1) {
+                    const bool more = q < g - 1;
+                    it = gitem<float>(A32[spec(item)], nullptr, prev.re, prev.im, mRt(item, tau), nullptr, nullptr,
+                                      more ? D(item, tau + 1).re : nullptr, more ? D(item, tau + 1).im : nullptr, n);
+                    if (more) {
                         it.dl_re = mS(item, tau + 1).re;
                         it.dl_im = mS(item, tau + 1).im;
                     }
                     it.dpart = part + ((long)item * nt + tau) * slots * n * 2;
+                } else {
+                    it = gitem<float>(A32[spec(item)], nullptr, prev.re, prev.im, mRt(item, tau), D(item, tau).re,
+                                      D(item, tau).im, nullptr, nullptr, n);
                 }
                 gi.push_back(it);
             }
-        b.add(fuse ? K::GEMM32_RC_PRE_DIAG : K::GEMM32_RC_PRE, gi, n, n);
+        b.add(fuse ? K::GEMM32_RC_DIAG : K::GEMM32_RC_PRE, gi, n, n);
     }
     if (fuse) {
         std::vector<DReduceItem> dr;

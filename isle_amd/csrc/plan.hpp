@@ -64,6 +64,9 @@ enum class K : int {
     GEMM32_RC_PRE_DIAG, // GEMM32_RC_PRE with the force diagonal's partial row sums fused (f32)
     DREDUCE,            // sum of those partial row sums
     RGATHER,            // rows of right-hand sides in the pivot order of an LU (max_m = rows, aux = columns)
+    GEMM32_RC_DIAG,     // f32 real x complex, no pre-scaling, force diagonal's row sums fused and the
+                        // stored rows scaled by the next slice's phase
+    TSCALE64TO32,       // transposed copy with conversion and complex row scaling
 };
 
 struct Step {
