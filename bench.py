@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--config4", action="store_true",
                     help="the 256-chain ensemble of uneven lattices (L in {3,4,6,8}, Nt in {16,32,64}, beta=Nt/8, "
                          "U=2, 20 steps): prints chain-trajectories/s instead of the config-3 line")
+    ap.add_argument("--right-chains", choices=("fp64", "fp32"), default="fp64",
+                    help="config 5: precision of the stable force's right chains")
     ap.add_argument("--config5", action="store_true",
                     help="the nanotube workload (12 x 16 cell honeycomb cylinder, Nx=384, Nt=256, beta=10, U=2.5, "
                          "mu=0.5, DIA, spin basis, 60 steps) with the stable block-cyclic form: prints "
@@ -90,6 +92,33 @@ def config4(args):
     dt = time.perf_counter() - t0
     pts = np.array(ens.trajPoints[args.warmup:], dtype=float)
     dh = np.array(ens.deltaH[args.warmup:], dtype=float)
+    # algorithmic work of one trajectory of every chain (per species: force
+    # (2Nt-1) 4Nx^3 + 8Nx^3 + 8 Nt Nx^2, eval (Nt-1) 4Nx^3 + 8/3 Nx^3; 21 forces + 1 eval)
+    def traj_flops(nx, nt):
+        force = (2 * nt - 1) * 4 * nx ** 3 + 8 * nx ** 3 + 8 * nt * nx ** 2
+        ev = (nt - 1) * 4 * nx ** 3 + 8 / 3 * nx ** 3
+        return 2 * (21 * force + ev)
+    work = sum(traj_flops(c.nx, c.nt) for c in chains)
+    achieved = work * args.steps / dt / 1e12
+    cpu = None
+    if not args.no_cpu:
+        # the oracle on the host: one force and one eval per (Nx, Nt) bucket on its first
+        # chain's start field, scaled to 21 forces + 1 eval per chain of the bucket
+        from oracle import hubbard_ref as ref
+        total = 0.0
+        for idx in ens.index:
+            c = chains[idx[0]]
+            fa = ref.HubbardFermiAction(c.kappa * c.beta / c.nt, 0.0, -1, ref.EXP)
+            t0 = time.perf_counter()
+            fa.force(c.phi0)
+            tf = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            fa.eval(c.phi0)
+            te = time.perf_counter() - t0
+            total += len(idx) * (21 * tf + te)
+        cpu = {"value": n_chains / total, "unit": "chain-traj/s", "cores": os.cpu_count(), "kind": "port",
+               "sample": "oracle (numpy complex128, OpenBLAS) force and eval once per (Nx, Nt) bucket, scaled to "
+                         "21 forces + 1 eval per chain"}
     print(json.dumps({
         "metric": "chain-trajectories/s, 256 uneven chains (config 4)", "value": n_chains * args.steps / dt,
         "unit": "chain-traj/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -99,7 +128,13 @@ def config4(args):
                                "particle-hole, sigmaKappa=-1, HMC length 1 with 20 steps",
                    "buckets": len(ens.buckets), "parallelism": "buckets on separate streams, one HIP graph"},
         "acceptance": float(pts.mean()) if pts.size else None,
-        "median_dH": float(np.median(dh)) if dh.size else None}))
+        "median_dH": float(np.median(dh)) if dh.size else None,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK["f64"], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK["f64"], "traffic": None,
+                     "note": "all kernels together: the algorithmic flops of the chains' trajectories "
+                             f"({work / 1e12:.3f} TFLOP per trajectory of every chain) per second, against the FP64 "
+                             "MFMA peak (the FP32 right chains could run at twice that)"},
+        "cpu_baseline": cpu}))
 
 
 # the dominant kind's kernel as it appears in the rocprofv3 PMC summary of this workload
@@ -274,7 +309,8 @@ def config5(args):
     kappa = ref.honeycomb(12, 16, periodic_j=False)
     nx = kappa.shape[0]
     kt, mt, ut = kappa * beta / nt, mu * beta / nt, u * beta / nt
-    ferm = A.HubbardFermiAction(kt, mt, -1, A.HFAHopping.DIA, A.HFABasis.SPIN, precision="stable", group=32)
+    ferm = A.HubbardFermiAction(kt, mt, -1, A.HFAHopping.DIA, A.HFABasis.SPIN, precision="stable", group=32,
+                                right_chains=args.right_chains)
     act = A.SumAction(A.HubbardGaugeAction(ut), ferm)
     rngs = [NumpyRNG(7 + c) for c in range(chains)]
     phi0 = np.stack([r.normal(0, np.sqrt(ut), nx * nt) + 0j for r in rngs])
@@ -294,6 +330,14 @@ def config5(args):
     f16 = A.HubbardFermiAction(kt, mt, -1, A.HFAHopping.DIA, A.HFABasis.SPIN, precision="stable",
                                group=16).force(phis[0]).cpu().numpy()
     f_err = float(np.linalg.norm(f32 - f16) / np.linalg.norm(f16))
+    # the stable force of one field, timed (both species)
+    ferm.force(phis[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        ferm.force(phis[0])
+    torch.cuda.synchronize()
+    force_ms = 1e3 * (time.perf_counter() - t0) / 5
     acts = np.array([act.eval(phis[b]) for b in range(chains)])
     lf = BatchedConstStepLeapfrog(act, LENGTH, nstep, rngs, graph=not args.no_graph)
     for _ in range(args.warmup):
@@ -310,14 +354,18 @@ def config5(args):
         "metric": "chain-trajectories/s, Nx=384 Nt=256 DIA spin-basis cylinder (config 5)",
         "value": chains * args.steps / dt, "unit": "chain-traj/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64" if args.right_chains == "fp64" else "f32 right chains + f64",
+        "data": "synthetic",
         "config": {"workload": "honeycomb cylinder 12 (periodic) x 16 (open) cells, Nx=384, Nt=256, beta=10, U=2.5, "
                                "mu=0.5, DIA, spin basis, sigmaKappa=-1, HMC length 1 with 60 steps",
-                   "chains": chains, "precision": "stable", "group": ferm._group(nt),
-                   "parallelism": "chains batched in every launch"},
+                   "chains": chains, "precision": "stable", "right_chains": args.right_chains,
+                   "group": ferm._group(nt), "parallelism": "chains batched in every launch"},
+        "force_ms_one_field": force_ms,
         "logdet_rel_err_vs_oracle": ld_err, "oracle_logdet_cpu_s": t_cpu,
         "reference_product_logdet_rel_err": float(plain_err),
         "force_rel_diff_group32_vs_group16": f_err,
+        "force_rel_diff_note": "the force used (groups of 32, right chains as configured) against FP64 chains over "
+                               "groups of 16",
         "acceptance": float(pts.mean()) if pts.size else None,
         "mean_dH": float(dh.mean()) if dh.size else None,
         "mean_exp_minus_dH": float(np.exp(-dh).mean()) if dh.size else None,
@@ -357,29 +405,40 @@ def force_only(args):
 
 def cpu_baseline(kappa_tilde, ut, nx):
     """The oracle (numpy/OpenBLAS complex128 restatement of the reference algorithm) on
-    the host: 2 forces + 2 evals at the full size, scaled to 41 forces + 1 eval."""
+    the host, at the full size: with every host core, 2 forces + 2 evals after a warm-up
+    force; with one BLAS thread (threadpoolctl), 1 force + 1 eval.  Each is scaled to
+    41 forces + 1 eval per trajectory."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from threadpoolctl import threadpool_limits
     from oracle import hubbard_ref as ref
     act = ref.SumAction(ref.HubbardGaugeAction(ut), ref.HubbardFermiAction(kappa_tilde, 0.0, -1, ref.EXP))
     phi = np.random.RandomState(7).normal(0, np.sqrt(ut), nx * NT) + 0j
+
+    def timed(reps):
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            act.force(phi)
+        tf = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            act.eval(phi)
+        return tf, (time.perf_counter() - t0) / reps
+
     act.force(phi)
-    t0 = time.perf_counter()
-    for _ in range(2):
-        act.force(phi)
-    tf = (time.perf_counter() - t0) / 2
-    t0 = time.perf_counter()
-    for _ in range(2):
-        act.eval(phi)
-    te = (time.perf_counter() - t0) / 2
-    traj = (NSTEP + 1) * tf + te
+    tf, te = timed(2)
+    with threadpool_limits(limits=1):
+        tf1, te1 = timed(1)
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count()
+    traj, traj1 = (NSTEP + 1) * tf + te, (NSTEP + 1) * tf1 + te1
     return {"value": 1.0 / traj, "unit": "traj/s", "cores": cores, "kind": "port",
             "sample": f"oracle (numpy complex128, OpenBLAS) force {tf:.3f} s and eval {te:.3f} s at Nx=288 "
                       f"Nt=128, each timed twice; trajectory = {NSTEP + 1} forces + 1 eval",
-            "force_s": tf, "eval_s": te}
+            "force_s": tf, "eval_s": te,
+            "one_thread": {"value": 1.0 / traj1, "unit": "traj/s", "cores": 1, "force_s": tf1, "eval_s": te1,
+                           "sample": "the same with one BLAS thread, each timed once"}}
 
 
 def main():
@@ -506,14 +565,16 @@ def main():
     pi0 = torch.as_tensor(np.stack([NumpyRNG(77 + c).normal(0, 1, nx * NT) for c in range(rev_n)]) + 0j, device=dev)
     x0 = start_phis[:rev_n]
     rev = {}
-    for name, action in (("mixed", make("mixed")[0]), ("fp64", make("fp64")[0])):
-        p1, q1, a1 = isle.leapfrog_batch(x0, pi0, action, LENGTH, NSTEP, 1.0)
-        p2, _, _ = isle.leapfrog_batch(p1, q1, action, LENGTH, NSTEP, -1.0, evaluate=False)
-        err = (torch.linalg.norm(p2 - x0, dim=1) / torch.linalg.norm(x0, dim=1)).cpu().numpy()
-        kin0 = (pi0.real ** 2).sum(dim=1).cpu().numpy() / 2
-        kin1 = (q1.real ** 2 + q1.imag ** 2).sum(dim=1).cpu().numpy() / 2
-        dh = np.real(np.asarray(a1) + kin1 - start_acts[:rev_n] - kin0)
-        rev[name] = {"err": err, "dH": dh}
+    actions = (("mixed", make("mixed")[0]), ("fp64", make("fp64")[0]))
+    for label, length, nstep in (("", LENGTH, NSTEP), ("_small_step", THERM_LENGTH, THERM_NSTEP)):
+        for name, action in actions:
+            p1, q1, a1 = isle.leapfrog_batch(x0, pi0, action, length, nstep, 1.0)
+            p2, _, _ = isle.leapfrog_batch(p1, q1, action, length, nstep, -1.0, evaluate=False)
+            err = (torch.linalg.norm(p2 - x0, dim=1) / torch.linalg.norm(x0, dim=1)).cpu().numpy()
+            kin0 = (pi0.real ** 2).sum(dim=1).cpu().numpy() / 2
+            kin1 = (q1.real ** 2 + q1.imag ** 2).sum(dim=1).cpu().numpy() / 2
+            dh = np.real(np.asarray(a1) + kin1 - start_acts[:rev_n] - kin0)
+            rev[name + label] = {"err": err, "dH": dh}
     regular = np.abs(rev["fp64"]["dH"]) < 10.0  # trajectories that passed no near-zero of det M
     reversibility = {
         "chains": rev_n, "regular_chains": int(regular.sum()),
@@ -522,6 +583,12 @@ def main():
         "mixed_max_regular": float(rev["mixed"]["err"][regular].max()) if regular.any() else None,
         "fp64_max_regular": float(rev["fp64"]["err"][regular].max()) if regular.any() else None,
         "dH_fp64": [round(float(x), 5) for x in rev["fp64"]["dH"]],
+        "small_step": {"length": THERM_LENGTH, "nstep": THERM_NSTEP,
+                       "mixed_max": float(rev["mixed_small_step"]["err"].max()),
+                       "fp64_max": float(rev["fp64_small_step"]["err"].max()),
+                       "dH_fp64_max_abs": float(np.abs(rev["fp64_small_step"]["dH"]).max()),
+                       "dH_mixed_minus_fp64_max_abs": float(np.abs(rev["mixed_small_step"]["dH"]
+                                                                   - rev["fp64_small_step"]["dH"]).max())},
     }
 
     # ---------------- kernel roofline (mixed force of the whole chain batch)
@@ -589,9 +656,11 @@ def main():
         "reversibility": reversibility,
         "reversibility_err": reversibility["mixed_max_regular"],
         "reversibility_err_plain_fp64": reversibility["fp64_max_regular"],
-        "reversibility_note": "|phi(fwd then reversed) - phi0| / |phi0| per chain on thermalised fields; max over "
-                              "the chains whose FP64 trajectory has |dH| < 10 (the others pass near a zero of det M, "
-                              "where the trajectory itself is chaotic), medians over all",
+        "reversibility_note": "|phi(fwd then reversed) - phi0| / |phi0| per chain on thermalised fields, same momenta "
+                              "for both forces; at length 1 / 40 steps: max over the chains whose FP64 trajectory has "
+                              "|dH| < 10 (the others pass near a zero of det M, where the trajectory itself is "
+                              "chaotic), medians over all; small_step: the same at length 0.25 / 100 steps, where "
+                              "every trajectory is regular",
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -43,6 +43,12 @@ int isle_hfm_create(isle_hfm** out, int hopping, int nx, const double* kappa_til
 int isle_hfm_destroy(isle_hfm* h);
 /* stream: a hipStream_t (e.g. torch.cuda.current_stream().cuda_stream) */
 int isle_hfm_set_stream(isle_hfm* h, void* stream);
+/* Memory schedule of the mixed-precision force (no reference counterpart; the reference keeps
+ * every left product, hubbardFermiAction.cpp:30-40): k > 0 keeps the FP64 suffix chains only
+ * every k slices inside each group and recomputes the rest segment by segment (k must divide
+ * the group size and nx % 32 == 0, else every product is kept); 0 keeps them all.  Applies to
+ * force operations first used after the call. */
+int isle_hfm_set_recompute(isle_hfm* h, int k);
 int isle_hfm_sync(isle_hfm* h);
 
 /* HubbardFermiMatrixExp::expKappa(species, inv) / HubbardFermiMatrixDia::K(species)
@@ -75,7 +81,8 @@ int isle_logdetM_batch(isle_hfm* h, const double* phi, int nt, int batch, int mo
  * mode 0: plain FP64 reference algorithm; mode 1: FP32 chains over groups of `group`
  * slices on MFMA with an FP64 combination of the group products; mode 2: FP64 chains over
  * the groups with the combination taken from the elimination of mode 2 of logdetM_batch
- * (same size limits).
+ * (same size limits); mode 3: mode 2 with the right chains in FP32 on MFMA (two groups or
+ * more).
  * out_dev: device [batch][nspecies][nx*nt] complex128. */
 int isle_force_parts(isle_hfm* h, const double* phi, int nt, int batch, int mode, int group, int nspecies,
                      int spin, double* out_dev);

@@ -28,12 +28,13 @@ from . import action, autograd, drivers, ensemble, evolver, meas, runfile, util
 from .core import (QLU, CDVector, HubbardFermiMatrixDia, HubbardFermiMatrixExp, Species, Vector, applyM, as_field,
                    getQLU, invert, logdet, logdetM, logdetQ, solveM, solveQ, to_first_log_branch)
 from .integrator import leapfrog, leapfrog_batch, rungeKutta4Flow
-from .lattice import Lattice, honeycomb, isBipartite, loadLattice
+from .lattice import LATTICES, Lattice, honeycomb, isBipartite, loadLattice
 
 random = evolver  # isle.random.NumpyRNG lives in isle_amd.evolver
 
 __all__ = [
-    "CDVector", "HubbardFermiMatrixDia", "HubbardFermiMatrixExp", "Lattice", "QLU", "Species", "Vector", "action",
+    "CDVector", "HubbardFermiMatrixDia", "HubbardFermiMatrixExp", "LATTICES", "Lattice", "QLU", "Species", "Vector",
+    "action",
     "autograd", "getQLU", "logdetQ", "solveQ",
     "applyM", "drivers", "ensemble", "meas", "runfile", "util",
     "as_field", "evolver", "honeycomb", "invert", "isBipartite", "leapfrog", "leapfrog_batch", "loadLattice",

@@ -10,11 +10,16 @@ Besides the reference's single-field ``eval``/``force``, HubbardFermiAction offe
 
 * ``"fp64"``  - the reference DIRECT_SINGLE algorithm, every product in FP64;
 * ``"mixed"`` - force from FP32 chains over groups of ``group`` time slices on MFMA,
-  combined in FP64; eval (the Metropolis action) from FP64 group products.
+  combined in FP64; eval (the Metropolis action) from FP64 group products.  With
+  ``recompute=k`` (k dividing ``group``, Nx a multiple of 32) the force keeps the FP64
+  suffix chains only every k slices and recomputes the others when they are needed:
+  about a third of the force's memory for about half again its FP64 work.
 * ``"stable"`` - for ill-conditioned products (large beta, spin basis, mu != 0, where the
   product of all time slices spans more decades than FP64 holds): eval and force from FP64
   group products combined through the structured pivoted elimination of their block-cyclic
-  matrix, never through the product of all groups.
+  matrix, never through the product of all groups.  ``right_chains="fp32"`` runs the
+  force's per-group right chains in FP32 on MFMA (with the force diagonal fused into them);
+  the elimination, the suffix chains and the action stay FP64.
 """
 import enum
 
@@ -157,9 +162,13 @@ class HubbardFermiAction(Action):
 
     def __init__(self, kappaTilde, muTilde, sigmaKappa, hopping=HFAHopping.DIA,
                  basis=HFABasis.PARTICLE_HOLE, algorithm=HFAAlgorithm.DIRECT_SINGLE,
-                 allowShortcut=False, precision="fp64", group=32):
+                 allowShortcut=False, precision="fp64", group=32, recompute=0, right_chains="fp64"):
         if precision not in ("fp64", "mixed", "stable"):
             raise ValueError("precision must be 'fp64', 'mixed' or 'stable'")
+        if right_chains not in ("fp64", "fp32"):
+            raise ValueError("right_chains must be 'fp64' or 'fp32'")
+        if int(recompute) < 0:
+            raise ValueError("recompute must be >= 0")
         if HFAAlgorithm(algorithm) == HFAAlgorithm.DIRECT_SQUARE and precision != "fp64":
             raise ValueError("HFAAlgorithm.DIRECT_SQUARE runs in FP64 only")
         kappa = kappaTilde.hopping() if isinstance(kappaTilde, Lattice) else np.asarray(kappaTilde, dtype=np.float64)
@@ -170,6 +179,14 @@ class HubbardFermiAction(Action):
         self.hfm = cls(kappa, muTilde, sigmaKappa)
         self.precision = precision
         self.group = int(group)
+        # memory schedule of the mixed force: keep the FP64 suffix chains every `recompute`
+        # slices and recompute the rest (0: keep all; see isle_hfm_set_recompute)
+        self.recompute = int(recompute)
+        # the stable force's right chains: FP64 (default) or FP32 on MFMA with the force
+        # diagonal fused into the chain products (as in the mixed form)
+        self.right_chains = right_chains
+        if self.recompute:
+            _lib.check(_lib.load().isle_hfm_set_recompute(self.hfm.handle, self.recompute))
         # log of the condition-number bound of one time slice: the FP32 right chains of
         # the mixed force span at most `group` slices and are kept within cond <= e^12
         r = float(np.max(np.sum(np.abs(kappa), axis=1))) if kappa.size else 0.0
@@ -228,7 +245,9 @@ class HubbardFermiAction(Action):
         f = f.reshape(-1)
         return -1j * f if self.basis == HFABasis.SPIN else f
 
-    def _mode(self):
+    def _mode(self, force=False):
+        if force and self.precision == "stable" and self.right_chains == "fp32":
+            return 3
         return {"fp64": 0, "mixed": 1, "stable": 2}[self.precision]
 
     def _group(self, nt):
@@ -291,7 +310,7 @@ class HubbardFermiAction(Action):
         nx = self.hfm.nx()
         nsp = 1 if self.shortcut else 2
         parts = torch.empty((batch, nsp, nx * nt), dtype=torch.complex128, device=phis.device)
-        _lib.check(_lib.load().isle_force_parts(self.hfm.handle, _lib.ptr(phis), nt, batch, self._mode(),
+        _lib.check(_lib.load().isle_force_parts(self.hfm.handle, _lib.ptr(phis), nt, batch, self._mode(force=True),
                                                 self._group(nt), nsp, int(self.basis == HFABasis.SPIN),
                                                 _lib.ptr(parts)))
         return parts

@@ -369,11 +369,15 @@ __global__ void transpose_items_kernel(const TCopyItem<S, D>* __restrict__ items
     }
 }
 
-// dst = diag(d) src^T for split-plane complex n x n matrices: FP64 src (re plane, then the
-// im plane n*n further), FP32 dst, d complex (d_re, d_im; FP64).  The product is formed in
-// FP64 and rounded once.  32 x 32 tiles of both planes through LDS; grid (64, 1, items).
+// dst = sign diag(d) src^T for complex n x n matrices: FP64 src as re / im planes with row
+// stride lds (a block of a larger matrix is fine), FP32 dst as split planes (im n*n after
+// re), d complex (d_re, d_im; FP64).  The product is formed in FP64 and rounded once.
+// 32 x 32 tiles of both planes through LDS; grid (64, 1, items).
 struct TScaleItem {
-    const double* src;
+    const double* s_re;
+    const double* s_im;
+    long lds;
+    double sign;
     float* dst;
     const double* d_re;
     const double* d_im;
@@ -392,14 +396,15 @@ __global__ __launch_bounds__(256) void transpose_scale_kernel(const TScaleItem* 
         for (int r = ty; r < 32; r += 8) {
             const int i = bi * 32 + r, j = bj * 32 + tx;
             const bool in = i < n && j < n;
-            tr[r][tx] = in ? it.src[(long)i * n + j] : 0.0;
-            ti[r][tx] = in ? it.src[nn + (long)i * n + j] : 0.0;
+            tr[r][tx] = in ? it.s_re[(long)i * it.lds + j] : 0.0;
+            ti[r][tx] = in ? it.s_im[(long)i * it.lds + j] : 0.0;
         }
         __syncthreads();
         for (int r = ty; r < 32; r += 8) {
-            const int i = bj * 32 + r, j = bi * 32 + tx;  // dst[i][j] = d[i] src[j][i]
+            const int i = bj * 32 + r, j = bi * 32 + tx;  // dst[i][j] = sign d[i] src[j][i]
             if (i < n && j < n) {
-                const double vr = tr[tx][r], vi = ti[tx][r], dr = it.d_re[i], di = it.d_im[i];
+                const double vr = it.sign * tr[tx][r], vi = it.sign * ti[tx][r];
+                const double dr = it.d_re[i], di = it.d_im[i];
                 it.dst[(long)i * n + j] = (float)(vr * dr - vi * di);
                 it.dst[nn + (long)i * n + j] = (float)(vr * di + vi * dr);
             }

@@ -89,8 +89,10 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
     const int n0 = tile_n * BN;
     if (m0 >= it.m || n0 >= it.n) return;
 
-    __shared__ T As_re[2][GEMM_BK][BM];
-    __shared__ T As_im[2][AC ? GEMM_BK : 1][AC ? BM : 1];
+    // A is stored transposed (k rows): consecutive threads write consecutive k, one LDS row
+    // apart; one element of padding per row puts those writes in different banks
+    __shared__ T As_re[2][GEMM_BK][BM + 1];
+    __shared__ T As_im[2][AC ? GEMM_BK : 1][AC ? BM + 1 : 1];
     __shared__ T Bs_re[2][GEMM_BK][BN];
     __shared__ T Bs_im[2][BC ? GEMM_BK : 1][BC ? BN : 1];
 
@@ -648,7 +650,7 @@ inline bool launch_gemm_rcd(const GemmItem<T>* d_items, int nitems, int max_m, i
 
 template <typename T, bool AC, bool BC, int BM, int BN, int BK>
 constexpr size_t gemm_lds_bytes() {
-    return sizeof(T) * 2 * BK * (BM * (AC ? 2 : 1) + BN * (BC ? 2 : 1));
+    return sizeof(T) * 2 * BK * ((BM + 1) * (AC ? 2 : 1) + BN * (BC ? 2 : 1));
 }
 
 // Launches the GEMM with tile shape `tile`; returns false if that shape does not fit

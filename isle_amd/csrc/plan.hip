@@ -369,6 +369,7 @@ void Plan::profile(const double* phi, int opk, int nt, int batch, int group, int
                   : opk == 2 ? get_logdet_plain(nt, batch, nspecies, spin)
                   : opk == 3 ? get_logdet_grouped(nt, batch, group, nspecies, spin)
                   : opk == 4 ? get_force_stable(nt, batch, group, nspecies, spin)
+                  : opk == 6 ? get_force_stable(nt, batch, group, nspecies, spin, true)
                              : get_logdet_stable(nt, batch, group, nspecies, spin);
     run(op, phi);  // warm up
     const size_t ns = op.steps.size();
@@ -656,9 +657,17 @@ OpCache& Plan::get_force_plain(int nt, int batch, int nspecies, int spin) {
 //   f[e_j] = diag(RT_{e_j}).
 // This is the reference's lefts/right factorisation applied inside each group with the
 // global factors folded into K_j; FP32 chains never span more than g slices.
+//
+// Memory: with n % 32 == 0 (the force diagonal fused into the right-chain products) a right
+// chain keeps only its last two products.  With a recompute interval k (set_recompute;
+// 0 < k < g, k | g) the suffix chains keep only Suf at s_j + m k (m = 0 .. g/k - 1; m = 0 is
+// P_j) and recompute the k - 1 matrices between two of them just before the right chain
+// needs them, segment by segment: about g more FP64 products per group for
+// (g/k + k + 1) instead of g stored suffix matrices.
 OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin) {
     if (g <= 0 || nt % g != 0) throw std::invalid_argument("group size must divide nt");
-    auto key = std::make_tuple(1, nt, batch, g, nspecies, spin);
+    const int ck = (recompute_ > 0 && recompute_ < g && g % recompute_ == 0 && nx_ % 32 == 0) ? recompute_ : 0;
+    auto key = std::make_tuple(1 + 16 * ck, nt, batch, g, nspecies, spin);
     auto found = cache_.find(key);
     if (found != cache_.end()) return *found->second;
     auto op = std::make_unique<OpCache>();
@@ -673,8 +682,14 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
     const int ng = nt / g;
     float* ph = b.d32((size_t)nb * 2 * nt * n);
     double* ph64 = b.d64((size_t)nb * 2 * nt * n);
-    double* Suf = b.d64((size_t)nb * nt * 2 * nn);
-    float* Rt = b.d32((size_t)nb * nt * 2 * nn);
+    const bool fuse = n % 32 == 0;
+    const int ncp = ck ? g / ck : g;  // stored suffix matrices per group (checkpoints)
+    // suffix storage: all of them, or the checkpoints + 2 ping-pong matrices of the first
+    // pass + k - 1 recomputed matrices of the current segment, per (item, group)
+    const int suf_per_group = ck ? ncp + 2 + (ck - 1) : g;
+    double* Suf = b.d64((size_t)nb * ng * suf_per_group * 2 * nn);
+    // right chains: every product, or the last two per group when the diagonal is fused
+    float* Rt = b.d32((size_t)nb * (fuse ? 2 * ng : nt) * 2 * nn);
     double* GL = b.d64((size_t)nb * ng * 2 * nn);
     double* RR = b.d64((size_t)nb * ng * 2 * nn);
     double* KK = b.d64((size_t)nb * ng * 2 * nn);
@@ -688,8 +703,28 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
         return CMat<double>{ph64 + (long)item * 2 * nt * n + (long)t * n,
                             ph64 + (long)item * 2 * nt * n + (long)nt * n + (long)t * n};
     };
-    auto mS = [&](int item, int t) { return cmat(Suf, (long)item * nt + t, nn); };
-    auto mRt = [&](int item, int t) { return cmat(Rt, (long)item * nt + t, nn); };
+    // slot s of group j's suffix storage
+    auto sufSlot = [&](int item, int j, int slot) {
+        return cmat(Suf, ((long)item * ng + j) * suf_per_group + slot, nn);
+    };
+    // where the first (full) pass keeps Suf_t: every t, or checkpoints / ping-pong
+    auto mS = [&](int item, int t) {
+        const int j = t / g, r = t % g;
+        if (!ck) return sufSlot(item, j, r);
+        return r % ck == 0 ? sufSlot(item, j, r / ck) : sufSlot(item, j, ncp + (t & 1));
+    };
+    // Suf_t as the right chain reads it (t = s_j + 1 .. e_j): stored, or recomputed into the
+    // segment slots (the top of a segment below the last is the next checkpoint)
+    auto mSL = [&](int item, int t) {
+        const int j = t / g, r = t % g;
+        if (!ck) return sufSlot(item, j, r);
+        const int m = (r - 1) / ck;
+        if (r == (m + 1) * ck) return sufSlot(item, j, m + 1);
+        return sufSlot(item, j, ncp + 2 + (r - m * ck - 1));
+    };
+    auto mRt = [&](int item, int t) {
+        return fuse ? cmat(Rt, ((long)item * ng + t / g) * 2 + (t & 1), nn) : cmat(Rt, (long)item * nt + t, nn);
+    };
     auto m64 = [&](double* base, int item, int j) { return cmat(base, (long)item * ng + j, nn); };
     auto mP = [&](int item, int j) { return mS(item, j * g); };  // group product P_j = Suf_{s_j}
     auto mK32 = [&](int item, int j) { return cmat(K32, (long)item * ng + j, nn); };
@@ -824,7 +859,6 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
     //      in the conversion of K_j (K32 = D_{s_j} K_j^T), the next slice's D_{tau+1} to the
     //      rows a chain product stores (its row sums use the product before that scaling).
     //      Otherwise the phase scales the complex operand along k as it is read.
-    const bool fuse = n % 32 == 0;
     const int slots = n / 16;
     double* part = fuse ? b.d64((size_t)nb * nt * slots * n * 2) : nullptr;
     if (fuse) {
@@ -832,7 +866,7 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
         for (int item = 0; item < nb; ++item)
             for (int j = 0; j < ng; ++j) {
                 const double* src = (j < ng - 1) ? m64(KK, item, j).re : m64(RR, item, ng - 1).re;
-                cv.push_back({src, mK32(item, j).re, D64(item, j * g).re, D64(item, j * g).im, n});
+                cv.push_back({src, src + nn, n, 1.0, mK32(item, j).re, D64(item, j * g).re, D64(item, j * g).im, n});
             }
         b.add(K::TSCALE64TO32, cv);
     } else {
@@ -845,6 +879,34 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
         b.add(K::TCVT64TO32, cv);
     }
     for (int q = 0; q < g; ++q) {
+        if (ck && q % ck == 0) {
+            // recompute the suffix matrices the next ck right-chain steps read:
+            // Suf_t, t = s_j + q + 1 .. s_j + q + ck (the top one is a checkpoint or, in the
+            // last segment, X_{e_j} itself)
+            const int top = q + ck;  // offset of the segment's top in its group
+            if (top == g) {
+                std::vector<ScaleItem<double>> sc;
+                for (int item = 0; item < nb; ++item)
+                    for (int j = 0; j < ng; ++j) {
+                        const int e = j * g + g - 1;
+                        sc.push_back({A64[spec(item)], D64(item, e).re, D64(item, e).im, mSL(item, e).re,
+                                      mSL(item, e).im, n, SCALE_ROW});
+                    }
+                b.add(K::SCALE64, sc);
+            }
+            const int first = top == g ? top - 2 : top - 1;  // offset of the first product to form
+            for (int r = first; r > q; --r) {
+                std::vector<GemmItem<double>> gi;
+                for (int item = 0; item < nb; ++item)
+                    for (int j = 0; j < ng; ++j) {
+                        const int t = j * g + r;
+                        const CMat<double> src = mSL(item, t + 1);
+                        gi.push_back(gitem<double>(A64[spec(item)], nullptr, src.re, src.im, mSL(item, t), nullptr,
+                                                   nullptr, D64(item, t).re, D64(item, t).im, n));
+                    }
+                b.add(K::GEMM64_RC_ROW, gi, n, n);
+            }
+        }
         std::vector<GemmItem<float>> gi;
         for (int item = 0; item < nb; ++item)
             for (int j = 0; j < ng; ++j) {
@@ -856,8 +918,8 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
                     it = gitem<float>(A32[spec(item)], nullptr, prev.re, prev.im, mRt(item, tau), nullptr, nullptr,
                                       more ? D(item, tau + 1).re : nullptr, more ? D(item, tau + 1).im : nullptr, n);
                     if (more) {
-                        it.dl_re = mS(item, tau + 1).re;
-                        it.dl_im = mS(item, tau + 1).im;
+                        it.dl_re = mSL(item, tau + 1).re;
+                        it.dl_im = mSL(item, tau + 1).im;
                     }
                     it.dpart = part + ((long)item * nt + tau) * slots * n * 2;
                 } else {
@@ -1209,9 +1271,10 @@ OpCache& Plan::get_logdet_stable(int nt, int batch, int g, int nspecies, int spi
 // (ng-1, 0, 1, ..., ng-2) so that every step works on a prefix of the columns; then the FP64
 // right chains Rt_tau = A_s (D_tau Rt_{tau-1}) from Rt_{s_j - 1} = K_j^T and
 // f[tau] = diag(Suf_{tau+1} R_tau) as in the other forms.
-OpCache& Plan::get_force_stable(int nt, int batch, int g, int nspecies, int spin) {
+OpCache& Plan::get_force_stable(int nt, int batch, int g, int nspecies, int spin, bool right32) {
     if (g <= 0 || nt % g != 0) throw std::invalid_argument("group size must divide nt");
-    auto key = std::make_tuple(4, nt, batch, g, nspecies, spin);
+    if (right32 && nt / g < 2) throw std::invalid_argument("FP32 right chains of the stable force need two groups or more");
+    auto key = std::make_tuple(right32 ? 6 : 4, nt, batch, g, nspecies, spin);
     auto found = cache_.find(key);
     if (found != cache_.end()) return *found->second;
     auto op = std::make_unique<OpCache>();
@@ -1226,9 +1289,12 @@ OpCache& Plan::get_force_stable(int nt, int batch, int g, int nspecies, int spin
     const int ng = nt / g;
     const long wd = (long)ng * n;  // columns of the block solves
     double* ph = b.d64((size_t)nb * 2 * nt * n);
+    float* ph32 = right32 ? b.d32((size_t)nb * 2 * nt * n) : nullptr;
     double* Suf = b.d64((size_t)nb * nt * 2 * nn);
-    double* Rt = b.d64((size_t)nb * nt * 2 * nn);
-    double* Kt = b.d64((size_t)nb * ng * 2 * nn);
+    double* Rt = right32 ? nullptr : b.d64((size_t)nb * nt * 2 * nn);
+    float* Rt32 = right32 ? b.d32((size_t)nb * ng * 2 * 2 * nn) : nullptr;  // last two products per group
+    double* Kt = right32 ? nullptr : b.d64((size_t)nb * ng * 2 * nn);
+    float* K32 = right32 ? b.d32((size_t)nb * ng * 2 * nn) : nullptr;
     double* Tb = b.d64((size_t)nb * ng * 2 * n * wd);
     op->parts = b.d64((size_t)nb * nt * n * 2);
     auto D = [&](int item, int t) {
@@ -1242,9 +1308,14 @@ OpCache& Plan::get_force_stable(int nt, int batch, int g, int nspecies, int spin
     auto spec = [&](int item) { return item % nspecies; };
     const double* A64[2] = {a64_[0]->as<double>(), a64_[1]->as<double>()};
 
+    auto D32 = [&](int item, int t) {
+        return CMat<float>{ph32 + (long)item * 2 * nt * n + (long)t * n,
+                           ph32 + (long)item * 2 * nt * n + (long)nt * n + (long)t * n};
+    };
     std::vector<PhaseItem> phs;
     for (int item = 0; item < nb; ++item)
-        phs.push_back({item / nspecies, spin, sigma_of(spec(item)), D(item, 0).re, D(item, 0).im, nullptr, nullptr, n, nt});
+        phs.push_back({item / nspecies, spin, sigma_of(spec(item)), D(item, 0).re, D(item, 0).im,
+                       right32 ? D32(item, 0).re : nullptr, right32 ? D32(item, 0).im : nullptr, n, nt});
     b.add(K::PHASE, phs);
     {
         std::vector<ScaleItem<double>> sc;
@@ -1391,14 +1462,67 @@ OpCache& Plan::get_force_stable(int nt, int batch, int g, int nspecies, int spin
             b.add(K::BLOCK, cp, 0, 0, 0, n * w);
             add_trsm(b, 1, n, w, tu);
         }
-        std::vector<TBlockItem> tb;
+        if (right32) {
+            // K_j^T straight to FP32 with the group's first phase folded in: D_{s_j} K_j^T
+            std::vector<TScaleItem> cv;
+            for (int item = 0; item < nb; ++item)
+                for (int j = 0; j < ng; ++j) {
+                    const CMat<double> x = j < ng - 1 ? mT(item, j + 1) : mT(item, 0);
+                    const long col = j < ng - 1 ? (long)(j + 1) * n : 0;
+                    float* dst = K32 + ((long)item * ng + j) * 2 * nn;
+                    cv.push_back({x.re + col, x.im + col, wd, j < ng - 1 ? -1.0 : 1.0, dst, D(item, j * g).re,
+                                  D(item, j * g).im, n});
+                }
+            b.add(K::TSCALE64TO32, cv);
+        } else {
+            std::vector<TBlockItem> tb;
+            for (int item = 0; item < nb; ++item)
+                for (int j = 0; j < ng; ++j) {
+                    const CMat<double> x = j < ng - 1 ? mT(item, j + 1) : mT(item, 0);
+                    const long col = j < ng - 1 ? (long)(j + 1) * n : 0;
+                    tb.push_back({x.re + col, x.im + col, mK(item, j).re, wd, n, j < ng - 1 ? -1.0 : 1.0});
+                }
+            b.add(K::TBLOCK, tb);
+        }
+    }
+    if (right32) {
+        // ---- local right chains in FP32 on MFMA with the force diagonal fused (as in the
+        //      mixed form): RT_tau = A_s (D_tau RT_{tau-1}), the phase of the next slice
+        //      applied to the stored rows, f[tau] = rowdot(Suf_{tau+1}, RT_tau)
+        const float* A32[2] = {a32_[0]->as<float>(), a32_[1]->as<float>()};
+        const int slots = n / 16;
+        double* part = b.d64((size_t)nb * nt * slots * n * 2);
+        auto mR32 = [&](int item, int t) { return cmat(Rt32, ((long)item * ng + t / g) * 2 + (t & 1), nn); };
+        for (int q = 0; q < g; ++q) {
+            std::vector<GemmItem<float>> gi;
+            for (int item = 0; item < nb; ++item)
+                for (int j = 0; j < ng; ++j) {
+                    const int tau = j * g + q;
+                    const bool more = q < g - 1;
+                    const CMat<float> prev = q == 0 ? cmat(K32, (long)item * ng + j, nn) : mR32(item, tau - 1);
+                    GemmItem<float> it = gitem<float>(A32[spec(item)], nullptr, prev.re, prev.im, mR32(item, tau),
+                                                      nullptr, nullptr, more ? D32(item, tau + 1).re : nullptr,
+                                                      more ? D32(item, tau + 1).im : nullptr, n);
+                    if (more) {
+                        it.dl_re = mS(item, tau + 1).re;
+                        it.dl_im = mS(item, tau + 1).im;
+                    }
+                    it.dpart = part + ((long)item * nt + tau) * slots * n * 2;
+                    gi.push_back(it);
+                }
+            b.add(K::GEMM32_RC_DIAG, gi, n, n);
+        }
+        std::vector<DReduceItem> dr;
         for (int item = 0; item < nb; ++item)
-            for (int j = 0; j < ng; ++j) {
-                const CMat<double> x = j < ng - 1 ? mT(item, j + 1) : mT(item, 0);
-                const long col = j < ng - 1 ? (long)(j + 1) * n : 0;
-                tb.push_back({x.re + col, x.im + col, mK(item, j).re, wd, n, j < ng - 1 ? -1.0 : 1.0});
-            }
-        b.add(K::TBLOCK, tb);
+            for (int tau = 0; tau < nt; ++tau)
+                dr.push_back({part + ((long)item * nt + tau) * slots * n * 2, op->parts + ((long)item * nt + tau) * 2 * n,
+                              n, slots});
+        b.add(K::DREDUCE, dr);
+        b.finish();
+        tune(*op);
+        auto& ref = *op;
+        cache_[key] = std::move(op);
+        return ref;
     }
     // ---- local right chains (FP64), transposed
     for (int q = 0; q < g; ++q) {
@@ -1477,7 +1601,7 @@ void Plan::force_parts(const double* phi, int nt, int batch, int mode, int group
     if (nt < 2) throw std::invalid_argument("nt < 2 in HubbardFermiAction algorithm DIRECT_SINGLE not supported");
     OpCache& op = mode == 0   ? get_force_plain(nt, batch, nspecies, spin)
                   : mode == 1 ? get_force_mixed(nt, batch, group, nspecies, spin)
-                              : get_force_stable(nt, batch, group, nspecies, spin);
+                              : get_force_stable(nt, batch, group, nspecies, spin, mode == 3);
     run(op, phi);
     const size_t bytes = (size_t)batch * nspecies * nt * nx_ * 2 * 8;
     ISLE_HIP(hipMemcpyAsync(out_dev, op.parts, bytes, hipMemcpyDeviceToDevice, stream_));
@@ -1846,6 +1970,10 @@ int isle_hfm_set_stream(isle_hfm* h, void* stream) {
     return guarded([&] { P(h)->set_stream(reinterpret_cast<hipStream_t>(stream)); });
 }
 
+int isle_hfm_set_recompute(isle_hfm* h, int k) {
+    return guarded([&] { P(h)->set_recompute(k); });
+}
+
 int isle_hfm_slice_matrix(isle_hfm* h, int species, int inv, double* out_host) {
     return guarded([&] {
         check_species(species);
@@ -1883,7 +2011,7 @@ int isle_force_parts(isle_hfm* h, const double* phi, int nt, int batch, int mode
                      double* out_dev) {
     return guarded([&] {
         if (batch < 1 || (nspecies != 1 && nspecies != 2)) throw std::invalid_argument("bad batch/nspecies");
-        if (mode < 0 || mode > 2) throw std::invalid_argument("mode must be 0, 1 or 2");
+        if (mode < 0 || mode > 3) throw std::invalid_argument("mode must be 0, 1, 2 or 3");
         P(h)->force_parts(phi, nt, batch, mode, group, nspecies, spin, out_dev);
     });
 }

@@ -99,6 +99,12 @@ public:
     int nx() const { return nx_; }
     int hopping() const { return hop_; }
     void set_stream(hipStream_t s) { stream_ = s; }
+    // Recompute interval k of the mixed force's suffix chains (0: keep them all); applies to
+    // force operations built afterwards (see get_force_mixed).
+    void set_recompute(int k) {
+        if (k < 0) throw std::invalid_argument("recompute interval must be >= 0");
+        recompute_ = k;
+    }
     hipStream_t stream() const { return stream_; }
 
     // log det M for a batch of fields (consecutive, nx*nt complex each) and nspecies
@@ -136,6 +142,7 @@ private:
     int sk_;
     std::vector<double> kappa_;
     hipStream_t stream_ = nullptr;
+    int recompute_ = 0;
     // inverse-form slice matrices A_s (EXP: exp(-k+mu), exp(-s k-mu); DIA: K_p, K_h)
     std::unique_ptr<DevBuf> a64_[2], a32_[2];
     // EXP non-inverse matrices exp(k-mu), exp(s k+mu)
@@ -162,7 +169,7 @@ private:
     OpCache& get_force_mixed(int nt, int batch, int group, int nspecies, int spin);
     OpCache& get_logdet_plain(int nt, int batch, int nspecies, int spin);
     OpCache& get_logdet_grouped(int nt, int batch, int group, int nspecies, int spin);
-    OpCache& get_force_stable(int nt, int batch, int group, int nspecies, int spin);
+    OpCache& get_force_stable(int nt, int batch, int group, int nspecies, int spin, bool right32 = false);
     OpCache& get_logdet_stable(int nt, int batch, int group, int nspecies, int spin);
     void run(OpCache& op, const double* phi);
     bool launch_step(OpCache& op, const Step& st, const double* phi);

@@ -4,6 +4,9 @@ Mirrors src/isle/cpp/lattice.hpp:25-163 (Lattice), src/isle/cpp/lattice.cpp:16-8
 (isBipartite) and src/isle/yamlio.py:16-50 (the ``!lattice`` YAML format).
 """
 
+import json
+import pathlib
+
 import numpy as np
 import yaml
 
@@ -152,6 +155,49 @@ def loadLattice(text_or_file):
     """Parse a ``!lattice`` YAML document (string or open file) with a safe loader."""
     text = text_or_file.read() if hasattr(text_or_file, "read") else text_or_file
     return yaml.load(text, Loader=_LatticeLoader)
+
+
+class _BuiltinLattices:
+    """The lattices that ship with the package (src/isle/__init__.py:38-80): ``keys()``
+    lists their names, ``[name]`` builds one (nt = 0) and ``loadExternal(fname)`` reads a
+    ``!lattice`` YAML file.  The data is isle_amd/resources/lattices.json."""
+
+    _DATA = pathlib.Path(__file__).resolve().parent / "resources" / "lattices.json"
+
+    def __init__(self):
+        self._table = None
+
+    def _entries(self):
+        if self._table is None:
+            if not self._DATA.is_file():
+                raise RuntimeError(f"the built-in lattice data {self._DATA} is missing from this installation")
+            self._table = json.loads(self._DATA.read_text())
+        return self._table
+
+    def keys(self):
+        return list(self._entries())
+
+    def __contains__(self, name):
+        return name in self._entries()
+
+    def __getitem__(self, name):
+        entry = self._entries().get(name)
+        if entry is None:
+            raise ValueError(f"no built-in lattice named {name!r}; known: {', '.join(self.keys())} "
+                             "(LATTICES.loadExternal reads a lattice file)")
+        lat = Lattice(0, len(entry["positions"]), entry["name"], entry["comment"])
+        for (i, j), strength in zip(entry["adjacency"], entry["hopping"]):
+            lat.setNeighbor(i, j, strength)
+        for i, pos in enumerate(entry["positions"]):
+            lat.position(i, *pos)
+        return lat
+
+    def loadExternal(self, fname):
+        with open(fname) as f:
+            return loadLattice(f)
+
+
+LATTICES = _BuiltinLattices()
 
 
 def honeycomb(l_i, l_j=None, periodic_j=True, kappa=1.0, nt=0):

@@ -15,17 +15,18 @@ mu=0, EXP, chain c seeded NumpyRNG(1000 + c), the 12 (L, Nt) buckets of the benc
 draw): the ensemble advances all 256 chains one trajectory (length 1, 20 steps; mixed
 force, FP64 action); for the chains of every bucket, in order, the oracle's
 const_step_leapfrog from the same seed gives the same accept decision, and Delta H within
-1e-4 (absolute) until one chain per bucket has a regular trajectory (|Delta H| < 5; 3 of
-the 12 first chains pass near a zero of det M, where Delta H ~ 1e1..1e4 differs between any
-two evaluation orders and only the decision is compared).  The bucket's mixed force at
+5e-5 max(1, |Delta H|) until one chain per bucket has a regular trajectory (|Delta H| < 50;
+2 of the 12 first chains and most random starts at Nx=128, Nt=64 pass near a zero of det M,
+where Delta H ~ 1e3..1e4 differs between any two evaluation orders and only the decision is
+compared).  The bucket's mixed force at
 its first chain's start field is within 1e-5 and its FP64 action within 1e-10 of the
 oracle's.
 
 Config 5 (honeycomb cylinder, 12 cells around, 16 along, open: Nx=384; Nt=256, beta=10,
 U=2.5, mu=0.5, DIA, spin basis; field from RandomState(7)): the stable log dets of both
 species within 1e-10 of oracle.logdetM_stable, the stable force parts within 1e-7 of
-oracle.force_part_stable (groups of 32 slices on both sides; the oracle's g=32 and g=16
-agree to 8e-11 there).  The reference's own product is checked to be off there, so the
+oracle.force_part_stable with FP64 right chains and within 1e-5 with FP32 right chains on
+MFMA (groups of 32 slices on both sides; the oracle's g=32 and g=16 agree to 8e-11 there).  The reference's own product is checked to be off there, so the
 comparison is not vacuous.
 """
 import numpy as np
@@ -84,7 +85,7 @@ def test_config4_buckets_against_oracle():
     dh, points = ens.deltaH[-1], ens.trajPoints[-1]
     for idx, bucket in zip(ens.index, ens.buckets):
         regular = 0
-        for c in idx[:4]:
+        for c in idx[:8]:
             ch = chains[c]
             kt = ch.kappa * ch.beta / ch.nt
             oracle = ref.HubbardFermiAction(kt, 0.0, -1, ref.EXP)
@@ -96,8 +97,8 @@ def test_config4_buckets_against_oracle():
             want_dh = float(np.real(e1 - e0))
             print(f"  Nx={ch.nx} Nt={ch.nt} chain {c}: dH {dh[c]:.6f} oracle {want_dh:.6f}", flush=True)
             assert points[c] == tp, (ch.nx, ch.nt, c)
-            if abs(want_dh) < 5.0:  # a trajectory that passes no near-zero of det M
-                assert abs(dh[c] - want_dh) < 1e-4, (ch.nx, ch.nt, c, dh[c], want_dh)
+            if abs(want_dh) < 50.0:  # a trajectory that passes no near-zero of det M
+                assert abs(dh[c] - want_dh) < 5e-5 * max(1.0, abs(want_dh)), (ch.nx, ch.nt, c, dh[c], want_dh)
                 regular += 1
             if c == idx[0]:
                 dev = torch.as_tensor(phi0, device="cuda")
@@ -130,8 +131,11 @@ def test_config5_stable_against_oracle():
             plain = ref.logdetM(hfm, aux, 0)
             assert mod2pi_diff(plain, want) > 1e-3 * abs(want)
     parts = g.force_parts(dev)[0].cpu().numpy()
+    g32 = act.HubbardFermiAction(kt, mt, -1, act.HFAHopping.DIA, act.HFABasis.SPIN, precision="stable", group=32,
+                                 right_chains="fp32")
+    parts32 = g32.force_parts(dev)[0].cpu().numpy()
     for s in (0, 1):
         want = ref.force_part_stable(hfm, aux, s, g=32)
-        err = rel_l2(parts[s], want)
-        print(f"  species {s}: force part L2 {err:.2e}", flush=True)
-        assert err < 1e-7
+        err, err32 = rel_l2(parts[s], want), rel_l2(parts32[s], want)
+        print(f"  species {s}: force part L2 {err:.2e} (FP64 chains), {err32:.2e} (FP32 chains)", flush=True)
+        assert err < 1e-7 and err32 < 1e-5

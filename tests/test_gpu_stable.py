@@ -95,3 +95,22 @@ def test_stable_rejects_unsupported_nx():
     g = act.HubbardFermiAction(kappa * 0.25, 0.0, -1, act.HFAHopping.EXP, precision="stable", group=4)
     with pytest.raises(Exception):
         g.logdets(torch.zeros(18 * 16, dtype=torch.complex128, device="cuda"))
+
+
+@pytest.mark.parametrize("nt,beta,group", [(128, 10.0, 16), (128, 10.0, 32)])
+def test_stable_fp32_right_chains(nt, beta, group):
+    """right_chains="fp32": the stable force with FP32 right chains (fused diagonal) on the
+    ill-conditioned cylinder, within 1e-5 of the stabilised oracle; the FP64-chain form is
+    the reference point (1e-9 / 1e-6)."""
+    print(f"stable force, FP32 right chains, Nx=64 Nt={nt} group {group}", flush=True)
+    kt, mt, phi = _cylinder(nt, beta=beta)
+    hfm = ref.HFMDia(kt, mt, -1)
+    aux = -1j * phi
+    g = act.HubbardFermiAction(kt, mt, -1, act.HFAHopping.DIA, act.HFABasis.SPIN, precision="stable", group=group,
+                               right_chains="fp32")
+    parts = g.force_parts(torch.as_tensor(phi, device="cuda"))[0].cpu().numpy()
+    for species in (0, 1):
+        want = ref.force_part_stable(hfm, aux, species, 8)
+        err = rel_l2(parts[species], want)
+        print(f"  species {species}: L2 {err:.2e}", flush=True)
+        assert err < 1e-5, (species, err)

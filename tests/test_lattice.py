@@ -57,3 +57,31 @@ def test_large_lattice_needs_no_recursion():
     idx = np.arange(n)
     h[idx, (idx + 1) % n] = h[(idx + 1) % n, idx] = 1.0
     assert isBipartite(h)
+
+
+def test_builtin_lattice_registry(lattice, tmp_path):
+    """isle_amd.LATTICES (src/isle/__init__.py:38-80): every built-in lattice of the
+    reference, with the hopping matrix of the test fixture (written independently from the
+    same files), positions, names; unknown names raise ValueError; loadExternal reads a
+    lattice file written by dumpLattice."""
+    import json
+    import pytest
+    from isle_amd import LATTICES
+    from isle_amd.lattice import dumpLattice
+    from tests.conftest import GOLDEN
+    names = set(json.loads((GOLDEN / "lattices.json").read_text()))
+    assert set(LATTICES.keys()) == names and len(names) == 19
+    for name in names:
+        lat = LATTICES[name]
+        assert np.array_equal(lat.hopping(), lattice(name)), name
+        assert lat.nt() == 0
+    c60 = LATTICES["c60_ipr"]
+    assert c60.nx() == 60 and np.allclose(np.sum(c60.hopping() != 0, axis=1), 3)
+    assert LATTICES["four_sites"].name == "four sites"
+    assert np.allclose(LATTICES["four_sites"].position(2), (1.5, 0.866025403784439, 0))
+    with pytest.raises(ValueError, match="no built-in lattice"):
+        LATTICES["no_such_lattice"]
+    path = tmp_path / "tube.yml"
+    path.write_text(dumpLattice(LATTICES["tube_3-3_1"]))
+    back = LATTICES.loadExternal(path)
+    assert np.array_equal(back.hopping(), LATTICES["tube_3-3_1"].hopping())
