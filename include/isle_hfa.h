@@ -151,6 +151,15 @@ int isle_profile_kernels(isle_hfm* h, const double* phi, int op, int nt, int bat
  *   src/isle/cpp/math.hpp:265-334 (bound as isle.logdet).  a: device n*n complex128. */
 int isle_logdet(const double* a, int n, double* out_host, void* stream);
 
+/* log det of one matrix as isle_logdet, without allocating: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc): the LU's scratch is a static device array of about
+ * 5 MiB per GPU, held until the process ends; n must be <= 512 (larger n is rejected with
+ * ISLE_ERR_VALUE before anything is enqueued).  Calls in one process are serialised and return
+ * after `stream` is synchronised; they share that GPU's array, so overlapping calls from
+ * outside this lock would race.  a: device n*n complex128; out_host: 2 doubles. */
+int isle_logdet_noalloc(const double* a, int n, double* out_host, void* stream);
+
 /* invert(matrix): getrf + getri equivalent (src/isle/cpp/math.hpp:222-226).
  * a, out: device n*n complex128 (a is not modified). */
 int isle_invert(const double* a, int n, double* out, void* stream);

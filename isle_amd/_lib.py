@@ -37,6 +37,7 @@ SIGNATURES = {
     "isle_leapfrog_drift": (_c_int, [_vp, _vp, _c_long, _c_double, _vp]),
     "isle_cscale": (_c_int, [_vp, _vp, _c_long, _c_double, _c_double, _vp]),
     "isle_logdet": (_c_int, [_vp, _c_int, _dp, _vp]),
+    "isle_logdet_noalloc": (_c_int, [_vp, _c_int, _dp, _vp]),
     "isle_profile_kernels": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _dp,
                                       _c_int]),
     "isle_invert": (_c_int, [_vp, _c_int, _vp, _vp]),

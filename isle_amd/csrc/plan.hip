@@ -2160,6 +2160,78 @@ static int invert_impl(const double* a, int n, double* out, double* logdet_host,
     });
 }
 
+// ---- log det of one matrix without allocating: the scratch of the LU is a static device
+// array (one copy per GPU, held until the process ends), found per device with
+// hipGetSymbolAddress on the first call there.  Calls in one process are serialised by a
+// lock and return after their stream is synchronised; calls that overlap on one GPU from
+// other threads without that lock would share the array and race.
+static constexpr int NOALLOC_MAX_N = 512;
+static constexpr size_t noalloc_align(size_t b) { return (b + 255) & ~size_t(255); }
+static constexpr size_t NOALLOC_WORK =
+    (size_t)isle::LU_DINV * ((NOALLOC_MAX_N + isle::LU_NB - 1) / isle::LU_NB) + 2 * (size_t)isle::LU_NB * NOALLOC_MAX_N +
+    2 * isle::LU_NB + 2;  // lu_work_doubles(NOALLOC_MAX_N)
+static constexpr size_t NOALLOC_OFF_W = noalloc_align(2 * sizeof(double) * NOALLOC_MAX_N * NOALLOC_MAX_N);
+static constexpr size_t NOALLOC_OFF_PIV = NOALLOC_OFF_W + noalloc_align(NOALLOC_WORK * sizeof(double));
+static constexpr size_t NOALLOC_OFF_ITEM = NOALLOC_OFF_PIV + noalloc_align(NOALLOC_MAX_N * sizeof(int));
+static constexpr size_t NOALLOC_OFF_LD = NOALLOC_OFF_ITEM + noalloc_align(sizeof(isle::LuItem));
+static constexpr size_t NOALLOC_BYTES = NOALLOC_OFF_LD + noalloc_align(2 * sizeof(double));
+static constexpr int NOALLOC_DEVICES = 64;
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+static std::mutex noalloc_mu;
+static unsigned char* noalloc_base[NOALLOC_DEVICES];  // device address of `scratch` per device
+
+// The current device's copy of `scratch` (caller holds noalloc_mu).
+static unsigned char* noalloc_scratch() {
+    int dev = 0;
+    ISLE_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_DEVICES) throw std::out_of_range("device index beyond the scratch table");
+    if (!noalloc_base[dev]) {
+        void* p = nullptr;
+        ISLE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        noalloc_base[dev] = static_cast<unsigned char*>(p);
+    }
+    return noalloc_base[dev];
+}
+
+// Waits for the stream when the call unwinds after enqueueing work, before the lock goes.
+struct NoallocDrain {
+    hipStream_t s;
+    int unwinding = std::uncaught_exceptions();
+    ~NoallocDrain() {
+        if (std::uncaught_exceptions() > unwinding) (void)hipStreamSynchronize(s);
+    }
+};
+
+int isle_logdet_noalloc(const double* a, int n, double* out_host, void* stream) {
+    return guarded([&] {
+        if (n < 1) throw std::invalid_argument("empty matrix");
+        if (n > NOALLOC_MAX_N)
+            throw std::invalid_argument("isle_logdet_noalloc: n = " + std::to_string(n) +
+                                        " exceeds 512, the order its static scratch holds");
+        if (isle::lu_work_doubles(n) > NOALLOC_WORK) throw std::logic_error("LU work beyond the static scratch");
+        hipStream_t s = (hipStream_t)stream;
+        const long nn = (long)n * n;
+        std::lock_guard<std::mutex> lock(noalloc_mu);
+        unsigned char* base = noalloc_scratch();
+        NoallocDrain drain{s};
+        double* m = reinterpret_cast<double*>(base);
+        double* w = reinterpret_cast<double*>(base + NOALLOC_OFF_W);
+        int* piv = reinterpret_cast<int*>(base + NOALLOC_OFF_PIV);
+        isle::LuItem* item = reinterpret_cast<isle::LuItem*>(base + NOALLOC_OFF_ITEM);
+        double* ld = reinterpret_cast<double*>(base + NOALLOC_OFF_LD);
+        hipLaunchKernelGGL(isle::interleaved_to_split_kernel, dim3(isle::ew_blocks(nn)), dim3(256), 0, s, a, m,
+                           m + nn, nn);
+        ISLE_HIP(hipGetLastError());
+        isle::LuItem it{m, m + nn, piv, ld, w, n, n};
+        ISLE_HIP(hipMemcpyAsync(item, &it, sizeof(it), hipMemcpyHostToDevice, s));
+        isle::launch_lu(item, 1, n, 0, s);
+        ISLE_HIP(hipMemcpyAsync(out_host, ld, 16, hipMemcpyDeviceToHost, s));
+        ISLE_HIP(hipStreamSynchronize(s));
+    });
+}
+
 int isle_invert(const double* a, int n, double* out, void* stream) {
     return invert_impl(a, n, out, nullptr, stream);
 }
