@@ -26,7 +26,8 @@ PEAK = {"f32": 157.3, "f64": 78.6}  # dense MFMA TFLOP/s, MI355X
 KIND_NAMES = ["gemm64_rc_pre", "gemm64_rc_row", "gemm64_cr_pre", "gemm64_cc", "gemm32_rc_row", "gemm32_rc_pre",
               "scale64", "scale32", "lu", "lu_solve", "diag64", "diag32", "copy64", "cvt32to64", "cvt64to32",
               "phase", "tcvt64to32", "diag64x32", "gemm64_cc_sub", "lu_shape", "block", "sigma", "gather",
-              "tblock", "trsm", "gemm32_rc_pre_diag", "dreduce", "rgather", "gemm32_rc_diag", "tscale64to32"]
+              "tblock", "trsm", "gemm32_rc_pre_diag", "dreduce", "rgather", "gemm32_rc_diag", "tscale64to32",
+              "gemm32_rck_diag", "colscale64to32"]
 
 
 def parse():
@@ -37,7 +38,7 @@ def parse():
     ap.add_argument("--chains", type=int, default=128, help="independent chains advanced together")
     ap.add_argument("--group", type=int, default=32, help="time slices per group of the mixed force")
     ap.add_argument("--plain-steps", type=int, default=3, help="timed trajectories of the plain FP64 forms")
-    ap.add_argument("--therm", type=int, default=2, help="thermalisation trajectories (length 0.25, 100 steps) "
+    ap.add_argument("--therm", type=int, default=8, help="thermalisation trajectories (length 0.25, 100 steps) "
                     "before the timed ones")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")

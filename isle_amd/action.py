@@ -252,10 +252,11 @@ class HubbardFermiAction(Action):
 
     def _group(self, nt):
         """Slices per group: the largest divisor of nt not above ``group`` that keeps the
-        slice-conditioning bound g * spread <= 12 (cond <= e^12 over one group, for the
-        FP32 right chains of the mixed force) or <= 24 (FP64 chains of "stable")."""
+        slice-conditioning bound g * spread <= 12 (cond <= e^12 over one group, for FP32
+        right chains: the mixed force, "stable" with right_chains="fp32") or <= 24 (FP64
+        chains of "stable")."""
         g = min(self.group, nt)
-        bound = 24.0 if self.precision == "stable" else 12.0
+        bound = 24.0 if self.precision == "stable" and self.right_chains == "fp64" else 12.0
         if self._slice_spread > 0:
             g = max(1, min(g, int(bound / self._slice_spread)))
         while nt % g:

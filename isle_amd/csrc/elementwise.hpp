@@ -413,6 +413,33 @@ __global__ __launch_bounds__(256) void transpose_scale_kernel(const TScaleItem* 
     }
 }
 
+// dst = sign src diag(d) for complex n x n matrices: FP64 src as re / im planes with row
+// stride lds (a block of a larger matrix is fine), FP32 dst as split planes (im n*n after
+// re), d complex (FP64); the product is formed in FP64 and rounded once.  Grid-stride over
+// the n*n elements of each item (grid.y = item).
+struct ColScaleItem {
+    const double* s_re;
+    const double* s_im;
+    long lds;
+    double sign;
+    float* dst;
+    const double* d_re;
+    const double* d_im;
+    int n;
+};
+
+__global__ __launch_bounds__(256) void colscale_convert_kernel(const ColScaleItem* __restrict__ items) {
+    const ColScaleItem it = items[blockIdx.y];
+    const long nn = (long)it.n * it.n;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < nn; e += (long)gridDim.x * 256) {
+        const long i = e / it.n, j = e % it.n;
+        const double vr = it.sign * it.s_re[i * it.lds + j], vi = it.sign * it.s_im[i * it.lds + j];
+        const double dr = it.d_re[j], di = it.d_im[j];
+        it.dst[e] = (float)(vr * dr - vi * di);
+        it.dst[nn + e] = (float)(vr * di + vi * dr);
+    }
+}
+
 // Right-hand sides between the caller's layout (nrhs interleaved rows of nt*n complex) and
 // blocks [t][re/im][n][nrhs]: 32 x 32 tiles through LDS so that both the row reads and the
 // block writes are contiguous.  Grid (ceil(n/32), ceil(nrhs/32), nt), 256 threads.

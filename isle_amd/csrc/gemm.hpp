@@ -334,9 +334,10 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* _
 // Because A = A^T, the k-rows of A are its columns, so both operand slabs are plain copies
 // of row segments: slab rows k0..k0+BK-1 of A (columns m0..) and of B (columns n0..).  The
 // copies go straight into LDS (no staging registers), three slabs in flight.  In LDS a row
-// of 16-byte chunks is XOR-swizzled (chunk ^ 4 on odd rows) so the k-rows read by one MFMA
-// step fall into different banks; the swizzle is applied by choosing each lane's source
-// chunk, since a wave's direct load writes 64 consecutive 16-byte chunks.
+// of 16-byte chunks is XOR-swizzled (chunk ^ 4, or ^ 8 for FP64, on odd rows; rcd_xor) so
+// the k-rows read by one MFMA step fall into different banks; the swizzle is applied by
+// choosing each lane's source chunk, since a wave's direct load writes 64 consecutive
+// 16-byte chunks.
 // Requirements (the caller vouches): k % BK == 0; m, n, lda, ldb, ldc multiples of 4;
 // 16-byte aligned operands and phase vectors.
 template <typename T, int BM, int BN, int BK, int RCD_STAGES = 3, int NW = 4>
@@ -349,9 +350,18 @@ struct RcdLayout {
     static constexpr size_t BYTES = sizeof(T) * STAGE * RCD_STAGES;
 };
 
-template <int EPC>
+// XOR applied to the chunk index of odd k rows: 8 chunks (128 bytes) for FP64 rows of a
+// multiple of 16 chunks, so that two k rows read in one LDS cycle fall in disjoint halves of
+// a 64-bank line (4 chunks overlapped half of them: 27% of the FP64 kernel's LDS cycles were
+// bank conflicts); 4 chunks otherwise.  ROW: elements per slab row (BM or BN).
+template <int EPC, int ROW>
+constexpr int rcd_xor() {
+    return (EPC == 2 && (ROW / EPC) % 16 == 0) ? 8 : 4;
+}
+
+template <int EPC, int ROW>
 __device__ __forceinline__ int rcd_swz(int k, int m) {
-    return (((m / EPC) ^ ((k & 1) << 2)) * EPC) + (m % EPC);
+    return (((m / EPC) ^ ((k & 1) * rcd_xor<EPC, ROW>())) * EPC) + (m % EPC);
 }
 
 // Issues this wave's direct copies of slab `slab` (A rows, B re/im rows, and with PRE the
@@ -373,7 +383,7 @@ __device__ __forceinline__ void rcd_issue(const GemmItem<T>& it, T* lds, int sla
     for (int q = 0; q < AQ; ++q) {
         const int c = (wave * AQ + q) * 64 + lane;  // chunk slot of this lane
         const int k = c / (BM / EPC), slot = c % (BM / EPC);
-        int col = m0 + EPC * (slot ^ ((k & 1) << 2));
+        int col = m0 + EPC * (slot ^ ((k & 1) * rcd_xor<EPC, BM>()));
         if (col >= it.m) col = 0;  // only feeds rows >= m of C
         __builtin_amdgcn_global_load_lds(it.a_re + (long)(k0 + k) * it.lda + col, st + (wave * AQ + q) * CH, 16, 0, 0);
     }
@@ -381,7 +391,7 @@ __device__ __forceinline__ void rcd_issue(const GemmItem<T>& it, T* lds, int sla
     for (int q = 0; q < BQ; ++q) {
         const int c = (wave * BQ + q) * 64 + lane;
         const int k = c / (BN / EPC), slot = c % (BN / EPC);
-        int col = n0 + EPC * (slot ^ ((k & 1) << 2));
+        int col = n0 + EPC * (slot ^ ((k & 1) * rcd_xor<EPC, BN>()));
         if (col >= it.n) col = 0;  // only feeds columns >= n of C
         const long off = (long)(k0 + k) * it.ldb + col;
         __builtin_amdgcn_global_load_lds(it.b_re + off, st + OFF_BR + (wave * BQ + q) * CH, 16, 0, 0);
@@ -483,7 +493,7 @@ void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items, int tiles_n_max) {
                 const int kk = kg + ks * 4 + lk;
                 T a[MI], br[NJ], bi[NJ];
 #pragma unroll
-                for (int i = 0; i < MI; ++i) a[i] = As[kk * BM + rcd_swz<EPC>(kk, wm + i * 16 + lr)];
+                for (int i = 0; i < MI; ++i) a[i] = As[kk * BM + rcd_swz<EPC, BM>(kk, wm + i * 16 + lr)];
                 T cr = 1, ci = 0;
                 if (PRE) {
                     cr = pw[kk];
@@ -491,7 +501,7 @@ void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items, int tiles_n_max) {
                 }
 #pragma unroll
                 for (int j = 0; j < NJ; ++j) {
-                    const int o = kk * BN + rcd_swz<EPC>(kk, wn + j * 16 + lr);
+                    const int o = kk * BN + rcd_swz<EPC, BN>(kk, wn + j * 16 + lr);
                     T vr = Br[o], vi = Bi[o];
                     if (PRE) {
                         const T t = vr * cr - vi * ci;
@@ -612,6 +622,244 @@ void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items, int tiles_n_max) {
                     it.c_im[off] = vi;
                 }
             }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// FP32 chain product with k-contiguous operand slabs (the fused right chains of the force):
+//     C = A B,  A real (row-major, shared by all items), B complex given by its transpose
+//     Bt (n x k, row-major split planes),
+// so both slabs are row segments contiguous in k: rows m of A and rows n of Bt, columns
+// k0 .. k0 + BK.  In LDS a slab row holds BK / 4 16-byte chunks; logical chunk c of row r sits
+// at position c ^ rck_swz(r), which puts the 16 rows read by one MFMA step in distinct banks
+// for 32- and for 64-bank LDS.  Lane (lr, lk) feeds k = kg + 4 lk + ks to MFMA step ks of a
+// 16-k group, so one ds_read_b128 brings a lane's operand for four MFMAs (a quarter of the LDS
+// instructions of gemm_rcd_kernel).  The product is stored transposed, Ct = C^T (n x m, the
+// next chain step's Bt), one 16-byte store per lane and 16 x 16 block, with FUSED_DIAG: the
+// row sums of L .* C per 16-column block (as gemm_rcd_kernel) from the unscaled product, and
+// C^T scaled by post[m] when post_re is set.
+// Requirements (the caller vouches): k % BK == 0, m % 4 == 0, the row strides lda, ldb
+// (of Bt) and ldc (of Ct) multiples of 4, 16-byte aligned operands.
+template <int BM, int BN, int BK, int STAGES>
+struct RckLayout {
+    static constexpr int A = BM * BK;  // floats per slab plane
+    static constexpr int B = BN * BK;
+    static constexpr int STAGE = A + 2 * B;
+    static constexpr size_t BYTES = sizeof(float) * STAGE * STAGES;
+};
+
+__device__ __forceinline__ int rck_swz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
+
+template <int BM, int BN, int BK, int WM, int WN>
+constexpr bool rck_shape_ok() {
+    constexpr int NW = WM * WN, RPI = 64 / (BK / 4);  // rows written by one wave instruction
+    return BK == 32 && BM % RPI == 0 && BN % RPI == 0 && (BM / RPI) % NW == 0 && (BN / RPI) % NW == 0 &&
+           BM % (16 * WM) == 0 && BN % (16 * WN) == 0;
+}
+
+// This wave's direct copies of slab `slab` into stage slab % STAGES of the LDS ring.
+template <int BM, int BN, int BK, int STAGES, int NW>
+__device__ __forceinline__ void rck_issue(const GemmItem<float>& it, float* lds, int slab, int m0, int n0, int wave,
+                                          int lane) {
+    typedef RckLayout<BM, BN, BK, STAGES> LY;
+    constexpr int CPR = BK / 4, RPI = 64 / CPR;
+    constexpr int AQ = BM / RPI / NW, BQ = BN / RPI / NW;
+    float* st = lds + (slab % STAGES) * LY::STAGE;
+    const int k0 = slab * BK;
+    const int rr = lane / CPR, pos = lane % CPR;
+#pragma unroll
+    for (int q = 0; q < AQ; ++q) {
+        const int ins = wave * AQ + q, r = ins * RPI + rr;
+        int row = m0 + r;
+        if (row >= it.m) row = 0;  // only feeds rows >= m of C
+        const int c = pos ^ rck_swz(r);
+        __builtin_amdgcn_global_load_lds(it.a_re + (long)row * it.lda + k0 + 4 * c, st + ins * 256, 16, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) {
+        const int ins = wave * BQ + q, r = ins * RPI + rr;
+        int row = n0 + r;
+        if (row >= it.n) row = 0;  // only feeds columns >= n of C
+        const int c = pos ^ rck_swz(r);
+        const long off = (long)row * it.ldb + k0 + 4 * c;
+        __builtin_amdgcn_global_load_lds(it.b_re + off, st + LY::A + ins * 256, 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(it.b_im + off, st + LY::A + LY::B + ins * 256, 16, 0, 0);
+    }
+}
+
+template <int BM, int BN, int BK, int STAGES, int WM = 2, int WN = 2>
+__global__ __launch_bounds__(64 * WM * WN, (WM * WN >= 12 ? 6 : 1))
+void gemm_rck_kernel(const GemmItem<float>* __restrict__ items, int tiles_n_max) {
+    constexpr int NW = WM * WN;
+    typedef RckLayout<BM, BN, BK, STAGES> LY;
+    static_assert(rck_shape_ok<BM, BN, BK, WM, WN>(), "tile shape");
+    static_assert(STAGES >= 2 && STAGES <= 4, "one to three slabs in flight");
+    constexpr int MI = BM / (16 * WM), NJ = BN / (16 * WN);
+    constexpr int RPI = 64 / (BK / 4);
+    constexpr int PER_SLAB = BM / RPI / NW + 2 * (BN / RPI / NW);  // direct loads per wave and slab
+    extern __shared__ __align__(16) unsigned char rck_raw[];
+    float* lds = reinterpret_cast<float*>(rck_raw);
+
+    int tile, item;
+    xcd_map(gridDim.x, gridDim.y, tile, item);
+    const GemmItem<float> it = items[item];
+    const int m0 = (tile / tiles_n_max) * BM, n0 = (tile % tiles_n_max) * BN;
+    if (m0 >= it.m || n0 >= it.n) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave / WN) * (BM / WM), wn = (wave % WN) * (BN / WN);
+    const int lr = lane & 15, lk = lane >> 4;
+    const int nslab = it.k / BK;
+
+    flt4 acc_re[MI][NJ], acc_im[MI][NJ];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) acc_re[i][j] = acc_im[i][j] = flt4{0, 0, 0, 0};
+
+    for (int j = 0; j < STAGES - 1 && j < nslab; ++j)
+        rck_issue<BM, BN, BK, STAGES, NW>(it, lds, j, m0, n0, wave, lane);
+    for (int s = 0; s < nslab; ++s) {
+        // this wave's copies of slab s done (later slabs stay in flight), then all waves'
+        const int ahead = min(STAGES - 2, nslab - 1 - s);
+        if (ahead >= 2)
+            asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(2 * PER_SLAB) : "memory");
+        else if (ahead == 1)
+            asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(PER_SLAB) : "memory");
+        else
+            asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
+        if (s + STAGES - 1 < nslab)  // into the buffer every wave finished reading
+            rck_issue<BM, BN, BK, STAGES, NW>(it, lds, s + STAGES - 1, m0, n0, wave, lane);
+        const float* st = lds + (s % STAGES) * LY::STAGE;
+        const float* As = st;
+        const float* Br = st + LY::A;
+        const float* Bi = st + LY::A + LY::B;
+#pragma unroll
+        for (int kg = 0; kg < BK; kg += 16) {
+            const int c = kg / 4 + lk;
+            flt4 a[MI], br[NJ], bi[NJ];
+#pragma unroll
+            for (int i = 0; i < MI; ++i) {
+                const int r = wm + i * 16 + lr;
+                a[i] = *reinterpret_cast<const flt4*>(As + r * BK + 4 * (c ^ rck_swz(r)));
+            }
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                const int r = wn + j * 16 + lr;
+                const int o = r * BK + 4 * (c ^ rck_swz(r));
+                br[j] = *reinterpret_cast<const flt4*>(Br + o);
+                bi[j] = *reinterpret_cast<const flt4*>(Bi + o);
+            }
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+                for (int i = 0; i < MI; ++i)
+#pragma unroll
+                    for (int j = 0; j < NJ; ++j) {
+                        acc_re[i][j] = mfma16(a[i][ks], br[j][ks], acc_re[i][j]);
+                        acc_im[i][j] = mfma16(a[i][ks], bi[j][ks], acc_im[i][j]);
+                    }
+        }
+    }
+    // Epilogue per 16-row block: row factors and left-factor entries loaded first (one
+    // memory latency per block), then the row sums of the unscaled product, then the scaled
+    // transposed store.  Lane (lr, lk) holds C[4 lk + r][lr] of each 16 x 16 block.
+    const bool scaled = it.post_re != nullptr;  // uniform over the workgroup
+    const bool identity = it.dl_re == nullptr;
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+        const int rowb = m0 + wm + i * 16 + 4 * lk;  // rows rowb .. rowb + 3
+        const bool rows_in = rowb < it.m;           // m % 4 == 0: all four or none
+        float fr[4], fi[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            fr[r] = 1.0f;
+            fi[r] = 0.0f;
+        }
+        if (scaled) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int di = rows_in ? rowb + r : 0;
+                fr[r] = it.post_re[di];
+                fi[r] = it.post_im[di];
+            }
+        }
+        double lre[NJ][4], lim[NJ][4];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = n0 + wn + j * 16 + lr;
+                const bool in = rows_in && col < it.n;
+                lre[j][r] = in && rowb + r == col ? 1.0 : 0.0;
+                lim[j][r] = 0;
+            }
+        if (!identity) {
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = n0 + wn + j * 16 + lr;
+                    const bool in = rows_in && col < it.n;
+                    const long o = in ? (long)(rowb + r) * it.n + col : 0;
+                    const double x = it.dl_re[o], y = it.dl_im[o];
+                    lre[j][r] = in ? x : 0.0;
+                    lim[j][r] = in ? y : 0.0;
+                }
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int col = n0 + wn + j * 16 + lr;
+            flt4 outr, outi;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float vr = acc_re[i][j][r], vi = acc_im[i][j][r];
+                const double sr = row16_sum(lre[j][r] * (double)vr - lim[j][r] * (double)vi);
+                const double si = row16_sum(lre[j][r] * (double)vi + lim[j][r] * (double)vr);
+                const int slot = (n0 + wn + j * 16) / 16;
+                if (lr == 15 && rows_in && slot < it.n / 16) {
+                    it.dpart[((long)slot * it.m + rowb + r) * 2] = sr;
+                    it.dpart[((long)slot * it.m + rowb + r) * 2 + 1] = si;
+                }
+                outr[r] = vr * fr[r] - vi * fi[r];
+                outi[r] = vr * fi[r] + vi * fr[r];
+            }
+            if (rows_in && col < it.n) {
+                const long off = (long)col * it.ldc + rowb;
+                *reinterpret_cast<flt4*>(it.c_re + off) = outr;
+                *reinterpret_cast<flt4*>(it.c_im + off) = outi;
+            }
+        }
+    }
+}
+
+template <int BM, int BN, int BK, int ST, int WM, int WN>
+inline bool launch_gemm_rck(const GemmItem<float>* d_items, int nitems, int max_m, int max_n, hipStream_t s) {
+    if constexpr (!rck_shape_ok<BM, BN, BK, WM, WN>()) {
+        return false;
+    } else {
+        const size_t lds = RckLayout<BM, BN, BK, ST>::BYTES;
+        if (lds > 160 * 1024) return false;
+        auto kern = gemm_rck_kernel<BM, BN, BK, ST, WM, WN>;
+        ISLE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const int tm = (max_m + BM - 1) / BM, tn = (max_n + BN - 1) / BN;
+        hipLaunchKernelGGL(kern, dim3(tm * tn, nitems), dim3(64 * WM * WN), lds, s, d_items, tn);
+        ISLE_HIP(hipGetLastError());
+        return true;
+    }
+}
+
+// Tile shapes of the k-contiguous FP32 chain product (the autotuner times them per step):
+//   0: 96x96, 12 waves of 48x16, 2 stages   1: the same, 3 stages   2: 96x96, 6 waves of 48x32,
+//   2 stages   3: 64x64, 4 waves of 32x32, 3 stages; other values report "does not fit".
+inline bool launch_gemm_rck_tile(const GemmItem<float>* d_items, int nitems, int max_m, int max_n, hipStream_t s,
+                                 int tile) {
+    switch (tile) {
+    case 0: return launch_gemm_rck<96, 96, 32, 2, 2, 6>(d_items, nitems, max_m, max_n, s);
+    case 1: return launch_gemm_rck<96, 96, 32, 3, 2, 6>(d_items, nitems, max_m, max_n, s);
+    case 2: return launch_gemm_rck<96, 96, 32, 2, 2, 3>(d_items, nitems, max_m, max_n, s);
+    case 3: return launch_gemm_rck<64, 64, 32, 3, 2, 2>(d_items, nitems, max_m, max_n, s);
+    default: return false;
     }
 }
 

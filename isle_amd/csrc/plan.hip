@@ -324,7 +324,7 @@ void Plan::run(OpCache& op, const double* phi) {
 static bool is_gemm(K k) {
     return k == K::GEMM64_RC_PRE || k == K::GEMM64_RC_ROW || k == K::GEMM64_CR_PRE || k == K::GEMM64_CC ||
            k == K::GEMM32_RC_ROW || k == K::GEMM32_RC_PRE || k == K::GEMM64_CC_SUB || k == K::GEMM32_RC_PRE_DIAG ||
-           k == K::TRSM || k == K::GEMM32_RC_DIAG;
+           k == K::TRSM || k == K::GEMM32_RC_DIAG || k == K::GEMM32_RCK_DIAG;
 }
 
 void Plan::tune(OpCache& op) {
@@ -498,6 +498,15 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
                                                                st.max_m, st.max_n, s, st.tile, n % 4 == 0 ? n : 0))
                 return false;
             break;
+        case K::GEMM32_RCK_DIAG:
+            if (n % 32 != 0 || n % 4 != 0) return false;
+            if (!launch_gemm_rck_tile((const GemmItem<float>*)st.items, st.count, st.max_m, st.max_n, s, st.tile))
+                return false;
+            break;
+        case K::COLSCALE64TO32:
+            hipLaunchKernelGGL(colscale_convert_kernel, dim3(std::min<unsigned>(cdiv(nn, 256), 1024), st.count),
+                               dim3(256), 0, s, (const ColScaleItem*)st.items);
+            break;
         case K::TSCALE64TO32:
             hipLaunchKernelGGL(transpose_scale_kernel, dim3(64, 1, st.count), dim3(256), 0, s,
                                (const TScaleItem*)st.items);
@@ -652,7 +661,8 @@ OpCache& Plan::get_force_plain(int nt, int batch, int nspecies, int spin) {
 //   GL_{ng-1} = I, GL_j = P_{j+1} GL_{j+1};  Mid = (1 + P_0 GL_0)^{-1}
 //   RR_0 = Mid, RR_j = RR_{j-1} P_{j-1};     K_j = GL_j RR_j
 // Per group again (FP32):  Rt_{s_j} = K_j X_{s_j}, Rt_tau = Rt_{tau-1} X_tau, carried
-// transposed (A_s symmetric): RT_{s_j} = A_s (D_{s_j} K_j^T), RT_tau = A_s (D_tau RT_{tau-1});
+// transposed (A_s symmetric): RT_{s_j} = A_s (D_{s_j} K_j^T), RT_tau = A_s (D_tau RT_{tau-1})
+// (with n % 32 == 0 the products read and store the untransposed Rt, see below);
 //   f[tau] = diag(Suf_{tau+1} Rt_tau) = sum_k Suf_{tau+1}[i][k] RT_tau[i][k]  (tau < e_j),
 //   f[e_j] = diag(RT_{e_j}).
 // This is the reference's lefts/right factorisation applied inside each group with the
@@ -856,19 +866,22 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
     //      produce the force diagonal f[tau] = rowdot(Suf_{tau+1}, Rt_tau) (diag Rt at a group's
     //      last slice) as partial row sums per 16 columns, summed by one reduction at the end,
     //      and every phase is applied outside the MFMA loop: the group's first phase D_{s_j}
-    //      in the conversion of K_j (K32 = D_{s_j} K_j^T), the next slice's D_{tau+1} to the
-    //      rows a chain product stores (its row sums use the product before that scaling).
+    //      in the conversion of K_j (K32 = K_j D_{s_j}), the next slice's D_{tau+1} to the
+    //      product as it is stored (its row sums use the product before that scaling).
     //      Otherwise the phase scales the complex operand along k as it is read.
     const int slots = n / 16;
     double* part = fuse ? b.d64((size_t)nb * nt * slots * n * 2) : nullptr;
     if (fuse) {
-        std::vector<TScaleItem> cv;
+        // the right chains are carried untransposed in the fused path: Rt_{s_j} starts from
+        // K_j D_{s_j} (FP32), Rt_tau = (Rt_{tau-1} D_tau) A_s, the next phase applied as the
+        // product is stored
+        std::vector<ColScaleItem> cv;
         for (int item = 0; item < nb; ++item)
             for (int j = 0; j < ng; ++j) {
                 const double* src = (j < ng - 1) ? m64(KK, item, j).re : m64(RR, item, ng - 1).re;
                 cv.push_back({src, src + nn, n, 1.0, mK32(item, j).re, D64(item, j * g).re, D64(item, j * g).im, n});
             }
-        b.add(K::TSCALE64TO32, cv);
+        b.add(K::COLSCALE64TO32, cv);
     } else {
         std::vector<TCopyItem<double, float>> cv;
         for (int item = 0; item < nb; ++item)
@@ -928,7 +941,7 @@ OpCache& Plan::get_force_mixed(int nt, int batch, int g, int nspecies, int spin)
                 }
                 gi.push_back(it);
             }
-        b.add(fuse ? K::GEMM32_RC_DIAG : K::GEMM32_RC_PRE, gi, n, n);
+        b.add(fuse ? K::GEMM32_RCK_DIAG : K::GEMM32_RC_PRE, gi, n, n);
     }
     if (fuse) {
         std::vector<DReduceItem> dr;
@@ -1463,8 +1476,8 @@ OpCache& Plan::get_force_stable(int nt, int batch, int g, int nspecies, int spin
             add_trsm(b, 1, n, w, tu);
         }
         if (right32) {
-            // K_j^T straight to FP32 with the group's first phase folded in: D_{s_j} K_j^T
-            std::vector<TScaleItem> cv;
+            // K_j = sign X (untransposed) to FP32 with the group's first phase: K_j D_{s_j}
+            std::vector<ColScaleItem> cv;
             for (int item = 0; item < nb; ++item)
                 for (int j = 0; j < ng; ++j) {
                     const CMat<double> x = j < ng - 1 ? mT(item, j + 1) : mT(item, 0);
@@ -1473,7 +1486,7 @@ OpCache& Plan::get_force_stable(int nt, int batch, int g, int nspecies, int spin
                     cv.push_back({x.re + col, x.im + col, wd, j < ng - 1 ? -1.0 : 1.0, dst, D(item, j * g).re,
                                   D(item, j * g).im, n});
                 }
-            b.add(K::TSCALE64TO32, cv);
+            b.add(K::COLSCALE64TO32, cv);
         } else {
             std::vector<TBlockItem> tb;
             for (int item = 0; item < nb; ++item)
@@ -1510,7 +1523,7 @@ OpCache& Plan::get_force_stable(int nt, int batch, int g, int nspecies, int spin
                     it.dpart = part + ((long)item * nt + tau) * slots * n * 2;
                     gi.push_back(it);
                 }
-            b.add(K::GEMM32_RC_DIAG, gi, n, n);
+            b.add(K::GEMM32_RCK_DIAG, gi, n, n);
         }
         std::vector<DReduceItem> dr;
         for (int item = 0; item < nb; ++item)

@@ -67,6 +67,9 @@ enum class K : int {
     GEMM32_RC_DIAG,     // f32 real x complex, no pre-scaling, force diagonal's row sums fused and the
                         // stored rows scaled by the next slice's phase
     TSCALE64TO32,       // transposed copy with conversion and complex row scaling
+    GEMM32_RCK_DIAG,    // f32 real x complex on k-contiguous slabs (operand and product stored
+                        // transposed), force diagonal's row sums fused, rows scaled by the next phase
+    COLSCALE64TO32,     // copy with conversion and complex column scaling
 };
 
 struct Step {

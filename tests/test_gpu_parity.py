@@ -286,10 +286,15 @@ def test_config2_gradient_check():
         e[i] = h
         d = g.eval(phi + e) - g.eval(phi - e)
         return -complex(d.real, np.remainder(d.imag + np.pi, 2 * np.pi) - np.pi) / (2 * h)
+    worst_h6 = worst_rich = 0.0
     for i in idx:
         assert abs(cdiff(i, 1e-6) - f[i]) < 5 * s0 * 1e-13 / 1e-6, (i, cdiff(i, 1e-6), f[i])
         rich = (4 * cdiff(i, 1e-4) - cdiff(i, 2e-4)) / 3
         assert abs(rich - f[i]) < 5 * s0 * 1e-13 / 1e-4, (i, rich, f[i])
+        worst_h6 = max(worst_h6, abs(cdiff(i, 1e-6) - f[i]))
+        worst_rich = max(worst_rich, abs(rich - f[i]))
+    print(f"config 2 FD: |S| = {s0:.1f}, max |FD - F| at h = 1e-6: {worst_h6:.2e}, Richardson h = 1e-4: "
+          f"{worst_rich:.2e}, max |F| = {np.abs(f[idx]).max():.2e}", flush=True)
     for s in range(2000, 2016):
         phi = np.random.RandomState(s).normal(0, np.sqrt(ut), nx * nt) + 0j
         assert rel_l2(g.force(phi).cpu().numpy(), r.force(phi)) < 1e-10

@@ -100,14 +100,16 @@ def test_stable_rejects_unsupported_nx():
 @pytest.mark.parametrize("nt,beta,group", [(128, 10.0, 16), (128, 10.0, 32)])
 def test_stable_fp32_right_chains(nt, beta, group):
     """right_chains="fp32": the stable force with FP32 right chains (fused diagonal) on the
-    ill-conditioned cylinder, within 1e-5 of the stabilised oracle; the FP64-chain form is
-    the reference point (1e-9 / 1e-6)."""
+    ill-conditioned cylinder, within 1e-5 of the stabilised oracle.  The FP32 chains keep
+    the mixed form's conditioning cap (g * spread <= 12): a requested group of 32 slices is
+    cut to 16 here (spread 0.56 per slice; at 32 slices the error is 1.8e-4)."""
     print(f"stable force, FP32 right chains, Nx=64 Nt={nt} group {group}", flush=True)
     kt, mt, phi = _cylinder(nt, beta=beta)
     hfm = ref.HFMDia(kt, mt, -1)
     aux = -1j * phi
     g = act.HubbardFermiAction(kt, mt, -1, act.HFAHopping.DIA, act.HFABasis.SPIN, precision="stable", group=group,
                                right_chains="fp32")
+    assert g._group(nt) == 16
     parts = g.force_parts(torch.as_tensor(phi, device="cuda"))[0].cpu().numpy()
     for species in (0, 1):
         want = ref.force_part_stable(hfm, aux, species, 8)
