@@ -278,9 +278,10 @@ def test_config2_gradient_check():
     idx = rng.choice(nx * nt, 64, replace=False)
     # Central differences: truncation O(h^2) plus round-off |S| * eta / h with eta ~ 1e-13 the
     # relative accuracy of the FP64 action.  h = 1e-6 (the prescribed step) is round-off
-    # limited (~|S| 1e-7).  The tight check extrapolates h = 2e-4 and 1e-4 (Richardson,
-    # truncation O(h^4)), leaving the round-off bound 5 |S| 1e-13 / 1e-4 (1.7e-6 at |S| = 350;
-    # the truncation of the plain h = 1e-4 difference alone reaches 1.7e-6 on this field).
+    # limited: the bound 5 |S| 1e-13 / 1e-6 = 1.7e-4 at |S| = 350, measured 8.3e-5 (the action
+    # carries ~2.4e-13 relative rounding).  The tight check extrapolates h = 2e-4 and 1e-4
+    # (Richardson, truncation O(h^4)), leaving the round-off bound 5 |S| 1e-13 / 1e-4 (1.7e-6;
+    # measured 1.1e-6; the truncation of the plain h = 1e-4 difference alone reaches 1.7e-6).
     def cdiff(i, h):
         e = np.zeros(nx * nt, dtype=complex)
         e[i] = h
