@@ -92,6 +92,13 @@ int isle_force_parts(isle_hfm* h, const double* phi, int nt, int batch, int mode
  * rhss, out: device [nrhs][nx*nt] complex128. */
 int isle_solveM(isle_hfm* h, const double* phi, int nt, int species, const double* rhss, int nrhs, double* out);
 
+/* M(phi) applied to each of nrhs rows: the product that the reference forms with the sparse
+ * matrix of HubbardFermiMatrixExp::M / HubbardFermiMatrixDia::M (hubbardFermiMatrixExp.cpp:
+ * 150-166, hubbardFermiMatrixDia.cpp:140-160; bind_hubbardFermiMatrix.cpp:30-31), used for the
+ * residuals of solveM.  x, out: device nrhs x (nx*nt) complex128 rows (time-major, as solveM's
+ * right-hand sides); phi: device nx*nt complex128. */
+int isle_applyM(isle_hfm* h, const double* phi, int nt, int species, const double* x, long nrhs, double* out);
+
 /* solveM with the factorization kept between calls (no counterpart in the reference: for
  * callers that solve many right-hand sides against one field, such as iterative refinement).
  * reuse = 0: as isle_solveM (the handle keeps the factorization: partial products and their

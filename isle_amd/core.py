@@ -299,28 +299,20 @@ def applyM(hfm, phi, species, x):
     """M(phi) applied to every row of x (NRHS x NxNt, time-major), in row form: with v a
     row split into time slices, (M v)_t = K v_t - F_t v_{t-1} for t >= 1 and
     (M v)_0 = K v_0 + F_0 v_{nt-1} (hubbardFermiMatrixExp.cpp:150-166, K = 1;
-    hubbardFermiMatrixDia.cpp:140-160, F diagonal).  One real GEMM of all slices' real and
-    imaginary parts with the (real) slice matrix on the device; the slice matrices are the
-    plan's own."""
+    hubbardFermiMatrixDia.cpp:140-160, F diagonal).  On the GPU (isle_applyM): one complex x
+    real GEMM per time slice over all rows with the plan's slice matrix, between kernels that
+    form the shifted, phase-scaled rows and interleave the result."""
     nx = hfm.nx()
     phi = as_field(phi, nx)
     nt = phi.shape[-1] // nx
     x = torch.as_tensor(np.asarray(x) if not isinstance(x, torch.Tensor) else x)
     x = x.to(device=phi.device, dtype=torch.complex128)
-    rows = x.reshape(-1, nt, nx)
-    sign = 1j if Species(species) == Species.PARTICLE else -1j  # F(tp) = ... e^{sign i phi_{tp-1}}
-    ph = torch.exp(sign * torch.roll(phi.reshape(nt, nx), 1, 0))
-    y = torch.roll(rows, 1, 1) * ph
-    y[:, 0] = -y[:, 0]  # + F_0 v_{nt-1}: the antiperiodic wrap
-    s = torch.as_tensor(hfm._slice(species, False), device=phi.device).to(torch.float64).T.contiguous()
-
-    def times_s(v):  # v @ s for complex v and real s: one real 2-D GEMM over the re/im parts
-        shape = v.shape
-        parts = torch.view_as_real(v).transpose(-1, -2).reshape(-1, nx)  # copied: rows (.., re/im)
-        prod = torch.mm(parts, s).reshape(*shape[:-1], 2, nx)
-        return torch.view_as_complex(prod.transpose(-1, -2).contiguous())
-    out = rows - times_s(y) if hfm.HOPPING == 1 else times_s(rows) - y
-    return out.reshape(x.shape)
+    shape = x.shape
+    rows = x.reshape(-1, nx * nt).contiguous()
+    out = torch.empty_like(rows)
+    _lib.check(_lib.load().isle_applyM(hfm.handle, _lib.ptr(phi.contiguous()), nt, int(species), _lib.ptr(rows),
+                                       rows.shape[0], _lib.ptr(out)))
+    return out.reshape(shape)
 
 
 def solveM(hfm, phi, species, rhss, reuse_factor=False):

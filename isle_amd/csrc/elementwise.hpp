@@ -440,6 +440,75 @@ __global__ __launch_bounds__(256) void colscale_convert_kernel(const ColScaleIte
     }
 }
 
+// M(phi) applied to rows (row form of hubbardFermiMatrixExp.cpp:150-166 / Dia.cpp:140-160):
+// rows v = x[r] (interleaved complex, time-major nt x n), y_t = sgn_t D_t v_{t-1} with
+// D_t = exp(s i phi_{t-1}) (s = +1 particle, -1 hole), sgn_0 = -1 (the antiperiodic wrap),
+// sgn_t = +1 otherwise.  EXP: (M v)_t = v_t - S y_t; DIA: (M v)_t = K v_t - y_t (S, K the real
+// symmetric slice matrix; the product is a GEMM over the rows of each slice).
+// applym_prep_kernel writes the GEMM operands as split planes [re/im][t][r][x]:
+//   EXP: ya = y, c = v (the GEMM then subtracts ya S from c);  DIA: ya = v.
+__device__ __forceinline__ void applym_y(const double* __restrict__ x, const double* __restrict__ phi, int n, int nt,
+                                         double sgn, long r, int t, int i, double& yr, double& yi) {
+    const int tp = t == 0 ? nt - 1 : t - 1;
+    const long src = 2 * ((r * nt + tp) * (long)n + i);
+    const double vr = x[src], vi = x[src + 1];
+    const double a = phi[2 * ((long)tp * n + i)], b = phi[2 * ((long)tp * n + i) + 1];
+    // exp(s i (a + i b)) = exp(-s b) (cos(s a) + i sin(s a))
+    const double mag = exp(-sgn * b) * (t == 0 ? -1.0 : 1.0);
+    double sn, cs;
+    sincos(sgn * a, &sn, &cs);
+    const double pr = mag * cs, pi = mag * sn;
+    yr = pr * vr - pi * vi;
+    yi = pr * vi + pi * vr;
+}
+
+__global__ __launch_bounds__(256) void applym_prep_kernel(const double* __restrict__ x, const double* __restrict__ phi,
+                                                          int n, int nt, long nrhs, double sgn, int exp_hop,
+                                                          double* __restrict__ ya, double* __restrict__ c) {
+    const long total = nrhs * nt * (long)n, plane = total;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const int i = (int)(e % n);
+        const long rt = e / n;
+        const int t = (int)(rt % nt);
+        const long r = rt / nt;
+        const long dst = ((long)t * nrhs + r) * n + i;  // [t][r][x]
+        if (exp_hop) {
+            double yr, yi;
+            applym_y(x, phi, n, nt, sgn, r, t, i, yr, yi);
+            ya[dst] = yr;
+            ya[plane + dst] = yi;
+            c[dst] = x[2 * e];
+            c[plane + dst] = x[2 * e + 1];
+        } else {
+            ya[dst] = x[2 * e];
+            ya[plane + dst] = x[2 * e + 1];
+        }
+    }
+}
+
+// out[r][t][x] = c (EXP) or c - y (DIA), interleaved.
+__global__ __launch_bounds__(256) void applym_finish_kernel(const double* __restrict__ x, const double* __restrict__ phi,
+                                                            int n, int nt, long nrhs, double sgn, int exp_hop,
+                                                            const double* __restrict__ c, double* __restrict__ out) {
+    const long total = nrhs * nt * (long)n, plane = total;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const int i = (int)(e % n);
+        const long rt = e / n;
+        const int t = (int)(rt % nt);
+        const long r = rt / nt;
+        const long src = ((long)t * nrhs + r) * n + i;
+        double vr = c[src], vi = c[plane + src];
+        if (!exp_hop) {
+            double yr, yi;
+            applym_y(x, phi, n, nt, sgn, r, t, i, yr, yi);
+            vr -= yr;
+            vi -= yi;
+        }
+        out[2 * e] = vr;
+        out[2 * e + 1] = vi;
+    }
+}
+
 // Right-hand sides between the caller's layout (nrhs interleaved rows of nt*n complex) and
 // blocks [t][re/im][n][nrhs]: 32 x 32 tiles through LDS so that both the row reads and the
 // block writes are contiguous.  Grid (ceil(n/32), ceil(nrhs/32), nt), 256 threads.

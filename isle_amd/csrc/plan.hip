@@ -1620,6 +1620,61 @@ void Plan::force_parts(const double* phi, int nt, int batch, int mode, int group
     ISLE_HIP(hipMemcpyAsync(out_dev, op.parts, bytes, hipMemcpyDeviceToDevice, stream_));
 }
 
+// M x for rows x: per time slice one complex x real GEMM over all rows with the slice
+// matrix S (EXP: exp(kappa - mu)..., DIA: K), between an elementwise prologue (the shifted,
+// phase-scaled rows) and epilogue (back to interleaved rows); see applym_prep_kernel.
+void Plan::applyM(const double* phi, int nt, int species, const double* x, long nrhs, double* out) {
+    const int n = nx_;
+    if (nt < 1 || nrhs < 1) throw std::invalid_argument("applyM needs nt >= 1 and at least one row");
+    if (nrhs > (1L << 30) / ((long)nt * n)) throw std::invalid_argument("applyM: too many rows for one call");
+    hipStream_t s = stream_;
+    const long plane = nrhs * nt * (long)n;
+    const size_t need = (size_t)4 * plane;  // two split-plane arrays
+    if (need > apply_ws_doubles_) {
+        apply_ws_.reset();
+        apply_ws_.reset(new DevBuf(need * sizeof(double)));
+        apply_ws_doubles_ = need;
+    }
+    double* ya = apply_ws_->as<double>();
+    double* c = ya + 2 * plane;
+    const bool exp_hop = hop_ == EXP;
+    const double sgn = species == PARTICLE ? 1.0 : -1.0;
+    const unsigned blocks = (unsigned)std::min<long>((plane + 255) / 256, 16384);
+    hipLaunchKernelGGL(applym_prep_kernel, dim3(blocks), dim3(256), 0, s, x, phi, n, nt, nrhs, sgn, (int)exp_hop, ya,
+                       c);
+    ISLE_HIP(hipGetLastError());
+    if (apply_items_n_ < nt) {
+        apply_items_.reset();
+        apply_items_.reset(new DevBuf((size_t)nt * sizeof(GemmItem<double>)));
+        apply_items_n_ = nt;
+    }
+    const double* S = exp_hop ? e64_[species]->as<double>() : a64_[species]->as<double>();
+    std::vector<GemmItem<double>> items(nt);
+    for (int t = 0; t < nt; ++t) {
+        GemmItem<double>& g = items[t];
+        g = GemmItem<double>{};
+        g.a_re = ya + (long)t * nrhs * n;
+        g.a_im = ya + plane + (long)t * nrhs * n;
+        g.b_re = S;
+        g.c_re = c + (long)t * nrhs * n;
+        g.c_im = c + plane + (long)t * nrhs * n;
+        g.m = (int)nrhs;
+        g.n = n;
+        g.k = n;
+        g.lda = g.ldb = g.ldc = n;
+    }
+    ISLE_HIP(hipMemcpyAsync(apply_items_->p, items.data(), nt * sizeof(GemmItem<double>), hipMemcpyHostToDevice, s));
+    const auto* d_items = apply_items_->as<GemmItem<double>>();
+    const bool ok = exp_hop ? launch_gemm<double, true, false, false, SUBTRACT>(d_items, nt, (int)nrhs, n, s, 0)
+                            : launch_gemm<double, true, false, false, SCALE_NONE>(d_items, nt, (int)nrhs, n, s, 0);
+    if (!ok) throw std::runtime_error("applyM: GEMM tile does not fit");
+    hipLaunchKernelGGL(applym_finish_kernel, dim3(blocks), dim3(256), 0, s, x, phi, n, nt, nrhs, sgn, (int)exp_hop, c,
+                       out);
+    ISLE_HIP(hipGetLastError());
+    // the descriptors above live on the host stack: wait before they go
+    ISLE_HIP(hipStreamSynchronize(s));
+}
+
 double* Plan::solve_workspace(size_t doubles) {
     if (doubles > solve_ws_doubles_) {
         solve_ws_.reset();  // free before allocating the larger buffer
@@ -2043,6 +2098,13 @@ int isle_solveM_ex(isle_hfm* h, const double* phi, int nt, int species, const do
         check_species(species);
         if (nrhs < 1) throw std::invalid_argument("need at least one right hand side");
         P(h)->solveM(phi, nt, species, rhss, nrhs, out, reuse != 0);
+    });
+}
+
+int isle_applyM(isle_hfm* h, const double* phi, int nt, int species, const double* x, long nrhs, double* out) {
+    return guarded([&] {
+        check_species(species);
+        P(h)->applyM(phi, nt, species, x, nrhs, out);
     });
 }
 

@@ -129,6 +129,8 @@ public:
     void solveM(const double* phi, int nt, int species, const double* rhss, int nrhs, double* out,
                 bool reuse = false);
     void F(int tp, const double* phi, int nt, int species, int inv, double* out);
+    // M(phi) of `species` applied to nrhs rows x (nrhs x nx*nt interleaved complex) -> out
+    void applyM(const double* phi, int nt, int species, const double* x, long nrhs, double* out);
     void slice_matrix(int species, int inv, double* out_host);
     void sync() { ISLE_HIP(hipStreamSynchronize(stream_)); }
     // Time every launch of a cached force (op 0/1/4) or logdet (op 2/3/5) operation with HIP
@@ -164,6 +166,10 @@ private:
         std::unique_ptr<DevBuf> ph, mz, part, piv, luw;
     };
     SolveFactor sf_;
+    // applyM's split-plane operands and GEMM descriptors, kept and grown like solve_ws_
+    std::unique_ptr<DevBuf> apply_ws_, apply_items_;
+    size_t apply_ws_doubles_ = 0;
+    int apply_items_n_ = 0;
     size_t solve_ws_doubles_ = 0;
     double* solve_workspace(size_t doubles);
 
