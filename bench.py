@@ -93,6 +93,14 @@ def config4(args):
     dt = time.perf_counter() - t0
     pts = np.array(ens.trajPoints[args.warmup:], dtype=float)
     dh = np.array(ens.deltaH[args.warmup:], dtype=float)
+    launch = None
+    if not args.no_graph:
+        # after the timed region: the integration graph alone, and each bucket's alone
+        whole, alone = ens.time_buckets()
+        launch = {"graph_ms": whole, "longest_bucket_alone_ms": max(alone), "sum_buckets_alone_ms": sum(alone),
+                  "overlap": sum(alone) / whole,
+                  "buckets_alone_ms": {f"Nx={b.chains[0].nx} Nt={b.chains[0].nt} x{len(b.chains)}": round(t, 3)
+                                       for b, t in zip(ens.buckets, alone)}}
     # algorithmic work of one trajectory of every chain (per species: force
     # (2Nt-1) 4Nx^3 + 8Nx^3 + 8 Nt Nx^2, eval (Nt-1) 4Nx^3 + 8/3 Nx^3; 21 forces + 1 eval)
     def traj_flops(nx, nt):
@@ -130,6 +138,7 @@ def config4(args):
                    "buckets": len(ens.buckets), "parallelism": "buckets on separate streams, one HIP graph"},
         "acceptance": float(pts.mean()) if pts.size else None,
         "median_dH": float(np.median(dh)) if dh.size else None,
+        "launch_structure": launch,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK["f64"], "unit": "TFLOP/s",
                      "frac": achieved / PEAK["f64"], "traffic": None,
                      "note": "all kernels together: the algorithmic flops of the chains' trajectories "
@@ -178,7 +187,7 @@ def propagator(args):
     from isle_amd import meas
     from isle_amd.evolver import EvolutionStage, NumpyRNG
     from oracle import hubbard_ref as ref
-    kappa = ref.honeycomb(L)
+    kappa = isle.honeycomb(L).hopping()
     nx = kappa.shape[0]
     kt, ut = kappa * BETA / NT, U * BETA / NT
     n = nx * NT
@@ -250,7 +259,7 @@ def logdetq(args):
     import isle_amd as isle
     from isle_amd.evolver import NumpyRNG
     from oracle import hubbard_ref as ref
-    kappa = ref.honeycomb(L)
+    kappa = isle.honeycomb(L).hopping()
     nx = kappa.shape[0]
     kt, ut = kappa * BETA / NT, U * BETA / NT
     phi = NumpyRNG(42).normal(0, np.sqrt(ut), nx * NT) + 0j
@@ -307,7 +316,7 @@ def config5(args):
     from oracle import hubbard_ref as ref
     nt, beta, u, mu, nstep = 256, 10.0, 2.5, 0.5, 60
     chains = args.chains
-    kappa = ref.honeycomb(12, 16, periodic_j=False)
+    kappa = isle.honeycomb(12, 16, periodic_j=False).hopping()
     nx = kappa.shape[0]
     kt, mt, ut = kappa * beta / nt, mu * beta / nt, u * beta / nt
     ferm = A.HubbardFermiAction(kt, mt, -1, A.HFAHopping.DIA, A.HFABasis.SPIN, precision="stable", group=32,

@@ -49,6 +49,13 @@ int isle_hfm_set_stream(isle_hfm* h, void* stream);
  * the group size and nx % 32 == 0, else every product is kept); 0 keeps them all.  Applies to
  * force operations first used after the call. */
 int isle_hfm_set_recompute(isle_hfm* h, int k);
+/* Launch-shape pinning (no reference counterpart): every step of launch kind `kind` (the kinds
+ * of isle_profile_kernels) uses tile shape `tile` instead of the autotuner's choice, in the
+ * plan's operations built before and after the call; tile -1 returns to the tuned shape.  An
+ * operation with a pinned shape that does not fit one of its steps fails when run (return 3).
+ * For reproducible timings and for tests that cover every shape (FP32 chain products with the
+ * force diagonal fused, kind 30: tiles 0-3). */
+int isle_hfm_set_tile(isle_hfm* h, int kind, int tile);
 int isle_hfm_sync(isle_hfm* h);
 
 /* HubbardFermiMatrixExp::expKappa(species, inv) / HubbardFermiMatrixDia::K(species)
@@ -150,7 +157,9 @@ int isle_cscale(const double* in, double* out, long n, double cr, double ci, voi
  * triangular solve (GEMM launches), 25 GEMM32 real x complex with the force diagonal's row sums
  * fused, 26 sum of those row sums, 27 rows of right-hand sides in pivot order (solveM), 28 GEMM32
  * real x complex with the row sums fused and the stored rows scaled by the next slice's phase,
- * 29 transposed FP64 -> FP32 copy with complex row scaling. */
+ * 29 transposed FP64 -> FP32 copy with complex row scaling, 30 GEMM32 real x complex on
+ * k-contiguous slabs with the row sums fused (product stored transposed), 31 FP64 -> FP32 copy
+ * with complex column scaling. */
 int isle_profile_kernels(isle_hfm* h, const double* phi, int op, int nt, int batch, int group, int nspecies,
                          int spin, int reps, double* out_host, int nkinds);
 

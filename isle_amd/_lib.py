@@ -21,6 +21,7 @@ SIGNATURES = {
     "isle_hfm_destroy": (_c_int, [_vp]),
     "isle_hfm_set_stream": (_c_int, [_vp, _vp]),
     "isle_hfm_set_recompute": (_c_int, [_vp, _c_int]),
+    "isle_hfm_set_tile": (_c_int, [_vp, _c_int, _c_int]),
     "isle_hfm_sync": (_c_int, [_vp]),
     "isle_hfm_slice_matrix": (_c_int, [_vp, _c_int, _c_int, _dp]),
     "isle_hfm_F": (_c_int, [_vp, _c_int, _vp, _c_int, _c_int, _c_int, _vp]),

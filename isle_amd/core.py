@@ -117,6 +117,12 @@ class _HFMBase:
     def nx(self):
         return self._kappa.shape[0]
 
+    def pin_tile(self, kind, tile):
+        """Launch shape `tile` for every step of launch kind `kind` (isle_hfm_set_tile; -1:
+        the autotuner's choice again), for reproducible timings and for tests that cover
+        every shape.  HIP graphs captured before keep their shapes."""
+        _lib.check(_lib.load().isle_hfm_set_tile(self.handle, int(kind), int(tile)))
+
     def kappaTilde(self):
         return self._kappa.copy()
 

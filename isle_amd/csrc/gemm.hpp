@@ -657,14 +657,15 @@ constexpr bool rck_shape_ok() {
            BM % (16 * WM) == 0 && BN % (16 * WN) == 0;
 }
 
-// This wave's direct copies of slab `slab` into stage slab % STAGES of the LDS ring.
+// This wave's direct copies of k-slab `slab` (columns slab*BK ..) into stage `stage` of the
+// LDS ring.
 template <int BM, int BN, int BK, int STAGES, int NW>
-__device__ __forceinline__ void rck_issue(const GemmItem<float>& it, float* lds, int slab, int m0, int n0, int wave,
-                                          int lane) {
+__device__ __forceinline__ void rck_issue(const GemmItem<float>& it, float* lds, int stage, int slab, int m0, int n0,
+                                          int wave, int lane) {
     typedef RckLayout<BM, BN, BK, STAGES> LY;
     constexpr int CPR = BK / 4, RPI = 64 / CPR;
     constexpr int AQ = BM / RPI / NW, BQ = BN / RPI / NW;
-    float* st = lds + (slab % STAGES) * LY::STAGE;
+    float* st = lds + stage * LY::STAGE;
     const int k0 = slab * BK;
     const int rr = lane / CPR, pos = lane % CPR;
 #pragma unroll
@@ -684,6 +685,81 @@ __device__ __forceinline__ void rck_issue(const GemmItem<float>& it, float* lds,
         const long off = (long)row * it.ldb + k0 + 4 * c;
         __builtin_amdgcn_global_load_lds(it.b_re + off, st + LY::A + ins * 256, 16, 0, 0);
         __builtin_amdgcn_global_load_lds(it.b_im + off, st + LY::A + LY::B + ins * 256, 16, 0, 0);
+    }
+}
+
+// Epilogue operands of one 16-row block of a wave's part of the tile: the row factors
+// post[rowb .. rowb + 3] (1 when post_re is null) and the left-factor entries L[rowb + r][col]
+// of the block's NJ 16-column blocks (the identity when dl_re is null).  Lane (lr, lk) holds
+// C[4 lk + r][lr] of each 16 x 16 block; rowb = the block's first row + 4 lk.
+template <int NJ>
+struct RckEpi {
+    float fr[4], fi[4];
+    double lre[NJ][4], lim[NJ][4];
+};
+
+template <int NJ>
+__device__ __forceinline__ void rck_epi_load(const GemmItem<float>& it, int rowb, int colb, int lr, RckEpi<NJ>& e) {
+    const bool rows_in = rowb < it.m;  // m % 4 == 0: all four rows or none
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        e.fr[r] = 1.0f;
+        e.fi[r] = 0.0f;
+    }
+    if (it.post_re != nullptr) {  // uniform over the workgroup
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int di = rows_in ? rowb + r : 0;
+            e.fr[r] = it.post_re[di];
+            e.fi[r] = it.post_im[di];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int col = colb + j * 16 + lr;
+            const bool in = rows_in && col < it.n;
+            if (it.dl_re == nullptr) {
+                e.lre[j][r] = in && rowb + r == col ? 1.0 : 0.0;
+                e.lim[j][r] = 0.0;
+            } else {
+                const long o = in ? (long)(rowb + r) * it.n + col : 0;
+                const double x = it.dl_re[o], y = it.dl_im[o];
+                e.lre[j][r] = in ? x : 0.0;
+                e.lim[j][r] = in ? y : 0.0;
+            }
+        }
+}
+
+// The row sums of L .* C per 16-column block (from the unscaled product) into dpart, and the
+// scaled product stored transposed: Ct[col][rowb .. rowb + 3], one 16-byte store per lane.
+template <int NJ>
+__device__ __forceinline__ void rck_epi_store(const GemmItem<float>& it, int rowb, int colb, int lr,
+                                              const RckEpi<NJ>& e, const flt4 (&are)[NJ], const flt4 (&aim)[NJ]) {
+    const bool rows_in = rowb < it.m;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int col = colb + j * 16 + lr;
+        flt4 outr, outi;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float vr = are[j][r], vi = aim[j][r];
+            const double sr = row16_sum(e.lre[j][r] * (double)vr - e.lim[j][r] * (double)vi);
+            const double si = row16_sum(e.lre[j][r] * (double)vi + e.lim[j][r] * (double)vr);
+            const int slot = (colb + j * 16) / 16;
+            if (lr == 15 && rows_in && slot < it.n / 16) {
+                it.dpart[((long)slot * it.m + rowb + r) * 2] = sr;
+                it.dpart[((long)slot * it.m + rowb + r) * 2 + 1] = si;
+            }
+            outr[r] = vr * e.fr[r] - vi * e.fi[r];
+            outi[r] = vr * e.fi[r] + vi * e.fr[r];
+        }
+        if (rows_in && col < it.n) {
+            const long off = (long)col * it.ldc + rowb;
+            *reinterpret_cast<flt4*>(it.c_re + off) = outr;
+            *reinterpret_cast<flt4*>(it.c_im + off) = outi;
+        }
     }
 }
 
@@ -717,7 +793,7 @@ void gemm_rck_kernel(const GemmItem<float>* __restrict__ items, int tiles_n_max)
         for (int j = 0; j < NJ; ++j) acc_re[i][j] = acc_im[i][j] = flt4{0, 0, 0, 0};
 
     for (int j = 0; j < STAGES - 1 && j < nslab; ++j)
-        rck_issue<BM, BN, BK, STAGES, NW>(it, lds, j, m0, n0, wave, lane);
+        rck_issue<BM, BN, BK, STAGES, NW>(it, lds, j % STAGES, j, m0, n0, wave, lane);
     for (int s = 0; s < nslab; ++s) {
         // this wave's copies of slab s done (later slabs stay in flight), then all waves'
         const int ahead = min(STAGES - 2, nslab - 1 - s);
@@ -728,7 +804,7 @@ void gemm_rck_kernel(const GemmItem<float>* __restrict__ items, int tiles_n_max)
         else
             asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
         if (s + STAGES - 1 < nslab)  // into the buffer every wave finished reading
-            rck_issue<BM, BN, BK, STAGES, NW>(it, lds, s + STAGES - 1, m0, n0, wave, lane);
+            rck_issue<BM, BN, BK, STAGES, NW>(it, lds, (s + STAGES - 1) % STAGES, s + STAGES - 1, m0, n0, wave, lane);
         const float* st = lds + (s % STAGES) * LY::STAGE;
         const float* As = st;
         const float* Br = st + LY::A;
@@ -762,73 +838,13 @@ void gemm_rck_kernel(const GemmItem<float>* __restrict__ items, int tiles_n_max)
     }
     // Epilogue per 16-row block: row factors and left-factor entries loaded first (one
     // memory latency per block), then the row sums of the unscaled product, then the scaled
-    // transposed store.  Lane (lr, lk) holds C[4 lk + r][lr] of each 16 x 16 block.
-    const bool scaled = it.post_re != nullptr;  // uniform over the workgroup
-    const bool identity = it.dl_re == nullptr;
+    // transposed store.
 #pragma unroll
     for (int i = 0; i < MI; ++i) {
-        const int rowb = m0 + wm + i * 16 + 4 * lk;  // rows rowb .. rowb + 3
-        const bool rows_in = rowb < it.m;           // m % 4 == 0: all four or none
-        float fr[4], fi[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            fr[r] = 1.0f;
-            fi[r] = 0.0f;
-        }
-        if (scaled) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int di = rows_in ? rowb + r : 0;
-                fr[r] = it.post_re[di];
-                fi[r] = it.post_im[di];
-            }
-        }
-        double lre[NJ][4], lim[NJ][4];
-#pragma unroll
-        for (int j = 0; j < NJ; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int col = n0 + wn + j * 16 + lr;
-                const bool in = rows_in && col < it.n;
-                lre[j][r] = in && rowb + r == col ? 1.0 : 0.0;
-                lim[j][r] = 0;
-            }
-        if (!identity) {
-#pragma unroll
-            for (int j = 0; j < NJ; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int col = n0 + wn + j * 16 + lr;
-                    const bool in = rows_in && col < it.n;
-                    const long o = in ? (long)(rowb + r) * it.n + col : 0;
-                    const double x = it.dl_re[o], y = it.dl_im[o];
-                    lre[j][r] = in ? x : 0.0;
-                    lim[j][r] = in ? y : 0.0;
-                }
-        }
-#pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-            const int col = n0 + wn + j * 16 + lr;
-            flt4 outr, outi;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float vr = acc_re[i][j][r], vi = acc_im[i][j][r];
-                const double sr = row16_sum(lre[j][r] * (double)vr - lim[j][r] * (double)vi);
-                const double si = row16_sum(lre[j][r] * (double)vi + lim[j][r] * (double)vr);
-                const int slot = (n0 + wn + j * 16) / 16;
-                if (lr == 15 && rows_in && slot < it.n / 16) {
-                    it.dpart[((long)slot * it.m + rowb + r) * 2] = sr;
-                    it.dpart[((long)slot * it.m + rowb + r) * 2 + 1] = si;
-                }
-                outr[r] = vr * fr[r] - vi * fi[r];
-                outi[r] = vr * fi[r] + vi * fr[r];
-            }
-            if (rows_in && col < it.n) {
-                const long off = (long)col * it.ldc + rowb;
-                *reinterpret_cast<flt4*>(it.c_re + off) = outr;
-                *reinterpret_cast<flt4*>(it.c_im + off) = outi;
-            }
-        }
+        RckEpi<NJ> ep;
+        const int rowb = m0 + wm + i * 16 + 4 * lk;
+        rck_epi_load<NJ>(it, rowb, n0 + wn, lr, ep);
+        rck_epi_store<NJ>(it, rowb, n0 + wn, lr, ep, acc_re[i], acc_im[i]);
     }
 }
 

@@ -318,7 +318,27 @@ void Plan::slice_matrix(int species, int inv, double* out_host) {
 
 // ---------------------------------------------------------------------------
 void Plan::run(OpCache& op, const double* phi) {
-    for (const Step& st : op.steps) launch_step(op, st, phi);
+    for (const Step& st : op.steps)
+        if (!launch_step(op, st, phi)) throw std::runtime_error("a step's launch shape does not fit it");
+}
+
+void Plan::set_tile(int kind, int tile) {
+    if (kind < 0 || kind > (int)K::COLSCALE64TO32) throw std::invalid_argument("unknown step kind");
+    if (tile < -1 || tile >= N_TILES) throw std::invalid_argument("tile out of range");
+    if (tile >= 0)
+        forced_[kind] = tile;
+    else
+        forced_.erase(kind);
+    for (auto& kv : cache_)
+        for (Step& st : kv.second->steps) {
+            if ((int)st.kind != kind) continue;
+            if (tile >= 0) {
+                st.tile = tile;
+            } else {
+                auto f = tuned_.find(std::make_tuple((int)st.kind, st.max_m, st.max_n, st.count));
+                if (f != tuned_.end()) st.tile = f->second;
+            }
+        }
 }
 
 static bool is_gemm(K k) {
@@ -333,6 +353,11 @@ void Plan::tune(OpCache& op) {
     ISLE_HIP(hipEventCreate(&e1));
     for (Step& st : op.steps) {
         if (!is_gemm(st.kind)) continue;
+        auto pinned = forced_.find((int)st.kind);
+        if (pinned != forced_.end()) {
+            st.tile = pinned->second;
+            continue;
+        }
         auto key = std::make_tuple((int)st.kind, st.max_m, st.max_n, st.count);
         auto f = tuned_.find(key);
         if (f != tuned_.end()) {
@@ -2040,6 +2065,10 @@ int isle_hfm_set_stream(isle_hfm* h, void* stream) {
 
 int isle_hfm_set_recompute(isle_hfm* h, int k) {
     return guarded([&] { P(h)->set_recompute(k); });
+}
+
+int isle_hfm_set_tile(isle_hfm* h, int kind, int tile) {
+    return guarded([&] { P(h)->set_tile(kind, tile); });
 }
 
 int isle_hfm_slice_matrix(isle_hfm* h, int species, int inv, double* out_host) {

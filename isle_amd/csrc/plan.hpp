@@ -109,6 +109,11 @@ public:
         recompute_ = k;
     }
     hipStream_t stream() const { return stream_; }
+    // Pins the launch shape of every step of kind `kind` (a K value) to `tile` instead of the
+    // autotuner's choice, in operations built before and after; tile -1 returns to the tuned
+    // shape.  A pinned shape that does not fit a step makes its operation throw when run.
+    // HIP graphs captured earlier keep the shapes they recorded.
+    void set_tile(int kind, int tile);
 
     // log det M for a batch of fields (consecutive, nx*nt complex each) and nspecies
     // species (1: particle only, 2: particle and hole).  mode 0 follows the reference
@@ -148,6 +153,7 @@ private:
     std::vector<double> kappa_;
     hipStream_t stream_ = nullptr;
     int recompute_ = 0;
+    std::map<int, int> forced_;  // kind -> pinned tile
     // inverse-form slice matrices A_s (EXP: exp(-k+mu), exp(-s k-mu); DIA: K_p, K_h)
     std::unique_ptr<DevBuf> a64_[2], a32_[2];
     // EXP non-inverse matrices exp(k-mu), exp(s k+mu)

@@ -109,6 +109,35 @@ class EnsembleLeapfrog:
             for s in self._streams:
                 cap.wait_stream(s)
 
+    def time_buckets(self, reps=3):
+        """Milliseconds of one trajectory's integration (the replayed work, no host part):
+        ``(whole, alone)`` with ``whole`` the ensemble graph and ``alone[i]`` bucket i's
+        integration captured in a graph of its own and replayed with nothing beside it, on
+        the inputs of the last trajectory.  The longest ``alone`` is the ensemble's critical
+        path; ``sum(alone) / whole`` is how much the buckets' streams overlap."""
+        if self._graph is None:
+            self._capture()
+
+        def timed(replay):
+            replay()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                replay()
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / reps
+        whole = timed(self._graph.replay)
+        alone = []
+        for b, (p, q) in zip(self.buckets, self._in):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                leapfrog_batch(p, q, b.action, self.length, self.nstep, evaluate=False)
+            alone.append(timed(g.replay))
+            del g
+        return whole, alone
+
     def _momenta(self):
         """Every chain's momenta rng.normal(0, 1, n) (real parts), drawn on a thread pool: the
         chains' generators are independent and numpy releases the GIL while drawing."""
