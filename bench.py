@@ -42,7 +42,13 @@ def parse():
                     "before the timed ones")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--logdet", choices=["stable", "grouped"], default="stable",
+                    help="the mixed form's Metropolis log det: groups of 8 slices through the stabilising "
+                         "block-cyclic LU (default), or the force's group products multiplied together")
     ap.add_argument("--no-graph", action="store_true", help="launch the integration eagerly, not from a HIP graph")
+    ap.add_argument("--no-merge", action="store_true",
+                    help="config 4: every (Nx, Nt) bucket with its own launches, instead of the buckets of one "
+                         "lattice sharing theirs")
     ap.add_argument("--streams", type=int, default=2,
                     help="parts of the batch integrated on streams of their own inside the graph")
     ap.add_argument("--config4", action="store_true",
@@ -82,7 +88,8 @@ def config4(args):
     lats = {l: isle.honeycomb(int(l)).hopping() for l in (3, 4, 6, 8)}
     chains = [Chain(lats[int(l)], int(nt), nt / 8.0, 2.0, 0.0, -1, NumpyRNG(1000 + c))
               for c, (l, nt) in enumerate(zip(ls, nts))]
-    ens = EnsembleLeapfrog(chains, 1.0, 20, precision="mixed", group=32, graph=not args.no_graph)
+    ens = EnsembleLeapfrog(chains, 1.0, 20, precision="mixed", group=32, graph=not args.no_graph,
+                           merge=not args.no_merge)
     for _ in range(args.warmup):
         ens.evolve()
     torch.cuda.synchronize()
@@ -95,12 +102,13 @@ def config4(args):
     dh = np.array(ens.deltaH[args.warmup:], dtype=float)
     launch = None
     if not args.no_graph:
-        # after the timed region: the integration graph alone, and each bucket's alone
-        whole, alone = ens.time_buckets()
-        launch = {"graph_ms": whole, "longest_bucket_alone_ms": max(alone), "sum_buckets_alone_ms": sum(alone),
-                  "overlap": sum(alone) / whole,
-                  "buckets_alone_ms": {f"Nx={b.chains[0].nx} Nt={b.chains[0].nt} x{len(b.chains)}": round(t, 3)
-                                       for b, t in zip(ens.buckets, alone)}}
+        # after the timed region: the integration graph alone, and each stream's group of
+        # buckets alone (one bucket per stream without merging)
+        whole, alone = ens.time_groups()
+        times = [t for _, t in alone]
+        launch = {"graph_ms": whole, "streams": len(alone), "merged": not args.no_merge,
+                  "longest_stream_alone_ms": max(times), "sum_streams_alone_ms": sum(times),
+                  "overlap": sum(times) / whole, "streams_alone_ms": {lab: round(t, 3) for lab, t in alone}}
     # algorithmic work of one trajectory of every chain (per species: force
     # (2Nt-1) 4Nx^3 + 8Nx^3 + 8 Nt Nx^2, eval (Nt-1) 4Nx^3 + 8/3 Nx^3; 21 forces + 1 eval)
     def traj_flops(nx, nt):
@@ -135,7 +143,9 @@ def config4(args):
         "dtype": "f32 right chains + f64 suffix/top level/action", "data": "synthetic",
         "config": {"workload": "256 honeycomb tori, L in {3,4,6,8}, Nt in {16,32,64}, beta=Nt/8, U=2, mu=0, EXP, "
                                "particle-hole, sigmaKappa=-1, HMC length 1 with 20 steps",
-                   "buckets": len(ens.buckets), "parallelism": "buckets on separate streams, one HIP graph"},
+                   "buckets": len(ens.buckets),
+                   "parallelism": ("the buckets of one lattice share their force launches, one stream per lattice, "
+                                   "one HIP graph" if not args.no_merge else "buckets on separate streams, one HIP graph")},
         "acceptance": float(pts.mean()) if pts.size else None,
         "median_dH": float(np.median(dh)) if dh.size else None,
         "launch_structure": launch,
@@ -488,7 +498,9 @@ def main():
     B = args.chains
 
     def make(prec):
-        ferm = A.HubbardFermiAction(kt, 0.0, -1, A.HFAHopping.EXP, precision=prec, group=args.group)
+        # the mixed form's Metropolis action: FP64, stabilising block-cyclic LU every 8 slices
+        extra = {"logdet": args.logdet, "logdet_group": 8} if prec == "mixed" else {}
+        ferm = A.HubbardFermiAction(kt, 0.0, -1, A.HFAHopping.EXP, precision=prec, group=args.group, **extra)
         return A.SumAction(A.HubbardGaugeAction(ut), ferm), ferm
 
     def clone_rngs(rngs):
@@ -644,6 +656,8 @@ def main():
         "config": {"workload": "honeycomb 12x12 torus Nx=288 Nt=128 beta=8 U=3 mu=0 EXP particle-hole "
                                "sigmaKappa=-1, HMC length 1 with 40 leapfrog steps",
                    "nx": nx, "nt": NT, "chains": B, "group": g, "streams": args.streams if not args.no_graph else 1,
+                   "metropolis_logdet": ("FP64, stabilising block-cyclic LU every 8 slices" if args.logdet == "stable"
+                                         else f"FP64 products of {g} slices multiplied together"),
                    "parallelism": "independent chains, 1 GPU"},
         "plain_fp64_traj_per_s": plain,
         "speedup_vs_plain_fp64": (traj_per_s / plain) if plain else None,

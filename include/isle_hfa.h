@@ -94,6 +94,15 @@ int isle_logdetM_batch(isle_hfm* h, const double* phi, int nt, int batch, int mo
 int isle_force_parts(isle_hfm* h, const double* phi, int nt, int batch, int mode, int group, int nspecies,
                      int spin, double* out_dev);
 
+/* isle_force_parts for nops batches of fields of one plan at once (no reference counterpart:
+ * the reference evaluates one field at a time; for ensembles of chains of several time
+ * extents on one lattice).  Batch i: batches[i] fields of nts[i] slices at phis[i] (device),
+ * group groups[i], parts to outs[i] (device [batches[i]][nspecies][nx*nts[i]]).  The batches'
+ * operations run merged: steps of one kind and launch shape share one launch.  Two batches
+ * may not have the same (nt, batch, group). */
+int isle_force_parts_multi(isle_hfm* h, int nops, const double* const* phis, const int* nts, const int* batches,
+                           int mode, const int* groups, int nspecies, int spin, double* const* outs);
+
 /* solveM(hfm, phi, species, rhss): solves M x = b for every row b of rhss
  *   bind_hubbardFermiMatrix.cpp:53-55, hubbardFermiMatrixExp.cpp:645-694, hubbardFermiMatrixDia.cpp:617-673.
  * rhss, out: device [nrhs][nx*nt] complex128. */

@@ -28,6 +28,7 @@ SIGNATURES = {
     "isle_logdetM": (_c_int, [_vp, _vp, _c_int, _c_int, _dp]),
     "isle_logdetM_batch": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _dp]),
     "isle_force_parts": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp]),
+    "isle_force_parts_multi": (_c_int, [_vp, _c_int, _vp, _vp, _vp, _c_int, _vp, _c_int, _c_int, _vp]),
     "isle_solveM": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp]),
     "isle_solveM_ex": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int]),
     "isle_applyM": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _c_long, _vp]),

@@ -10,7 +10,9 @@ Besides the reference's single-field ``eval``/``force``, HubbardFermiAction offe
 
 * ``"fp64"``  - the reference DIRECT_SINGLE algorithm, every product in FP64;
 * ``"mixed"`` - force from FP32 chains over groups of ``group`` time slices on MFMA,
-  combined in FP64; eval (the Metropolis action) from FP64 group products.  With
+  combined in FP64; eval (the Metropolis action) from FP64 group products, multiplied
+  together (``logdet="grouped"``) or combined through the stabilising block-cyclic LU with
+  groups of ``logdet_group`` slices (``logdet="stable"``, default 8).  With
   ``recompute=k`` (k dividing ``group``, Nx a multiple of 32) the force keeps the FP64
   suffix chains only every k slices and recomputes the others when they are needed:
   about a third of the force's memory for about half again its FP64 work.
@@ -21,6 +23,7 @@ Besides the reference's single-field ``eval``/``force``, HubbardFermiAction offe
   force's per-group right chains in FP32 on MFMA (with the force diagonal fused into them);
   the elimination, the suffix chains and the action stay FP64.
 """
+import ctypes
 import enum
 
 import numpy as np
@@ -29,6 +32,10 @@ import torch
 from . import _lib
 from .core import (HubbardFermiMatrixDia, HubbardFermiMatrixExp, Lattice, as_field, invert, isBipartite, logdetQ,
                    to_first_log_branch)
+
+
+# device workspace bound of one stable log det call; larger batches are split into chunks
+_STABLE_LOGDET_BYTES = 24 << 30
 
 
 class HFAHopping(enum.IntEnum):
@@ -162,9 +169,17 @@ class HubbardFermiAction(Action):
 
     def __init__(self, kappaTilde, muTilde, sigmaKappa, hopping=HFAHopping.DIA,
                  basis=HFABasis.PARTICLE_HOLE, algorithm=HFAAlgorithm.DIRECT_SINGLE,
-                 allowShortcut=False, precision="fp64", group=32, recompute=0, right_chains="fp64"):
+                 allowShortcut=False, precision="fp64", group=32, recompute=0, right_chains="fp64",
+                 logdet="grouped", logdet_group=8):
         if precision not in ("fp64", "mixed", "stable"):
             raise ValueError("precision must be 'fp64', 'mixed' or 'stable'")
+        if logdet not in ("grouped", "stable"):
+            raise ValueError("logdet must be 'grouped' or 'stable'")
+        if logdet == "stable" and precision != "mixed":
+            raise ValueError("logdet='stable' selects the Metropolis log det of precision='mixed' "
+                             "('fp64' is the reference algorithm, 'stable' is stable throughout)")
+        if int(logdet_group) < 1:
+            raise ValueError("logdet_group must be >= 1")
         if right_chains not in ("fp64", "fp32"):
             raise ValueError("right_chains must be 'fp64' or 'fp32'")
         if int(recompute) < 0:
@@ -185,6 +200,11 @@ class HubbardFermiAction(Action):
         # the stable force's right chains: FP64 (default) or FP32 on MFMA with the force
         # diagonal fused into the chain products (as in the mixed form)
         self.right_chains = right_chains
+        # the mixed form's Metropolis log det: FP64 products of `group` slices multiplied
+        # together ("grouped"), or FP64 products of `logdet_group` slices combined through the
+        # stabilising block-cyclic LU ("stable"), never through the product of all slices
+        self.logdet_form = logdet
+        self.logdet_group = int(logdet_group)
         if self.recompute:
             _lib.check(_lib.load().isle_hfm_set_recompute(self.hfm.handle, self.recompute))
         # log of the condition-number bound of one time slice: the FP32 right chains of
@@ -248,7 +268,19 @@ class HubbardFermiAction(Action):
     def _mode(self, force=False):
         if force and self.precision == "stable" and self.right_chains == "fp32":
             return 3
+        if not force and self.precision == "mixed" and self.logdet_form == "stable":
+            return 2
         return {"fp64": 0, "mixed": 1, "stable": 2}[self.precision]
+
+    def _logdet_group(self, nt):
+        """Slices per group of the log det: the force's group, or for the mixed form's stable
+        log det the largest divisor of nt not above ``logdet_group``."""
+        if self.precision == "mixed" and self.logdet_form == "stable":
+            g = min(self.logdet_group, nt)
+            while nt % g:
+                g -= 1
+            return g
+        return self._group(nt)
 
     def _group(self, nt):
         """Slices per group: the largest divisor of nt not above ``group`` that keeps the
@@ -280,9 +312,20 @@ class HubbardFermiAction(Action):
         phis, batch, nt = self._shape(phis)
         nsp = 1 if self.shortcut else 2
         out = np.empty((batch, nsp, 2))
-        _lib.check(_lib.load().isle_logdetM_batch(
-            self.hfm.handle, _lib.ptr(phis), nt, batch, self._mode(), self._group(nt), nsp,
-            int(self.basis == HFABasis.SPIN), _lib.dptr(out)))
+        mode, g = self._mode(), self._logdet_group(nt)
+        step = batch
+        if mode == 2 and nt // g > 1:
+            # the block-cyclic factorisation keeps every elimination step's 2Nx x 3Nx block
+            # (12 Nx^2 doubles) and the group products (4 Nx^2 doubles) with LU work: about
+            # 136 Nx^2 bytes per group, field and species
+            nx = self.hfm.nx()
+            per_field = 136 * nx * nx * (nt // g) * nsp
+            step = max(1, min(batch, _STABLE_LOGDET_BYTES // per_field))
+        for b0 in range(0, batch, step):
+            b1 = min(batch, b0 + step)
+            _lib.check(_lib.load().isle_logdetM_batch(
+                self.hfm.handle, _lib.ptr(phis[b0:b1]), nt, b1 - b0, mode, g, nsp,
+                int(self.basis == HFABasis.SPIN), _lib.dptr(out[b0:b1])))
         return out[..., 0] + 1j * out[..., 1]
 
     def eval_batch(self, phis):
@@ -315,6 +358,46 @@ class HubbardFermiAction(Action):
                                                 self._group(nt), nsp, int(self.basis == HFABasis.SPIN),
                                                 _lib.ptr(parts)))
         return parts
+
+    def force_parts_multi(self, phis_list):
+        """force_parts of several batches of fields of this lattice with different time
+        extents, their operations merged into shared launches (isle_force_parts_multi: every
+        chain step is one launch over all batches' descriptors).  ``phis_list``: tensors
+        [batch_i, nx*nt_i] with distinct nt_i; returns [batch_i, nspecies, nx*nt_i] each."""
+        self._single_only()
+        shaped = [self._shape(p) for p in phis_list]
+        nx = self.hfm.nx()
+        nsp = 1 if self.shortcut else 2
+        parts = [torch.empty((b, nsp, nx * nt), dtype=torch.complex128, device=p.device) for p, b, nt in shaped]
+        k = len(shaped)
+        ptrs = ctypes.c_void_p * k
+        ints = ctypes.c_int * k
+        phis_a = ptrs(*[p.data_ptr() for p, _, _ in shaped])
+        outs_a = ptrs(*[t.data_ptr() for t in parts])
+        nts = ints(*[nt for _, _, nt in shaped])
+        batches = ints(*[b for _, b, _ in shaped])
+        groups = ints(*[self._group(nt) for _, _, nt in shaped])
+        as_vp = lambda a: ctypes.cast(a, ctypes.c_void_p)  # noqa: E731
+        _lib.check(_lib.load().isle_force_parts_multi(
+            self.hfm.handle, k, as_vp(phis_a), as_vp(nts), as_vp(batches), self._mode(force=True), as_vp(groups), nsp,
+            int(self.basis == HFABasis.SPIN), as_vp(outs_a)))
+        return parts
+
+    def force_batch_multi(self, phis_list, outs):
+        """force_batch of several batches of fields with different time extents (see
+        force_parts_multi), written to ``outs`` (tensors shaped like the fields)."""
+        parts = self.force_parts_multi(phis_list)
+        nsp = 1 if self.shortcut else 2
+        mode = 2 if self.basis == HFABasis.SPIN else (1 if self.shortcut else 0)
+        lib = _lib.load()
+        for pt, out in zip(parts, outs):
+            batch, n = pt.shape[0], pt.shape[2]
+            if out.stride(-1) != 1 or (batch > 1 and out.stride(0) < n):
+                raise ValueError("out must hold the fields' forces as rows of contiguous elements")
+            fh = pt[:, 1] if nsp == 2 else pt[:, 0]
+            _lib.check(lib.isle_combine_force_batch(_lib.ptr(pt[:, 0]), _lib.ptr(fh), _lib.ptr(out), n, batch, nsp * n,
+                                                    out.stride(0) if batch > 1 else 0, mode, 0, _lib.stream_ptr()))
+        return outs
 
     def force_batch(self, phis, out=None, accumulate=False):
         """Force -dS/dphi of each field (hubbardFermiAction.cpp:243-272), [batch, nx*nt]."""

@@ -24,6 +24,7 @@ struct Builder {
         int count, mm, mn, flag, aux;
         double flops;
         int tile;
+        size_t ibytes;
     };
     std::vector<Pending> pend;
 
@@ -35,7 +36,7 @@ struct Builder {
         size_t off = (blob.size() + 63) & ~size_t(63);
         blob.resize(off + items.size() * sizeof(It));
         std::memcpy(blob.data() + off, items.data(), items.size() * sizeof(It));
-        pend.push_back({kind, off, (int)items.size(), mm, mn, flag, aux, flops_of(kind, items), tile});
+        pend.push_back({kind, off, (int)items.size(), mm, mn, flag, aux, flops_of(kind, items), tile, sizeof(It)});
     }
     template <typename It>
     static double flops_of(K, const std::vector<It>&) { return 0.0; }
@@ -62,7 +63,9 @@ struct Builder {
         op.bufs.emplace_back(new DevBuf(blob.size() ? blob.size() : 64));
         char* dev = op.bufs.back()->as<char>();
         if (!blob.empty()) ISLE_HIP(hipMemcpy(dev, blob.data(), blob.size(), hipMemcpyHostToDevice));
-        for (auto& p : pend) op.steps.push_back({p.kind, dev + p.off, p.count, p.mm, p.mn, p.flag, p.flops, p.tile, p.aux});
+        for (auto& p : pend)
+            op.steps.push_back({p.kind, dev + p.off, p.count, p.mm, p.mn, p.flag, p.flops, p.tile, p.aux, p.ibytes, p.off});
+        op.blob = std::move(blob);
     }
 };
 
@@ -329,8 +332,8 @@ void Plan::set_tile(int kind, int tile) {
         forced_[kind] = tile;
     else
         forced_.erase(kind);
-    for (auto& kv : cache_)
-        for (Step& st : kv.second->steps) {
+    auto apply = [&](OpCache& op) {
+        for (Step& st : op.steps) {
             if ((int)st.kind != kind) continue;
             if (tile >= 0) {
                 st.tile = tile;
@@ -339,6 +342,9 @@ void Plan::set_tile(int kind, int tile) {
                 if (f != tuned_.end()) st.tile = f->second;
             }
         }
+    };
+    for (auto& kv : cache_) apply(*kv.second);
+    for (auto& kv : multi_) apply(kv.second->merged);
 }
 
 static bool is_gemm(K k) {
@@ -1634,15 +1640,163 @@ void Plan::logdetM(const double* phi, int nt, int batch, int mode, int group, in
     }
 }
 
+OpCache& Plan::get_force(int mode, int nt, int batch, int group, int nspecies, int spin) {
+    if (nt < 2) throw std::invalid_argument("nt < 2 in HubbardFermiAction algorithm DIRECT_SINGLE not supported");
+    return mode == 0   ? get_force_plain(nt, batch, nspecies, spin)
+           : mode == 1 ? get_force_mixed(nt, batch, group, nspecies, spin)
+                       : get_force_stable(nt, batch, group, nspecies, spin, mode == 3);
+}
+
 void Plan::force_parts(const double* phi, int nt, int batch, int mode, int group, int nspecies, int spin,
                        double* out_dev) {
-    if (nt < 2) throw std::invalid_argument("nt < 2 in HubbardFermiAction algorithm DIRECT_SINGLE not supported");
-    OpCache& op = mode == 0   ? get_force_plain(nt, batch, nspecies, spin)
-                  : mode == 1 ? get_force_mixed(nt, batch, group, nspecies, spin)
-                              : get_force_stable(nt, batch, group, nspecies, spin, mode == 3);
+    OpCache& op = get_force(mode, nt, batch, group, nspecies, spin);
     run(op, phi);
     const size_t bytes = (size_t)batch * nspecies * nt * nx_ * 2 * 8;
     ISLE_HIP(hipMemcpyAsync(out_dev, op.parts, bytes, hipMemcpyDeviceToDevice, stream_));
+}
+
+// Merging independent operations of one plan.  Each operation is a sequence of dependent
+// steps; operations share nothing but read-only slice matrices, so any interleaving that
+// keeps every operation's own order is correct.  Two steps can share a launch when they have
+// the same kind, descriptor size and launch parameters (GEMM kinds: any m, n, the launch
+// covers the largest) and neither binds the caller's fields (PHASE).  The sequences are
+// merged one after another into a shortest common supersequence under that relation (dynamic
+// programming over the two sequences), so the merged operation has as few launches as that
+// greedy order allows; each merged step's descriptors are the concatenation of its members'.
+std::unique_ptr<Plan::MultiOp> Plan::merge_ops(const std::vector<OpCache*>& ops) {
+    struct Member {
+        int op, step;
+    };
+    struct Slot {
+        std::vector<Member> members;
+    };
+    auto step_of = [&](const Member& mb) -> const Step& { return ops[mb.op]->steps[mb.step]; };
+    auto joinable = [&](const Step& a, const Step& b) {
+        if (a.kind != b.kind || a.kind == K::PHASE || a.ibytes != b.ibytes || a.flag != b.flag || a.aux != b.aux)
+            return false;
+        return is_gemm(a.kind) || (a.max_m == b.max_m && a.max_n == b.max_n);
+    };
+    std::vector<Slot> seq;
+    for (int i = 0; i < (int)ops[0]->steps.size(); ++i) seq.push_back({{{0, i}}});
+    for (int o = 1; o < (int)ops.size(); ++o) {
+        const auto& st = ops[o]->steps;
+        const int a = (int)seq.size(), bn = (int)st.size();
+        // len[i][j]: length of the shortest merge of seq[i..] and st[j..]
+        std::vector<std::vector<int>> len(a + 1, std::vector<int>(bn + 1, 0));
+        for (int i = a; i >= 0; --i)
+            for (int j = bn; j >= 0; --j) {
+                if (i == a || j == bn) {
+                    len[i][j] = (a - i) + (bn - j);
+                    continue;
+                }
+                int best = 1 + std::min(len[i + 1][j], len[i][j + 1]);
+                if (joinable(step_of(seq[i].members[0]), st[j])) best = std::min(best, 1 + len[i + 1][j + 1]);
+                len[i][j] = best;
+            }
+        std::vector<Slot> out;
+        int i = 0, j = 0;
+        while (i < a || j < bn) {
+            if (i < a && j < bn && joinable(step_of(seq[i].members[0]), st[j]) && len[i][j] == 1 + len[i + 1][j + 1]) {
+                Slot sl = seq[i];
+                sl.members.push_back({o, j});
+                out.push_back(sl);
+                ++i;
+                ++j;
+            } else if (i < a && (j == bn || len[i][j] == 1 + len[i + 1][j])) {
+                out.push_back(seq[i++]);
+            } else {
+                out.push_back({{{o, j}}});
+                ++j;
+            }
+        }
+        seq.swap(out);
+    }
+    auto mo = std::make_unique<MultiOp>();
+    mo->ops = ops;
+    std::vector<char> blob;
+    std::vector<size_t> offs;
+    for (const Slot& sl : seq) {
+        size_t off = (blob.size() + 63) & ~size_t(63);
+        if (sl.members.size() > 1) {
+            size_t total = 0;
+            for (const Member& mb : sl.members) total += step_of(mb).count * step_of(mb).ibytes;
+            blob.resize(off + total);
+            size_t pos = off;
+            for (const Member& mb : sl.members) {
+                const Step& s2 = step_of(mb);
+                const size_t bytes = s2.count * s2.ibytes;
+                std::memcpy(blob.data() + pos, ops[mb.op]->blob.data() + s2.off, bytes);
+                pos += bytes;
+            }
+        }
+        offs.push_back(off);
+    }
+    mo->merged.bufs.emplace_back(new DevBuf(blob.size() ? blob.size() : 64));
+    char* dev = mo->merged.bufs.back()->as<char>();
+    if (!blob.empty()) ISLE_HIP(hipMemcpy(dev, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    for (size_t q = 0; q < seq.size(); ++q) {
+        const Slot& sl = seq[q];
+        Step st = step_of(sl.members[0]);
+        if (sl.members.size() == 1) {  // the member's own descriptors and launch
+            mo->owner.push_back(st.kind == K::PHASE ? sl.members[0].op : -1);
+            mo->merged.steps.push_back(st);
+            continue;
+        }
+        st.items = dev + offs[q];
+        st.off = offs[q];
+        st.count = 0;
+        st.flops = 0;
+        for (const Member& mb : sl.members) {
+            const Step& s2 = step_of(mb);
+            st.count += s2.count;
+            st.flops += s2.flops;
+            st.max_m = std::max(st.max_m, s2.max_m);
+            st.max_n = std::max(st.max_n, s2.max_n);
+        }
+        mo->owner.push_back(-1);
+        mo->merged.steps.push_back(st);
+    }
+    mo->merged.blob = std::move(blob);
+    return mo;
+}
+
+void Plan::force_parts_multi(int nops, const double* const* phis, const int* nts, const int* batches, int mode,
+                             const int* groups, int nspecies, int spin, double* const* outs) {
+    if (nops < 1) throw std::invalid_argument("need at least one batch of fields");
+    std::vector<long> key = {mode, nspecies, spin, recompute_, nops};
+    for (int i = 0; i < nops; ++i) {
+        if (batches[i] < 1) throw std::invalid_argument("bad batch");
+        key.push_back(nts[i]);
+        key.push_back(batches[i]);
+        key.push_back(groups[i]);
+    }
+    auto found = multi_.find(key);
+    MultiOp* mo = nullptr;
+    if (found == multi_.end()) {
+        std::vector<OpCache*> ops;
+        for (int i = 0; i < nops; ++i) {
+            OpCache* op = &get_force(mode, nts[i], batches[i], groups[i], nspecies, spin);
+            for (OpCache* o : ops)
+                if (o == op) throw std::invalid_argument("two batches of one shape: join them into one batch");
+            ops.push_back(op);
+        }
+        auto made = merge_ops(ops);
+        tune(made->merged);
+        mo = made.get();
+        multi_[key] = std::move(made);
+    } else {
+        mo = found->second.get();
+    }
+    for (size_t q = 0; q < mo->merged.steps.size(); ++q) {
+        const int o = mo->owner[q];
+        const Step& st = mo->merged.steps[q];
+        const bool ok = o >= 0 ? launch_step(*mo->ops[o], st, phis[o]) : launch_step(mo->merged, st, nullptr);
+        if (!ok) throw std::runtime_error("a step's launch shape does not fit it");
+    }
+    for (int i = 0; i < nops; ++i) {
+        const size_t bytes = (size_t)batches[i] * nspecies * nts[i] * nx_ * 2 * 8;
+        ISLE_HIP(hipMemcpyAsync(outs[i], mo->ops[i]->parts, bytes, hipMemcpyDeviceToDevice, stream_));
+    }
 }
 
 // M x for rows x: per time slice one complex x real GEMM over all rows with the slice
@@ -2110,6 +2264,15 @@ int isle_force_parts(isle_hfm* h, const double* phi, int nt, int batch, int mode
         if (batch < 1 || (nspecies != 1 && nspecies != 2)) throw std::invalid_argument("bad batch/nspecies");
         if (mode < 0 || mode > 3) throw std::invalid_argument("mode must be 0, 1, 2 or 3");
         P(h)->force_parts(phi, nt, batch, mode, group, nspecies, spin, out_dev);
+    });
+}
+
+int isle_force_parts_multi(isle_hfm* h, int nops, const double* const* phis, const int* nts, const int* batches,
+                           int mode, const int* groups, int nspecies, int spin, double* const* outs) {
+    return guarded([&] {
+        if (nops < 1 || (nspecies != 1 && nspecies != 2)) throw std::invalid_argument("bad nops/nspecies");
+        if (mode < 0 || mode > 3) throw std::invalid_argument("mode must be 0, 1, 2 or 3");
+        P(h)->force_parts_multi(nops, phis, nts, batches, mode, groups, nspecies, spin, outs);
     });
 }
 

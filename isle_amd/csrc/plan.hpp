@@ -81,12 +81,15 @@ struct Step {
     double flops;        // algorithmic real flops of the launch (GEMMs: 2mnk per real product)
     int tile = 0;        // GEMM tile shape chosen by the autotuner (TILE_64 / TILE_32)
     int aux = 0;         // LU_SHAPE: nt; BLOCK / GATHER: largest element count per item
+    size_t ibytes = 0;   // bytes per descriptor
+    size_t off = 0;      // offset of the descriptors in the operation's host blob
 };
 
 // A cached operation: descriptor blob + workspaces + step list.
 struct OpCache {
     std::vector<std::unique_ptr<DevBuf>> bufs;
     std::vector<Step> steps;
+    std::vector<char> blob;    // host copy of the descriptors (for merged operations)
     long phi_stride = 0;       // doubles between consecutive fields of the batch
     double* logdet = nullptr;  // device [batch][nspecies][2]
     double* parts = nullptr;   // device [batch][nspecies][nx*nt] interleaved
@@ -130,6 +133,12 @@ public:
     // combination taken from the structured factorisation of the block-cyclic matrix.
     void force_parts(const double* phi, int nt, int batch, int mode, int group, int nspecies, int spin,
                      double* out_dev);
+    // force_parts for several batches of fields at once (nops of them, batch i with nts[i]
+    // slices, batches[i] fields at phis[i], group groups[i], parts to outs[i]): their
+    // operations' steps are aligned by kind and every aligned step is one launch over the
+    // descriptors of all of them (the phase steps, which bind the fields, stay separate).
+    void force_parts_multi(int nops, const double* const* phis, const int* nts, const int* batches, int mode,
+                           const int* groups, int nspecies, int spin, double* const* outs);
     // reuse: phi, nt and species equal those of the previous call (its factorization is kept)
     void solveM(const double* phi, int nt, int species, const double* rhss, int nrhs, double* out,
                 bool reuse = false);
@@ -192,6 +201,16 @@ private:
     // keep the fastest for that step.
     void tune(OpCache& op);
     std::map<std::tuple<int, int, int, int>, int> tuned_;
+    // a merged operation (force_parts_multi): its own steps (descriptors in merged.bufs) and,
+    // per step, the operation whose field it binds (-1: none) and that operation
+    struct MultiOp {
+        OpCache merged;
+        std::vector<int> owner;
+        std::vector<OpCache*> ops;
+    };
+    std::map<std::vector<long>, std::unique_ptr<MultiOp>> multi_;
+    OpCache& get_force(int mode, int nt, int batch, int group, int nspecies, int spin);
+    std::unique_ptr<MultiOp> merge_ops(const std::vector<OpCache*>& ops);
 };
 
 }  // namespace isle

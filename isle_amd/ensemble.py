@@ -3,13 +3,17 @@
 The reference runs one chain per process (``src/isle/drivers/hmc.py``) with
 ``ConstStepLeapfrog`` (``src/isle/evolver/leapfrog.py:38-66``) and ``BinarySelector``
 (``src/isle/evolver/selector.py:20-30``).  Here chains are grouped into buckets of equal
-(Nx, Nt); every bucket is a homogeneous batch with its own plan (so its launch shapes are
-tuned per (Nx, Nt) by the plan's autotuner), all buckets run their leapfrog integration
-on separate streams, and one trajectory of the whole ensemble is captured once in a HIP
-graph (fork/join over the bucket streams) and replayed.  Momenta and the Metropolis
-decision of every chain come from that chain's own NumpyRNG exactly as a lone
-ConstStepLeapfrog would draw them, so each chain reproduces a single-chain run with the
-same seed up to the arithmetic of the batched kernels.
+(Nx, Nt) and parameters; every bucket is a homogeneous batch.  Buckets on one lattice (the
+same hopping matrix and chemical potential per time slice, so the same slice matrices) share
+one fermion action and one plan, and with ``merge=True`` (the default) they integrate
+together: every kick evaluates the fermion force of all of them in shared launches (each
+chain step one launch over the descriptors of all their chains), with each bucket's gauge
+force added in its own kick.  Each such group runs on its own stream, and one trajectory of
+the whole ensemble is captured once in a HIP graph (fork/join over the streams) and
+replayed.  Launch shapes come from the plans' autotuner per (Nx, Nt) and per merged step.
+Momenta and the Metropolis decision of every chain come from that chain's own NumpyRNG
+exactly as a lone ConstStepLeapfrog would draw them, so each chain reproduces a single-chain
+run with the same seed up to the arithmetic of the batched kernels.
 """
 from collections import OrderedDict
 from concurrent.futures import ThreadPoolExecutor
@@ -20,7 +24,7 @@ import torch
 from . import _lib
 from .action import HFAHopping, HubbardFermiAction, HubbardGaugeAction, SumAction
 from .evolver import BinarySelector
-from .integrator import eval_batch, leapfrog_batch
+from .integrator import eval_batch, leapfrog_batch, leapfrog_multi
 
 
 class Chain:
@@ -40,20 +44,28 @@ class Chain:
     def key(self):
         return (self.nx, self.nt, self.beta, self.U, self.mu, self.sigma_kappa, self.kappa.tobytes())
 
+    def lattice_key(self):
+        """Chains with equal keys have the same slice matrices (kappa beta/nt, mu beta/nt)."""
+        kt = self.kappa * self.beta / self.nt
+        return (kt.shape, kt.tobytes(), self.mu * self.beta / self.nt, self.sigma_kappa)
+
 
 class _Bucket:
-    def __init__(self, chains, hopping, precision, group):
+    def __init__(self, chains, fermion):
         c0 = chains[0]
         self.chains = chains
-        kt = c0.kappa * c0.beta / c0.nt
-        self.fermion = HubbardFermiAction(kt, c0.mu * c0.beta / c0.nt, c0.sigma_kappa, hopping,
-                                          precision=precision, group=group)
-        self.action = SumAction(HubbardGaugeAction(c0.utilde), self.fermion)
+        self.fermion = fermion
+        self.utilde = c0.utilde
+        self.action = SumAction(HubbardGaugeAction(c0.utilde), fermion)
         self.n = c0.nx * c0.nt
         dev = _lib.device()
         self.phi = torch.as_tensor(np.stack([c.phi0 for c in chains]), device=dev)
         self.acts = np.asarray(eval_batch(self.action, self.phi), dtype=complex)
         self.selectors = [BinarySelector(c.rng) for c in chains]
+
+    def label(self):
+        c0 = self.chains[0]
+        return f"Nx={c0.nx} Nt={c0.nt} x{len(self.chains)}"
 
 
 class EnsembleLeapfrog:
@@ -62,16 +74,38 @@ class EnsembleLeapfrog:
     ``chains``: list of Chain.  ``evolve()`` advances every chain by one trajectory and
     returns the accept (1) / reject (0) decision of each chain in input order; ``deltaH``
     and ``trajPoints`` keep the history.  With ``graph=True`` the integration of all
-    buckets is one replayed HIP graph."""
+    buckets is one replayed HIP graph; with ``merge=True`` the buckets of one lattice share
+    their force launches (``groups`` lists the buckets integrated together)."""
 
-    def __init__(self, chains, length, nstep, hopping=HFAHopping.EXP, precision="mixed", group=32, graph=True):
+    def __init__(self, chains, length, nstep, hopping=HFAHopping.EXP, precision="mixed", group=32, graph=True,
+                 merge=True):
         self.chains = list(chains)
         self.length, self.nstep = float(length), int(nstep)
-        groups = OrderedDict()
+        keyed = OrderedDict()
         for i, c in enumerate(self.chains):
-            groups.setdefault(c.key(), []).append(i)
-        self.index = list(groups.values())  # chain indices of each bucket
-        self.buckets = [_Bucket([self.chains[i] for i in idx], hopping, precision, group) for idx in self.index]
+            keyed.setdefault(c.key(), []).append(i)
+        self.index = list(keyed.values())  # chain indices of each bucket
+        fermions = {}
+        self.buckets = []
+        self.groups = []  # bucket indices integrated together (one stream each)
+        group_of = {}
+        for idx in self.index:
+            c0 = self.chains[idx[0]]
+            lk = c0.lattice_key() if merge else len(self.buckets)
+            if lk not in fermions:
+                fermions[lk] = HubbardFermiAction(c0.kappa * c0.beta / c0.nt, c0.mu * c0.beta / c0.nt,
+                                                  c0.sigma_kappa, hopping, precision=precision, group=group)
+            bi = len(self.buckets)
+            self.buckets.append(_Bucket([self.chains[i] for i in idx], fermions[lk]))
+            # a group holds buckets of one lattice with distinct time extents
+            for gi in group_of.get(lk, []):
+                if all(self.buckets[b].chains[0].nt != c0.nt for b in self.groups[gi]):
+                    self.groups[gi].append(bi)
+                    break
+            else:
+                group_of.setdefault(lk, []).append(len(self.groups))
+                self.groups.append([bi])
+        self.merge = bool(merge)
         self.graph = bool(graph)
         self._graph = None
         self._pool = ThreadPoolExecutor(max_workers=16)
@@ -86,35 +120,47 @@ class EnsembleLeapfrog:
                 out[i] = host[j]
         return out
 
+    def _integrate(self, gi, phis, pis):
+        """One trajectory's integration of group gi from the given fields and momenta (one
+        per bucket of the group); returns [(phi', pi')] in the group's bucket order."""
+        members = [self.buckets[b] for b in self.groups[gi]]
+        if not self.merge:
+            b = members[0]
+            p1, q1, _ = leapfrog_batch(phis[0], pis[0], b.action, self.length, self.nstep, evaluate=False)
+            return [(p1, q1)]
+        p1, q1 = leapfrog_multi(phis, pis, members[0].fermion, [b.utilde for b in members], self.length, self.nstep)
+        return list(zip(p1, q1))
+
     def _capture(self):
         main = torch.cuda.current_stream()
-        self._streams = [torch.cuda.Stream() for _ in self.buckets]
+        self._streams = [torch.cuda.Stream() for _ in self.groups]
         self._in = [(torch.zeros_like(b.phi), torch.zeros_like(b.phi)) for b in self.buckets]
-        # warm-up on the bucket streams: builds and tunes every plan outside the capture
-        for s, b, (p, q) in zip(self._streams, self.buckets, self._in):
+        ins = [([self._in[b][0] for b in g], [self._in[b][1] for b in g]) for g in self.groups]
+        # warm-up on the group streams: builds and tunes every plan outside the capture
+        for gi, s in enumerate(self._streams):
             s.wait_stream(main)
             with torch.cuda.stream(s):
-                leapfrog_batch(p, q, b.action, self.length, self.nstep, evaluate=False)
+                self._integrate(gi, *ins[gi])
             main.wait_stream(s)
         torch.cuda.synchronize()
         self._graph = torch.cuda.CUDAGraph()
-        self._out = []
+        self._out = [None] * len(self.buckets)
         with torch.cuda.graph(self._graph):
             cap = torch.cuda.current_stream()
-            for s, b, (p, q) in zip(self._streams, self.buckets, self._in):
+            for gi, s in enumerate(self._streams):
                 s.wait_stream(cap)
                 with torch.cuda.stream(s):
-                    p1, q1, _ = leapfrog_batch(p, q, b.action, self.length, self.nstep, evaluate=False)
-                self._out.append((p1, q1))
+                    for b, res in zip(self.groups[gi], self._integrate(gi, *ins[gi])):
+                        self._out[b] = res
             for s in self._streams:
                 cap.wait_stream(s)
 
-    def time_buckets(self, reps=3):
+    def time_groups(self, reps=3):
         """Milliseconds of one trajectory's integration (the replayed work, no host part):
-        ``(whole, alone)`` with ``whole`` the ensemble graph and ``alone[i]`` bucket i's
-        integration captured in a graph of its own and replayed with nothing beside it, on
-        the inputs of the last trajectory.  The longest ``alone`` is the ensemble's critical
-        path; ``sum(alone) / whole`` is how much the buckets' streams overlap."""
+        ``(whole, alone)`` with ``whole`` the ensemble graph and ``alone`` a list of
+        (label, ms): each group's integration captured in a graph of its own and replayed
+        with nothing beside it, on the inputs of the last trajectory.  The longest is the
+        ensemble's critical path; ``sum(alone) / whole`` is how much the streams overlap."""
         if self._graph is None:
             self._capture()
 
@@ -130,11 +176,12 @@ class EnsembleLeapfrog:
             return e0.elapsed_time(e1) / reps
         whole = timed(self._graph.replay)
         alone = []
-        for b, (p, q) in zip(self.buckets, self._in):
+        for gi, members in enumerate(self.groups):
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                leapfrog_batch(p, q, b.action, self.length, self.nstep, evaluate=False)
-            alone.append(timed(g.replay))
+                self._integrate(gi, [self._in[b][0] for b in members], [self._in[b][1] for b in members])
+            label = ", ".join(self.buckets[b].label() for b in members)
+            alone.append((label, timed(g.replay)))
             del g
         return whole, alone
 
@@ -162,12 +209,15 @@ class EnsembleLeapfrog:
             self._graph.replay()
             results = [(p1.clone(), q1.clone()) for p1, q1 in self._out]
         else:
-            results = []
-            for b, ph in zip(self.buckets, pis_host):
-                q = torch.zeros_like(b.phi)
-                q.real.copy_(torch.from_numpy(ph))
-                p1, q1, _ = leapfrog_batch(b.phi, q, b.action, self.length, self.nstep, evaluate=False)
-                results.append((p1, q1))
+            results = [None] * len(self.buckets)
+            for gi, members in enumerate(self.groups):
+                qs = []
+                for b in members:
+                    q = torch.zeros_like(self.buckets[b].phi)
+                    q.real.copy_(torch.from_numpy(pis_host[b]))
+                    qs.append(q)
+                for b, res in zip(members, self._integrate(gi, [self.buckets[b].phi for b in members], qs)):
+                    results[b] = res
         points = [None] * len(self.chains)
         dhs = [None] * len(self.chains)
         for idx, b, (p1, q1), ph in zip(self.index, self.buckets, results, pis_host):

@@ -78,6 +78,37 @@ def leapfrog_batch(phis, pis, action, length, nsteps, direction=1.0, evaluate=Tr
     return phi, pi, acts
 
 
+def leapfrog_multi(phis, pis, fermion, utildes, length, nsteps, direction=1.0):
+    """Leapfrog for several batches of chains on one lattice with different time extents:
+    lists of [batch_i, nx*nt_i] fields and momenta, one HubbardFermiAction for all of them,
+    the gauge action's utilde of each batch.  Every kick evaluates the fermion force of all
+    batches in shared launches (HubbardFermiAction.force_batch_multi) and adds each batch's
+    gauge force in its kick kernel.  Returns the lists (phis', pis')."""
+    if nsteps < 1:
+        raise ValueError("nsteps must be at least 1")
+    phi = [as_field(p).clone().reshape(-1, p.shape[-1]) for p in phis]
+    pi = [as_field(p).clone().reshape(-1, p.shape[-1]) for p in pis]
+    eps = direction * length / float(nsteps)
+    fbufs = [torch.empty_like(p) for p in phi]
+
+    def kick(scale):
+        fermion.force_batch_multi(phi, fbufs)
+        for p, q, f, ut in zip(phi, pi, fbufs, utildes):
+            _kick(q, f, p, eps * scale, ut)
+
+    def drift():
+        for p, q in zip(phi, pi):
+            _drift(p, q, eps)
+
+    kick(0.5)
+    drift()
+    for _ in range(nsteps - 1):
+        kick(1.0)
+        drift()
+    kick(0.5)
+    return phi, pi
+
+
 def eval_batch(action, phi):
     """Action of each field of a [batch, nx*nt] stack as a numpy complex array; gauge +
     fermion sums are evaluated with the batched kernels."""

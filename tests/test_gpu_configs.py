@@ -3,8 +3,9 @@
 Config 3 (honeycomb 12x12 torus, Nx=288, Nt=128, beta=8, U=3, mu=0, EXP, particle-hole
 basis, sigmaKappa=-1; fields N(0, U beta/Nt) from RandomState(1000..1003)):
 * log det M of both species within 1e-10 (relative; imaginary parts modulo 2 pi) of
-  oracle.logdetM, for the plain FP64 form (the reference's product) and for the grouped
-  FP64 form the mixed action evaluates for Metropolis;
+  oracle.logdetM, for the plain FP64 form (the reference's product) and for both FP64 forms
+  the mixed action can evaluate for Metropolis: the grouped products multiplied together,
+  and products of 8 slices combined through the stabilising block-cyclic LU;
 * the mixed-precision force (FP32 right chains on MFMA) within 1e-5 of the oracle's force
   (relative L2 norm and max-norm ratio), the FP64 force within 1e-6.  The oracle's own
   force is the reference's unstabilised FP64 algorithm, 7e-8 from the stabilised
@@ -54,11 +55,11 @@ def test_config3_logdet_and_force(seed):
     hfm = ref.HFMExp(kt, 0.0, -1)
     want = [ref.logdetM(hfm, phi, s) for s in (0, 1)]
     dev = torch.as_tensor(phi, device="cuda")
-    for precision in ("fp64", "mixed"):
-        a = act.HubbardFermiAction(kt, 0.0, -1, act.HFAHopping.EXP, precision=precision)
+    for precision, form in (("fp64", "grouped"), ("mixed", "grouped"), ("mixed", "stable")):
+        a = act.HubbardFermiAction(kt, 0.0, -1, act.HFAHopping.EXP, precision=precision, logdet=form)
         got = a.logdets(dev)[0]
         for s in (0, 1):
-            assert mod2pi_diff(got[s], want[s]) < 1e-10 * abs(want[s]), (precision, s, got[s], want[s])
+            assert mod2pi_diff(got[s], want[s]) < 1e-10 * abs(want[s]), (precision, form, s, got[s], want[s])
     f_want = ref.HubbardFermiAction(kt, 0.0, -1, ref.EXP).force(phi)
     f_mixed = act.HubbardFermiAction(kt, 0.0, -1, act.HFAHopping.EXP, precision="mixed").force(dev).cpu().numpy()
     f_plain = act.HubbardFermiAction(kt, 0.0, -1, act.HFAHopping.EXP).force(dev).cpu().numpy()

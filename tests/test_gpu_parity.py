@@ -437,11 +437,13 @@ def test_graph_split_over_streams(precision, tol):
         assert rel_l2(g_pi.cpu().numpy(), e_pi.cpu().numpy()) < tol
 
 
+@pytest.mark.parametrize("merge", [True, False])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("mixed", 5e-5)])
-def test_ensemble_uneven_chains_match_single_runs(precision, tol):
-    """Chains of uneven (Nx, Nt) advanced together (buckets on separate streams, one HIP
-    graph per trajectory) reproduce each chain run alone with the same seed: identical
-    accept/reject decisions and Delta H within `tol` (absolute + relative) over 3
+def test_ensemble_uneven_chains_match_single_runs(precision, tol, merge):
+    """Chains of uneven (Nx, Nt) advanced together (one HIP graph per trajectory; with merge
+    the two time extents of each lattice share their force launches on one stream, without
+    it every bucket has its own stream) reproduce each chain run alone with the same seed:
+    identical accept/reject decisions and Delta H within `tol` (absolute + relative) over 3
     trajectories; small sizes of the config-4 family (L in {3, 4}, Nt in {16, 32},
     beta = Nt/8, U = 2).  Batches of different size may run different tile shapes, so
     the mixed form's FP32 rounding differs between the runs (Delta H to ~1e-5)."""
@@ -450,7 +452,9 @@ def test_ensemble_uneven_chains_match_single_runs(precision, tol):
     specs = [(3, 16), (4, 32), (3, 32), (4, 16), (3, 16), (4, 32)]
     def chains():
         return [Chain(ref.honeycomb(l), nt, nt / 8.0, 2.0, 0.0, -1, NumpyRNG(900 + i)) for i, (l, nt) in enumerate(specs)]
-    ens = EnsembleLeapfrog(chains(), 1.0, 8, precision=precision, group=32, graph=True)
+    print(f"ensemble vs single chains: {precision}, merge={merge}", flush=True)
+    ens = EnsembleLeapfrog(chains(), 1.0, 8, precision=precision, group=32, graph=True, merge=merge)
+    assert len(ens.groups) == (2 if merge else 4)
     for _ in range(3):
         ens.evolve()
     for i, c in enumerate(chains()):
