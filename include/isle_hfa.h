@@ -77,7 +77,7 @@ int isle_logdetM(isle_hfm* h, const double* phi, int nt, int species, double* ou
  * mode 0: reference algorithm in FP64; mode 1: FP64 products of groups of `group` slices
  * formed in parallel first; mode 2: those group products combined by the pivoted
  * elimination of their block-cyclic matrix (stable where the product of all slices is not
- * representable in FP64; needs nx % 32 == 0 and nx <= 480 when nt/group > 1).
+ * representable in FP64; needs nx % 32 == 0 when nt/group > 1).
  * nspecies 1 (particle) or 2 (particle, hole); spin != 0 evaluates at aux = -i*phi (spin
  * basis).  out_host: [batch][nspecies][2]. */
 int isle_logdetM_batch(isle_hfm* h, const double* phi, int nt, int batch, int mode, int group, int nspecies,

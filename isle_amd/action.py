@@ -163,7 +163,7 @@ class HubbardFermiAction(Action):
     DIRECT_SQUARE (hubbardFermiAction.cpp:62-108, 230-272) is the reference's small-size
     cross-check: eval = -logdetQ through the block LU of Q = M_p M_h^T (any size), and the
     force from the blocks of the dense (NxNt)^2 Q^-1 next to the off-diagonal blocks T+ and
-    T- of Q, inverted by the HIP LU, so the force needs Nx*Nt <= 512.  FP64 only; no
+    T- of Q, inverted by the HIP LU, so the force needs Nx*Nt <= 16383.  FP64 only; no
     per-species parts (logdets, force_parts).
     """
 
@@ -222,7 +222,7 @@ class HubbardFermiAction(Action):
             self.shortcut = False  # Q holds both species
 
     # ---- DIRECT_SQUARE (hubbardFermiAction.cpp:62-108): Q = M_p M_h^T
-    SQUARE_MAX = 512  # order of the dense LU of the force's Q^-1
+    SQUARE_MAX = 16383  # order of the dense LU of the force's Q^-1 (the HIP LU's limit)
 
     def _field(self, phi):
         return -1j * phi if self.basis == HFABasis.SPIN else phi

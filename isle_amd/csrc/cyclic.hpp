@@ -60,10 +60,15 @@ struct SigmaItem {
 };
 
 __global__ __launch_bounds__(256) void sigma_kernel(const SigmaItem* __restrict__ items) {
-    __shared__ int pv[1024];
+    constexpr int PV_LDS = 1024;  // pivot lists up to this length are staged in LDS
+    __shared__ int pv_lds[PV_LDS];
     const SigmaItem it = items[blockIdx.x];
-    for (int i = threadIdx.x; i < it.npiv; i += 256) pv[i] = it.piv[i];
-    __syncthreads();
+    const bool staged = it.npiv <= PV_LDS;  // uniform over the workgroup
+    if (staged) {
+        for (int i = threadIdx.x; i < it.npiv; i += 256) pv_lds[i] = it.piv[i];
+        __syncthreads();
+    }
+    const int* pv = staged ? pv_lds : it.piv;
     for (int r = threadIdx.x; r < it.m; r += 256) {
         int pos = r;
         for (int k = 0; k < it.npiv; ++k) {

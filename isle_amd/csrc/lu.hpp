@@ -131,6 +131,97 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k)
     return ((unsigned long long)hi << 32) | lo;
 }
 
+// L11^{-1} and U11^{-1} of a factorised panel by recursive doubling (lt: L11 unit lower,
+// ut: U11 with its diagonal already inverted, both [NB][NB + 1] in LDS, tl / tu temporaries),
+// then written to the panel's slots of the work buffer.  Ends with a barrier-free copy; the
+// caller's threads must all take part (block-wide loops and barriers inside).
+//   [A 0; C D]^{-1} = [A^-1 0; -D^-1 C A^-1  D^-1],  [A B; 0 D]^{-1} = [A^-1 -A^-1 B D^-1; 0 D^-1]
+template <int NB>
+__device__ void lu_invert_panel_blocks(const LuItem& it, int panel, double (*lt_re)[NB + 1], double (*lt_im)[NB + 1],
+                                       double (*ut_re)[NB + 1], double (*ut_im)[NB + 1], double (*tl_re)[NB + 1],
+                                       double (*tl_im)[NB + 1], double (*tu_re)[NB + 1], double (*tu_im)[NB + 1]) {
+    const int tid = threadIdx.x, nth = blockDim.x;
+    for (int b = 1; b < NB; b *= 2) {
+        const int work = (NB / (2 * b)) * b * b;
+        for (int e = tid; e < work; e += nth) {
+            const int blk = e / (b * b), i = (e / b) % b, j = e % b, o = blk * 2 * b;
+            double tr = 0, ti = 0, sr = 0, si = 0;
+            for (int q = j; q < b; ++q) {  // C A^-1
+                const double cr = lt_re[o + b + i][o + q], ci = lt_im[o + b + i][o + q];
+                const double xr = lt_re[o + q][o + j], xi = lt_im[o + q][o + j];
+                tr += cr * xr - ci * xi;
+                ti += cr * xi + ci * xr;
+            }
+            for (int q = 0; q <= j; ++q) {  // B D^-1
+                const double br = ut_re[o + i][o + b + q], bi = ut_im[o + i][o + b + q];
+                const double xr = ut_re[o + b + q][o + b + j], xi = ut_im[o + b + q][o + b + j];
+                sr += br * xr - bi * xi;
+                si += br * xi + bi * xr;
+            }
+            tl_re[o + b + i][o + j] = tr;
+            tl_im[o + b + i][o + j] = ti;
+            tu_re[o + i][o + b + j] = sr;
+            tu_im[o + i][o + b + j] = si;
+        }
+        __syncthreads();
+        for (int e = tid; e < work; e += nth) {
+            const int blk = e / (b * b), i = (e / b) % b, j = e % b, o = blk * 2 * b;
+            double xr_ = 0, xi_ = 0, yr = 0, yi = 0;
+            for (int q = 0; q <= i; ++q) {  // D^-1 (C A^-1)
+                const double dr = lt_re[o + b + i][o + b + q], di = lt_im[o + b + i][o + b + q];
+                const double tr = tl_re[o + b + q][o + j], ti = tl_im[o + b + q][o + j];
+                xr_ += dr * tr - di * ti;
+                xi_ += dr * ti + di * tr;
+            }
+            for (int q = i; q < b; ++q) {  // A^-1 (B D^-1)
+                const double ar_ = ut_re[o + i][o + q], ai_ = ut_im[o + i][o + q];
+                const double tr = tu_re[o + q][o + b + j], ti = tu_im[o + q][o + b + j];
+                yr += ar_ * tr - ai_ * ti;
+                yi += ar_ * ti + ai_ * tr;
+            }
+            lt_re[o + b + i][o + j] = -xr_;
+            lt_im[o + b + i][o + j] = -xi_;
+            ut_re[o + i][o + b + j] = -yr;
+            ut_im[o + i][o + b + j] = -yi;
+        }
+        __syncthreads();
+    }
+    double* li_re = lu_linv(it, panel);
+    double* li_im = li_re + NB * NB;
+    double* ui_re = lu_uinv(it, panel);
+    double* ui_im = ui_re + NB * NB;
+    for (int e = tid; e < NB * NB; e += nth) {
+        li_re[e] = lt_re[e / NB][e % NB];
+        li_im[e] = lt_im[e / NB][e % NB];
+        ui_re[e] = ut_re[e / NB][e % NB];
+        ui_im[e] = ut_im[e / NB][e % NB];
+    }
+}
+
+// On the item's last panel (k0 + NB >= kf), where every U_ii and pivot of the kf columns is
+// final: log det = sum log U_ii + i pi (number of interchanges), into it.logdet.  rsum: >= 16
+// doubles of LDS.
+__device__ inline void lu_panel_logdet(const LuItem& it, int k0, int nb, int kf, double* rsum) {
+    if (!it.logdet || k0 + nb < kf) return;
+    __syncthreads();
+    const int tid = threadIdx.x, nth = blockDim.x, ld = it.ld;
+    double lr_ = 0, la = 0, pc = 0;
+    for (int i = tid; i < kf; i += nth) {
+        const double dr = it.re[(long)i * ld + i], di = it.im[(long)i * ld + i];
+        lr_ += log(hypot(dr, di));
+        la += atan2(di, dr);
+        pc += (it.piv[i] != i) ? 1.0 : 0.0;
+    }
+    lr_ = block_sum(lr_, rsum);
+    la = block_sum(la, rsum);
+    pc = block_sum(pc, rsum);
+    if (tid == 0) {
+        if (((long)pc) & 1) la += M_PI;
+        it.logdet[0] = lr_;
+        it.logdet[1] = remainder(la, 2.0 * M_PI);
+    }
+}
+
 template <int NB>
 __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __restrict__ items, int k0,
                                                             int add_identity) {
@@ -299,85 +390,167 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
     }
     if (tid == 0) perm[0] = cnt;
 
-    // L11^{-1} and U11^{-1} by recursive doubling:
-    //   [A 0; C D]^{-1} = [A^-1 0; -D^-1 C A^-1  D^-1],  [A B; 0 D]^{-1} = [A^-1 -A^-1 B D^-1; 0 D^-1]
-    for (int b = 1; b < NB; b *= 2) {
-        const int work = (NB / (2 * b)) * b * b;
-        for (int e = tid; e < work; e += nth) {
-            const int blk = e / (b * b), i = (e / b) % b, j = e % b, o = blk * 2 * b;
-            double tr = 0, ti = 0, sr = 0, si = 0;
-            for (int q = j; q < b; ++q) {  // C A^-1
-                const double cr = lt_re[o + b + i][o + q], ci = lt_im[o + b + i][o + q];
-                const double xr = lt_re[o + q][o + j], xi = lt_im[o + q][o + j];
-                tr += cr * xr - ci * xi;
-                ti += cr * xi + ci * xr;
-            }
-            for (int q = 0; q <= j; ++q) {  // B D^-1
-                const double br = ut_re[o + i][o + b + q], bi = ut_im[o + i][o + b + q];
-                const double xr = ut_re[o + b + q][o + b + j], xi = ut_im[o + b + q][o + b + j];
-                sr += br * xr - bi * xi;
-                si += br * xi + bi * xr;
-            }
-            tl_re[o + b + i][o + j] = tr;
-            tl_im[o + b + i][o + j] = ti;
-            tu_re[o + i][o + b + j] = sr;
-            tu_im[o + i][o + b + j] = si;
-        }
-        __syncthreads();
-        for (int e = tid; e < work; e += nth) {
-            const int blk = e / (b * b), i = (e / b) % b, j = e % b, o = blk * 2 * b;
-            double xr_ = 0, xi_ = 0, yr = 0, yi = 0;
-            for (int q = 0; q <= i; ++q) {  // D^-1 (C A^-1)
-                const double dr = lt_re[o + b + i][o + b + q], di = lt_im[o + b + i][o + b + q];
-                const double tr = tl_re[o + b + q][o + j], ti = tl_im[o + b + q][o + j];
-                xr_ += dr * tr - di * ti;
-                xi_ += dr * ti + di * tr;
-            }
-            for (int q = i; q < b; ++q) {  // A^-1 (B D^-1)
-                const double ar_ = ut_re[o + i][o + q], ai_ = ut_im[o + i][o + q];
-                const double tr = tu_re[o + q][o + b + j], ti = tu_im[o + q][o + b + j];
-                yr += ar_ * tr - ai_ * ti;
-                yi += ar_ * ti + ai_ * tr;
-            }
-            lt_re[o + b + i][o + j] = -xr_;
-            lt_im[o + b + i][o + j] = -xi_;
-            ut_re[o + i][o + b + j] = -yr;
-            ut_im[o + i][o + b + j] = -yi;
-        }
-        __syncthreads();
-    }
-    {
-        double* li_re = lu_linv(it, panel);
-        double* li_im = li_re + NB * NB;
-        double* ui_re = lu_uinv(it, panel);
-        double* ui_im = ui_re + NB * NB;
-        for (int e = tid; e < NB * NB; e += nth) {
-            li_re[e] = lt_re[e / NB][e % NB];
-            li_im[e] = lt_im[e / NB][e % NB];
-            ui_re[e] = ut_re[e / NB][e % NB];
-            ui_im[e] = ut_im[e / NB][e % NB];
-        }
-    }
+    lu_invert_panel_blocks<NB>(it, panel, lt_re, lt_im, ut_re, ut_im, tl_re, tl_im, tu_re, tu_im);
+    lu_panel_logdet(it, k0, NB, kf, rsum);
+}
 
-    if (it.logdet && k0 + NB >= kf) {
-        // the item's last panel: every U_ii and pivot of the kf columns is final
-        __syncthreads();
-        double lr_ = 0, la = 0, pc = 0;
-        for (int i = tid; i < kf; i += nth) {
-            const double dr = re[(long)i * ld + i], di = im[(long)i * ld + i];
-            lr_ += log(hypot(dr, di));
-            la += atan2(di, dr);
-            pc += (it.piv[i] != i) ? 1.0 : 0.0;
+// Panels with more candidate rows than one workgroup holds in registers (lu_rows > LU_MAX_N,
+// up to LU_TALL_MAX): the same factorisation with the panel in global memory (in place; an
+// n x NB panel is a few MB, resident in L2), each thread taking every blockDim-th candidate
+// row.  Per column: pivot search (keys as in pivot_key, 14 position bits), the pivot row
+// swapped with row kk across the panel's columns and copied to LDS, then the multipliers
+// and the rank-1 update of the rows below.  The interchange record, L11^{-1} / U11^{-1} and
+// the log-determinant are those of lu_panel_kernel.
+constexpr int LU_TALL_MAX = 16383;
+constexpr int LU_TALL_THREADS = 1024;
+
+__device__ __forceinline__ unsigned long long pivot_key_tall(double v, int pos) {
+    return (static_cast<unsigned long long>(__double_as_longlong(v)) & ~0x3FFFull) |
+           static_cast<unsigned long long>(0x3FFF - pos);
+}
+
+template <int NB>
+__global__ __launch_bounds__(LU_TALL_THREADS) void lu_panel_tall_kernel(const LuItem* __restrict__ items, int k0,
+                                                                        int add_identity) {
+    __shared__ unsigned long long wkey[LU_TALL_THREADS / 64];
+    __shared__ int pvt[NB];
+    __shared__ double xrow[2 * NB];  // the pivot row (re, im interleaved)
+    __shared__ double lt_re[NB][NB + 1], lt_im[NB][NB + 1];
+    __shared__ double ut_re[NB][NB + 1], ut_im[NB][NB + 1];
+    __shared__ double tl_re[NB][NB + 1], tl_im[NB][NB + 1];
+    __shared__ double tu_re[NB][NB + 1], tu_im[NB][NB + 1];
+    __shared__ double rsum[LU_TALL_THREADS / 64];
+
+    const LuItem it = items[blockIdx.x];
+    const int n = it.n, m = lu_m(it), kf = lu_kf(it), ntop = it.nt;
+    if (k0 >= kf) return;
+    const int ld = it.ld, tid = threadIdx.x, nth = blockDim.x;
+    const int rows = lu_rows(k0, m, ntop), kb = min(NB, kf - k0);
+    const int panel = k0 / NB;
+    double* re = it.re;
+    double* im = it.im;
+    if (add_identity)
+        for (int i = tid; i < n; i += nth) re[(long)i * ld + i] += 1.0;
+    __syncthreads();
+
+    for (int kk = 0; kk < kb; ++kk) {
+        unsigned long long key = 0ull;
+        for (int p = kk + tid; p < rows; p += nth) {
+            const long o = (long)lu_row(k0, ntop, p) * ld + k0 + kk;
+            const unsigned long long k2 = pivot_key_tall(cabs1(re[o], im[o]), p);
+            key = k2 > key ? k2 : key;
         }
-        lr_ = block_sum(lr_, rsum);
-        la = block_sum(la, rsum);
-        pc = block_sum(pc, rsum);
-        if (tid == 0) {
-            if (((long)pc) & 1) la += M_PI;
-            it.logdet[0] = lr_;
-            it.logdet[1] = remainder(la, 2.0 * M_PI);
+        key = wave_max_u64(key);
+        if ((tid & 63) == 0) wkey[tid >> 6] = key;
+        __syncthreads();
+        unsigned long long kmax = 0ull;
+        for (int w = 0; w < (nth >> 6); ++w) kmax = wkey[w] > kmax ? wkey[w] : kmax;
+        // an exactly zero column keeps row kk in place
+        const int p = kmax ? 0x3FFF - static_cast<int>(kmax & 0x3FFFull) : kk;
+        if (tid < kb) {  // exchange positions kk and p across the panel's columns
+            const long okk = (long)lu_row(k0, ntop, kk) * ld + k0 + tid;
+            const long op = (long)lu_row(k0, ntop, p) * ld + k0 + tid;
+            const double ar = re[okk], ai = im[okk];
+            const double br = re[op], bi = im[op];
+            if (p != kk) {
+                re[okk] = br;
+                im[okk] = bi;
+                re[op] = ar;
+                im[op] = ai;
+            }
+            xrow[2 * tid] = br;
+            xrow[2 * tid + 1] = bi;
+        }
+        if (tid == 0) pvt[kk] = p;
+        __syncthreads();
+        const double ur = xrow[2 * kk], ui = xrow[2 * kk + 1];
+        const bool nz = ur != 0.0 || ui != 0.0;
+        double ir = 0.0, ii = 0.0;
+        if (nz) {
+            const double rden = 1.0 / (ur * ur + ui * ui);
+            ir = ur * rden;
+            ii = -ui * rden;
+        }
+        for (int q = kk + 1 + tid; q < rows; q += nth) {
+            const long ob = (long)lu_row(k0, ntop, q) * ld + k0;
+            double lr = re[ob + kk], li = im[ob + kk];
+            if (nz) {
+                const double t = lr * ir - li * ii;
+                li = lr * ii + li * ir;
+                lr = t;
+                re[ob + kk] = lr;
+                im[ob + kk] = li;
+            }
+            for (int j = kk + 1; j < kb; ++j) {
+                const double br = xrow[2 * j], bi = xrow[2 * j + 1];
+                re[ob + j] -= lr * br - li * bi;
+                im[ob + j] -= lr * bi + li * br;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < kb) it.piv[k0 + tid] = lu_row(k0, ntop, pvt[tid]);
+    if (tid == 0) {
+        // composed interchange of the panel: (final position, original position) of every
+        // row that moved, replayed from the pivot sequence on at most 2 NB positions
+        int pos[2 * NB], org[2 * NB], cnt = 0;
+        for (int kk = 0; kk < kb; ++kk) {
+            const int p = pvt[kk];
+            if (p == kk) continue;
+            int a = -1, b = -1;
+            for (int e = 0; e < cnt; ++e) {
+                if (pos[e] == kk) a = e;
+                if (pos[e] == p) b = e;
+            }
+            if (a < 0) {
+                pos[cnt] = kk;
+                org[cnt] = kk;
+                a = cnt++;
+            }
+            if (b < 0) {
+                pos[cnt] = p;
+                org[cnt] = p;
+                b = cnt++;
+            }
+            const int t = org[a];
+            org[a] = org[b];
+            org[b] = t;
+        }
+        int* perm = lu_perm(it);
+        int w = 0;
+        for (int e = 0; e < cnt; ++e)
+            if (org[e] != pos[e]) {
+                perm[1 + w] = pos[e];
+                perm[1 + 2 * NB + w] = org[e];
+                ++w;
+            }
+        perm[0] = w;
+    }
+    // L11 and U11 (diagonal inverted) from the rows now at positions 0..NB-1; positions
+    // kb..NB-1 of a partial last panel are padded with the identity
+    if (tid < NB) {
+        const int r = tid;
+        const bool in = r < kb;
+        const long ob = (long)lu_row(k0, ntop, r) * ld + k0;
+#pragma unroll 4
+        for (int c = 0; c < NB; ++c) {
+            const bool cin = in && c < kb;
+            const double ar = cin ? re[ob + c] : 0.0, ai = cin ? im[ob + c] : 0.0;
+            lt_re[r][c] = (cin && c < r) ? ar : (c == r ? 1.0 : 0.0);
+            lt_im[r][c] = (cin && c < r) ? ai : 0.0;
+            double vr = (cin && c >= r) ? ar : (c == r ? 1.0 : 0.0), vi = (cin && c >= r) ? ai : 0.0;
+            if (c == r) {  // the diagonal of U11^{-1} is 1/u_ii
+                const double den = vr * vr + vi * vi;
+                vr = vr / den;
+                vi = -vi / den;
+            }
+            ut_re[r][c] = vr;
+            ut_im[r][c] = vi;
         }
     }
+    __syncthreads();
+    lu_invert_panel_blocks<NB>(it, panel, lt_re, lt_im, ut_re, ut_im, tl_re, tl_im, tu_re, tu_im);
+    lu_panel_logdet(it, k0, NB, kf, rsum);
 }
 
 // The panel's interchanges on the columns right of it, on every CU: the new rows
@@ -589,19 +762,25 @@ __global__ __launch_bounds__(LU_UPD_THREADS) void lu_update_kernel(const LuItem*
 }
 
 // Factorise the first kf columns of nitems m x ncols matrices of one shape (identity rows
-// above nt as described at the top; nt = 0 for a general matrix).  Candidate rows of every
-// panel must fit one workgroup: lu_rows(k0, m, nt) <= LU_MAX_N.
+// above nt as described at the top; nt = 0 for a general matrix).  Panels with at most
+// LU_MAX_N candidate rows run in registers (lu_panel_kernel), larger ones up to LU_TALL_MAX
+// in global memory (lu_panel_tall_kernel).
 inline void launch_lu_shape(const LuItem* d_items, int nitems, int m, int ncols, int kf, int nt, int add_identity,
                             hipStream_t s) {
     if (kf > ncols || kf > m || (nt > 0 && (kf % LU_NB != 0 || nt > m)))
         throw std::invalid_argument("inconsistent LU shape");
     for (int k0 = 0; k0 < kf; k0 += LU_NB)
-        if (lu_rows(k0, m, nt) > LU_MAX_N) throw std::invalid_argument("matrix too large for the LU kernel");
+        if (lu_rows(k0, m, nt) > LU_TALL_MAX) throw std::invalid_argument("matrix too large for the LU kernel");
     for (int k0 = 0; k0 < kf; k0 += LU_NB) {
         const int rows = lu_rows(k0, m, nt);
-        const int threads = ((rows + 63) / 64) * 64;
-        hipLaunchKernelGGL(lu_panel_kernel<LU_NB>, dim3(nitems), dim3(threads), 0, s, d_items, k0,
-                           k0 == 0 ? add_identity : 0);
+        if (rows > LU_MAX_N) {  // the panel in global memory
+            hipLaunchKernelGGL(lu_panel_tall_kernel<LU_NB>, dim3(nitems), dim3(LU_TALL_THREADS), 0, s, d_items, k0,
+                               k0 == 0 ? add_identity : 0);
+        } else {
+            const int threads = ((rows + 63) / 64) * 64;
+            hipLaunchKernelGGL(lu_panel_kernel<LU_NB>, dim3(nitems), dim3(threads), 0, s, d_items, k0,
+                               k0 == 0 ? add_identity : 0);
+        }
         ISLE_HIP(hipGetLastError());
         const int ntr = rows - LU_NB, ntc = ncols - k0 - LU_NB;
         if (ntc > 0) {
@@ -655,11 +834,14 @@ inline size_t lu_solve_lds(int n) {
     return 2 * np * SOLVE_COLS * sizeof(double) + 2 * (size_t)n * sizeof(int);
 }
 
-// Blocked forward/back substitution on 16 right-hand sides per workgroup (X in LDS).
+// Blocked forward/back substitution on 16 right-hand sides per workgroup.
 // Per block of 16 rows: the product with the already solved rows runs on FP64 MFMA (the
 // K range split over the 4 waves, partial sums reduced through LDS), then the block is
 // multiplied by the inverse of its 16x16 diagonal block of L (or U), which is a diagonal
-// sub-block of the panel inverses the LU stored.
+// sub-block of the panel inverses the LU stored.  GX = false: X staged in LDS (orders up to
+// LU_MAX_N), B and X may be the same array.  GX = true: X kept in the output array itself
+// (larger orders; only the row permutation in LDS), so B must be null or another array.
+template <bool GX>
 __global__ __launch_bounds__(SOLVE_THREADS) void lu_solve_kernel(const SolveItem* __restrict__ items) {
     extern __shared__ double lds[];
     __shared__ double part_r[4][16][17], part_i[4][16][17];
@@ -670,10 +852,37 @@ __global__ __launch_bounds__(SOLVE_THREADS) void lu_solve_kernel(const SolveItem
     const int c0 = blockIdx.x * SOLVE_COLS;
     if (c0 >= it.nrhs) return;
     const int nb = (n + 15) / 16, np = nb * 16;
-    double* xr = lds;  // [np][16]
-    double* xi = lds + (size_t)np * SOLVE_COLS;
-    int* sigma = reinterpret_cast<int*>(lds + 2 * (size_t)np * SOLVE_COLS);  // [n]
-    int* pv = sigma + n;                                                      // [n]
+    double* xr = lds;  // LDS form: [np][16]
+    double* xi = lds + (GX ? 0 : (size_t)np * SOLVE_COLS);
+    int* sigma = reinterpret_cast<int*>(lds + (GX ? 0 : 2 * (size_t)np * SOLVE_COLS));  // [n]
+    int* pv = sigma + n;                                                                 // [n]
+    // element (row, column c0 + c) of X
+    auto xget = [&](int row, int c, double& vr, double& vi) {
+        if constexpr (GX) {
+            const int gc = c0 + c;
+            if (row < n && gc < it.nrhs) {
+                vr = it.x_re[gc * it.xsc + row * it.xsr];
+                vi = it.x_im[gc * it.xsc + row * it.xsr];
+            } else {
+                vr = vi = 0.0;
+            }
+        } else {
+            vr = xr[row * SOLVE_COLS + c];
+            vi = xi[row * SOLVE_COLS + c];
+        }
+    };
+    auto xset = [&](int row, int c, double vr, double vi) {
+        if constexpr (GX) {
+            const int gc = c0 + c;
+            if (row < n && gc < it.nrhs) {
+                it.x_re[gc * it.xsc + row * it.xsr] = vr;
+                it.x_im[gc * it.xsc + row * it.xsr] = vi;
+            }
+        } else {
+            xr[row * SOLVE_COLS + c] = vr;
+            xi[row * SOLVE_COLS + c] = vi;
+        }
+    };
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const double* lre = it.lu_re;
@@ -698,7 +907,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void lu_solve_kernel(const SolveItem
     }
     __syncthreads();
     for (int e = tid; e < np * SOLVE_COLS; e += SOLVE_THREADS) {
-        const int row = e / SOLVE_COLS, gc = c0 + e % SOLVE_COLS;
+        const int row = e / SOLVE_COLS, cc = e % SOLVE_COLS, gc = c0 + cc;
         double vr = 0, vi = 0;
         if (row < n && gc < it.nrhs) {
             const int src = sigma[row];
@@ -709,8 +918,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void lu_solve_kernel(const SolveItem
                 vr = (src == gc) ? 1.0 : 0.0;
             }
         }
-        xr[e] = vr;
-        xi[e] = vi;
+        xset(row, cc, vr, vi);
     }
     __syncthreads();
 
@@ -731,7 +939,8 @@ __global__ __launch_bounds__(SOLVE_THREADS) void lu_solve_kernel(const SolveItem
                 const bool ok = arow < n && k < n;
                 const double a_r = ok ? lre[(long)arow * ld + k] : 0.0;
                 const double a_i = ok ? lim[(long)arow * ld + k] : 0.0;
-                const double b_r = xr[k * SOLVE_COLS + li], b_i = xi[k * SOLVE_COLS + li];
+                double b_r, b_i;
+                xget(k, li, b_r, b_i);
                 rr = mfma16(a_r, b_r, rr);
                 ii = mfma16(a_i, b_i, ii);
                 ri = mfma16(a_r, b_i, ri);
@@ -749,10 +958,10 @@ __global__ __launch_bounds__(SOLVE_THREADS) void lu_solve_kernel(const SolveItem
                 dg_i[ei][ec] = d[LU_NB * LU_NB + (off + ei) * LU_NB + off + ec];
             }
             __syncthreads();
-            y_r[ei][ec] = xr[(r0 + ei) * SOLVE_COLS + ec] -
-                          (part_r[0][ei][ec] + part_r[1][ei][ec] + part_r[2][ei][ec] + part_r[3][ei][ec]);
-            y_i[ei][ec] = xi[(r0 + ei) * SOLVE_COLS + ec] -
-                          (part_i[0][ei][ec] + part_i[1][ei][ec] + part_i[2][ei][ec] + part_i[3][ei][ec]);
+            double xv_r, xv_i;
+            xget(r0 + ei, ec, xv_r, xv_i);
+            y_r[ei][ec] = xv_r - (part_r[0][ei][ec] + part_r[1][ei][ec] + part_r[2][ei][ec] + part_r[3][ei][ec]);
+            y_i[ei][ec] = xv_i - (part_i[0][ei][ec] + part_i[1][ei][ec] + part_i[2][ei][ec] + part_i[3][ei][ec]);
             __syncthreads();
             if (wave == 0) {  // X block = Dinv * Y
                 dbl4 pr = {0, 0, 0, 0}, pi = pr, qr = pr, qi = pr;
@@ -766,31 +975,37 @@ __global__ __launch_bounds__(SOLVE_THREADS) void lu_solve_kernel(const SolveItem
                     qi = mfma16(a_i, b_r, qi);
                 }
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = lk + 4 * r;
-                    xr[(r0 + row) * SOLVE_COLS + li] = pr[r] - pi[r];
-                    xi[(r0 + row) * SOLVE_COLS + li] = qr[r] + qi[r];
-                }
+                for (int r = 0; r < 4; ++r) xset(r0 + lk + 4 * r, li, pr[r] - pi[r], qr[r] + qi[r]);
             }
             __syncthreads();
         }
     }
-    for (int e = tid; e < n * SOLVE_COLS; e += SOLVE_THREADS) {
-        const int row = e / SOLVE_COLS, cc = e % SOLVE_COLS;
-        const int gc = c0 + cc;
-        if (gc < it.nrhs) {
-            it.x_re[gc * it.xsc + row * it.xsr] = xr[row * SOLVE_COLS + cc];
-            it.x_im[gc * it.xsc + row * it.xsr] = xi[row * SOLVE_COLS + cc];
+    if constexpr (!GX) {
+        for (int e = tid; e < n * SOLVE_COLS; e += SOLVE_THREADS) {
+            const int row = e / SOLVE_COLS, cc = e % SOLVE_COLS;
+            const int gc = c0 + cc;
+            if (gc < it.nrhs) {
+                it.x_re[gc * it.xsc + row * it.xsr] = xr[row * SOLVE_COLS + cc];
+                it.x_im[gc * it.xsc + row * it.xsr] = xi[row * SOLVE_COLS + cc];
+            }
         }
     }
 }
 
 inline void launch_lu_solve(const SolveItem* d_items, int nitems, int max_n, int max_rhs, hipStream_t s) {
-    const size_t lds = lu_solve_lds(max_n);
-    if (max_n > LU_MAX_N || lds > 134 * 1024) throw std::invalid_argument("matrix too large for the LU solve kernel");
-    ISLE_HIP(hipFuncSetAttribute((const void*)lu_solve_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     dim3 grid((max_rhs + SOLVE_COLS - 1) / SOLVE_COLS, nitems);
-    hipLaunchKernelGGL(lu_solve_kernel, grid, dim3(SOLVE_THREADS), lds, s, d_items);
+    const size_t lds = lu_solve_lds(max_n);
+    if (max_n <= LU_MAX_N && lds <= 134 * 1024) {
+        ISLE_HIP(hipFuncSetAttribute((const void*)lu_solve_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds));
+        hipLaunchKernelGGL(lu_solve_kernel<false>, grid, dim3(SOLVE_THREADS), lds, s, d_items);
+    } else {
+        if (max_n > LU_TALL_MAX) throw std::invalid_argument("matrix too large for the LU solve kernel");
+        const size_t lds_gx = 2 * (size_t)max_n * sizeof(int);
+        ISLE_HIP(hipFuncSetAttribute((const void*)lu_solve_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds_gx));
+        hipLaunchKernelGGL(lu_solve_kernel<true>, grid, dim3(SOLVE_THREADS), lds_gx, s, d_items);
+    }
     ISLE_HIP(hipGetLastError());
 }
 

@@ -1179,8 +1179,8 @@ struct Cyclic {
 // P(item, j) -> CMat<double> of group product j; logdet slots [nb][ng-1][2] (or null)
 template <typename PF>
 Cyclic add_cyclic_factor(Builder& b, int n, int ng, int nb, PF P, double* logdet) {
-    if (ng < 2 || n % LU_NB != 0 || n + LU_NB > LU_MAX_N || 2 * n > 1024)
-        throw std::invalid_argument("block-cyclic factorisation needs nx % 32 == 0 and nx <= 480");
+    if (ng < 2 || n % LU_NB != 0 || n + LU_NB > LU_TALL_MAX)
+        throw std::invalid_argument("block-cyclic factorisation needs nx % 32 == 0 and nx <= 16351");
     Cyclic c;
     c.n = n;
     c.ng = ng;
@@ -2379,7 +2379,7 @@ int isle_cscale(const double* in, double* out, long n, double cr, double ci, voi
 int isle_logdet(const double* a, int n, double* out_host, void* stream) {
     return guarded([&] {
         if (n < 1) throw std::invalid_argument("empty matrix");
-        if (n > isle::LU_MAX_N) throw std::invalid_argument("matrix too large for the LU kernel (n > 512)");
+        if (n > isle::LU_TALL_MAX) throw std::invalid_argument("matrix too large for the LU kernel (n > 16383)");
         hipStream_t s = (hipStream_t)stream;
         const long nn = (long)n * n;
         DenseScratch sc(n, s);
@@ -2398,7 +2398,7 @@ int isle_logdet(const double* a, int n, double* out_host, void* stream) {
 static int invert_impl(const double* a, int n, double* out, double* logdet_host, void* stream) {
     return guarded([&] {
         if (n < 1) throw std::invalid_argument("empty matrix");
-        if (n > isle::LU_MAX_N) throw std::invalid_argument("matrix too large for the LU kernel (n > 512)");
+        if (n > isle::LU_TALL_MAX) throw std::invalid_argument("matrix too large for the LU kernel (n > 16383)");
         hipStream_t s = (hipStream_t)stream;
         const long nn = (long)n * n;
         DenseScratch sc(n, s);

@@ -25,7 +25,7 @@ result is then the exact solve to rounding.  ``refine=0`` returns solveM's resul
 * ``Logdet(hfm, alpha=1)(stage, itr)`` - records log det M of particles and holes.  With
   alpha = 0 (spin basis: the matrices of -i phi), where the reference calls a dense
   slogdet for stability, the log dets come from the pivoted block-cyclic elimination of
-  the grouped products (precision "stable"; nx a multiple of 32, at most 480) and else
+  the grouped products (precision "stable"; nx a multiple of 32) and else
   from the reference's product algorithm.
 
 Results stay in memory (``values``: name -> list, one entry per call) and ``save(path)``
@@ -176,7 +176,7 @@ class Logdet(_Measurement):
     def _spin_action(self):
         if self._spin is None:
             nx = self.hfm.nx()
-            stable = nx % 32 == 0 and nx <= 480
+            stable = nx % 32 == 0 and nx <= 16351
             self._spin = HubbardFermiAction(self.hfm.kappaTilde(), self.hfm.muTilde(), self.hfm.sigmaKappa(),
                                             HFAHopping(self.hfm.HOPPING),
                                             HFABasis.SPIN, precision="stable" if stable else "fp64")

@@ -7,8 +7,8 @@ numpy inverse; pinned in tests/test_oracle_*.py) and the GPU DIRECT_SINGLE ones,
 1e-10 relative (action: imaginary part modulo 2 pi), on a honeycomb 3x3 torus (Nx=18),
 Nt=16, beta=4 (N = 288) for EXP / DIA hopping, particle-hole / spin basis, sigmaKappa =
 +-1 and mu = 0 / 0.1, with a complex field.  The action goes through the block LU of Q
-(logdetQ, tests/test_gpu_qlu.py); the force's dense Q^{-1} beyond N = 512 and graph
-capture are refused.
+(logdetQ, tests/test_gpu_qlu.py); graph capture is refused.  Beyond N = 512 the force's
+dense Q^{-1} runs on the tall-panel LU: N = 576 (Nx=18, Nt=32) against the oracle.
 """
 import numpy as np
 import pytest
@@ -41,12 +41,20 @@ def test_direct_square_matches_oracle(hop, basis, sk, mu):
     assert rel_l2(f_gpu, single.force(phi).cpu().numpy()) < 1e-10
 
 
+def test_direct_square_force_beyond_512():
+    print("direct square force, N = 576", flush=True)
+    kappa = ref.honeycomb(3) * 4.0 / 32
+    rs = np.random.RandomState(22)
+    phi = rs.normal(0, 0.35, 18 * 32) + 0.1j * rs.normal(0, 1, 18 * 32)
+    sq = A.HubbardFermiAction(kappa, 0.0, -1, A.HFAHopping.EXP, A.HFABasis.PARTICLE_HOLE, A.HFAAlgorithm.DIRECT_SQUARE)
+    r = ref.HubbardFermiAction(kappa, 0.0, -1, ref.EXP, ref.PARTICLE_HOLE, ref.DIRECT_SQUARE)
+    assert rel_l2(sq.force(phi).cpu().numpy(), r.force(phi)) < 1e-9
+
+
 def test_direct_square_limits():
     kappa = ref.honeycomb(3) * 0.25
     sq = A.HubbardFermiAction(kappa, 0.0, -1, A.HFAHopping.EXP, A.HFABasis.PARTICLE_HOLE,
                               A.HFAAlgorithm.DIRECT_SQUARE)
-    with pytest.raises(ValueError):
-        sq.force(np.zeros(18 * 32, complex))  # N = 576 > 512: the dense Q^{-1} is refused
     with pytest.raises(ValueError):
         sq.logdets(np.zeros(18 * 16, complex))
     with pytest.raises(ValueError):

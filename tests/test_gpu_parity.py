@@ -4,6 +4,7 @@ Tolerances (stated per test): logdet / eval / plain FP64 force 1e-10 relative
 (imaginary parts compared modulo 2*pi); mixed-precision force 1e-5 relative (L2).
 """
 import itertools
+import os
 
 import numpy as np
 import pytest
@@ -259,7 +260,11 @@ def test_hmc_fixed_seed_matches_oracle():
 def test_config2_gradient_check():
     """Nx=72, Nt=64, beta=6, U=3: the gradient of the action through torch.autograd (its
     backward is the force sweep) vs central FD (h=1e-6) on 64 entries, and the force vs
-    the oracle force on 16 seeded fields."""
+    the oracle force on 16 seeded fields.  The h = 1e-6 differences are taken twice: of the
+    GPU's own FP64 action (round-off limited, see below) and, from the committed fixture
+    tests/golden/fd_config2_h1e-6.npz, of the same action formula in extended precision
+    (accurate to its O(h^2) truncation, ~1e-8): the force agrees with the latter within 3e-8
+    absolute (measured 9.8e-9 on forces up to 13)."""
     kappa = ref.honeycomb(6)
     nx, nt, beta, u = 72, 64, 6.0, 3.0
     kt = kappa * beta / nt
@@ -276,6 +281,11 @@ def test_config2_gradient_check():
     assert rel_l2(f, g.force(phi).cpu().numpy()) < 1e-14
     s0 = abs(g.eval(phi))
     idx = rng.choice(nx * nt, 64, replace=False)
+    golden = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fd_config2_h1e-6.npz"))
+    assert np.array_equal(golden["idx"], idx)
+    err_golden = np.abs(f[idx] - golden["fd"]).max()
+    print(f"config 2 FD (extended precision, h = 1e-6): max |FD - F| = {err_golden:.2e}", flush=True)
+    assert err_golden < 3e-8
     # Central differences: truncation O(h^2) plus round-off |S| * eta / h with eta ~ 1e-13 the
     # relative accuracy of the FP64 action.  h = 1e-6 (the prescribed step) is round-off
     # limited: the bound 5 |S| 1e-13 / 1e-6 = 1.7e-4 at |S| = 350, measured 8.3e-5 (the action

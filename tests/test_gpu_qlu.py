@@ -13,8 +13,8 @@ log det and solve).
   Taylor series differ by rounding, which cond Q amplifies; at beta = 3, Nt = 1 it is 7e7);
 * at Nx=72, Nt=64 (honeycomb 6x6, beta=6), beyond a dense check, log det Q equals
   log det M_p + log det M_h (det Q = det M_p det M_h) from the GPU logdetM to 1e-9;
-* DIRECT_SQUARE's action goes through logdetQ and so has no size limit: at Nx*Nt = 576
-  it equals DIRECT_SINGLE's.
+* DIRECT_SQUARE beyond Nx*Nt = 512: at 576 its action (through logdetQ) and its force
+  (dense Q^{-1} on the tall-panel LU) equal DIRECT_SINGLE's.
 """
 import itertools
 
@@ -22,7 +22,7 @@ import numpy as np
 import pytest
 
 from oracle import hubbard_ref as ref
-from tests.helpers import mod2pi_diff, random_phi
+from tests.helpers import mod2pi_diff, random_phi, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -104,7 +104,7 @@ def test_logdetQ_is_sum_of_logdetM():
         assert mod2pi_diff(got, want) < 1e-9 * abs(want), cls.__name__
 
 
-def test_direct_square_eval_has_no_size_limit():
+def test_direct_square_beyond_512():
     kappa = ref.honeycomb(3) * 4.0 / 32
     phi = random_phi(np.random.RandomState(3), 18 * 32)
     for hop in (A.HFAHopping.EXP, A.HFAHopping.DIA):
@@ -112,5 +112,5 @@ def test_direct_square_eval_has_no_size_limit():
         single = A.HubbardFermiAction(kappa, 0.0, -1, hop, A.HFABasis.PARTICLE_HOLE, A.HFAAlgorithm.DIRECT_SINGLE)
         s_sq, s_single = sq.eval(phi), single.eval(phi)
         assert mod2pi_diff(s_sq, s_single) < 1e-10 * abs(s_single)
-        with pytest.raises(ValueError):
-            sq.force(phi)  # the force needs the dense Q^{-1}
+        # beyond N = 512 the force's dense Q^{-1} runs on the tall-panel LU
+        assert rel_l2(sq.force(phi).cpu().numpy(), single.force(phi).cpu().numpy()) < 1e-9
