@@ -27,7 +27,7 @@ KIND_NAMES = ["gemm64_rc_pre", "gemm64_rc_row", "gemm64_cr_pre", "gemm64_cc", "g
               "scale64", "scale32", "lu", "lu_solve", "diag64", "diag32", "copy64", "cvt32to64", "cvt64to32",
               "phase", "tcvt64to32", "diag64x32", "gemm64_cc_sub", "lu_shape", "block", "sigma", "gather",
               "tblock", "trsm", "gemm32_rc_pre_diag", "dreduce", "rgather", "gemm32_rc_diag", "tscale64to32",
-              "gemm32_rck_diag", "colscale64to32"]
+              "gemm32_rck_diag", "colscale64to32", "lu_shape_defer"]
 
 
 def parse():
@@ -35,7 +35,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="timed trajectories (per chain) of the batched form")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--chains", type=int, default=128, help="independent chains advanced together")
+    ap.add_argument("--chains", type=int, default=None,
+                    help="independent chains advanced together (default 128; 4 for --config5)")
     ap.add_argument("--group", type=int, default=32, help="time slices per group of the mixed force")
     ap.add_argument("--plain-steps", type=int, default=3, help="timed trajectories of the plain FP64 forms")
     ap.add_argument("--therm", type=int, default=8, help="thermalisation trajectories (length 0.25, 100 steps) "
@@ -325,7 +326,7 @@ def config5(args):
     from isle_amd.evolver import BatchedConstStepLeapfrog, NumpyRNG
     from oracle import hubbard_ref as ref
     nt, beta, u, mu, nstep = 256, 10.0, 2.5, 0.5, 60
-    chains = args.chains
+    chains = args.chains or 4
     kappa = isle.honeycomb(12, 16, periodic_j=False).hopping()
     nx = kappa.shape[0]
     kt, mt, ut = kappa * beta / nt, mu * beta / nt, u * beta / nt
@@ -406,7 +407,8 @@ def force_only(args):
     kt = lat.hopping() * BETA / NT
     ut = U * BETA / NT
     ferm = A.HubbardFermiAction(kt, 0.0, -1, A.HFAHopping.EXP, precision="mixed", group=args.group)
-    phis = torch.as_tensor(np.random.RandomState(5).normal(0, np.sqrt(ut), (args.chains, nx * NT)) + 0j, device=dev)
+    phis = torch.as_tensor(np.random.RandomState(5).normal(0, np.sqrt(ut), (args.chains or 128, nx * NT)) + 0j,
+                           device=dev)
     for _ in range(args.warmup):
         ferm.force_batch(phis)
     torch.cuda.synchronize()
@@ -419,7 +421,7 @@ def force_only(args):
     torch.cuda.synchronize()
     dst = src.clone()
     torch.cuda.synchronize()
-    print(json.dumps({"force_only": True, "chains": args.chains, "force_ms": 1e3 * t_force,
+    print(json.dumps({"force_only": True, "chains": args.chains or 128, "force_ms": 1e3 * t_force,
                       "copy_bytes": int(dst.numel() * 8)}))
 
 
@@ -495,7 +497,7 @@ def main():
     nx = lat.nx()
     kt = lat.hopping() * BETA / NT
     ut = U * BETA / NT
-    B = args.chains
+    B = args.chains or 128
 
     def make(prec):
         # the mixed form's Metropolis action: FP64, stabilising block-cyclic LU every 8 slices

@@ -168,7 +168,8 @@ int isle_cscale(const double* in, double* out, long n, double cr, double ci, voi
  * real x complex with the row sums fused and the stored rows scaled by the next slice's phase,
  * 29 transposed FP64 -> FP32 copy with complex row scaling, 30 GEMM32 real x complex on
  * k-contiguous slabs with the row sums fused (product stored transposed), 31 FP64 -> FP32 copy
- * with complex column scaling. */
+ * with complex column scaling, 32 LU of a block-cyclic elimination step whose columns beyond
+ * the factorised ones are eliminated by a blocked triangular solve and one GEMM. */
 int isle_profile_kernels(isle_hfm* h, const double* phi, int op, int nt, int batch, int group, int nspecies,
                          int spin, int reps, double* out_host, int nkinds);
 

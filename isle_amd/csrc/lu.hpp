@@ -48,10 +48,13 @@ struct LuItem {
     int ld;
     int m = 0, ncols = 0, kf = 0;  // rows, columns, columns to factorise (kf % LU_NB == 0 if nt > 0)
     int nt = 0;      // > 0: rows [k0 + LU_NB, nt) are identity rows, never candidates (see above)
+    int nupd = 0;    // > 0: columns [nupd, ncols) take the row interchanges only (their
+                     // elimination is left to the caller: U12 = L11^{-1} A12, A22 -= L21 U12)
 };
 __host__ __device__ inline int lu_m(const LuItem& it) { return it.m ? it.m : it.n; }
 __host__ __device__ inline int lu_ncols(const LuItem& it) { return it.ncols ? it.ncols : it.n; }
 __host__ __device__ inline int lu_kf(const LuItem& it) { return it.kf ? it.kf : it.n; }
+__host__ __device__ inline int lu_nupd(const LuItem& it) { return it.nupd ? it.nupd : lu_ncols(it); }
 // Candidate rows of the panel at k0 and the matrix row of panel position p.
 __host__ __device__ inline bool lu_structured(int k0, int nt) { return nt > 0 && k0 + LU_NB < nt; }
 __host__ __device__ inline int lu_rows(int k0, int m, int nt) {
@@ -626,6 +629,19 @@ __global__ __launch_bounds__(256) void lu_rswap_kernel(const LuItem* __restrict_
             im[o] = wi[j];
         }
     }
+    // columns left to the caller's elimination (col >= nupd): the panel rows take their new
+    // content as it is (the reads of phase two, which may read panel rows, are done first)
+    if (nrt > 0 && k0 + NB + c0 + CW > lu_nupd(it)) {  // uniform over the workgroup
+        __syncthreads();
+        if (ok && col >= lu_nupd(it)) {
+#pragma unroll
+            for (int j = 0; j < QP; ++j) {
+                const int q = part + PARTS * j;
+                re[(long)(k0 + q) * ld + col] = st_re[(long)q * ncols + c];
+                im[(long)(k0 + q) * ld + col] = st_im[(long)q * ncols + c];
+            }
+        }
+    }
 }
 
 // grid.x: tm*tm trailing tiles (tm = tiles per dimension of the largest item's A22), then
@@ -670,7 +686,7 @@ __global__ __launch_bounds__(LU_UPD_THREADS) void lu_update_kernel(const LuItem*
         return;
     }
     // ---- trailing tile (I, J) of A22 = panel positions NB.. x columns c0.. of the matrix
-    const int c0 = k0 + NB, ntr = lu_rows(k0, m, ntop) - NB, ntc = ncols - c0;
+    const int c0 = k0 + NB, ntr = lu_rows(k0, m, ntop) - NB, ntc = lu_nupd(it) - c0;
     const int I = blockIdx.x / tmc, J = blockIdx.x % tmc;
     if (I * T >= ntr || J * T >= ntc) return;
     __shared__ double sa_re[T][T + 1], sa_im[T][T + 1];  // L11^{-1}
@@ -766,7 +782,7 @@ __global__ __launch_bounds__(LU_UPD_THREADS) void lu_update_kernel(const LuItem*
 // LU_MAX_N candidate rows run in registers (lu_panel_kernel), larger ones up to LU_TALL_MAX
 // in global memory (lu_panel_tall_kernel).
 inline void launch_lu_shape(const LuItem* d_items, int nitems, int m, int ncols, int kf, int nt, int add_identity,
-                            hipStream_t s) {
+                            hipStream_t s, int nupd = 0) {
     if (kf > ncols || kf > m || (nt > 0 && (kf % LU_NB != 0 || nt > m)))
         throw std::invalid_argument("inconsistent LU shape");
     for (int k0 = 0; k0 < kf; k0 += LU_NB)
@@ -782,9 +798,10 @@ inline void launch_lu_shape(const LuItem* d_items, int nitems, int m, int ncols,
                                k0 == 0 ? add_identity : 0);
         }
         ISLE_HIP(hipGetLastError());
-        const int ntr = rows - LU_NB, ntc = ncols - k0 - LU_NB;
-        if (ntc > 0) {
-            hipLaunchKernelGGL(lu_rswap_kernel, dim3((ntc + 63) / 64, nitems), dim3(256), 0, s, d_items, k0);
+        const int ntr = rows - LU_NB, nsw = ncols - k0 - LU_NB;  // columns right of the panel
+        const int ntc = (nupd > 0 ? std::min(nupd, ncols) : ncols) - k0 - LU_NB;  // of them updated
+        if (nsw > 0) {
+            hipLaunchKernelGGL(lu_rswap_kernel, dim3((nsw + 63) / 64, nitems), dim3(256), 0, s, d_items, k0);
             ISLE_HIP(hipGetLastError());
         }
         const int tmr = (ntr > 0 && ntc > 0) ? (ntr + LU_TILE - 1) / LU_TILE : 0;

@@ -326,7 +326,7 @@ void Plan::run(OpCache& op, const double* phi) {
 }
 
 void Plan::set_tile(int kind, int tile) {
-    if (kind < 0 || kind > (int)K::COLSCALE64TO32) throw std::invalid_argument("unknown step kind");
+    if (kind < 0 || kind > (int)K::LU_SHAPE_DEFER) throw std::invalid_argument("unknown step kind");
     if (tile < -1 || tile >= N_TILES) throw std::invalid_argument("tile out of range");
     if (tile >= 0)
         forced_[kind] = tile;
@@ -505,6 +505,9 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
             break;
         case K::LU_SHAPE:
             launch_lu_shape((const LuItem*)st.items, st.count, st.max_m, st.max_n, st.flag, st.aux, 0, s);
+            break;
+        case K::LU_SHAPE_DEFER:  // the columns beyond kf (= flag) are eliminated by later steps
+            launch_lu_shape((const LuItem*)st.items, st.count, st.max_m, st.max_n, st.flag, st.aux, 0, s, st.flag);
             break;
         case K::BLOCK:
             hipLaunchKernelGGL(block_fill_kernel, dim3(ew_blocks(st.aux), st.count), dim3(256), 0, s,
@@ -1236,6 +1239,11 @@ Cyclic add_cyclic_factor(Builder& b, int n, int ng, int nb, PF P, double* logdet
             }
         }
         b.add(K::BLOCK, v, 0, 0, 0, (int)nn);
+        // Steps that factorise only the first kf columns leave the columns beyond to one
+        // blocked triangular solve and one GEMM with k = kf (U12 = L11^{-1} A12 in place,
+        // A22 -= L21 U12) instead of a rank-32 update of them per panel: a third of the
+        // trailing-update traffic, and the bulk of the flops on the tuned GEMM tiles.
+        const bool defer = c.kf(k) < c.ldw(k);
         std::vector<LuItem> lu;
         for (int item = 0; item < nb; ++item) {
             LuItem it{c.wre(item, k), c.wim(item, k), c.pv(item, k),
@@ -1244,9 +1252,34 @@ Cyclic add_cyclic_factor(Builder& b, int n, int ng, int nb, PF P, double* logdet
             it.ncols = c.ldw(k);
             it.kf = c.kf(k);
             it.nt = n;
+            if (defer) it.nupd = c.kf(k);
             lu.push_back(it);
         }
-        b.add(K::LU_SHAPE, lu, 2 * n, c.ldw(k), c.kf(k), n);
+        b.add(defer ? K::LU_SHAPE_DEFER : K::LU_SHAPE, lu, 2 * n, c.ldw(k), c.kf(k), n);
+        if (defer) {
+            const int kf = c.kf(k), ld = c.ldw(k), w = ld - kf, mb = 2 * n - kf;
+            std::vector<TrsmMat> tm;
+            std::vector<GemmItem<double>> gs;
+            for (int item = 0; item < nb; ++item) {
+                double* wre = c.wre(item, k);
+                double* wim = c.wim(item, k);
+                tm.push_back({wre, wim, ld, c.wk(item, k), wre + kf, wim + kf, ld, wre + kf, wim + kf, ld});
+                GemmItem<double> g{};
+                g.a_re = wre + (long)kf * ld;  // L21: rows kf.., columns 0..kf-1
+                g.a_im = wim + (long)kf * ld;
+                g.b_re = wre + kf;  // U12: rows 0..kf-1, columns kf..
+                g.b_im = wim + kf;
+                g.c_re = wre + (long)kf * ld + kf;  // A22
+                g.c_im = wim + (long)kf * ld + kf;
+                g.m = mb;
+                g.n = w;
+                g.k = kf;
+                g.lda = g.ldb = g.ldc = ld;
+                gs.push_back(g);
+            }
+            add_trsm(b, 0, kf, w, tm);
+            b.add(K::GEMM64_CC_SUB, gs, mb, w);
+        }
     }
     return c;
 }

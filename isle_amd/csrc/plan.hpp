@@ -70,6 +70,7 @@ enum class K : int {
     GEMM32_RCK_DIAG,    // f32 real x complex on k-contiguous slabs (operand and product stored
                         // transposed), force diagonal's row sums fused, rows scaled by the next phase
     COLSCALE64TO32,     // copy with conversion and complex column scaling
+    LU_SHAPE_DEFER,     // LU_SHAPE whose columns beyond kf take the row interchanges only
 };
 
 struct Step {

@@ -13,7 +13,9 @@ the whole ensemble is captured once in a HIP graph (fork/join over the streams) 
 replayed.  Launch shapes come from the plans' autotuner per (Nx, Nt) and per merged step.
 Momenta and the Metropolis decision of every chain come from that chain's own NumpyRNG
 exactly as a lone ConstStepLeapfrog would draw them, so each chain reproduces a single-chain
-run with the same seed up to the arithmetic of the batched kernels.
+run with the same seed up to the arithmetic of the batched kernels.  A lattice that carries
+more than half of the ensemble's work is split over two streams (config 4: Nx=128, 85% of
+the flops).
 """
 from collections import OrderedDict
 from concurrent.futures import ThreadPoolExecutor
@@ -105,11 +107,36 @@ class EnsembleLeapfrog:
             else:
                 group_of.setdefault(lk, []).append(len(self.groups))
                 self.groups.append([bi])
+        if merge:
+            self.groups = self._balance(self.groups)
         self.merge = bool(merge)
         self.graph = bool(graph)
         self._graph = None
         self._pool = ThreadPoolExecutor(max_workers=16)
         self.trajPoints, self.deltaH = [], []
+
+    def _cost(self, b):
+        """Work of one trajectory of bucket b, up to a constant: chains x Nt x Nx^3."""
+        c0 = self.buckets[b].chains[0]
+        return len(self.buckets[b].chains) * c0.nt * float(c0.nx) ** 3
+
+    def _balance(self, groups):
+        """A group with more than half of the ensemble's work and at least two buckets is
+        the critical path of the graph: its buckets go to two streams, largest first onto the
+        lighter one (each stream still shares its launches)."""
+        total = sum(self._cost(b) for g in groups for b in g)
+        out = []
+        for g in groups:
+            if len(g) >= 2 and sum(self._cost(b) for b in g) > 0.5 * total:
+                parts, loads = [[], []], [0.0, 0.0]
+                for b in sorted(g, key=self._cost, reverse=True):
+                    i = 0 if loads[0] <= loads[1] else 1
+                    parts[i].append(b)
+                    loads[i] += self._cost(b)
+                out.extend(sorted(p) for p in parts if p)
+            else:
+                out.append(g)
+        return out
 
     def phis(self):
         """Current field of every chain (input order), as host arrays."""
