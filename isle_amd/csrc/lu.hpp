@@ -230,8 +230,9 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
                                                             int add_identity) {
     __shared__ unsigned long long pkey[NB];
     __shared__ int pvt[NB];  // pivot position of each column
-    __shared__ double xrow[2 * NB];  // the pivot row (re, im interleaved); a column's writes and reads
-                                     // are separated by that column's barriers
+    // the pivot row (re, im pairs); a column's writes and reads are separated by that
+    // column's barriers
+    __shared__ __align__(16) double xrow[2 * NB];
     __shared__ double lt_re[NB][NB + 1], lt_im[NB][NB + 1];  // L11, inverted in place
     __shared__ double ut_re[NB][NB + 1], ut_im[NB][NB + 1];  // U11, inverted in place
     __shared__ double tl_re[NB][NB + 1], tl_im[NB][NB + 1];  // doubling temporaries
@@ -291,12 +292,12 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
             const int p = kmax ? 511 - static_cast<int>(kmax & 0x1FFull) : kk;
             if (own && pos == p) {
                 // slots NB-kk.. of the pivot row hold its own earlier multipliers: broadcast
-                // zeros there, so that the update below needs no per-slot branch
+                // zeros there, so that the update below subtracts nothing there
+                double2* dst = reinterpret_cast<double2*>(xrow);
 #pragma unroll
                 for (int j = 0; j < NB; ++j) {
                     const bool u = j < NB - kk;
-                    xrow[2 * j] = u ? ar[j] : 0.0;
-                    xrow[2 * j + 1] = u ? ai[j] : 0.0;
+                    dst[j] = make_double2(u ? ar[j] : 0.0, u ? ai[j] : 0.0);
                 }
             }
             if (own) {  // the interchange of positions kk and p
@@ -308,7 +309,9 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
             if (tid == 0) pvt[kk] = p;  // global stores here would stall every LDS wait (FLAT)
             __syncthreads();
             if (own && pos > kk) {  // rows not yet pivoted: multiplier and rank-1 update
-                const double ur = xrow[0], ui = xrow[1];
+                const double2* xv = reinterpret_cast<const double2*>(xrow);
+                const double2 u0 = xv[0];
+                const double ur = u0.x, ui = u0.y;
                 if (ur != 0.0 || ui != 0.0) {
                     const double rden = 1.0 / (ur * ur + ui * ui);
                     const double ir = ur * rden, ii = -ui * rden;
@@ -316,23 +319,22 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
                     ni = nr * ii + ni * ir;
                     nr = t;
                 }
-                // slots NB-kk.. (earlier multipliers) subtract 0; the pivot row is read from
-                // LDS in batches of 8 entries so that a batch's reads are in flight together
+                // the pivot row in batches of 8 entries (a batch's reads in flight together);
+                // batches from NB-kk on hold only broadcast zeros and are skipped
                 constexpr int RB = 8;
 #pragma unroll
                 for (int j0 = 0; j0 < NB; j0 += RB) {
-                    double br[RB], bi[RB];
+                    if (j0 < NB - kk) {
+                        double2 bv[RB];
 #pragma unroll
-                    for (int q = 0; q < RB; ++q) {
-                        br[q] = xrow[2 * (j0 + q)];
-                        bi[q] = xrow[2 * (j0 + q) + 1];
-                    }
+                        for (int q = 0; q < RB; ++q) bv[q] = xv[j0 + q];
 #pragma unroll
-                    for (int q = 0; q < RB; ++q) {
-                        const int j = j0 + q;
-                        if (j >= 1) {
-                            ar[j] -= nr * br[q] - ni * bi[q];
-                            ai[j] -= nr * bi[q] + ni * br[q];
+                        for (int q = 0; q < RB; ++q) {
+                            const int j = j0 + q;
+                            if (j >= 1) {
+                                ar[j] -= nr * bv[q].x - ni * bv[q].y;
+                                ai[j] -= nr * bv[q].y + ni * bv[q].x;
+                            }
                         }
                     }
                 }
