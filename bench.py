@@ -27,7 +27,8 @@ KIND_NAMES = ["gemm64_rc_pre", "gemm64_rc_row", "gemm64_cr_pre", "gemm64_cc", "g
               "scale64", "scale32", "lu", "lu_solve", "diag64", "diag32", "copy64", "cvt32to64", "cvt64to32",
               "phase", "tcvt64to32", "diag64x32", "gemm64_cc_sub", "lu_shape", "block", "sigma", "gather",
               "tblock", "trsm", "gemm32_rc_pre_diag", "dreduce", "rgather", "gemm32_rc_diag", "tscale64to32",
-              "gemm32_rck_diag", "colscale64to32", "lu_shape_defer"]
+              "gemm32_rck_diag", "colscale64to32", "lu_shape_defer",
+              "gemm64_cc4", "gemm64_cc4_sub"]
 
 
 def parse():

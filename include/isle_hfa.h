@@ -115,6 +115,18 @@ int isle_solveM(isle_hfm* h, const double* phi, int nt, int species, const doubl
  * right-hand sides); phi: device nx*nt complex128. */
 int isle_applyM(isle_hfm* h, const double* phi, int nt, int species, const double* x, long nrhs, double* out);
 
+/* Block LU of Q = M_p M_h^T (getQLU, hubbardFermiMatrixExp.cpp:419-479 for nt > 2; the Dia
+ * file repeats it; the QLU struct is hubbardFermiMatrixExp.hpp / .cpp:282-360), nt >= 3, as one
+ * cached device operation: the off-diagonal blocks, then the recursion with one LU per
+ * diagonal block and complex GEMMs for the products; the log dets of the d_i stay on the
+ * device until the end; the block products use four real products each (the three-product
+ * form's rounding, amplified by the recursion, would move log det Q by ~1e-8 relative at
+ * nx=288, nt=128).  phi: device nx*nt complex128.  Outputs (device, interleaved
+ * complex128, row-major nx x nx blocks): dinv [nt], u and l [nt-1], v and h [nt-2];
+ * logdet_host: 2 nt doubles (re, im of log det d_i).  Synchronises the handle's stream once. */
+int isle_qlu(isle_hfm* h, const double* phi, int nt, double* dinv, double* u, double* l, double* v, double* hh,
+             double* logdet_host);
+
 /* solveM with the factorization kept between calls (no counterpart in the reference: for
  * callers that solve many right-hand sides against one field, such as iterative refinement).
  * reuse = 0: as isle_solveM (the handle keeps the factorization: partial products and their
@@ -169,7 +181,8 @@ int isle_cscale(const double* in, double* out, long n, double cr, double ci, voi
  * 29 transposed FP64 -> FP32 copy with complex row scaling, 30 GEMM32 real x complex on
  * k-contiguous slabs with the row sums fused (product stored transposed), 31 FP64 -> FP32 copy
  * with complex column scaling, 32 LU of a block-cyclic elimination step whose columns beyond
- * the factorised ones are eliminated by a blocked triangular solve and one GEMM. */
+ * the factorised ones are eliminated by a blocked triangular solve and one GEMM, 33 GEMM64
+ * complex x complex with four real products (block LU of Q), 34 the same, subtracted. */
 int isle_profile_kernels(isle_hfm* h, const double* phi, int op, int nt, int batch, int group, int nspecies,
                          int spin, int reps, double* out_host, int nkinds);
 

@@ -420,8 +420,10 @@ def getQLU(hfm, phi):
 
     The recursion is sequential in the time slice: each diagonal block
     d_i = P - l_{i-1} u_{i-1} is inverted by the HIP LU (which also gives log det d_i)
-    before l_i and h_i can be formed; the block products run as complex GEMMs on the
-    device and sum_i h_i v_i is one GEMM over the stacked blocks."""
+    before l_i and h_i can be formed.  For nt >= 3 the whole recursion is one native
+    operation (isle_qlu): the off-diagonal blocks, the block products as complex GEMMs and
+    one LU and solve per diagonal block, launched back to back with the log dets left on the
+    device, and sum_i h_i v_i one GEMM over the stacked blocks; nt = 1, 2 are single blocks."""
     nx = hfm.nx()
     phi = as_field(phi, nx)
     if phi.dim() != 1:
@@ -429,12 +431,17 @@ def getQLU(hfm, phi):
     nt = phi.shape[0] // nx
     dev = phi.device
     c128 = torch.complex128
-    p = torch.as_tensor(hfm.P(), device=dev).to(c128)
     dinv = torch.empty((nt, nx, nx), dtype=c128, device=dev)
     u = torch.empty((max(nt - 1, 0), nx, nx), dtype=c128, device=dev)
     l = torch.empty_like(u)
     v = torch.empty((max(nt - 2, 0), nx, nx), dtype=c128, device=dev)
     h = torch.empty_like(v)
+    if nt >= 3:
+        ld = np.empty(2 * nt)
+        _lib.check(_lib.load().isle_qlu(hfm.handle, _lib.ptr(phi.contiguous()), nt, _lib.ptr(dinv), _lib.ptr(u),
+                                        _lib.ptr(l), _lib.ptr(v), _lib.ptr(h), _lib.dptr(ld)))
+        return QLU(dinv, u, l, v, h, [complex(ld[2 * i], ld[2 * i + 1]) for i in range(nt)])
+    p = torch.as_tensor(hfm.P(), device=dev).to(c128)
     lds = []
 
     def inv(i, d):
@@ -444,25 +451,10 @@ def getQLU(hfm, phi):
         inv(0, p + hfm.Tplus(0, phi) + hfm.Tminus(0, phi))
         return QLU(dinv, u, l, v, h, lds)
     inv(0, p)
-    if nt == 2:
-        l[0] = (hfm.Tplus(1, phi) + hfm.Tminus(1, phi)) @ dinv[0]
-        u[0] = hfm.Tplus(0, phi) + hfm.Tminus(0, phi)
-        inv(1, p - l[0] @ u[0])
-        return QLU(dinv, u, l, v, h, lds)
-    u[0] = hfm.Tminus(0, phi)
-    l[0] = hfm.Tplus(1, phi) @ dinv[0]
-    v[0] = hfm.Tplus(0, phi)
-    h[0] = hfm.Tminus(nt - 1, phi) @ dinv[0]
-    for i in range(1, nt - 2):
-        inv(i, p - l[i - 1] @ u[i - 1])
-        l[i] = hfm.Tplus(i + 1, phi) @ dinv[i]
-        h[i] = -(h[i - 1] @ u[i - 1]) @ dinv[i]
-        v[i] = -(l[i - 1] @ v[i - 1])
-        u[i] = hfm.Tminus(i, phi)
-    inv(nt - 2, p - l[nt - 3] @ u[nt - 3])
-    u[nt - 2] = hfm.Tminus(nt - 2, phi) - l[nt - 3] @ v[nt - 3]
-    l[nt - 2] = (hfm.Tplus(nt - 1, phi) - h[nt - 3] @ u[nt - 3]) @ dinv[nt - 2]
-    inv(nt - 1, p - l[nt - 2] @ u[nt - 2] - _hcat(h) @ _vcat(v))
+    # nt == 2
+    l[0] = (hfm.Tplus(1, phi) + hfm.Tminus(1, phi)) @ dinv[0]
+    u[0] = hfm.Tplus(0, phi) + hfm.Tminus(0, phi)
+    inv(1, p - l[0] @ u[0])
     return QLU(dinv, u, l, v, h, lds)
 
 

@@ -195,6 +195,19 @@ __global__ void split_to_interleaved_kernel(const double* __restrict__ re, const
 }
 
 // interleaved complex128 -> split planes
+// Stacks of n x n blocks in split planes to interleaved complex [nblk][n][n]: block k starts at
+// element k * boff of the planes, row stride ld.  Grid (ceil(n*n / 256), nblk).
+__global__ __launch_bounds__(256) void blocks_to_interleaved_kernel(const double* __restrict__ re,
+                                                                     const double* __restrict__ im, long boff,
+                                                                     long ld, int n, double* __restrict__ out) {
+    const long k = blockIdx.y, nn = (long)n * n;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < nn; e += (long)gridDim.x * 256) {
+        const long r = e / n, c = e % n, src = k * boff + r * ld + c;
+        out[2 * (k * nn + e)] = re[src];
+        out[2 * (k * nn + e) + 1] = im[src];
+    }
+}
+
 __global__ void interleaved_to_split_kernel(const double* __restrict__ in, double* __restrict__ re,
                                             double* __restrict__ im, long n) {
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {

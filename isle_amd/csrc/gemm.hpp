@@ -69,7 +69,7 @@ __device__ __forceinline__ double row16_sum(double v) {
 // One BM x BN output tile of item `it` (tile index `tile` in a ceil(m/BM) x tiles_n_max
 // grid).  AC/BC: A/B complex.  PRE: scale the (single) complex operand along k.
 // POST: SCALE_* or SUBTRACT.  Must be called by all 256 threads of the block.
-template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int GEMM_BK>
+template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int GEMM_BK, bool G4 = false>
 __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int tiles_n_max) {
     static_assert(!(PRE && AC && BC), "pre-scaling needs exactly one complex operand");
     static_assert(BM % 32 == 0 && BN % 32 == 0, "tile must be a multiple of 32");
@@ -105,7 +105,10 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
     const int lk = lane >> 4;
 
     typedef typename Acc4<T>::type V4;
-    constexpr int NP2 = (AC && BC) ? MI * NJ : 1;  // third product of the complex x complex case
+    // complex x complex: three real products (Gauss) unless G4 asks for the four of the
+    // textbook product (componentwise error bound: for recursions that amplify rounding)
+    constexpr bool GAUSS = AC && BC && !G4;
+    constexpr int NP2 = GAUSS ? MI * NJ : 1;  // third product of the Gauss form
     V4 acc_re[MI][NJ], acc_im[MI][NJ], acc_p2[NP2];
 #pragma unroll
     for (int i = 0; i < MI; ++i)
@@ -186,11 +189,10 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
 
 #pragma unroll
         for (int kg = 0; kg < GEMM_BK; kg += 16) {
-            // complex x complex uses three real products (Gauss):
-            //   P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 - P2
+            // Gauss: P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 - P2
             constexpr int PB = BLOCKED ? MI * NJ : 1;
             V4 part_re[BLOCKED ? MI : 1][BLOCKED ? NJ : 1], part_im[BLOCKED ? MI : 1][BLOCKED ? NJ : 1];
-            V4 part_p2[(BLOCKED && AC && BC) ? PB : 1];
+            V4 part_p2[(BLOCKED && GAUSS) ? PB : 1];
             if constexpr (BLOCKED) {
 #pragma unroll
                 for (int i = 0; i < MI; ++i)
@@ -198,7 +200,7 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
                     for (int j = 0; j < NJ; ++j) {
                         part_re[i][j] = V4{0, 0, 0, 0};
                         part_im[i][j] = V4{0, 0, 0, 0};
-                        if (AC && BC) part_p2[i * NJ + j] = V4{0, 0, 0, 0};
+                        if (GAUSS) part_p2[i * NJ + j] = V4{0, 0, 0, 0};
                     }
             }
 #pragma unroll
@@ -221,11 +223,16 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
                     for (int j = 0; j < NJ; ++j) {
                         V4& cr = BLOCKED ? part_re[BLOCKED ? i : 0][BLOCKED ? j : 0] : acc_re[i][j];
                         V4& ci = BLOCKED ? part_im[BLOCKED ? i : 0][BLOCKED ? j : 0] : acc_im[i][j];
-                        if (AC && BC) {
-                            V4& c2 = BLOCKED ? part_p2[BLOCKED ? i * NJ + j : 0] : acc_p2[(AC && BC) ? i * NJ + j : 0];
+                        if (GAUSS) {
+                            V4& c2 = BLOCKED ? part_p2[BLOCKED ? i * NJ + j : 0] : acc_p2[GAUSS ? i * NJ + j : 0];
                             cr = mfma16(ar[i], br[j], cr);
                             c2 = mfma16(ai[i], bi[j], c2);
                             ci = mfma16(ar[i] + ai[i], br[j] + bi[j], ci);
+                        } else if (AC && BC) {
+                            cr = mfma16(ar[i], br[j], cr);
+                            cr = mfma16(-ai[i], bi[j], cr);
+                            ci = mfma16(ar[i], bi[j], ci);
+                            ci = mfma16(ai[i], br[j], ci);
                         } else if (AC) {
                             cr = mfma16(ar[i], br[j], cr);
                             ci = mfma16(ai[i], br[j], ci);
@@ -242,7 +249,7 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
                 for (int i = 0; i < MI; ++i)
 #pragma unroll
                     for (int j = 0; j < NJ; ++j) {
-                        if (AC && BC) {
+                        if (GAUSS) {
                             acc_re[i][j] += part_re[i][j] - part_p2[i * NJ + j];
                             acc_im[i][j] += part_im[i][j] - part_re[i][j] - part_p2[i * NJ + j];
                         } else {
@@ -256,7 +263,7 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
         __syncthreads();
         buf ^= 1;
     }
-    if constexpr (!BLOCKED && AC && BC) {  // Gauss recombination of the three products
+    if constexpr (!BLOCKED && GAUSS) {  // Gauss recombination of the three products
 #pragma unroll
         for (int i = 0; i < MI; ++i)
 #pragma unroll
@@ -319,13 +326,13 @@ __device__ __forceinline__ void xcd_map(int tiles, int nitems, int& tile, int& i
 }
 
 // grid: x = tile index over (ceil(m/BM) x ceil(n/BN)) of the largest item, y = item.
-template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int GEMM_BK>
+template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int GEMM_BK, bool G4 = false>
 __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* __restrict__ items,
                                                             int tiles_n_max) {
     int tile, item;
     xcd_map(gridDim.x, gridDim.y, tile, item);
     const GemmItem<T> it = items[item];
-    gemm_tile<T, AC, BC, PRE, POST, BM, BN, GEMM_BK>(it, tile, tiles_n_max);
+    gemm_tile<T, AC, BC, PRE, POST, BM, BN, GEMM_BK, G4>(it, tile, tiles_n_max);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -919,13 +926,13 @@ constexpr size_t gemm_lds_bytes() {
 
 // Launches the GEMM with tile shape `tile`; returns false if that shape does not fit
 // this operand type (LDS budget), in which case nothing is launched.
-template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int BK>
+template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int BK, bool G4 = false>
 inline bool launch_gemm_tile(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s) {
     if constexpr (gemm_lds_bytes<T, AC, BC, BM, BN, BK>() > 96 * 1024) {
         return false;
     } else {
         const int tm = (max_m + BM - 1) / BM, tn = (max_n + BN - 1) / BN;
-        hipLaunchKernelGGL((gemm_kernel<T, AC, BC, PRE, POST, BM, BN, BK>), dim3(tm * tn, nitems),
+        hipLaunchKernelGGL((gemm_kernel<T, AC, BC, PRE, POST, BM, BN, BK, G4>), dim3(tm * tn, nitems),
                            dim3(GEMM_THREADS), 0, s, d_items, tn);
         ISLE_HIP(hipGetLastError());
         return true;
@@ -937,7 +944,7 @@ inline bool launch_gemm_tile(const GemmItem<T>* d_items, int nitems, int max_m, 
 // phase pointers (the caller vouches for all of it by passing direct_k, the K dimension
 // shared by every item, or 0 when not eligible); K must be a multiple of the tile's K slab,
 // which is checked here.
-template <typename T, bool AC, bool BC, bool PRE, int POST>
+template <typename T, bool AC, bool BC, bool PRE, int POST, bool G4 = false>
 inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s,
                         int tile = 0, int direct_k = 0) {
     if ((tile >= 6 && tile < 18) || tile >= 20) {
@@ -972,14 +979,14 @@ inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int m
         return false;  // only the direct-to-LDS tiles fuse the row sums
     } else {
         switch (tile) {
-        case 1: return launch_gemm_tile<T, AC, BC, PRE, POST, 32, 32, 32>(d_items, nitems, max_m, max_n, s);
-        case 2: return launch_gemm_tile<T, AC, BC, PRE, POST, 32, 32, 64>(d_items, nitems, max_m, max_n, s);
-        case 3: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 32>(d_items, nitems, max_m, max_n, s);
-        case 4: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 96, 32>(d_items, nitems, max_m, max_n, s);
-        case 5: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 96, 16>(d_items, nitems, max_m, max_n, s);
-        case 18: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 32, 32>(d_items, nitems, max_m, max_n, s);
-        case 19: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 32, 16>(d_items, nitems, max_m, max_n, s);
-        default: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 16>(d_items, nitems, max_m, max_n, s);
+        case 1: return launch_gemm_tile<T, AC, BC, PRE, POST, 32, 32, 32, G4>(d_items, nitems, max_m, max_n, s);
+        case 2: return launch_gemm_tile<T, AC, BC, PRE, POST, 32, 32, 64, G4>(d_items, nitems, max_m, max_n, s);
+        case 3: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 32, G4>(d_items, nitems, max_m, max_n, s);
+        case 4: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 96, 32, G4>(d_items, nitems, max_m, max_n, s);
+        case 5: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 96, 16, G4>(d_items, nitems, max_m, max_n, s);
+        case 18: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 32, 32, G4>(d_items, nitems, max_m, max_n, s);
+        case 19: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 32, 16, G4>(d_items, nitems, max_m, max_n, s);
+        default: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 16, G4>(d_items, nitems, max_m, max_n, s);
         }
     }
 }

@@ -42,7 +42,8 @@ struct Builder {
     static double flops_of(K, const std::vector<It>&) { return 0.0; }
     template <typename T>
     static double flops_of(K kind, const std::vector<GemmItem<T>>& items) {
-        const double reals = (kind == K::GEMM64_CC || kind == K::GEMM64_CC_SUB || kind == K::TRSM) ? 4.0 : 2.0;
+        const double reals = (kind == K::GEMM64_CC || kind == K::GEMM64_CC_SUB || kind == K::TRSM ||
+                              kind == K::GEMM64_CC4 || kind == K::GEMM64_CC4_SUB) ? 4.0 : 2.0;
         double f = 0;
         for (const auto& g : items) f += reals * 2.0 * g.m * (double)g.n * g.k;
         return f;
@@ -326,7 +327,7 @@ void Plan::run(OpCache& op, const double* phi) {
 }
 
 void Plan::set_tile(int kind, int tile) {
-    if (kind < 0 || kind > (int)K::LU_SHAPE_DEFER) throw std::invalid_argument("unknown step kind");
+    if (kind < 0 || kind >= (int)K::KIND_END) throw std::invalid_argument("unknown step kind");
     if (tile < -1 || tile >= N_TILES) throw std::invalid_argument("tile out of range");
     if (tile >= 0)
         forced_[kind] = tile;
@@ -350,7 +351,8 @@ void Plan::set_tile(int kind, int tile) {
 static bool is_gemm(K k) {
     return k == K::GEMM64_RC_PRE || k == K::GEMM64_RC_ROW || k == K::GEMM64_CR_PRE || k == K::GEMM64_CC ||
            k == K::GEMM32_RC_ROW || k == K::GEMM32_RC_PRE || k == K::GEMM64_CC_SUB || k == K::GEMM32_RC_PRE_DIAG ||
-           k == K::TRSM || k == K::GEMM32_RC_DIAG || k == K::GEMM32_RCK_DIAG;
+           k == K::TRSM || k == K::GEMM32_RC_DIAG || k == K::GEMM32_RCK_DIAG || k == K::GEMM64_CC4 ||
+           k == K::GEMM64_CC4_SUB;
 }
 
 void Plan::tune(OpCache& op) {
@@ -502,6 +504,14 @@ bool Plan::launch_step(OpCache& op, const Step& st, const double* phi) {
         case K::GEMM64_CC_SUB:
             if (!launch_gemm<double, true, true, false, SUBTRACT>((const GemmItem<double>*)st.items, st.count,
                                                                st.max_m, st.max_n, s, st.tile)) return false;
+            break;
+        case K::GEMM64_CC4:
+            if (!launch_gemm<double, true, true, false, SCALE_NONE, true>((const GemmItem<double>*)st.items, st.count,
+                                                                       st.max_m, st.max_n, s, st.tile)) return false;
+            break;
+        case K::GEMM64_CC4_SUB:
+            if (!launch_gemm<double, true, true, false, SUBTRACT, true>((const GemmItem<double>*)st.items, st.count,
+                                                                     st.max_m, st.max_n, s, st.tile)) return false;
             break;
         case K::LU_SHAPE:
             launch_lu_shape((const LuItem*)st.items, st.count, st.max_m, st.max_n, st.flag, st.aux, 0, s);
@@ -1832,6 +1842,245 @@ void Plan::force_parts_multi(int nops, const double* const* phis, const int* nts
     }
 }
 
+// Block LU of Q = M_p M_h^T (getQLU, hubbardFermiMatrixExp.cpp:419-479, the Dia file
+// repeats it), nt >= 3, as one cached operation: the off-diagonal blocks T+(t), T-(t) for all
+// t first (a real matrix, signed, times a phase vector on its rows or columns), then the
+// recursion, sequential in t:
+//   d_0 = P;  u_0 = T-(0), l_0 = T+(1) d_0^{-1}, v_0 = T+(0), h_0 = T-(nt-1) d_0^{-1}
+//   d_i = P - l_{i-1} u_{i-1};  l_i = T+(i+1) d_i^{-1}, h_i = -h_{i-1} u_{i-1} d_i^{-1},
+//         v_i = -l_{i-1} v_{i-1}, u_i = T-(i)                                  (1 <= i <= nt-3)
+//   d_{nt-2} = P - l_{nt-3} u_{nt-3};  u_{nt-2} = T-(nt-2) - l_{nt-3} v_{nt-3},
+//         l_{nt-2} = (T+(nt-1) - h_{nt-3} u_{nt-3}) d_{nt-2}^{-1}
+//   d_{nt-1} = P - l_{nt-2} u_{nt-2} - sum_i h_i v_i
+// Each d_i^{-1} comes from one LU with partial pivoting (its log det kept on the device)
+// and its triangular solves; products are complex GEMMs tuned per shape, with the four real
+// products of the textbook form (the Gauss form's larger error in the imaginary parts,
+// carried through the recursion and the final sum, moved log det Q by ~1e-8 relative at
+// Nx=288, Nt=128, against ~3e-10 for the four-product form).  The h_i are
+// stored side by side (one nx x (nt-2)nx matrix) and the v_i stacked, so the last sum is
+// one GEMM with k = (nt-2) nx.
+OpCache& Plan::get_qlu(int nt) {
+    if (nt < 3) throw std::invalid_argument("the native QLU needs nt >= 3");
+    auto key = std::make_tuple(7, nt, 0, 0, 0, 0);
+    auto found = cache_.find(key);
+    if (found != cache_.end()) return *found->second;
+    auto op = std::make_unique<OpCache>();
+    op->nt = nt;
+    op->batch = 1;
+    op->phi_stride = 2L * nx_ * nt;
+    Builder b(*op);
+    const int n = nx_;
+    const long nn = (long)n * n;
+    const int nh = nt - 2;  // h and v blocks
+    // real factors of the off-diagonal blocks, and P, on the host
+    std::vector<double> rp(nn), rm(nn), pm(nn, 0.0);
+    if (hop_ == EXP) {
+        // T+(t) = -+ exp(kappa - mu) diag(e^{i phi_{t-1}}),  T-(t) = -+ diag(e^{-i phi_t}) exp(sk kappa + mu)
+        ISLE_HIP(hipMemcpy(rp.data(), e64_[PARTICLE]->p, nn * 8, hipMemcpyDeviceToHost));
+        ISLE_HIP(hipMemcpy(rm.data(), e64_[HOLE]->p, nn * 8, hipMemcpyDeviceToHost));
+        if (sk_ == -1) {
+            for (int i = 0; i < n; ++i) pm[(long)i * n + i] = 2.0;
+        } else {
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < n; ++k) {
+                    const double a = rp[(long)i * n + k];
+                    for (int j = 0; j < n; ++j) pm[(long)i * n + j] += a * rm[(long)k * n + j];
+                }
+            for (int i = 0; i < n; ++i) pm[(long)i * n + i] += 1.0;
+        }
+    } else {
+        // T+(t) = +- diag(e^{i phi_{t-1}}) (sk kappa - (1 - mu)),  T-(t) = +- (kappa - (1 + mu)) diag(e^{-i phi_t})
+        for (long e = 0; e < nn; ++e) {
+            rp[e] = sk_ * kappa_[e];
+            rm[e] = kappa_[e];
+        }
+        for (int i = 0; i < n; ++i) {
+            rp[(long)i * n + i] -= 1.0 - mu_;
+            rm[(long)i * n + i] -= 1.0 + mu_;
+        }
+        // P = (2 - mu^2) - (sk (1 + mu) + 1 - mu) kappa + sk kappa^2
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < n; ++k) {
+                const double a = kappa_[(long)i * n + k];
+                for (int j = 0; j < n; ++j) pm[(long)i * n + j] += sk_ * a * kappa_[(long)k * n + j];
+            }
+        for (long e = 0; e < nn; ++e) pm[e] -= (sk_ * (1 + mu_) + 1 - mu_) * kappa_[e];
+        for (int i = 0; i < n; ++i) pm[(long)i * n + i] += 2 - mu_ * mu_;
+    }
+    const bool expo = hop_ == EXP;
+    double* R = b.d64(4 * nn);  // rp, -rp, rm, -rm
+    {
+        std::vector<double> host(4 * nn);
+        for (long e = 0; e < nn; ++e) {
+            host[e] = rp[e];
+            host[nn + e] = -rp[e];
+            host[2 * nn + e] = rm[e];
+            host[3 * nn + e] = -rm[e];
+        }
+        ISLE_HIP(hipMemcpy(R, host.data(), 4 * nn * 8, hipMemcpyHostToDevice));
+    }
+    double* Pb = b.d64(2 * nn);  // P (imaginary part zero)
+    {
+        std::vector<double> host(2 * nn, 0.0);
+        std::copy(pm.begin(), pm.end(), host.begin());
+        ISLE_HIP(hipMemcpy(Pb, host.data(), 2 * nn * 8, hipMemcpyHostToDevice));
+    }
+    double* ph = b.d64(2 * 2 * (size_t)nt * n);  // phases e^{+i phi_{t-1}} and e^{-i phi_{t-1}}
+    auto D = [&](int sgn, int t) {
+        const double* base = ph + (sgn > 0 ? 0 : 2L * nt * n);
+        return CMat<double>{const_cast<double*>(base) + (long)t * n, const_cast<double*>(base) + (long)nt * n + (long)t * n};
+    };
+    double* Tp = b.d64((size_t)nt * 2 * nn);
+    double* Tm = b.d64((size_t)nt * 2 * nn);
+    double* Dw = b.d64(2 * nn);       // d_i, factorised in place
+    double* tmp = b.d64(2 * nn);
+    double* dinv = b.d64((size_t)nt * 2 * nn);
+    double* U = b.d64((size_t)(nt - 1) * 2 * nn);
+    double* L = b.d64((size_t)(nt - 1) * 2 * nn);
+    double* Vre = b.d64((size_t)nh * nn);
+    double* Vim = b.d64((size_t)nh * nn);
+    double* Hre = b.d64((size_t)nh * nn);  // n x (nh n), row stride nh n
+    double* Him = b.d64((size_t)nh * nn);
+    int* piv = b.di(n);
+    double* luw = b.d64(lu_work_doubles(n));
+    op->logdet = b.d64(2 * (size_t)nt);
+    op->extra = {dinv, U, L, Vre, Vim, Hre, Him};
+    const long ldh = (long)nh * n;
+    auto tp = [&](int t) { return cmat(Tp, t, nn); };
+    auto tm = [&](int t) { return cmat(Tm, t, nn); };
+    auto di = [&](int i) { return cmat(dinv, i, nn); };
+    auto uu = [&](int i) { return cmat(U, i, nn); };
+    auto ll = [&](int i) { return cmat(L, i, nn); };
+    auto vv = [&](int i) { return CMat<double>{Vre + (long)i * nn, Vim + (long)i * nn}; };
+    auto hh = [&](int i) { return CMat<double>{Hre + (long)i * n, Him + (long)i * n}; };  // row stride ldh
+    const CMat<double> dw = cmat(Dw, 0, nn), tw = cmat(tmp, 0, nn), pb = cmat(Pb, 0, nn);
+
+    std::vector<PhaseItem> phs = {{0, 0, 1.0, D(1, 0).re, D(1, 0).im, nullptr, nullptr, n, nt},
+                                  {0, 0, -1.0, D(-1, 0).re, D(-1, 0).im, nullptr, nullptr, n, nt}};
+    b.add(K::PHASE, phs);
+    {
+        std::vector<ScaleItem<double>> sc;
+        for (int t = 0; t < nt; ++t) {
+            // signs: EXP T+ is + at t = 0, T- + at t = nt-1, - elsewhere; DIA the opposite
+            const bool wrap_p = t == 0, wrap_m = t == nt - 1;
+            const bool neg_p = expo ? !wrap_p : wrap_p, neg_m = expo ? !wrap_m : wrap_m;
+            const double* ap = R + (neg_p ? nn : 0);
+            const double* am = R + 2 * nn + (neg_m ? nn : 0);
+            const CMat<double> dp = D(1, t), dm = D(-1, (t + 1) % nt);  // e^{i phi_{t-1}}, e^{-i phi_t}
+            sc.push_back({ap, dp.re, dp.im, tp(t).re, tp(t).im, n, expo ? SCALE_COL : SCALE_ROW});
+            sc.push_back({am, dm.re, dm.im, tm(t).re, tm(t).im, n, expo ? SCALE_ROW : SCALE_COL});
+        }
+        b.add(K::SCALE64, sc);
+    }
+    auto cc = [&](CMat<double> a, long lda, CMat<double> bb, long ldb, CMat<double> c, long ldc, int k) {
+        GemmItem<double> g{};
+        g.a_re = a.re;
+        g.a_im = a.im;
+        g.b_re = bb.re;
+        g.b_im = bb.im;
+        g.c_re = c.re;
+        g.c_im = c.im;
+        g.m = n;
+        g.n = n;
+        g.k = k;
+        g.lda = (int)lda;
+        g.ldb = (int)ldb;
+        g.ldc = (int)ldc;
+        return std::vector<GemmItem<double>>{g};
+    };
+    auto blk = [&](CMat<double> dst, long ldd, const CMat<double>* src, long lds, int mode) {
+        std::vector<BlockItem> v = {{dst.re, dst.im, src ? src->re : nullptr, src ? src->im : nullptr, ldd, lds, n, n,
+                                     mode}};
+        b.add(K::BLOCK, v, 0, 0, 0, (int)nn);
+    };
+    auto invert = [&](int i) {  // dinv[i] = d^{-1} and log det d_i from one LU of Dw
+        std::vector<LuItem> lu = {{dw.re, dw.im, piv, op->logdet + 2L * i, luw, n, n}};
+        b.add(K::LU, lu, n, 0, 0);
+        SolveItem si{};
+        si.lu_re = dw.re;
+        si.lu_im = dw.im;
+        si.piv = piv;
+        si.dinv = luw;
+        si.x_re = di(i).re;
+        si.x_im = di(i).im;
+        si.xsr = n;
+        si.xsc = 1;
+        si.n = n;
+        si.ld = n;
+        si.nrhs = n;
+        std::vector<SolveItem> sv = {si};
+        b.add(K::SOLVE, sv, n, n);
+    };
+    // i = 0
+    blk(dw, n, &pb, n, 2);
+    invert(0);
+    {
+        const CMat<double> t0 = tm(0), p0 = tp(0);
+        blk(uu(0), n, &t0, n, 2);
+        b.add(K::GEMM64_CC4, cc(tp(1), n, di(0), n, ll(0), n, n), n, n);
+        blk(vv(0), n, &p0, n, 2);
+        b.add(K::GEMM64_CC4, cc(tm(nt - 1), n, di(0), n, hh(0), ldh, n), n, n);
+    }
+    for (int i = 1; i <= nt - 3; ++i) {
+        blk(dw, n, &pb, n, 2);
+        b.add(K::GEMM64_CC4_SUB, cc(ll(i - 1), n, uu(i - 1), n, dw, n, n), n, n);
+        invert(i);
+        b.add(K::GEMM64_CC4, cc(tp(i + 1), n, di(i), n, ll(i), n, n), n, n);
+        b.add(K::GEMM64_CC4, cc(hh(i - 1), ldh, uu(i - 1), n, tw, n, n), n, n);
+        blk(hh(i), ldh, nullptr, 0, 0);
+        b.add(K::GEMM64_CC4_SUB, cc(tw, n, di(i), n, hh(i), ldh, n), n, n);
+        blk(vv(i), n, nullptr, 0, 0);
+        b.add(K::GEMM64_CC4_SUB, cc(ll(i - 1), n, vv(i - 1), n, vv(i), n, n), n, n);
+        const CMat<double> ti = tm(i);
+        blk(uu(i), n, &ti, n, 2);
+    }
+    // i = nt - 2
+    blk(dw, n, &pb, n, 2);
+    b.add(K::GEMM64_CC4_SUB, cc(ll(nt - 3), n, uu(nt - 3), n, dw, n, n), n, n);
+    invert(nt - 2);
+    {
+        const CMat<double> t2 = tm(nt - 2), t1 = tp(nt - 1);
+        blk(uu(nt - 2), n, &t2, n, 2);
+        b.add(K::GEMM64_CC4_SUB, cc(ll(nt - 3), n, vv(nt - 3), n, uu(nt - 2), n, n), n, n);
+        blk(tw, n, &t1, n, 2);
+        b.add(K::GEMM64_CC4_SUB, cc(hh(nt - 3), ldh, uu(nt - 3), n, tw, n, n), n, n);
+        b.add(K::GEMM64_CC4, cc(tw, n, di(nt - 2), n, ll(nt - 2), n, n), n, n);
+    }
+    // i = nt - 1
+    blk(dw, n, &pb, n, 2);
+    b.add(K::GEMM64_CC4_SUB, cc(ll(nt - 2), n, uu(nt - 2), n, dw, n, n), n, n);
+    b.add(K::GEMM64_CC4_SUB, cc(CMat<double>{Hre, Him}, ldh, CMat<double>{Vre, Vim}, n, dw, n, nh * n), n, n);
+    invert(nt - 1);
+    b.finish();
+    tune(*op);
+    auto& ref = *op;
+    cache_[key] = std::move(op);
+    return ref;
+}
+
+void Plan::qlu(const double* phi, int nt, double* dinv, double* u, double* l, double* v, double* h,
+               double* logdet_host) {
+    OpCache& op = get_qlu(nt);
+    run(op, phi);
+    const int n = nx_;
+    const long nn = (long)n * n;
+    hipStream_t s = stream_;
+    auto out = [&](const double* re, const double* im, long boff, long ld, int nblk, double* dst) {
+        if (nblk <= 0) return;
+        hipLaunchKernelGGL(blocks_to_interleaved_kernel, dim3(std::min<long>((nn + 255) / 256, 1024), nblk), dim3(256),
+                           0, s, re, im, boff, ld, n, dst);
+        ISLE_HIP(hipGetLastError());
+    };
+    const auto& e = op.extra;  // dinv, U, L, Vre, Vim, Hre, Him
+    out(e[0], e[0] + nn, 2 * nn, n, nt, dinv);
+    out(e[1], e[1] + nn, 2 * nn, n, nt - 1, u);
+    out(e[2], e[2] + nn, 2 * nn, n, nt - 1, l);
+    out(e[3], e[4], nn, n, nt - 2, v);
+    out(e[5], e[6], n, (long)(nt - 2) * n, nt - 2, h);
+    ISLE_HIP(hipMemcpyAsync(logdet_host, op.logdet, 2 * sizeof(double) * nt, hipMemcpyDeviceToHost, s));
+    ISLE_HIP(hipStreamSynchronize(s));
+}
+
 // M x for rows x: per time slice one complex x real GEMM over all rows with the slice
 // matrix S (EXP: exp(kappa - mu)..., DIA: K), between an elementwise prologue (the shifted,
 // phase-scaled rows) and epilogue (back to interleaved rows); see applym_prep_kernel.
@@ -2297,6 +2546,14 @@ int isle_force_parts(isle_hfm* h, const double* phi, int nt, int batch, int mode
         if (batch < 1 || (nspecies != 1 && nspecies != 2)) throw std::invalid_argument("bad batch/nspecies");
         if (mode < 0 || mode > 3) throw std::invalid_argument("mode must be 0, 1, 2 or 3");
         P(h)->force_parts(phi, nt, batch, mode, group, nspecies, spin, out_dev);
+    });
+}
+
+int isle_qlu(isle_hfm* h, const double* phi, int nt, double* dinv, double* u, double* l, double* v, double* hh,
+             double* logdet_host) {
+    return guarded([&] {
+        if (nt < 3) throw std::invalid_argument("isle_qlu needs nt >= 3");
+        P(h)->qlu(phi, nt, dinv, u, l, v, hh, logdet_host);
     });
 }
 

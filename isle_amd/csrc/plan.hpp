@@ -71,6 +71,9 @@ enum class K : int {
                         // transposed), force diagonal's row sums fused, rows scaled by the next phase
     COLSCALE64TO32,     // copy with conversion and complex column scaling
     LU_SHAPE_DEFER,     // LU_SHAPE whose columns beyond kf take the row interchanges only
+    GEMM64_CC4,         // complex x complex with four real products       (f64)
+    GEMM64_CC4_SUB,     // C -= A B, the same                              (f64)
+    KIND_END
 };
 
 struct Step {
@@ -91,6 +94,7 @@ struct OpCache {
     std::vector<std::unique_ptr<DevBuf>> bufs;
     std::vector<Step> steps;
     std::vector<char> blob;    // host copy of the descriptors (for merged operations)
+    std::vector<double*> extra;  // operation-specific result buffers (the QLU blocks)
     long phi_stride = 0;       // doubles between consecutive fields of the batch
     double* logdet = nullptr;  // device [batch][nspecies][2]
     double* parts = nullptr;   // device [batch][nspecies][nx*nt] interleaved
@@ -140,6 +144,11 @@ public:
     // descriptors of all of them (the phase steps, which bind the fields, stay separate).
     void force_parts_multi(int nops, const double* const* phis, const int* nts, const int* batches, int mode,
                            const int* groups, int nspecies, int spin, double* const* outs);
+    // Block LU of Q = M_p M_h^T (getQLU, nt >= 3): the stacks d_i^{-1} (nt), u_i, l_i (nt-1),
+    // v_i, h_i (nt-2) written interleaved to dinv, u, l, v, h (device [k][nx][nx] complex128)
+    // and log det d_i to logdet_host (nt x 2); one synchronisation at the end.
+    void qlu(const double* phi, int nt, double* dinv, double* u, double* l, double* v, double* h,
+             double* logdet_host);
     // reuse: phi, nt and species equal those of the previous call (its factorization is kept)
     void solveM(const double* phi, int nt, int species, const double* rhss, int nrhs, double* out,
                 bool reuse = false);
@@ -196,6 +205,7 @@ private:
     OpCache& get_logdet_grouped(int nt, int batch, int group, int nspecies, int spin);
     OpCache& get_force_stable(int nt, int batch, int group, int nspecies, int spin, bool right32 = false);
     OpCache& get_logdet_stable(int nt, int batch, int group, int nspecies, int spin);
+    OpCache& get_qlu(int nt);
     void run(OpCache& op, const double* phi);
     bool launch_step(OpCache& op, const Step& st, const double* phi);
     // Autotuner: time every tile shape of each GEMM step once per (kind, m, n, items) and

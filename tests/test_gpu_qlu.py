@@ -12,7 +12,8 @@ log det and solve).
   the reference test (cond Q <= 2e3: the oracle's exp(kappa) by eigh and the GPU's by a
   Taylor series differ by rounding, which cond Q amplifies; at beta = 3, Nt = 1 it is 7e7);
 * at Nx=72, Nt=64 (honeycomb 6x6, beta=6), beyond a dense check, log det Q equals
-  log det M_p + log det M_h (det Q = det M_p det M_h) from the GPU logdetM to 1e-9;
+  log det M_p + log det M_h (det Q = det M_p det M_h) from the GPU logdetM to 1e-9, and at
+  Nx=288, Nt=128 (the config-3 torus) to 2e-9;
 * DIRECT_SQUARE beyond Nx*Nt = 512: at 576 its action (through logdetQ) and its force
   (dense Q^{-1} on the tall-panel LU) equal DIRECT_SINGLE's.
 """
@@ -102,6 +103,21 @@ def test_logdetQ_is_sum_of_logdetM():
         want = isle.logdetM(hfm, phi, isle.Species.PARTICLE) + isle.logdetM(hfm, phi, isle.Species.HOLE)
         got = isle.logdetQ(hfm, phi)
         assert mod2pi_diff(got, want) < 1e-9 * abs(want), cls.__name__
+
+
+def test_logdetQ_config3_size():
+    # Nx=288 (honeycomb 12x12), Nt=128, beta=8, U=3: the recursion runs 128 sequential block
+    # inversions and its rounding grows with them; the oracle's own qlu differs from
+    # log det M_p + log det M_h by 2e-10 relative on this field and the GPU's (four-product
+    # block GEMMs) by ~4e-10.  A three-product (Gauss) complex GEMM would give ~1e-8 here.
+    print("logdetQ vs logdetM, Nx=288 Nt=128", flush=True)
+    nt, beta, U = 128, 8.0, 3.0
+    kappa = ref.honeycomb(12) * beta / nt
+    phi = np.random.RandomState(42).normal(0, np.sqrt(U * beta / nt), 288 * nt) + 0j
+    hfm = isle.HubbardFermiMatrixExp(kappa, 0.0, -1)
+    want = isle.logdetM(hfm, phi, isle.Species.PARTICLE) + isle.logdetM(hfm, phi, isle.Species.HOLE)
+    got = isle.logdetQ(hfm, phi)
+    assert mod2pi_diff(got, want) < 2e-9 * abs(want), (got, want)
 
 
 def test_direct_square_beyond_512():
