@@ -290,15 +290,10 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
             // pivot position; an exactly zero column (no nonzero key) keeps row kk in place
             const unsigned long long kmax = pkey[kk];
             const int p = kmax ? 511 - static_cast<int>(kmax & 0x1FFull) : kk;
-            if (own && pos == p) {
-                // slots NB-kk.. of the pivot row hold its own earlier multipliers: broadcast
-                // zeros there, so that the update below subtracts nothing there
+            if (own && pos == p) {  // the raw register row (see the update for slots NB-kk..)
                 double2* dst = reinterpret_cast<double2*>(xrow);
 #pragma unroll
-                for (int j = 0; j < NB; ++j) {
-                    const bool u = j < NB - kk;
-                    dst[j] = make_double2(u ? ar[j] : 0.0, u ? ai[j] : 0.0);
-                }
+                for (int j = 0; j < NB; ++j) dst[j] = make_double2(ar[j], ai[j]);
             }
             if (own) {  // the interchange of positions kk and p
                 if (pos == p)
@@ -319,21 +314,35 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
                     ni = nr * ii + ni * ir;
                     nr = t;
                 }
-                // the pivot row in batches of 8 entries (a batch's reads in flight together);
-                // batches from NB-kk on hold only broadcast zeros and are skipped
+                // the pivot row in batches of 8 entries (a batch's reads in flight together).
+                // Slots NB-kk.. of the pivot row hold its own earlier multipliers: batches from
+                // there on are skipped and the one batch that straddles NB-kk is masked (the
+                // owner writes its row unmasked, a single lane's stores on the critical path)
                 constexpr int RB = 8;
+                const int live = NB - kk;
 #pragma unroll
                 for (int j0 = 0; j0 < NB; j0 += RB) {
-                    if (j0 < NB - kk) {
+                    if (j0 < live) {
                         double2 bv[RB];
 #pragma unroll
                         for (int q = 0; q < RB; ++q) bv[q] = xv[j0 + q];
+                        if (j0 + RB <= live) {
 #pragma unroll
-                        for (int q = 0; q < RB; ++q) {
-                            const int j = j0 + q;
-                            if (j >= 1) {
-                                ar[j] -= nr * bv[q].x - ni * bv[q].y;
-                                ai[j] -= nr * bv[q].y + ni * bv[q].x;
+                            for (int q = 0; q < RB; ++q) {
+                                const int j = j0 + q;
+                                if (j >= 1) {
+                                    ar[j] -= nr * bv[q].x - ni * bv[q].y;
+                                    ai[j] -= nr * bv[q].y + ni * bv[q].x;
+                                }
+                            }
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < RB; ++q) {
+                                const int j = j0 + q;
+                                if (j >= 1 && j < live) {
+                                    ar[j] -= nr * bv[q].x - ni * bv[q].y;
+                                    ai[j] -= nr * bv[q].y + ni * bv[q].x;
+                                }
                             }
                         }
                     }
@@ -376,17 +385,17 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
             lt_re[r][c] = (in && c < r) ? ar[c] : (c == r ? 1.0 : 0.0);
             lt_im[r][c] = (in && c < r) ? ai[c] : 0.0;
             const bool u = in && c >= r && c < kb;
-            double vr = u ? ar[c] : (c == r ? 1.0 : 0.0), vi = u ? ai[c] : 0.0;
-            if (c == r) {  // the diagonal of U11^{-1} is 1/u_ii
-                const double den = vr * vr + vi * vi;
-                vr = vr / den;
-                vi = -vi / den;
-            }
-            ut_re[r][c] = vr;
-            ut_im[r][c] = vi;
+            ut_re[r][c] = u ? ar[c] : (c == r ? 1.0 : 0.0);
+            ut_im[r][c] = u ? ai[c] : 0.0;
         }
     }
     __syncthreads();
+    if (tid < NB) {  // the diagonal of U11^{-1} is 1/u_ii (one division per row, not per column)
+        const double vr = ut_re[tid][tid], vi = ut_im[tid][tid];
+        const double den = vr * vr + vi * vi;
+        ut_re[tid][tid] = vr / den;
+        ut_im[tid][tid] = -vi / den;
+    }
     const int cnt = min(slot_cnt, 2 * NB);
     int* perm = lu_perm(it);
     for (int e = tid; e < cnt; e += nth) {
@@ -394,6 +403,7 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
         perm[1 + 2 * NB + e] = slot_src[e];
     }
     if (tid == 0) perm[0] = cnt;
+    __syncthreads();
 
     lu_invert_panel_blocks<NB>(it, panel, lt_re, lt_im, ut_re, ut_im, tl_re, tl_im, tu_re, tu_im);
     lu_panel_logdet(it, k0, NB, kf, rsum);
@@ -795,7 +805,8 @@ inline void launch_lu_shape(const LuItem* d_items, int nitems, int m, int ncols,
             hipLaunchKernelGGL(lu_panel_tall_kernel<LU_NB>, dim3(nitems), dim3(LU_TALL_THREADS), 0, s, d_items, k0,
                                k0 == 0 ? add_identity : 0);
         } else {
-            const int threads = ((rows + 63) / 64) * 64;
+            // at least 4 waves: the L11 / U11 inversion after the column loop uses every thread
+            const int threads = std::max(256, ((rows + 63) / 64) * 64);
             hipLaunchKernelGGL(lu_panel_kernel<LU_NB>, dim3(nitems), dim3(threads), 0, s, d_items, k0,
                                k0 == 0 ? add_identity : 0);
         }
