@@ -318,6 +318,8 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
                 // Slots NB-kk.. of the pivot row hold its own earlier multipliers: batches from
                 // there on are skipped and the one batch that straddles NB-kk is masked (the
                 // owner writes its row unmasked, a single lane's stores on the critical path)
+                // Two fused multiply-adds per real part: a_j - m b_j = (a_j - m_r b_r) + m_i b_i
+                // (and likewise for the imaginary part).
                 constexpr int RB = 8;
                 const int live = NB - kk;
 #pragma unroll
@@ -331,8 +333,8 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
                             for (int q = 0; q < RB; ++q) {
                                 const int j = j0 + q;
                                 if (j >= 1) {
-                                    ar[j] -= nr * bv[q].x - ni * bv[q].y;
-                                    ai[j] -= nr * bv[q].y + ni * bv[q].x;
+                                    ar[j] = fma(ni, bv[q].y, fma(-nr, bv[q].x, ar[j]));
+                                    ai[j] = fma(-ni, bv[q].x, fma(-nr, bv[q].y, ai[j]));
                                 }
                             }
                         } else {
@@ -340,8 +342,8 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
                             for (int q = 0; q < RB; ++q) {
                                 const int j = j0 + q;
                                 if (j >= 1 && j < live) {
-                                    ar[j] -= nr * bv[q].x - ni * bv[q].y;
-                                    ai[j] -= nr * bv[q].y + ni * bv[q].x;
+                                    ar[j] = fma(ni, bv[q].y, fma(-nr, bv[q].x, ar[j]));
+                                    ai[j] = fma(-ni, bv[q].x, fma(-nr, bv[q].y, ai[j]));
                                 }
                             }
                         }
