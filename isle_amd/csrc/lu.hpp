@@ -272,92 +272,111 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
     __syncthreads();
     // Column loop.  Rows never move between threads: thread t keeps original panel row t in
     // registers and `pos` is its current position (LAPACK's interchange of rows KK and p is
-    // the exchange of two positions).  The register row is rotated by one slot per column:
-    // slot 0 always holds column KK, and the slot freed at the end takes the multiplier
-    // (rows still to be pivoted) or the departing column (rows already pivoted), so after
-    // NB rotations every row is back in column order, L multipliers then U entries.  This
-    // keeps all register indices static inside a real loop (a fully unrolled column loop
-    // does not fit the instruction cache).
-    for (int kk = 0; kk < NB; ++kk) {
-        double nr = ar[0], ni = ai[0];  // enters the last slot: the multiplier or the departing value
-        if (kk < kb) {
-            {  // max over the wave first (lanes to one LDS address serialise), then one atomic per wave
-                const unsigned long long key =
-                    wave_max_u64((own && pos >= kk) ? pivot_key(cabs1(ar[0], ai[0]), pos) : 0ull);
-                if ((tid & 63) == 0 && key != 0ull) atomicMax(&pkey[kk], key);
-            }
-            __syncthreads();
-            // pivot position; an exactly zero column (no nonzero key) keeps row kk in place
-            const unsigned long long kmax = pkey[kk];
-            const int p = kmax ? 511 - static_cast<int>(kmax & 0x1FFull) : kk;
-            if (own && pos == p) {  // the raw register row (see the update for slots NB-kk..)
-                double2* dst = reinterpret_cast<double2*>(xrow);
+    // the exchange of two positions).  Columns go in groups of LU_CG: in the group starting at
+    // column K2 slot Q (0 .. LU_CG-1) holds column K2 + Q, and after the group the register
+    // row is rotated by LU_CG slots, the freed slots taking their columns' multipliers (rows
+    // still to be pivoted) or departing values (rows already pivoted).  After NB / LU_CG
+    // rotations every row is back in column order, L multipliers then U entries.  This keeps
+    // all register indices static inside a real loop (a fully unrolled column loop does not
+    // fit the instruction cache) at 1 / LU_CG of the register moves of a rotation per column.
+    constexpr int LU_CG = 8;
+    static_assert(NB % LU_CG == 0, "columns go in groups");
+    for (int k2 = 0; k2 < NB; k2 += LU_CG) {
+        const int live = NB - k2;  // slots Q+1 .. live-1 hold the columns right of column K2 + Q
 #pragma unroll
-                for (int j = 0; j < NB; ++j) dst[j] = make_double2(ar[j], ai[j]);
-            }
-            if (own) {  // the interchange of positions kk and p
-                if (pos == p)
-                    pos = kk;
-                else if (pos == kk)
-                    pos = p;
-            }
-            if (tid == 0) pvt[kk] = p;  // global stores here would stall every LDS wait (FLAT)
-            __syncthreads();
-            if (own && pos > kk) {  // rows not yet pivoted: multiplier and rank-1 update
-                const double2* xv = reinterpret_cast<const double2*>(xrow);
-                const double2 u0 = xv[0];
-                const double ur = u0.x, ui = u0.y;
-                if (ur != 0.0 || ui != 0.0) {
-                    const double rden = 1.0 / (ur * ur + ui * ui);
-                    const double ir = ur * rden, ii = -ui * rden;
-                    const double t = nr * ir - ni * ii;
-                    ni = nr * ii + ni * ir;
-                    nr = t;
+        for (int Q = 0; Q < LU_CG; ++Q) {
+            const int kk = k2 + Q;
+            double nr = ar[Q], ni = ai[Q];  // stays in slot Q: the multiplier or the departing value
+            if (kk < kb) {
+                {  // max over the wave first (lanes to one LDS address serialise), then one atomic per wave
+                    const unsigned long long key =
+                        wave_max_u64((own && pos >= kk) ? pivot_key(cabs1(ar[Q], ai[Q]), pos) : 0ull);
+                    if ((tid & 63) == 0 && key != 0ull) atomicMax(&pkey[kk], key);
                 }
-                // the pivot row in batches of 8 entries (a batch's reads in flight together).
-                // Slots NB-kk.. of the pivot row hold its own earlier multipliers: batches from
-                // there on are skipped and the one batch that straddles NB-kk is masked (the
-                // owner writes its row unmasked, a single lane's stores on the critical path)
-                // Two fused multiply-adds per real part: a_j - m b_j = (a_j - m_r b_r) + m_i b_i
-                // (and likewise for the imaginary part).
-                constexpr int RB = 8;
-                const int live = NB - kk;
+                __syncthreads();
+                // pivot position; an exactly zero column (no nonzero key) keeps row kk in place
+                const unsigned long long kmax = pkey[kk];
+                const int p = kmax ? 511 - static_cast<int>(kmax & 0x1FFull) : kk;
+                if (own && pos == p) {  // the raw register row (see the update for the other slots)
+                    double2* dst = reinterpret_cast<double2*>(xrow);
 #pragma unroll
-                for (int j0 = 0; j0 < NB; j0 += RB) {
-                    if (j0 < live) {
-                        double2 bv[RB];
+                    for (int j = 0; j < NB; ++j) dst[j] = make_double2(ar[j], ai[j]);
+                }
+                if (own) {  // the interchange of positions kk and p
+                    if (pos == p)
+                        pos = kk;
+                    else if (pos == kk)
+                        pos = p;
+                }
+                if (tid == 0) pvt[kk] = p;  // global stores here would stall every LDS wait (FLAT)
+                __syncthreads();
+                if (own && pos > kk) {  // rows not yet pivoted: multiplier and rank-1 update
+                    const double2* xv = reinterpret_cast<const double2*>(xrow);
+                    const double2 u0 = xv[Q];
+                    const double ur = u0.x, ui = u0.y;
+                    if (ur != 0.0 || ui != 0.0) {
+                        const double rden = 1.0 / (ur * ur + ui * ui);
+                        const double ir = ur * rden, ii = -ui * rden;
+                        const double t = nr * ir - ni * ii;
+                        ni = nr * ii + ni * ir;
+                        nr = t;
+                    }
+                    // the pivot row in batches of 8 entries (a batch's reads in flight together).
+                    // Slots 0..Q and live.. of the pivot row hold its own multipliers (or, for
+                    // slots below Q, entries of columns already done): batches
+                    // from live on are skipped and the one batch that straddles live is masked
+                    // (the owner writes its row unmasked, a single lane's stores on the critical
+                    // path).  Two fused multiply-adds per real part:
+                    // a_j - m b_j = (a_j - m_r b_r) + m_i b_i, likewise for the imaginary part.
+                    constexpr int RB = 8;
 #pragma unroll
-                        for (int q = 0; q < RB; ++q) bv[q] = xv[j0 + q];
-                        if (j0 + RB <= live) {
+                    for (int j0 = 0; j0 < NB; j0 += RB) {
+                        if (j0 < live) {
+                            double2 bv[RB];
 #pragma unroll
-                            for (int q = 0; q < RB; ++q) {
-                                const int j = j0 + q;
-                                if (j >= 1) {
-                                    ar[j] = fma(ni, bv[q].y, fma(-nr, bv[q].x, ar[j]));
-                                    ai[j] = fma(-ni, bv[q].x, fma(-nr, bv[q].y, ai[j]));
+                            for (int q = 0; q < RB; ++q) bv[q] = xv[j0 + q];
+                            if (j0 + RB <= live) {
+#pragma unroll
+                                for (int q = 0; q < RB; ++q) {
+                                    const int j = j0 + q;
+                                    if (j > Q) {
+                                        ar[j] = fma(ni, bv[q].y, fma(-nr, bv[q].x, ar[j]));
+                                        ai[j] = fma(-ni, bv[q].x, fma(-nr, bv[q].y, ai[j]));
+                                    }
                                 }
-                            }
-                        } else {
+                            } else {
 #pragma unroll
-                            for (int q = 0; q < RB; ++q) {
-                                const int j = j0 + q;
-                                if (j >= 1 && j < live) {
-                                    ar[j] = fma(ni, bv[q].y, fma(-nr, bv[q].x, ar[j]));
-                                    ai[j] = fma(-ni, bv[q].x, fma(-nr, bv[q].y, ai[j]));
+                                for (int q = 0; q < RB; ++q) {
+                                    const int j = j0 + q;
+                                    if (j > Q && j < live) {
+                                        ar[j] = fma(ni, bv[q].y, fma(-nr, bv[q].x, ar[j]));
+                                        ai[j] = fma(-ni, bv[q].x, fma(-nr, bv[q].y, ai[j]));
+                                    }
                                 }
                             }
                         }
                     }
                 }
             }
+            ar[Q] = nr;
+            ai[Q] = ni;
+        }
+        double hr[LU_CG], hi[LU_CG];
+#pragma unroll
+        for (int q = 0; q < LU_CG; ++q) {
+            hr[q] = ar[q];
+            hi[q] = ai[q];
         }
 #pragma unroll
-        for (int j = 0; j < NB - 1; ++j) {
-            ar[j] = ar[j + 1];
-            ai[j] = ai[j + 1];
+        for (int j = 0; j < NB - LU_CG; ++j) {
+            ar[j] = ar[j + LU_CG];
+            ai[j] = ai[j + LU_CG];
         }
-        ar[NB - 1] = nr;
-        ai[NB - 1] = ni;
+#pragma unroll
+        for (int q = 0; q < LU_CG; ++q) {
+            ar[NB - LU_CG + q] = hr[q];
+            ai[NB - LU_CG + q] = hi[q];
+        }
     }
     if (tid < kb) it.piv[k0 + tid] = lu_row(k0, ntop, pvt[tid]);
     if (own) {
