@@ -19,6 +19,11 @@ import time
 import numpy as np
 
 METRIC = "HMC traj/s at Nx=288 Nt=128: batched FP32-force vs plain FP64; force rel err"
+
+
+def progress(msg):
+    """One line on stderr as each part of the default bench ends (stdout carries the JSON)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 L, NT, BETA, U, NSTEP, LENGTH = 12, 128, 8.0, 3.0, 40, 1.0
 THERM_LENGTH, THERM_NSTEP = 0.25, 100  # thermalisation trajectories: step 0.0025 instead of 0.025
 REV_CHAINS = 8
@@ -555,6 +560,7 @@ def main():
     t_th = time.perf_counter() - t_th
     start_phis, start_acts, start_rngs = phis.clone(), acts.copy(), clone_rngs(rngs)
 
+    progress("thermalisation done")
     # ---------------- force error of the mixed form against plain FP64 on the thermalised fields
     act_p, ferm_p = make("fp64")
     f_m = ferm_m.force_batch(phis).cpu().numpy()
@@ -564,6 +570,7 @@ def main():
     force_err_max = float((np.abs(f_m - f_p).max(axis=1) / np.abs(f_p).max(axis=1)).max())
     del f_m, f_p
 
+    progress("force error done")
     # ---------------- batched mixed precision: the timed trajectories
     ev = BatchedConstStepLeapfrog(act_m, LENGTH, NSTEP, rngs, graph=not args.no_graph, split=split)
     phis, acts = run(ev, phis, acts, args.warmup)
@@ -575,6 +582,7 @@ def main():
     traj_per_s = B * args.steps / t_mixed
     mixed_stats = stats(ev, 0)
 
+    progress("timed mixed trajectories done")
     # ---------------- batched plain FP64 from the same thermalised fields and generator states
     plain_batched, plain_stats = None, {}
     if not args.no_plain:
@@ -588,6 +596,7 @@ def main():
         plain_stats = stats(ev_pb, 0)
         del ev_pb, pb_phis
 
+    progress("batched plain FP64 trajectories done")
     # ---------------- reversibility: forward then reversed trajectory on thermalised fields
     #                  (the first REV chains), mixed and plain FP64 forces, the same momenta
     rev_n = min(REV_CHAINS, B)
@@ -620,6 +629,7 @@ def main():
                                                                    - rev["fp64_small_step"]["dH"]).max())},
     }
 
+    progress("reversibility done")
     # ---------------- kernel roofline (mixed force of the whole chain batch)
     nk = len(KIND_NAMES)
     prof = np.zeros(4 * nk)
@@ -640,6 +650,7 @@ def main():
                 "force_ms": float(prof[:, 1].sum()),
                 "breakdown_ms": {KIND_NAMES[k]: round(float(prof[k, 1]), 4) for k in range(nk) if prof[k, 0] > 0}}
 
+    progress("roofline done")
     # ---------------- plain FP64, one chain, unbatched (from chain 0's thermalised field)
     plain = None
     if not args.no_plain:
