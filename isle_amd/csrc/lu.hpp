@@ -624,15 +624,17 @@ __global__ __launch_bounds__(256) void lu_rswap_kernel(const LuItem* __restrict_
     const int c = c0 + (tid % CW), part = tid / CW;
     const bool ok = c < nrt;
     const long col = k0 + NB + c;
+    const int prow = min(NB, lu_rows(k0, lu_m(it), ntop));  // positions that hold rows of the matrix
     double* st_re = lu_stage(it);
     double* st_im = st_re + (size_t)NB * ncols;
     {
         double vr[QP], vi[QP];
 #pragma unroll
         for (int j = 0; j < QP; ++j) {
-            const long o = (long)lu_row(k0, ntop, orig[part + PARTS * j]) * ld + col;
-            vr[j] = ok ? re[o] : 0.0;
-            vi[j] = ok ? im[o] : 0.0;
+            const bool in = ok && part + PARTS * j < prow;
+            const long o = in ? (long)lu_row(k0, ntop, orig[part + PARTS * j]) * ld + col : 0;
+            vr[j] = in ? re[o] : 0.0;
+            vi[j] = in ? im[o] : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < QP; ++j)
@@ -670,8 +672,10 @@ __global__ __launch_bounds__(256) void lu_rswap_kernel(const LuItem* __restrict_
 #pragma unroll
             for (int j = 0; j < QP; ++j) {
                 const int q = part + PARTS * j;
-                re[(long)(k0 + q) * ld + col] = st_re[(long)q * ncols + c];
-                im[(long)(k0 + q) * ld + col] = st_im[(long)q * ncols + c];
+                if (q < prow) {
+                    re[(long)(k0 + q) * ld + col] = st_re[(long)q * ncols + c];
+                    im[(long)(k0 + q) * ld + col] = st_im[(long)q * ncols + c];
+                }
             }
         }
     }
@@ -721,7 +725,9 @@ __global__ __launch_bounds__(LU_UPD_THREADS) void lu_update_kernel(const LuItem*
     // ---- trailing tile (I, J) of A22 = panel positions NB.. x columns c0.. of the matrix
     const int c0 = k0 + NB, ntr = lu_rows(k0, m, ntop) - NB, ntc = lu_nupd(it) - c0;
     const int I = blockIdx.x / tmc, J = blockIdx.x % tmc;
-    if (I * T >= ntr || J * T >= ntc) return;
+    // tile row 0 always runs when there are columns to update: it writes U12, also for a
+    // last panel with no rows below it (ntr <= 0: a matrix with columns beyond its kf)
+    if ((I > 0 && I * T >= ntr) || J * T >= ntc) return;
     __shared__ double sa_re[T][T + 1], sa_im[T][T + 1];  // L11^{-1}
     __shared__ double sl_re[T][T + 1], sl_im[T][T + 1];  // L21 tile
     __shared__ double sb_re[T][T + 1], sb_im[T][T + 1];  // staged A12 tile, then U12 tile
@@ -777,9 +783,10 @@ __global__ __launch_bounds__(LU_UPD_THREADS) void lu_update_kernel(const LuItem*
     }
     __syncthreads();
     if (I == 0) {  // U12 into rows k0.. of the matrix (read by no other workgroup of this launch)
+        const int prow = NB + min(ntr, 0);  // the panel's rows (fewer than NB in a short last panel)
         for (int e = tid; e < T * T; e += LU_UPD_THREADS) {
             const int r = e / T, c = e % T, col = J * T + c;
-            if (col < ntc) {
+            if (col < ntc && r < prow) {
                 re[(long)(k0 + r) * ld + c0 + col] = sb_re[r][c];
                 im[(long)(k0 + r) * ld + c0 + col] = sb_im[r][c];
             }
@@ -838,8 +845,8 @@ inline void launch_lu_shape(const LuItem* d_items, int nitems, int m, int ncols,
             hipLaunchKernelGGL(lu_rswap_kernel, dim3((nsw + 63) / 64, nitems), dim3(256), 0, s, d_items, k0);
             ISLE_HIP(hipGetLastError());
         }
-        const int tmr = (ntr > 0 && ntc > 0) ? (ntr + LU_TILE - 1) / LU_TILE : 0;
-        const int tmc = (ntr > 0 && ntc > 0) ? (ntc + LU_TILE - 1) / LU_TILE : 0;
+        const int tmr = ntc > 0 ? std::max(1, (ntr + LU_TILE - 1) / LU_TILE) : 0;
+        const int tmc = ntc > 0 ? (ntc + LU_TILE - 1) / LU_TILE : 0;
         const int swap_wgs = (k0 + LU_SWAP_COLS - 1) / LU_SWAP_COLS;
         if (tmr * tmc + swap_wgs > 0) {
             hipLaunchKernelGGL(lu_update_kernel, dim3(tmr * tmc + swap_wgs, nitems), dim3(LU_UPD_THREADS), 0, s,

@@ -1932,7 +1932,11 @@ OpCache& Plan::get_qlu(int nt) {
     };
     double* Tp = b.d64((size_t)nt * 2 * nn);
     double* Tm = b.d64((size_t)nt * 2 * nn);
-    double* Dw = b.d64(2 * nn);       // d_i, factorised in place
+    // With n % 32 == 0: [d_i | 1] (n x 2n), whose LU over the first n columns leaves L^{-1} P
+    // in the right block; otherwise d_i alone and both triangular solves
+    const bool aug = n % LU_NB == 0;
+    const long ldd = aug ? 2L * n : (long)n;
+    double* Dw = b.d64(2 * (size_t)n * ldd);  // factorised in place
     double* tmp = b.d64(2 * nn);
     double* dinv = b.d64((size_t)nt * 2 * nn);
     double* U = b.d64((size_t)(nt - 1) * 2 * nn);
@@ -1942,7 +1946,7 @@ OpCache& Plan::get_qlu(int nt) {
     double* Hre = b.d64((size_t)nh * nn);  // n x (nh n), row stride nh n
     double* Him = b.d64((size_t)nh * nn);
     int* piv = b.di(n);
-    double* luw = b.d64(lu_work_doubles(n));
+    double* luw = b.d64(lu_work_doubles(n, (int)ldd));
     op->logdet = b.d64(2 * (size_t)nt);
     op->extra = {dinv, U, L, Vre, Vim, Hre, Him};
     const long ldh = (long)nh * n;
@@ -1953,7 +1957,9 @@ OpCache& Plan::get_qlu(int nt) {
     auto ll = [&](int i) { return cmat(L, i, nn); };
     auto vv = [&](int i) { return CMat<double>{Vre + (long)i * nn, Vim + (long)i * nn}; };
     auto hh = [&](int i) { return CMat<double>{Hre + (long)i * n, Him + (long)i * n}; };  // row stride ldh
-    const CMat<double> dw = cmat(Dw, 0, nn), tw = cmat(tmp, 0, nn), pb = cmat(Pb, 0, nn);
+    const long dpl = (long)n * ldd;  // one plane of [d | 1] (or of d)
+    const CMat<double> dw{Dw, Dw + dpl}, dr{Dw + n, Dw + dpl + n};  // d and (aug) the identity block
+    const CMat<double> tw = cmat(tmp, 0, nn), pb = cmat(Pb, 0, nn);
 
     std::vector<PhaseItem> phs = {{0, 0, 1.0, D(1, 0).re, D(1, 0).im, nullptr, nullptr, n, nt},
                                   {0, 0, -1.0, D(-1, 0).re, D(-1, 0).im, nullptr, nullptr, n, nt}};
@@ -1993,26 +1999,46 @@ OpCache& Plan::get_qlu(int nt) {
                                      mode}};
         b.add(K::BLOCK, v, 0, 0, 0, (int)nn);
     };
-    auto invert = [&](int i) {  // dinv[i] = d^{-1} and log det d_i from one LU of Dw
-        std::vector<LuItem> lu = {{dw.re, dw.im, piv, op->logdet + 2L * i, luw, n, n}};
-        b.add(K::LU, lu, n, 0, 0);
+    // d_i = P and (aug) the identity beside it, in one launch
+    auto load_d = [&]() {
+        std::vector<BlockItem> v = {{dw.re, dw.im, pb.re, pb.im, ldd, n, n, n, 2}};
+        if (aug) v.push_back({dr.re, dr.im, nullptr, nullptr, ldd, 0, n, n, 1});
+        b.add(K::BLOCK, v, 0, 0, 0, (int)nn);
+    };
+    // dinv[i] = d^{-1} and log det d_i: the LU of [d | 1] over its first n columns leaves
+    // L^{-1} P in the right block (the interchanges and trailing updates of the LU apply to it
+    // on every CU), so only the backward solve with U remains (aug); else both solves
+    auto invert = [&](int i) {
+        LuItem li{dw.re, dw.im, piv, op->logdet + 2L * i, luw, n, (int)ldd};
+        li.m = n;
+        li.ncols = (int)ldd;
+        li.kf = n;
+        std::vector<LuItem> lu = {li};
+        b.add(K::LU_SHAPE, lu, n, (int)ldd, n, 0);
         SolveItem si{};
         si.lu_re = dw.re;
         si.lu_im = dw.im;
         si.piv = piv;
         si.dinv = luw;
+        if (aug) {  // X = U^{-1} (L^{-1} P), else X = A^{-1} I
+            si.b_re = dr.re;
+            si.b_im = dr.im;
+            si.bsr = ldd;
+            si.bsc = 1;
+            si.mode = SOLVE_U;
+        }
         si.x_re = di(i).re;
         si.x_im = di(i).im;
         si.xsr = n;
         si.xsc = 1;
         si.n = n;
-        si.ld = n;
+        si.ld = (int)ldd;
         si.nrhs = n;
         std::vector<SolveItem> sv = {si};
         b.add(K::SOLVE, sv, n, n);
     };
     // i = 0
-    blk(dw, n, &pb, n, 2);
+    load_d();
     invert(0);
     {
         const CMat<double> t0 = tm(0), p0 = tp(0);
@@ -2022,8 +2048,8 @@ OpCache& Plan::get_qlu(int nt) {
         b.add(K::GEMM64_CC4, cc(tm(nt - 1), n, di(0), n, hh(0), ldh, n), n, n);
     }
     for (int i = 1; i <= nt - 3; ++i) {
-        blk(dw, n, &pb, n, 2);
-        b.add(K::GEMM64_CC4_SUB, cc(ll(i - 1), n, uu(i - 1), n, dw, n, n), n, n);
+        load_d();
+        b.add(K::GEMM64_CC4_SUB, cc(ll(i - 1), n, uu(i - 1), n, dw, ldd, n), n, n);
         invert(i);
         b.add(K::GEMM64_CC4, cc(tp(i + 1), n, di(i), n, ll(i), n, n), n, n);
         b.add(K::GEMM64_CC4, cc(hh(i - 1), ldh, uu(i - 1), n, tw, n, n), n, n);
@@ -2035,8 +2061,8 @@ OpCache& Plan::get_qlu(int nt) {
         blk(uu(i), n, &ti, n, 2);
     }
     // i = nt - 2
-    blk(dw, n, &pb, n, 2);
-    b.add(K::GEMM64_CC4_SUB, cc(ll(nt - 3), n, uu(nt - 3), n, dw, n, n), n, n);
+    load_d();
+    b.add(K::GEMM64_CC4_SUB, cc(ll(nt - 3), n, uu(nt - 3), n, dw, ldd, n), n, n);
     invert(nt - 2);
     {
         const CMat<double> t2 = tm(nt - 2), t1 = tp(nt - 1);
@@ -2047,9 +2073,9 @@ OpCache& Plan::get_qlu(int nt) {
         b.add(K::GEMM64_CC4, cc(tw, n, di(nt - 2), n, ll(nt - 2), n, n), n, n);
     }
     // i = nt - 1
-    blk(dw, n, &pb, n, 2);
-    b.add(K::GEMM64_CC4_SUB, cc(ll(nt - 2), n, uu(nt - 2), n, dw, n, n), n, n);
-    b.add(K::GEMM64_CC4_SUB, cc(CMat<double>{Hre, Him}, ldh, CMat<double>{Vre, Vim}, n, dw, n, nh * n), n, n);
+    load_d();
+    b.add(K::GEMM64_CC4_SUB, cc(ll(nt - 2), n, uu(nt - 2), n, dw, ldd, n), n, n);
+    b.add(K::GEMM64_CC4_SUB, cc(CMat<double>{Hre, Him}, ldh, CMat<double>{Vre, Vim}, n, dw, ldd, nh * n), n, n);
     invert(nt - 1);
     b.finish();
     tune(*op);

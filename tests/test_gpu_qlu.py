@@ -11,6 +11,8 @@ log det and solve).
   test uses 4, 8, 32; 1 and 2 are its special cases) with kappaTilde = kappa / Nt as in
   the reference test (cond Q <= 2e3: the oracle's exp(kappa) by eigh and the GPU's by a
   Taylor series differ by rounding, which cond Q amplifies; at beta = 3, Nt = 1 it is 7e7);
+* the same at Nx=32 (honeycomb 4x4), where the blocks are inverted through the LU of
+  [d | 1], Nt = 3, 8;
 * at Nx=72, Nt=64 (honeycomb 6x6, beta=6), beyond a dense check, log det Q equals
   log det M_p + log det M_h (det Q = det M_p det M_h) from the GPU logdetM to 1e-9, and at
   Nx=288, Nt=128 (the config-3 torus) to 2e-9;
@@ -91,6 +93,25 @@ def test_qlu_logdet_solve(hop, nt):
         xs = np.linalg.solve(q, bs.T).T
         got = _np(isle.solveQ(lu, bs))
         assert got.shape == bs.shape and np.abs(got - xs).max() < 1e-10 * np.abs(xs).max()
+
+
+@pytest.mark.parametrize("hop", ["EXP", "DIA"])
+@pytest.mark.parametrize("nt", [3, 8])
+def test_qlu_blocks_nx32(hop, nt):
+    # Nx % 32 == 0: each diagonal block is inverted through the LU of [d | 1] (its right block
+    # ends as L^{-1} P, then one backward solve), checked against the oracle as above
+    print(f"QLU {hop} Nx=32 Nt={nt}", flush=True)
+    rng = np.random.RandomState(300 + nt)
+    for mu, sk in ((0.0, -1), (0.1, 1)):
+        g, r = _pair(hop, nt, mu, sk, L=4, beta=1.0)
+        phi = random_phi(rng, 32 * nt)
+        lu = isle.getQLU(g, phi)
+        want = ref.qlu(r, phi)
+        for part in ("dinv", "u", "l", "v", "h"):
+            got, exp = _np(getattr(lu, part)), np.array(getattr(want, part)).reshape(-1, 32, 32)
+            assert np.abs(got - exp).max() < 1e-10 * np.abs(exp).max(), (part, mu, sk)
+        q = r.Q(phi)
+        assert mod2pi_diff(isle.logdetQ(lu), ref.ilogdet(q)) < 1e-10 * max(1.0, abs(ref.ilogdet(q)))
 
 
 def test_logdetQ_is_sum_of_logdetM():
