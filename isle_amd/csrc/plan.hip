@@ -1994,8 +1994,8 @@ OpCache& Plan::get_qlu(int nt) {
         g.ldc = (int)ldc;
         return std::vector<GemmItem<double>>{g};
     };
-    auto blk = [&](CMat<double> dst, long ldd, const CMat<double>* src, long lds, int mode) {
-        std::vector<BlockItem> v = {{dst.re, dst.im, src ? src->re : nullptr, src ? src->im : nullptr, ldd, lds, n, n,
+    auto blk = [&](CMat<double> dst, long ldst, const CMat<double>* src, long lds, int mode) {
+        std::vector<BlockItem> v = {{dst.re, dst.im, src ? src->re : nullptr, src ? src->im : nullptr, ldst, lds, n, n,
                                      mode}};
         b.add(K::BLOCK, v, 0, 0, 0, (int)nn);
     };
