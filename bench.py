@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=3, help="timed trajectories (per chain) of the batched form")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--chains", type=int, default=None,
-                    help="independent chains advanced together (default 128; 4 for --config5)")
+                    help="independent chains advanced together (default 128; 16 for --config5)")
     ap.add_argument("--group", type=int, default=None,
                     help="time slices per group (default 32)")
     ap.add_argument("--plain-steps", type=int, default=3, help="timed trajectories of the plain FP64 forms")
@@ -62,8 +62,8 @@ def parse():
     ap.add_argument("--config4", action="store_true",
                     help="the 256-chain ensemble of uneven lattices (L in {3,4,6,8}, Nt in {16,32,64}, beta=Nt/8, "
                          "U=2, 20 steps): prints chain-trajectories/s instead of the config-3 line")
-    ap.add_argument("--right-chains", choices=("fp64", "fp32"), default="fp64",
-                    help="config 5: precision of the stable force's right chains")
+    ap.add_argument("--right-chains", choices=("fp64", "fp32"), default="fp32",
+                    help="config 5: precision of the stable force's right chains (FP32 on MFMA by default)")
     ap.add_argument("--config5", action="store_true",
                     help="the nanotube workload (12 x 16 cell honeycomb cylinder, Nx=384, Nt=256, beta=10, U=2.5, "
                          "mu=0.5, DIA, spin basis, 60 steps) with the stable block-cyclic form: prints "
@@ -324,8 +324,10 @@ def config5(args):
     sigmaKappa=-1, HMC length 1 with 60 steps; chain c starts from N(0, U beta/Nt) drawn from
     NumpyRNG(7 + c).  The product of all 256 slices spans ~33 decades here, so eval and force
     use precision="stable" (FP64 groups of --group slices, default 32, combined by the pivoted
-    block-cyclic elimination; groups of 16 double its sequential steps and run 2.2x slower).  One step = one trajectory of every
-    chain.  Checks: log det M of chain 0's start field against the oracle's stabilised
+    block-cyclic elimination; groups of 16 double its sequential steps and run 2.2x slower).
+    One step = one trajectory of every chain; 16 chains by default (the elimination's LU
+    panels are one workgroup per matrix, so more chains per launch fill more of the GPU: 4
+    chains give 1.61, 16 give 2.12, 32 give 2.23 chain-trajectories/s).  Checks: log det M of chain 0's start field against the oracle's stabilised
     restatement (CPU, groups of 16), and the force against FP64 chains over another grouping
     of the same elimination (32 slices, or 16 when the force uses 32)."""
     import torch
@@ -334,7 +336,7 @@ def config5(args):
     from isle_amd.evolver import BatchedConstStepLeapfrog, NumpyRNG
     from oracle import hubbard_ref as ref
     nt, beta, u, mu, nstep = 256, 10.0, 2.5, 0.5, 60
-    chains = args.chains or 4
+    chains = args.chains or 16
     kappa = isle.honeycomb(12, 16, periodic_j=False).hopping()
     nx = kappa.shape[0]
     kt, mt, ut = kappa * beta / nt, mu * beta / nt, u * beta / nt
