@@ -652,6 +652,15 @@ def main():
                 "force_ms": float(prof[:, 1].sum()),
                 "breakdown_ms": {KIND_NAMES[k]: round(float(prof[k, 1]), 4) for k in range(nk) if prof[k, 0] > 0}}
 
+    # the FP32 right-chain step, the chain kernel furthest below its peak, beside the dominant one
+    k32 = KIND_NAMES.index("gemm32_rck_diag")
+    if prof[k32, 0] > 0:
+        a32 = prof[k32, 2] / prof[k32, 1] * 1e3 / 1e12
+        roofline["fp32_right_chain"] = {"kernel": "gemm32_rck_diag", "achieved": a32, "peak": PEAK["f32"],
+                                        "unit": "TFLOP/s", "frac": a32 / PEAK["f32"],
+                                        "avg_launch_ms": float(prof[k32, 1] / prof[k32, 0]),
+                                        "note": "epilogue-bound: the force diagonal reads the FP64 suffix "
+                                                "entries (16 B per product element) after the K loop"}
     progress("roofline done")
     # ---------------- plain FP64, one chain, unbatched (from chain 0's thermalised field)
     plain = None
