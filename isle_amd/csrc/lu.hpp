@@ -257,16 +257,36 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
 
     double ar[NB], ai[NB];
     int pos = tid;  // current position of this thread's original panel row
+    // full panels of 16-byte aligned rows move in 16-byte pieces (uniform over the workgroup)
+    const bool vec = kb == NB && ld % 2 == 0 && ((reinterpret_cast<uintptr_t>(re) | reinterpret_cast<uintptr_t>(im)) & 15) == 0;
+    if (vec) {
+        const long o = own ? (long)lu_row(k0, ntop, tid) * ld + k0 : 0;
 #pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        double vr = 0, vi = 0;
-        if (own && c < kb) {
-            vr = re[(long)lu_row(k0, ntop, tid) * ld + k0 + c];
-            vi = im[(long)lu_row(k0, ntop, tid) * ld + k0 + c];
-            if (add_identity && c == tid) vr += 1.0;
+        for (int c = 0; c < NB; c += 2) {
+            const double2 vr = own ? *reinterpret_cast<const double2*>(re + o + c) : make_double2(0.0, 0.0);
+            const double2 vi = own ? *reinterpret_cast<const double2*>(im + o + c) : make_double2(0.0, 0.0);
+            ar[c] = vr.x;
+            ar[c + 1] = vr.y;
+            ai[c] = vi.x;
+            ai[c + 1] = vi.y;
         }
-        ar[c] = vr;
-        ai[c] = vi;
+        if (add_identity && own && tid < NB) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c == tid) ar[c] += 1.0;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            double vr = 0, vi = 0;
+            if (own && c < kb) {
+                vr = re[(long)lu_row(k0, ntop, tid) * ld + k0 + c];
+                vi = im[(long)lu_row(k0, ntop, tid) * ld + k0 + c];
+                if (add_identity && c == tid) vr += 1.0;
+            }
+            ar[c] = vr;
+            ai[c] = vi;
+        }
     }
 
     __syncthreads();
@@ -381,11 +401,19 @@ __global__ __launch_bounds__(LU_MAX_N) void lu_panel_kernel(const LuItem* __rest
     if (tid < kb) it.piv[k0 + tid] = lu_row(k0, ntop, pvt[tid]);
     if (own) {
         const long orow = lu_row(k0, ntop, pos);
+        if (vec) {
 #pragma unroll
-        for (int c = 0; c < NB; ++c) {
-            if (c < kb) {
-                re[orow * ld + k0 + c] = ar[c];
-                im[orow * ld + k0 + c] = ai[c];
+            for (int c = 0; c < NB; c += 2) {
+                *reinterpret_cast<double2*>(re + orow * ld + k0 + c) = make_double2(ar[c], ar[c + 1]);
+                *reinterpret_cast<double2*>(im + orow * ld + k0 + c) = make_double2(ai[c], ai[c + 1]);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                if (c < kb) {
+                    re[orow * ld + k0 + c] = ar[c];
+                    im[orow * ld + k0 + c] = ai[c];
+                }
             }
         }
         if (pos != tid) {  // at most 2*NB positions change: the panel rows and their partners
