@@ -327,7 +327,7 @@ def config5(args):
     block-cyclic elimination; groups of 16 double its sequential steps and run 2.2x slower).
     One step = one trajectory of every chain; 16 chains by default (the elimination's LU
     panels are one workgroup per matrix, so more chains per launch fill more of the GPU: 4
-    chains give 1.61, 16 give 2.12, 32 give 2.23 chain-trajectories/s).  Checks: log det M of chain 0's start field against the oracle's stabilised
+    chains gave 1.61, 16 gave 2.12, 32 gave 2.23 chain-trajectories/s in one build).  Checks: log det M of chain 0's start field against the oracle's stabilised
     restatement (CPU, groups of 16), and the force against FP64 chains over another grouping
     of the same elimination (32 slices, or 16 when the force uses 32)."""
     import torch

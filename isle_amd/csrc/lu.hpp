@@ -144,22 +144,29 @@ __device__ void lu_invert_panel_blocks(const LuItem& it, int panel, double (*lt_
                                        double (*ut_re)[NB + 1], double (*ut_im)[NB + 1], double (*tl_re)[NB + 1],
                                        double (*tl_im)[NB + 1], double (*tu_re)[NB + 1], double (*tu_im)[NB + 1]) {
     const int tid = threadIdx.x, nth = blockDim.x;
+    // Each level's dot products run over the whole block length, the terms outside the
+    // triangle replaced by zero (fixed trip counts: the loads of a product are issued together
+    // instead of one loop iteration at a time; every index stays inside the NB x NB blocks).
+#pragma unroll
     for (int b = 1; b < NB; b *= 2) {
         const int work = (NB / (2 * b)) * b * b;
         for (int e = tid; e < work; e += nth) {
             const int blk = e / (b * b), i = (e / b) % b, j = e % b, o = blk * 2 * b;
             double tr = 0, ti = 0, sr = 0, si = 0;
-            for (int q = j; q < b; ++q) {  // C A^-1
-                const double cr = lt_re[o + b + i][o + q], ci = lt_im[o + b + i][o + q];
-                const double xr = lt_re[o + q][o + j], xi = lt_im[o + q][o + j];
-                tr += cr * xr - ci * xi;
-                ti += cr * xi + ci * xr;
-            }
-            for (int q = 0; q <= j; ++q) {  // B D^-1
-                const double br = ut_re[o + i][o + b + q], bi = ut_im[o + i][o + b + q];
-                const double xr = ut_re[o + b + q][o + b + j], xi = ut_im[o + b + q][o + b + j];
-                sr += br * xr - bi * xi;
-                si += br * xi + bi * xr;
+#pragma unroll 8
+            for (int q = 0; q < b; ++q) {
+                {  // C A^-1: terms q >= j
+                    const double cr = lt_re[o + b + i][o + q], ci = lt_im[o + b + i][o + q];
+                    const double xr = lt_re[o + q][o + j], xi = lt_im[o + q][o + j];
+                    tr += q >= j ? cr * xr - ci * xi : 0.0;
+                    ti += q >= j ? cr * xi + ci * xr : 0.0;
+                }
+                {  // B D^-1: terms q <= j
+                    const double br = ut_re[o + i][o + b + q], bi = ut_im[o + i][o + b + q];
+                    const double xr = ut_re[o + b + q][o + b + j], xi = ut_im[o + b + q][o + b + j];
+                    sr += q <= j ? br * xr - bi * xi : 0.0;
+                    si += q <= j ? br * xi + bi * xr : 0.0;
+                }
             }
             tl_re[o + b + i][o + j] = tr;
             tl_im[o + b + i][o + j] = ti;
@@ -170,17 +177,20 @@ __device__ void lu_invert_panel_blocks(const LuItem& it, int panel, double (*lt_
         for (int e = tid; e < work; e += nth) {
             const int blk = e / (b * b), i = (e / b) % b, j = e % b, o = blk * 2 * b;
             double xr_ = 0, xi_ = 0, yr = 0, yi = 0;
-            for (int q = 0; q <= i; ++q) {  // D^-1 (C A^-1)
-                const double dr = lt_re[o + b + i][o + b + q], di = lt_im[o + b + i][o + b + q];
-                const double tr = tl_re[o + b + q][o + j], ti = tl_im[o + b + q][o + j];
-                xr_ += dr * tr - di * ti;
-                xi_ += dr * ti + di * tr;
-            }
-            for (int q = i; q < b; ++q) {  // A^-1 (B D^-1)
-                const double ar_ = ut_re[o + i][o + q], ai_ = ut_im[o + i][o + q];
-                const double tr = tu_re[o + q][o + b + j], ti = tu_im[o + q][o + b + j];
-                yr += ar_ * tr - ai_ * ti;
-                yi += ar_ * ti + ai_ * tr;
+#pragma unroll 8
+            for (int q = 0; q < b; ++q) {
+                {  // D^-1 (C A^-1): terms q <= i
+                    const double dr = lt_re[o + b + i][o + b + q], di = lt_im[o + b + i][o + b + q];
+                    const double tr = tl_re[o + b + q][o + j], ti = tl_im[o + b + q][o + j];
+                    xr_ += q <= i ? dr * tr - di * ti : 0.0;
+                    xi_ += q <= i ? dr * ti + di * tr : 0.0;
+                }
+                {  // A^-1 (B D^-1): terms q >= i
+                    const double ar_ = ut_re[o + i][o + q], ai_ = ut_im[o + i][o + q];
+                    const double tr = tu_re[o + q][o + b + j], ti = tu_im[o + q][o + b + j];
+                    yr += q >= i ? ar_ * tr - ai_ * ti : 0.0;
+                    yi += q >= i ? ar_ * ti + ai_ * tr : 0.0;
+                }
             }
             lt_re[o + b + i][o + j] = -xr_;
             lt_im[o + b + i][o + j] = -xi_;
