@@ -127,6 +127,14 @@ int isle_applyM(isle_hfm* h, const double* phi, int nt, int species, const doubl
 int isle_qlu(isle_hfm* h, const double* phi, int nt, double* dinv, double* u, double* l, double* v, double* hh,
              double* logdet_host);
 
+/* solveQ (hubbardFermiMatrixExp.cpp:494-537; bind_hubbardFermiMatrix.cpp:47-49) on the blocks of
+ * Q's LU as isle_qlu returns them (device interleaved complex128: dinv [nt], u and l [nt-1], v and
+ * h [nt-2] blocks of nx*nx): the forward substitution with l and h, then the backward one with
+ * dinv, u and v, every block product a complex GEMM on the device.  rhs, x: device nrhs x
+ * (nx*nt) complex128 rows (time-major).  nt >= 1. */
+int isle_solveq(isle_hfm* h, int nt, int nrhs, const double* dinv, const double* u, const double* l,
+                const double* v, const double* hh, const double* rhs, double* x);
+
 /* solveM with the factorization kept between calls (no counterpart in the reference: for
  * callers that solve many right-hand sides against one field, such as iterative refinement).
  * reuse = 0: as isle_solveM (the handle keeps the factorization: partial products and their

@@ -33,6 +33,7 @@ SIGNATURES = {
     "isle_solveM_ex": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int]),
     "isle_applyM": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _c_long, _vp]),
     "isle_qlu": (_c_int, [_vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "isle_solveq": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "isle_combine_force": (_c_int, [_vp, _vp, _vp, _c_long, _c_int, _c_int, _vp]),
     "isle_combine_force_batch": (_c_int, [_vp, _vp, _vp, _c_long, _c_int, _c_long, _c_long, _c_int, _c_int, _vp]),
     "isle_gauge_force": (_c_int, [_vp, _vp, _c_long, _c_double, _c_int, _vp]),

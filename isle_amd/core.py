@@ -357,9 +357,10 @@ class QLU:
     Device complex128 stacks: ``dinv`` [nt, nx, nx] (d_i^{-1}), ``u``, ``l`` [nt-1, nx, nx],
     ``v``, ``h`` [nt-2, nx, nx]; ``logdet_d`` holds log det d_i from the LU that inverted d_i."""
 
-    def __init__(self, dinv, u, l, v, h, logdet_d):
+    def __init__(self, dinv, u, l, v, h, logdet_d, hfm=None):
         self.dinv, self.u, self.l, self.v, self.h = dinv, u, l, v, h
         self.logdet_d = list(logdet_d)
+        self.hfm = hfm  # the matrix it factorises (getQLU sets it): solveQ then runs natively
 
     def nt(self):
         return self.dinv.shape[0]
@@ -440,7 +441,7 @@ def getQLU(hfm, phi):
         ld = np.empty(2 * nt)
         _lib.check(_lib.load().isle_qlu(hfm.handle, _lib.ptr(phi.contiguous()), nt, _lib.ptr(dinv), _lib.ptr(u),
                                         _lib.ptr(l), _lib.ptr(v), _lib.ptr(h), _lib.dptr(ld)))
-        return QLU(dinv, u, l, v, h, [complex(ld[2 * i], ld[2 * i + 1]) for i in range(nt)])
+        return QLU(dinv, u, l, v, h, [complex(ld[2 * i], ld[2 * i + 1]) for i in range(nt)], hfm)
     p = torch.as_tensor(hfm.P(), device=dev).to(c128)
     lds = []
 
@@ -449,13 +450,13 @@ def getQLU(hfm, phi):
 
     if nt == 1:
         inv(0, p + hfm.Tplus(0, phi) + hfm.Tminus(0, phi))
-        return QLU(dinv, u, l, v, h, lds)
+        return QLU(dinv, u, l, v, h, lds, hfm)
     inv(0, p)
     # nt == 2
     l[0] = (hfm.Tplus(1, phi) + hfm.Tminus(1, phi)) @ dinv[0]
     u[0] = hfm.Tplus(0, phi) + hfm.Tminus(0, phi)
     inv(1, p - l[0] @ u[0])
-    return QLU(dinv, u, l, v, h, lds)
+    return QLU(dinv, u, l, v, h, lds, hfm)
 
 
 def logdetQ(hfm, phi=None):
@@ -474,7 +475,10 @@ def solveQ(hfm, phi, rhs=None):
     """x with Q(phi) x = rhs (solveQ, hubbardFermiMatrixExp.cpp:494-537,
     bind_hubbardFermiMatrix.cpp:47-49): forward substitution with l and h, then backward
     with dinv, u and v.  ``solveQ(qlu, rhs)`` reuses a factorisation.  rhs: a spacetime
-    vector (NxNt); a 2-D rhs solves every row.  Returns a device tensor of rhs's shape."""
+    vector (NxNt); a 2-D rhs solves every row.  Returns a device tensor of rhs's shape.
+    With the QLU of getQLU (which records its matrix) the substitutions run as one native
+    operation (isle_solveq, complex GEMMs of four real products); a QLU assembled by hand
+    takes the same steps as torch products."""
     if isinstance(hfm, QLU):
         lu, rhs = hfm, phi
     if rhs is None:
@@ -488,7 +492,16 @@ def solveQ(hfm, phi, rhs=None):
     shape = b.shape
     if b.dim() not in (1, 2) or shape[-1] != nt * nx:
         raise RuntimeError("Right hand side does not have correct size (spacetime vector)")
-    # columns are right-hand sides: b[t] is nx x nrhs
+    if lu.hfm is not None and lu.hfm.nx() == nx:
+        # one native operation (isle_solveq): the substitutions as complex GEMMs on the device
+        rows = b.reshape(-1, nt * nx).to(torch.complex128).contiguous()
+        out = torch.empty_like(rows)
+        parts = [t.to(device=rows.device, dtype=torch.complex128).contiguous()
+                 for t in (lu.dinv, lu.u, lu.l, lu.v, lu.h)]
+        _lib.check(_lib.load().isle_solveq(lu.hfm.handle, nt, rows.shape[0], *[_lib.ptr(t) for t in parts],
+                                           _lib.ptr(rows), _lib.ptr(out)))
+        return out.reshape(shape)
+    # a QLU built by hand: the same substitutions as torch products; b[t] is nx x nrhs
     b = b.reshape(-1, nt, nx).permute(1, 2, 0).contiguous()
     y = torch.empty_like(b)
     y[0] = b[0]

@@ -195,6 +195,36 @@ __global__ void split_to_interleaved_kernel(const double* __restrict__ re, const
 }
 
 // interleaved complex128 -> split planes
+// Interleaved complex [nblk][n][n] to split planes (the inverse of blocks_to_interleaved_kernel):
+// block k of the planes starts at element k * boff, row stride ld.  Grid (ceil(n*n / 256), nblk).
+__global__ __launch_bounds__(256) void interleaved_to_blocks_kernel(const double* __restrict__ in, long boff, long ld,
+                                                                     int n, double* __restrict__ re,
+                                                                     double* __restrict__ im) {
+    const long k = blockIdx.y, nn = (long)n * n;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < nn; e += (long)gridDim.x * 256) {
+        const long r = e / n, c = e % n, dst = k * boff + r * ld + c;
+        re[dst] = in[2 * (k * nn + e)];
+        im[dst] = in[2 * (k * nn + e) + 1];
+    }
+}
+
+// Rows of right-hand sides (interleaved complex [nrhs][len]) to and from split planes holding
+// them as columns ([len][nrhs]).  to_planes = 1: rows -> planes; 0: planes -> rows.
+__global__ __launch_bounds__(256) void rhs_columns_kernel(double* __restrict__ rows, double* __restrict__ re,
+                                                          double* __restrict__ im, long len, int nrhs, int to_planes) {
+    const long total = len * nrhs;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const long i = e / nrhs, c = e % nrhs, o = 2 * (c * len + i);
+        if (to_planes) {
+            re[e] = rows[o];
+            im[e] = rows[o + 1];
+        } else {
+            rows[o] = re[e];
+            rows[o + 1] = im[e];
+        }
+    }
+}
+
 // Stacks of n x n blocks in split planes to interleaved complex [nblk][n][n]: block k starts at
 // element k * boff of the planes, row stride ld.  Grid (ceil(n*n / 256), nblk).
 __global__ __launch_bounds__(256) void blocks_to_interleaved_kernel(const double* __restrict__ re,

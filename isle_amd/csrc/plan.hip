@@ -2107,6 +2107,113 @@ void Plan::qlu(const double* phi, int nt, double* dinv, double* u, double* l, do
     ISLE_HIP(hipStreamSynchronize(s));
 }
 
+// solveQ (hubbardFermiMatrixExp.cpp:494-537) on the blocks of Q's LU, as one cached operation
+// per (nt, number of right-hand sides) whose workspace holds the blocks in split planes and the
+// right-hand sides as columns:
+//   forward   y_0 = b_0,  y_i = b_i - l_{i-1} y_{i-1} (i >= 1),  y_{nt-1} -= sum_{i<nt-2} h_i y_i
+//   backward  x_{nt-1} = d_{nt-1}^{-1} y_{nt-1},  x_{nt-2} = d_{nt-2}^{-1} (y_{nt-2} - u_{nt-2} x_{nt-1}),
+//             x_i = d_i^{-1} (y_i - u_i x_{i+1} - v_i x_{nt-1})  (i = nt-3 .. 0)
+// with every product a complex GEMM of four real products (as in getQLU); the h_i side by side
+// make the sum one GEMM with k = (nt-2) nx, and the v_i x_{nt-1} are one launch over all i.
+OpCache& Plan::get_solveq(int nt, int nrhs) {
+    if (nt < 1 || nrhs < 1) throw std::invalid_argument("solveQ needs nt >= 1 and at least one right-hand side");
+    auto key = std::make_tuple(8, nt, nrhs, 0, 0, 0);
+    auto found = cache_.find(key);
+    if (found != cache_.end()) return *found->second;
+    auto op = std::make_unique<OpCache>();
+    op->nt = nt;
+    op->batch = 1;
+    op->phi_stride = 0;
+    Builder b(*op);
+    const int n = nx_;
+    const long nn = (long)n * n;
+    const int nu = std::max(nt - 1, 0), nh = std::max(nt - 2, 0);
+    const long ldh = (long)std::max(nh, 1) * n;
+    double* DI = b.d64((size_t)nt * 2 * nn);
+    double* UU = b.d64((size_t)std::max(nu, 1) * 2 * nn);
+    double* LL = b.d64((size_t)std::max(nu, 1) * 2 * nn);
+    double* VV = b.d64((size_t)std::max(nh, 1) * 2 * nn);
+    double* HH = b.d64((size_t)std::max(nh, 1) * 2 * nn);  // n x (nh n): re plane, then im plane
+    const long plane = (long)nt * n * nrhs;                // right-hand sides as columns, row stride nrhs
+    double* Y = b.d64(2 * (size_t)plane);
+    double* X = b.d64(2 * (size_t)plane);
+    op->extra = {DI, UU, LL, VV, HH, Y, X};
+    auto blkm = [&](double* base, int k) { return cmat(base, k, nn); };
+    auto col = [&](double* base, int t) {
+        return CMat<double>{base + (long)t * n * nrhs, base + plane + (long)t * n * nrhs};
+    };
+    auto gi = [&](CMat<double> a, long lda, CMat<double> bb, CMat<double> c, int k) {
+        GemmItem<double> g{};
+        g.a_re = a.re;
+        g.a_im = a.im;
+        g.b_re = bb.re;
+        g.b_im = bb.im;
+        g.c_re = c.re;
+        g.c_im = c.im;
+        g.m = n;
+        g.n = nrhs;
+        g.k = k;
+        g.lda = (int)lda;
+        g.ldb = nrhs;
+        g.ldc = nrhs;
+        return g;
+    };
+    auto one = [&](K kind, GemmItem<double> g) {
+        std::vector<GemmItem<double>> v = {g};
+        b.add(kind, v, n, nrhs);
+    };
+    for (int i = 1; i < nt; ++i) one(K::GEMM64_CC4_SUB, gi(blkm(LL, i - 1), n, col(Y, i - 1), col(Y, i), n));
+    if (nt > 2)
+        one(K::GEMM64_CC4_SUB, gi(CMat<double>{HH, HH + (long)nh * nn}, ldh, col(Y, 0), col(Y, nt - 1), nh * n));
+    one(K::GEMM64_CC4, gi(blkm(DI, nt - 1), n, col(Y, nt - 1), col(X, nt - 1), n));
+    if (nt > 1) {
+        one(K::GEMM64_CC4_SUB, gi(blkm(UU, nt - 2), n, col(X, nt - 1), col(Y, nt - 2), n));
+        one(K::GEMM64_CC4, gi(blkm(DI, nt - 2), n, col(Y, nt - 2), col(X, nt - 2), n));
+    }
+    if (nt > 2) {
+        std::vector<GemmItem<double>> vx;
+        for (int i = 0; i < nh; ++i) vx.push_back(gi(blkm(VV, i), n, col(X, nt - 1), col(Y, i), n));
+        b.add(K::GEMM64_CC4_SUB, vx, n, nrhs);
+        for (int i = nt - 3; i >= 0; --i) {
+            one(K::GEMM64_CC4_SUB, gi(blkm(UU, i), n, col(X, i + 1), col(Y, i), n));
+            one(K::GEMM64_CC4, gi(blkm(DI, i), n, col(Y, i), col(X, i), n));
+        }
+    }
+    b.finish();
+    tune(*op);
+    auto& ref = *op;
+    cache_[key] = std::move(op);
+    return ref;
+}
+
+void Plan::solve_q(int nt, int nrhs, const double* dinv, const double* u, const double* l, const double* v,
+                   const double* h, const double* rhs, double* x) {
+    OpCache& op = get_solveq(nt, nrhs);
+    const int n = nx_;
+    const long nn = (long)n * n, plane = (long)nt * n * nrhs;
+    const int nh = std::max(nt - 2, 0);
+    hipStream_t s = stream_;
+    const auto& e = op.extra;  // DI, UU, LL, VV, HH, Y, X
+    auto in = [&](const double* src, int nblk, double* re, double* im, long boff, long ld) {
+        if (nblk <= 0) return;
+        hipLaunchKernelGGL(interleaved_to_blocks_kernel, dim3(std::min<long>((nn + 255) / 256, 1024), nblk), dim3(256),
+                           0, s, src, boff, ld, n, re, im);
+        ISLE_HIP(hipGetLastError());
+    };
+    in(dinv, nt, e[0], e[0] + nn, 2 * nn, n);
+    in(u, nt - 1, e[1], e[1] + nn, 2 * nn, n);
+    in(l, nt - 1, e[2], e[2] + nn, 2 * nn, n);
+    in(v, nt - 2, e[3], e[3] + nn, 2 * nn, n);
+    in(h, nt - 2, e[4], e[4] + (long)nh * nn, n, (long)nh * n);  // h_i side by side
+    const unsigned g = (unsigned)std::min<long>((plane + 255) / 256, 4096);
+    hipLaunchKernelGGL(rhs_columns_kernel, dim3(g), dim3(256), 0, s, const_cast<double*>(rhs), e[5], e[5] + plane,
+                       (long)nt * n, nrhs, 1);
+    ISLE_HIP(hipGetLastError());
+    run(op, nullptr);
+    hipLaunchKernelGGL(rhs_columns_kernel, dim3(g), dim3(256), 0, s, x, e[6], e[6] + plane, (long)nt * n, nrhs, 0);
+    ISLE_HIP(hipGetLastError());
+}
+
 // M x for rows x: per time slice one complex x real GEMM over all rows with the slice
 // matrix S (EXP: exp(kappa - mu)..., DIA: K), between an elementwise prologue (the shifted,
 // phase-scaled rows) and epilogue (back to interleaved rows); see applym_prep_kernel.
@@ -2572,6 +2679,14 @@ int isle_force_parts(isle_hfm* h, const double* phi, int nt, int batch, int mode
         if (batch < 1 || (nspecies != 1 && nspecies != 2)) throw std::invalid_argument("bad batch/nspecies");
         if (mode < 0 || mode > 3) throw std::invalid_argument("mode must be 0, 1, 2 or 3");
         P(h)->force_parts(phi, nt, batch, mode, group, nspecies, spin, out_dev);
+    });
+}
+
+int isle_solveq(isle_hfm* h, int nt, int nrhs, const double* dinv, const double* u, const double* l,
+                const double* v, const double* hh, const double* rhs, double* x) {
+    return guarded([&] {
+        if (nt < 1 || nrhs < 1) throw std::invalid_argument("isle_solveq needs nt >= 1 and nrhs >= 1");
+        P(h)->solve_q(nt, nrhs, dinv, u, l, v, hh, rhs, x);
     });
 }
 

@@ -149,6 +149,10 @@ public:
     // and log det d_i to logdet_host (nt x 2); one synchronisation at the end.
     void qlu(const double* phi, int nt, double* dinv, double* u, double* l, double* v, double* h,
              double* logdet_host);
+    // x with Q x = rhs from the blocks of Q's LU (interleaved complex128 device stacks as
+    // getQLU returns them); rhs, x: nrhs x (nt * nx) rows.  nt >= 1.
+    void solve_q(int nt, int nrhs, const double* dinv, const double* u, const double* l, const double* v,
+                 const double* h, const double* rhs, double* x);
     // reuse: phi, nt and species equal those of the previous call (its factorization is kept)
     void solveM(const double* phi, int nt, int species, const double* rhss, int nrhs, double* out,
                 bool reuse = false);
@@ -206,6 +210,7 @@ private:
     OpCache& get_force_stable(int nt, int batch, int group, int nspecies, int spin, bool right32 = false);
     OpCache& get_logdet_stable(int nt, int batch, int group, int nspecies, int spin);
     OpCache& get_qlu(int nt);
+    OpCache& get_solveq(int nt, int nrhs);
     void run(OpCache& op, const double* phi);
     bool launch_step(OpCache& op, const Step& st, const double* phi);
     // Autotuner: time every tile shape of each GEMM step once per (kind, m, n, items) and

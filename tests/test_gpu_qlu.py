@@ -12,7 +12,9 @@ log det and solve).
   the reference test (cond Q <= 2e3: the oracle's exp(kappa) by eigh and the GPU's by a
   Taylor series differ by rounding, which cond Q amplifies; at beta = 3, Nt = 1 it is 7e7);
 * the same at Nx=32 (honeycomb 4x4), where the blocks are inverted through the LU of
-  [d | 1], Nt = 3, 8;
+  [d | 1], Nt = 3, 8, with solveQ against numpy.linalg.solve;
+* solveQ on getQLU's factorisation (the native isle_solveq) equals the torch steps taken for
+  a QLU assembled by hand from the same blocks, to 1e-12, Nt = 1, 2, 3, 8;
 * at Nx=72, Nt=64 (honeycomb 6x6, beta=6), beyond a dense check, log det Q equals
   log det M_p + log det M_h (det Q = det M_p det M_h) from the GPU logdetM to 1e-9, and at
   Nx=288, Nt=128 (the config-3 torus) to 2e-9;
@@ -112,6 +114,28 @@ def test_qlu_blocks_nx32(hop, nt):
             assert np.abs(got - exp).max() < 1e-10 * np.abs(exp).max(), (part, mu, sk)
         q = r.Q(phi)
         assert mod2pi_diff(isle.logdetQ(lu), ref.ilogdet(q)) < 1e-10 * max(1.0, abs(ref.ilogdet(q)))
+        bs = np.stack([random_phi(rng, 32 * nt) for _ in range(2)])
+        xs = np.linalg.solve(q, bs.T).T
+        assert np.abs(_np(isle.solveQ(lu, bs)) - xs).max() < 1e-10 * np.abs(xs).max()
+
+
+@pytest.mark.parametrize("nt", [1, 2, 3, 8])
+def test_solveQ_native_equals_torch_steps(nt):
+    # solveQ on getQLU's factorisation runs natively (isle_solveq); the same blocks in a QLU
+    # assembled by hand take the torch steps: both do the same substitutions
+    print(f"solveQ native vs torch steps, Nx=18 Nt={nt}", flush=True)
+    rng = np.random.RandomState(500 + nt)
+    g, _ = _pair("DIA", nt, 0.1, 1, beta=1.0)
+    phi = random_phi(rng, 18 * nt)
+    lu = isle.getQLU(g, phi)
+    assert lu.hfm is g
+    by_hand = isle.QLU(lu.dinv, lu.u, lu.l, lu.v, lu.h, lu.logdet_d)
+    bs = np.stack([random_phi(rng, 18 * nt) for _ in range(5)])
+    native, steps = _np(isle.solveQ(lu, bs)), _np(isle.solveQ(by_hand, bs))
+    assert native.shape == bs.shape
+    assert np.abs(native - steps).max() < 1e-12 * np.abs(steps).max()
+    one = _np(isle.solveQ(lu, bs[2]))
+    assert one.shape == bs[2].shape and np.abs(one - steps[2]).max() < 1e-12 * np.abs(steps[2]).max()
 
 
 def test_logdetQ_is_sum_of_logdetM():
