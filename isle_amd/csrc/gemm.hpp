@@ -15,8 +15,6 @@
 // running sum (blocked summation).
 #pragma once
 
-#include <type_traits>
-
 #include "common.hpp"
 
 namespace isle {
@@ -642,17 +640,20 @@ void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items, int tiles_n_max) {
 // for 32- and for 64-bank LDS.  Lane (lr, lk) feeds k = kg + 4 lk + ks to MFMA step ks of a
 // 16-k group, so one ds_read_b128 brings a lane's operand for four MFMAs (a quarter of the LDS
 // instructions of gemm_rcd_kernel).  The product is stored transposed, Ct = C^T (n x m, the
-// next chain step's Bt), one 16-byte store per lane and 16 x 16 block, with FUSED_DIAG: the
-// row sums of L .* C per 16-column block (as gemm_rcd_kernel) from the unscaled product, and
-// C^T scaled by post[m] when post_re is set.
+// next chain step's Bt), one 16-byte store per lane and 16 x 16 block, with FUSED_DIAG (when
+// dpart is set): the row sums of L .* C per 16-column block (as gemm_rcd_kernel) from the
+// unscaled product, and C^T scaled by post[m] when post_re is set (the tile's row factors
+// staged in LDS).
 // Requirements (the caller vouches): k % BK == 0, m % 4 == 0, the row strides lda, ldb
-// (of Bt) and ldc (of Ct) multiples of 4, 16-byte aligned operands.
+// (of Bt) and ldc (of Ct) multiples of 4, 16-byte aligned operands, each operand plane below
+// 4 GiB.
 template <int BM, int BN, int BK, int STAGES>
 struct RckLayout {
     static constexpr int A = BM * BK;  // floats per slab plane
     static constexpr int B = BN * BK;
     static constexpr int STAGE = A + 2 * B;
-    static constexpr size_t BYTES = sizeof(float) * STAGE * STAGES;
+    static constexpr int POST = STAGES * STAGE;  // then the tile's row factors, 2 BM floats
+    static constexpr size_t BYTES = sizeof(float) * (STAGE * STAGES + 2 * BM);
 };
 
 __device__ __forceinline__ int rck_swz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
@@ -661,11 +662,24 @@ template <int BM, int BN, int BK, int WM, int WN>
 constexpr bool rck_shape_ok() {
     constexpr int NW = WM * WN, RPI = 64 / (BK / 4);  // rows written by one wave instruction
     return BK == 32 && BM % RPI == 0 && BN % RPI == 0 && (BM / RPI) % NW == 0 && (BN / RPI) % NW == 0 &&
-           BM % (16 * WM) == 0 && BN % (16 * WN) == 0;
+           BM % (16 * WM) == 0 && BN % (16 * WN) == 0 && 64 * NW >= 2 * BM;
+}
+
+// global-memory access through an address-space-1 pointer (a generic access would also count
+// against the LDS counter and wait for the slab copies into LDS); T without constructors
+template <typename T>
+using global_ptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ T ld_global(const T* p) {
+    return *(global_ptr<const T>)p;
+}
+template <typename T>
+__device__ __forceinline__ void st_global(T* p, const T& v) {
+    *(global_ptr<T>)p = v;
 }
 
 // This wave's direct copies of k-slab `slab` (columns slab*BK ..) into stage `stage` of the
-// LDS ring.
+// LDS ring: uniform slab bases plus 32-bit per-lane byte offsets (one address register each).
 template <int BM, int BN, int BK, int STAGES, int NW>
 __device__ __forceinline__ void rck_issue(const GemmItem<float>& it, float* lds, int stage, int slab, int m0, int n0,
                                           int wave, int lane) {
@@ -675,33 +689,34 @@ __device__ __forceinline__ void rck_issue(const GemmItem<float>& it, float* lds,
     float* st = lds + stage * LY::STAGE;
     const int k0 = slab * BK;
     const int rr = lane / CPR, pos = lane % CPR;
+    const char* ab = reinterpret_cast<const char*>(it.a_re + k0);
+    const char* bbr = reinterpret_cast<const char*>(it.b_re + k0);
+    const char* bbi = reinterpret_cast<const char*>(it.b_im + k0);
 #pragma unroll
     for (int q = 0; q < AQ; ++q) {
         const int ins = wave * AQ + q, r = ins * RPI + rr;
         int row = m0 + r;
         if (row >= it.m) row = 0;  // only feeds rows >= m of C
-        const int c = pos ^ rck_swz(r);
-        __builtin_amdgcn_global_load_lds(it.a_re + (long)row * it.lda + k0 + 4 * c, st + ins * 256, 16, 0, 0);
+        const unsigned off = 4u * (unsigned)(row * it.lda + 4 * (pos ^ rck_swz(r)));
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const float*>(ab + off), st + ins * 256, 16, 0, 0);
     }
 #pragma unroll
     for (int q = 0; q < BQ; ++q) {
         const int ins = wave * BQ + q, r = ins * RPI + rr;
         int row = n0 + r;
         if (row >= it.n) row = 0;  // only feeds columns >= n of C
-        const int c = pos ^ rck_swz(r);
-        const long off = (long)row * it.ldb + k0 + 4 * c;
-        __builtin_amdgcn_global_load_lds(it.b_re + off, st + LY::A + ins * 256, 16, 0, 0);
-        __builtin_amdgcn_global_load_lds(it.b_im + off, st + LY::A + LY::B + ins * 256, 16, 0, 0);
+        const unsigned off = 4u * (unsigned)(row * it.ldb + 4 * (pos ^ rck_swz(r)));
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const float*>(bbr + off), st + LY::A + ins * 256, 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const float*>(bbi + off), st + LY::A + LY::B + ins * 256, 16,
+                                         0, 0);
     }
 }
 
-// Epilogue operands of one 16-row block of a wave's part of the tile: the row factors
-// post[rowb .. rowb + 3] (1 when post_re is null) and the left-factor entries L[rowb + r][col]
-// of the block's NJ 16-column blocks (the identity when dl_re is null).  Lane (lr, lk) holds
-// C[4 lk + r][lr] of each 16 x 16 block; rowb = the block's first row + 4 lk.
+// Left-factor entries L[rowb + r][col] of one 16-row block's NJ 16-column blocks (the
+// identity when dl_re is null).  Lane (lr, lk) holds C[4 lk + r][lr] of each 16 x 16 block;
+// rowb = the block's first row + 4 lk.
 template <int NJ>
 struct RckEpi {
-    float fr[4], fi[4];
     double lre[NJ][4], lim[NJ][4];
 };
 
@@ -709,30 +724,17 @@ template <int NJ>
 __device__ __forceinline__ void rck_epi_load(const GemmItem<float>& it, int rowb, int colb, int lr, RckEpi<NJ>& e) {
     const bool rows_in = rowb < it.m;  // m % 4 == 0: all four rows or none
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        e.fr[r] = 1.0f;
-        e.fi[r] = 0.0f;
-    }
-    if (it.post_re != nullptr) {  // uniform over the workgroup
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int di = rows_in ? rowb + r : 0;
-            e.fr[r] = it.post_re[di];
-            e.fi[r] = it.post_im[di];
-        }
-    }
-#pragma unroll
     for (int j = 0; j < NJ; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int col = colb + j * 16 + lr;
             const bool in = rows_in && col < it.n;
-            if (it.dl_re == nullptr) {
+            if (it.dl_re == nullptr) {  // uniform over the workgroup
                 e.lre[j][r] = in && rowb + r == col ? 1.0 : 0.0;
                 e.lim[j][r] = 0.0;
             } else {
                 const long o = in ? (long)(rowb + r) * it.n + col : 0;
-                const double x = it.dl_re[o], y = it.dl_im[o];
+                const double x = ld_global(it.dl_re + o), y = ld_global(it.dl_im + o);
                 e.lre[j][r] = in ? x : 0.0;
                 e.lim[j][r] = in ? y : 0.0;
             }
@@ -740,11 +742,19 @@ __device__ __forceinline__ void rck_epi_load(const GemmItem<float>& it, int rowb
 }
 
 // The row sums of L .* C per 16-column block (from the unscaled product) into dpart, and the
-// scaled product stored transposed: Ct[col][rowb .. rowb + 3], one 16-byte store per lane.
+// product scaled by the row factors (pr, pi: the tile's, from LDS) stored transposed:
+// Ct[col][rowb .. rowb + 3], one 16-byte store per lane.
 template <int NJ>
-__device__ __forceinline__ void rck_epi_store(const GemmItem<float>& it, int rowb, int colb, int lr,
-                                              const RckEpi<NJ>& e, const flt4 (&are)[NJ], const flt4 (&aim)[NJ]) {
+__device__ __forceinline__ void rck_epi_store(const GemmItem<float>& it, int rowb, int rloc, int colb, int lr,
+                                              const float* pr, const float* pi, const RckEpi<NJ>& e,
+                                              const flt4 (&are)[NJ], const flt4 (&aim)[NJ]) {
     const bool rows_in = rowb < it.m;
+    float fr[4], fi[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        fr[r] = pr[rloc + r];
+        fi[r] = pi[rloc + r];
+    }
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
         const int col = colb + j * 16 + lr;
@@ -755,17 +765,17 @@ __device__ __forceinline__ void rck_epi_store(const GemmItem<float>& it, int row
             const double sr = row16_sum(e.lre[j][r] * (double)vr - e.lim[j][r] * (double)vi);
             const double si = row16_sum(e.lre[j][r] * (double)vi + e.lim[j][r] * (double)vr);
             const int slot = (colb + j * 16) / 16;
-            if (lr == 15 && rows_in && slot < it.n / 16) {
-                it.dpart[((long)slot * it.m + rowb + r) * 2] = sr;
-                it.dpart[((long)slot * it.m + rowb + r) * 2 + 1] = si;
+            if (it.dpart != nullptr && lr == 15 && rows_in && slot < it.n / 16) {
+                st_global(it.dpart + ((long)slot * it.m + rowb + r) * 2, sr);
+                st_global(it.dpart + ((long)slot * it.m + rowb + r) * 2 + 1, si);
             }
-            outr[r] = vr * e.fr[r] - vi * e.fi[r];
-            outi[r] = vr * e.fi[r] + vi * e.fr[r];
+            outr[r] = vr * fr[r] - vi * fi[r];
+            outi[r] = vr * fi[r] + vi * fr[r];
         }
         if (rows_in && col < it.n) {
             const long off = (long)col * it.ldc + rowb;
-            *reinterpret_cast<flt4*>(it.c_re + off) = outr;
-            *reinterpret_cast<flt4*>(it.c_im + off) = outi;
+            st_global(reinterpret_cast<flt4*>(it.c_re + off), outr);
+            st_global(reinterpret_cast<flt4*>(it.c_im + off), outi);
         }
     }
 }
@@ -788,7 +798,7 @@ void gemm_rck_kernel(const GemmItem<float>* __restrict__ items, int tiles_n_max)
     const GemmItem<float> it = items[item];
     const int m0 = (tile / tiles_n_max) * BM, n0 = (tile % tiles_n_max) * BN;
     if (m0 >= it.m || n0 >= it.n) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = (wave / WN) * (BM / WM), wn = (wave % WN) * (BN / WN);
     const int lr = lane & 15, lk = lane >> 4;
     const int nslab = it.k / BK;
@@ -801,6 +811,16 @@ void gemm_rck_kernel(const GemmItem<float>* __restrict__ items, int tiles_n_max)
 
     for (int j = 0; j < STAGES - 1 && j < nslab; ++j)
         rck_issue<BM, BN, BK, STAGES, NW>(it, lds, j % STAGES, j, m0, n0, wave, lane);
+    // the tile's row factors into LDS, read by the stores after the k loop's barriers (its
+    // load in flight with the first slabs)
+    if (tid < 2 * BM) {
+        const int r = tid % BM, row = m0 + r;
+        const float* src = tid < BM ? it.post_re : it.post_im;
+        float v = tid < BM ? 1.0f : 0.0f;
+        if (it.post_re != nullptr && row < it.m) v = ld_global(src + row);
+        lds[LY::POST + tid] = v;
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     for (int s = 0; s < nslab; ++s) {
         // this wave's copies of slab s done (later slabs stay in flight), then all waves'
         const int ahead = min(STAGES - 2, nslab - 1 - s);
@@ -810,8 +830,13 @@ void gemm_rck_kernel(const GemmItem<float>* __restrict__ items, int tiles_n_max)
             asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(PER_SLAB) : "memory");
         else
             asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
-        if (s + STAGES - 1 < nslab)  // into the buffer every wave finished reading
-            rck_issue<BM, BN, BK, STAGES, NW>(it, lds, (s + STAGES - 1) % STAGES, s + STAGES - 1, m0, n0, wave, lane);
+        if (s + STAGES - 1 < nslab) {  // into the buffer every wave finished reading
+            // the lane index read again (v_mbcnt, no inputs), so the copies' per-lane offsets
+            // are formed here instead of held in registers over the loop
+            int lanev;
+            asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lanev));
+            rck_issue<BM, BN, BK, STAGES, NW>(it, lds, (s + STAGES - 1) % STAGES, s + STAGES - 1, m0, n0, wave, lanev);
+        }
         const float* st = lds + (s % STAGES) * LY::STAGE;
         const float* As = st;
         const float* Br = st + LY::A;
@@ -841,17 +866,20 @@ void gemm_rck_kernel(const GemmItem<float>* __restrict__ items, int tiles_n_max)
                         acc_re[i][j] = mfma16(a[i][ks], br[j][ks], acc_re[i][j]);
                         acc_im[i][j] = mfma16(a[i][ks], bi[j][ks], acc_im[i][j]);
                     }
+            // keep the next group's operand reads below this group's MFMAs (six waves per SIMD
+            // hide their latency; hoisted, they would double the operand registers)
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
-    // Epilogue per 16-row block: row factors and left-factor entries loaded first (one
-    // memory latency per block), then the row sums of the unscaled product, then the scaled
-    // transposed store.
+    if (nslab == 0) __syncthreads();  // the row factors
+    // Epilogue per 16-row block: left-factor entries, then the row sums and the stores
+    const float* pr = lds + LY::POST;
 #pragma unroll
     for (int i = 0; i < MI; ++i) {
         RckEpi<NJ> ep;
-        const int rowb = m0 + wm + i * 16 + 4 * lk;
-        rck_epi_load<NJ>(it, rowb, n0 + wn, lr, ep);
-        rck_epi_store<NJ>(it, rowb, n0 + wn, lr, ep, acc_re[i], acc_im[i]);
+        const int rloc = wm + i * 16 + 4 * lk;
+        rck_epi_load<NJ>(it, m0 + rloc, n0 + wn, lr, ep);
+        rck_epi_store<NJ>(it, m0 + rloc, rloc, n0 + wn, lr, pr, pr + BM, ep, acc_re[i], acc_im[i]);
     }
 }
 
