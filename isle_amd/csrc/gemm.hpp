@@ -47,6 +47,20 @@ struct GemmItem {
 
 constexpr int GEMM_THREADS = 256;
 
+// global-memory access through an address-space-1 pointer (a generic access would also count
+// against the LDS counter and wait for the slab copies into LDS); T without constructors
+template <typename T>
+using global_ptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ T ld_global(const T* p) {
+    return *(global_ptr<const T>)p;
+}
+template <typename T>
+__device__ __forceinline__ void st_global(T* p, const T& v) {
+    *(global_ptr<T>)p = v;
+}
+
+
 // Sum over the 16 lanes of a DPP row (the lanes that share lane / 16); lane 15 of the row ends
 // with the total.  Both 32-bit halves of the double move with the same pattern.
 template <int SHR>
@@ -352,7 +366,9 @@ struct RcdLayout {
     static constexpr int B = BK * BN;            // per plane
     static constexpr int PRE = NW * 2 * BK;      // one copy of pre (re[BK], im[BK]) per wave
     static constexpr int STAGE = A + 2 * B + PRE;
-    static constexpr size_t BYTES = sizeof(T) * STAGE * RCD_STAGES;
+    static constexpr int FAC = STAGE * RCD_STAGES;  // then the tile's row or column factors
+    static constexpr int NFAC = BM > BN ? BM : BN;  // re[NFAC], im[NFAC]
+    static constexpr size_t BYTES = sizeof(T) * (FAC + 2 * NFAC);
 };
 
 // XOR applied to the chunk index of odd k rows: 8 chunks (128 bytes) for FP64 rows of a
@@ -466,6 +482,22 @@ void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items, int tiles_n_max) {
     // slabs are issued RCD_STAGES-1 ahead of the one being computed
     for (int j = 0; j < RCD_STAGES - 1 && j < nslab; ++j)
         rcd_issue<T, PRE, BM, BN, BK, RCD_STAGES, NW, POST>(it, rcd_lds, j, m0, n0, wave, lane);
+    // the tile's row factors (column factors with SCALE_COL) into LDS, read by the epilogue
+    // after the k loop's barriers (their loads in flight with the first slabs; 1 where absent)
+    constexpr bool SCALED = POST == SCALE_ROW || POST == SCALE_COL;
+    constexpr int NF = POST == SCALE_COL ? BN : BM;
+    T* fac = rcd_lds + LY::FAC;
+    if constexpr (SCALED || POST == FUSED_DIAG) {
+        const int base = POST == SCALE_COL ? n0 : m0, lim = POST == SCALE_COL ? it.n : it.m;
+        for (int t = tid; t < 2 * NF; t += 64 * NW) {
+            const int e = base + t % NF;
+            const T* src = t < NF ? it.post_re : it.post_im;
+            T v = t < NF ? T(1) : T(0);
+            if (it.post_re != nullptr && e < lim) v = ld_global(src + e);
+            fac[(t < NF ? 0 : LY::NFAC) + t % NF] = v;
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    }
     for (int s = 0; s < nslab; ++s) {
         // wait for this wave's copies of slab s (the later slabs already issued stay in
         // flight), then for all waves.  A plain s_barrier: __syncthreads() would also drain
@@ -541,35 +573,21 @@ void gemm_rcd_kernel(const GemmItem<T>* __restrict__ items, int tiles_n_max) {
     // factors, the left factor of the row sums) are all loaded, branch-free, before the first
     // of them is used: one memory latency per block instead of one per element.  Elements
     // outside the matrix load element 0 and discard it.
-    constexpr bool SCALED = POST == SCALE_ROW || POST == SCALE_COL;
     // FUSED_DIAG with a row factor (uniform over the workgroup): the stored rows are scaled
     const bool diag_rows = POST == FUSED_DIAG && it.post_re != nullptr;
+    if (nslab == 0) __syncthreads();  // the factors
 #pragma unroll
     for (int i = 0; i < MI; ++i) {
         T fr[NJ][4], fi[NJ][4];
         double lre[NJ][4], lim[NJ][4];
-        if (POST == FUSED_DIAG && diag_rows) {
+        if constexpr (SCALED || POST == FUSED_DIAG) {
 #pragma unroll
             for (int j = 0; j < NJ; ++j)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int row = m0 + wm + i * 16 + acc_row<T>(lk, r);
-                    const int di = row < it.m ? row : 0;
-                    fr[j][r] = it.post_re[di];
-                    fi[j][r] = it.post_im[di];
-                }
-        }
-        if constexpr (SCALED) {
-#pragma unroll
-            for (int j = 0; j < NJ; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = m0 + wm + i * 16 + acc_row<T>(lk, r);
-                    const int col = n0 + wn + j * 16 + lr;
-                    const bool in = row < it.m && col < it.n;
-                    const int di = in ? (POST == SCALE_ROW ? row : col) : 0;
-                    fr[j][r] = it.post_re[di];
-                    fi[j][r] = it.post_im[di];
+                    const int e = POST == SCALE_COL ? wn + j * 16 + lr : wm + i * 16 + acc_row<T>(lk, r);
+                    fr[j][r] = fac[e];
+                    fi[j][r] = fac[LY::NFAC + e];
                 }
         }
         if constexpr (POST == FUSED_DIAG) {
@@ -663,19 +681,6 @@ constexpr bool rck_shape_ok() {
     constexpr int NW = WM * WN, RPI = 64 / (BK / 4);  // rows written by one wave instruction
     return BK == 32 && BM % RPI == 0 && BN % RPI == 0 && (BM / RPI) % NW == 0 && (BN / RPI) % NW == 0 &&
            BM % (16 * WM) == 0 && BN % (16 * WN) == 0 && 64 * NW >= 2 * BM;
-}
-
-// global-memory access through an address-space-1 pointer (a generic access would also count
-// against the LDS counter and wait for the slab copies into LDS); T without constructors
-template <typename T>
-using global_ptr = __attribute__((address_space(1))) T*;
-template <typename T>
-__device__ __forceinline__ T ld_global(const T* p) {
-    return *(global_ptr<const T>)p;
-}
-template <typename T>
-__device__ __forceinline__ void st_global(T* p, const T& v) {
-    *(global_ptr<T>)p = v;
 }
 
 // This wave's direct copies of k-slab `slab` (columns slab*BK ..) into stage `stage` of the
