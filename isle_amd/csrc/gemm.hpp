@@ -81,7 +81,8 @@ __device__ __forceinline__ double row16_sum(double v) {
 // One BM x BN output tile of item `it` (tile index `tile` in a ceil(m/BM) x tiles_n_max
 // grid).  AC/BC: A/B complex.  PRE: scale the (single) complex operand along k.
 // POST: SCALE_* or SUBTRACT.  Must be called by all 256 threads of the block.
-template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int GEMM_BK, bool G4 = false>
+template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int GEMM_BK, bool G4 = false,
+          bool SB = false>
 __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int tiles_n_max) {
     static_assert(!(PRE && AC && BC), "pre-scaling needs exactly one complex operand");
     static_assert(BM % 32 == 0 && BN % 32 == 0, "tile must be a multiple of 32");
@@ -103,10 +104,13 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
 
     // A is stored transposed (k rows): consecutive threads write consecutive k, one LDS row
     // apart; one element of padding per row puts those writes in different banks
-    __shared__ T As_re[2][GEMM_BK][BM + 1];
-    __shared__ T As_im[2][AC ? GEMM_BK : 1][AC ? BM + 1 : 1];
-    __shared__ T Bs_re[2][GEMM_BK][BN];
-    __shared__ T Bs_im[2][BC ? GEMM_BK : 1][BC ? BN : 1];
+    // SB: one LDS buffer (the next slab waits in registers behind a second barrier): half the
+    // LDS, so twice the workgroups per CU
+    constexpr int NBUF = SB ? 1 : 2;
+    __shared__ T As_re[NBUF][GEMM_BK][BM + 1];
+    __shared__ T As_im[NBUF][AC ? GEMM_BK : 1][AC ? BM + 1 : 1];
+    __shared__ T Bs_re[NBUF][GEMM_BK][BN];
+    __shared__ T Bs_im[NBUF][BC ? GEMM_BK : 1][BC ? BN : 1];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -271,9 +275,15 @@ __device__ __forceinline__ void gemm_tile(const GemmItem<T>& it, int tile, int t
                     }
             }
         }
-        if (more) store_slab(buf ^ 1);
-        __syncthreads();
-        buf ^= 1;
+        if constexpr (SB) {
+            __syncthreads();  // every wave done reading the buffer
+            if (more) store_slab(0);
+            __syncthreads();
+        } else {
+            if (more) store_slab(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
     }
     if constexpr (!BLOCKED && GAUSS) {  // Gauss recombination of the three products
 #pragma unroll
@@ -338,13 +348,14 @@ __device__ __forceinline__ void xcd_map(int tiles, int nitems, int& tile, int& i
 }
 
 // grid: x = tile index over (ceil(m/BM) x ceil(n/BN)) of the largest item, y = item.
-template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int GEMM_BK, bool G4 = false>
+template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int GEMM_BK, bool G4 = false,
+          bool SB = false>
 __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmItem<T>* __restrict__ items,
                                                             int tiles_n_max) {
     int tile, item;
     xcd_map(gridDim.x, gridDim.y, tile, item);
     const GemmItem<T> it = items[item];
-    gemm_tile<T, AC, BC, PRE, POST, BM, BN, GEMM_BK, G4>(it, tile, tiles_n_max);
+    gemm_tile<T, AC, BC, PRE, POST, BM, BN, GEMM_BK, G4, SB>(it, tile, tiles_n_max);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -933,7 +944,9 @@ inline bool launch_gemm_rck_tile(const GemmItem<float>* d_items, int nitems, int
 //   direct, more waves sharing one LDS ring (48x16 per wave): 20: 96x96 slab S, 12 waves, 2 stages
 //   21: 96x96 slab S, 12 waves, 3 stages   22: 96x48 slab S, 6 waves, 2 stages (FP64 only)
 //   23: 96x96 slab S, 6 waves of 48x32, 2 stages   24: the same, 3 stages
-enum : int { N_TILES = 25 };
+//   register-staged with one LDS buffer (half the LDS: twice the workgroups per CU, a second
+//   barrier per slab): 25: 96x32, K slab 16   26: 96x32, K slab 32   27: 64x64, K slab 16
+enum : int { N_TILES = 28 };
 
 template <typename T, int POST, bool PRE, int BM, int BN, int BK, int ST = 3, int WM = 2, int WN = 2>
 inline bool launch_gemm_rcd(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s) {
@@ -952,20 +965,20 @@ inline bool launch_gemm_rcd(const GemmItem<T>* d_items, int nitems, int max_m, i
     }
 }
 
-template <typename T, bool AC, bool BC, int BM, int BN, int BK>
+template <typename T, bool AC, bool BC, int BM, int BN, int BK, bool SB = false>
 constexpr size_t gemm_lds_bytes() {
-    return sizeof(T) * 2 * BK * ((BM + 1) * (AC ? 2 : 1) + BN * (BC ? 2 : 1));
+    return sizeof(T) * (SB ? 1 : 2) * BK * ((BM + 1) * (AC ? 2 : 1) + BN * (BC ? 2 : 1));
 }
 
 // Launches the GEMM with tile shape `tile`; returns false if that shape does not fit
 // this operand type (LDS budget), in which case nothing is launched.
-template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int BK, bool G4 = false>
+template <typename T, bool AC, bool BC, bool PRE, int POST, int BM, int BN, int BK, bool G4 = false, bool SB = false>
 inline bool launch_gemm_tile(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s) {
-    if constexpr (gemm_lds_bytes<T, AC, BC, BM, BN, BK>() > 96 * 1024) {
+    if constexpr (gemm_lds_bytes<T, AC, BC, BM, BN, BK, SB>() > 96 * 1024) {
         return false;
     } else {
         const int tm = (max_m + BM - 1) / BM, tn = (max_n + BN - 1) / BN;
-        hipLaunchKernelGGL((gemm_kernel<T, AC, BC, PRE, POST, BM, BN, BK, G4>), dim3(tm * tn, nitems),
+        hipLaunchKernelGGL((gemm_kernel<T, AC, BC, PRE, POST, BM, BN, BK, G4, SB>), dim3(tm * tn, nitems),
                            dim3(GEMM_THREADS), 0, s, d_items, tn);
         ISLE_HIP(hipGetLastError());
         return true;
@@ -980,7 +993,7 @@ inline bool launch_gemm_tile(const GemmItem<T>* d_items, int nitems, int max_m, 
 template <typename T, bool AC, bool BC, bool PRE, int POST, bool G4 = false>
 inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int max_n, hipStream_t s,
                         int tile = 0, int direct_k = 0) {
-    if ((tile >= 6 && tile < 18) || tile >= 20) {
+    if ((tile >= 6 && tile < 18) || (tile >= 20 && tile < 25)) {
         if constexpr (!AC && BC && POST != SUBTRACT) {
             constexpr int BK = sizeof(T) == 4 ? 32 : 16;
             const int slab = (tile == 9 || tile == 12 || tile == 14 || tile == 17) ? 2 * BK : BK;
@@ -1019,6 +1032,9 @@ inline bool launch_gemm(const GemmItem<T>* d_items, int nitems, int max_m, int m
         case 5: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 96, 16, G4>(d_items, nitems, max_m, max_n, s);
         case 18: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 32, 32, G4>(d_items, nitems, max_m, max_n, s);
         case 19: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 32, 16, G4>(d_items, nitems, max_m, max_n, s);
+        case 25: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 32, 16, G4, true>(d_items, nitems, max_m, max_n, s);
+        case 26: return launch_gemm_tile<T, AC, BC, PRE, POST, 96, 32, 32, G4, true>(d_items, nitems, max_m, max_n, s);
+        case 27: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 16, G4, true>(d_items, nitems, max_m, max_n, s);
         default: return launch_gemm_tile<T, AC, BC, PRE, POST, 64, 64, 16, G4>(d_items, nitems, max_m, max_n, s);
         }
     }
