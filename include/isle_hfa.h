@@ -54,7 +54,8 @@ int isle_hfm_set_recompute(isle_hfm* h, int k);
  * plan's operations built before and after the call; tile -1 returns to the tuned shape.  An
  * operation with a pinned shape that does not fit one of its steps fails when run (return 3).
  * For reproducible timings and for tests that cover every shape (FP32 chain products with the
- * force diagonal fused, kind 30: tiles 0-3). */
+ * force diagonal fused, kind 30: tiles 0-3; the register-staged shapes with one LDS buffer of
+ * the complex products, kinds 3 and 18: tiles 25-27). */
 int isle_hfm_set_tile(isle_hfm* h, int kind, int tile);
 int isle_hfm_sync(isle_hfm* h);
 
